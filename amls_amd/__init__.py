@@ -1,0 +1,295 @@
+"""amls_amd: the AMLS eigensolver hot path on MI355X (gfx950).
+
+Python mirror of the reference's C++ API (src/amls/solver.hpp:18-52):
+
+    from amls_amd import AmlsSolver, AmlsOptions, Pencil
+    sol = AmlsSolver().solve(Pencil(K, M), AmlsOptions(nev=100))
+    sol.theta, sol.Z, sol.p, sol.timing
+
+All numerics run in libamls_hip.so through its C-ABI (include/amls_hip.h);
+this module only converts arrays.  `HipContext` exposes the per-phase entry
+points used by the parity tests.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Dict, Optional
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from ._lib import AmlsError, AmlsOptionsC, dptr, iptr
+
+__all__ = ["AmlsOptions", "AmlsSolver", "Solution", "Pencil", "HipContext", "AmlsError"]
+
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+@dataclass
+class AmlsOptions:
+    """AmlsOptions of src/amls/solver.hpp:18-30 (same names and defaults),
+    plus the GPU forms: precision 'fp64' | 'mixed', siSteps, siVectors."""
+    nLevels: int = 6
+    leafSize: int = 500
+    thresholdFactor: float = 2.0
+    fixedKeep: int = 0
+    nev: int = 100
+    denseProjectedMax: int = 5000
+    shift: float = 0.0
+    twoPass: bool = True
+    numThreads: int = 0
+    recoverVectors: bool = True
+    verbosity: int = 1
+    precision: str = "fp64"
+    siSteps: int = 0
+    siVectors: int = 0
+
+    def to_c(self) -> AmlsOptionsC:
+        o = AmlsOptionsC()
+        o.nLevels = self.nLevels
+        o.leafSize = self.leafSize
+        o.thresholdFactor = self.thresholdFactor
+        o.fixedKeep = self.fixedKeep
+        o.nev = self.nev
+        o.denseProjectedMax = self.denseProjectedMax
+        o.shift = self.shift
+        o.twoPass = int(bool(self.twoPass))
+        o.numThreads = self.numThreads
+        o.recoverVectors = int(bool(self.recoverVectors))
+        o.verbosity = self.verbosity
+        o.precision = PRECISIONS[self.precision]
+        o.siSteps = self.siSteps
+        o.siVectors = self.siVectors
+        return o
+
+
+@dataclass
+class Solution:
+    """Solution of src/amls/solver.hpp:38-43."""
+    theta: np.ndarray
+    Z: Optional[np.ndarray]
+    p: int
+    timing: Dict[str, float] = field(default_factory=dict)
+
+
+@dataclass
+class Pencil:
+    """Pencil{K, M} (src/core/types.hpp:24-28); coords: optional (n, 3) dof
+    coordinates that let the partitioner cut along node planes."""
+    K: sp.spmatrix
+    M: sp.spmatrix
+    coords: Optional[np.ndarray] = None
+
+
+def _csc(A):
+    A = sp.csc_matrix(A)
+    if not A.has_sorted_indices:
+        A = A.sorted_indices()
+    return (np.ascontiguousarray(A.indptr, dtype=np.int64),
+            np.ascontiguousarray(A.indices, dtype=np.int64),
+            np.ascontiguousarray(A.data, dtype=np.float64))
+
+
+class HipContext:
+    """One amls_ctx.  device < 0: host-only (partition and analysis only)."""
+
+    def __init__(self, device: int = 0):
+        self._L = _lib.lib()
+        h = C.c_void_p()
+        st = self._L.amls_hip_create(device, 0, C.byref(h))
+        if st != 0:
+            raise AmlsError(st, f"amls_hip_create(device={device}) failed")
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.amls_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st: int):
+        if st != 0:
+            msg = self._L.amls_hip_last_error(self._h)
+            raise AmlsError(st, msg.decode() if msg else "")
+
+    # ---- analysis --------------------------------------------------------
+    def set_problem(self, K, M, coords=None, nLevels: int = 6, leafSize: int = 500,
+                    shift: float = 0.0):
+        kp, ki, kx = _csc(K)
+        mp, mi, mx = _csc(M)
+        self._keep = (kp, ki, kx, mp, mi, mx)
+        cc = None if coords is None else np.ascontiguousarray(coords, dtype=np.float64)
+        self._chk(self._L.amls_hip_set_problem(self._h, K.shape[0], iptr(kp), iptr(ki), dptr(kx),
+                                               iptr(mp), iptr(mi), dptr(mx), dptr(cc),
+                                               nLevels, leafSize, shift))
+        self.n = K.shape[0]
+
+    def tree(self) -> dict:
+        nn, ml, n = C.c_int64(), C.c_int64(), C.c_int64()
+        self._chk(self._L.amls_hip_tree_info(self._h, C.byref(nn), C.byref(ml), C.byref(n)))
+        N = nn.value
+        out = {k: np.zeros(N, dtype=np.int64)
+               for k in ("treeId", "offset", "size", "parent", "level", "bsize")}
+        perm = np.zeros(n.value, dtype=np.int64)
+        self._chk(self._L.amls_hip_tree_get(self._h, iptr(out["treeId"]), iptr(out["offset"]),
+                                            iptr(out["size"]), iptr(out["parent"]),
+                                            iptr(out["level"]), iptr(out["bsize"]), iptr(perm)))
+        out["perm"] = perm
+        out["maxLevel"] = ml.value
+        return out
+
+    # ---- phases ----------------------------------------------------------
+    def ge(self, precision: str = "fp64"):
+        self._chk(self._L.amls_hip_ge(self._h, PRECISIONS[precision]))
+
+    def spectral_prepare(self):
+        self._chk(self._L.amls_hip_spectral_prepare(self._h))
+
+    def spectral(self, kind: str, value: float) -> np.ndarray:
+        N = self.tree()["size"].shape[0]
+        k = np.zeros(N, dtype=np.int64)
+        self._chk(self._L.amls_hip_spectral(self._h, {"fixed": 0, "threshold": 1}[kind],
+                                            float(value), iptr(k)))
+        return k
+
+    def project(self, precision: str = "fp64") -> int:
+        p = C.c_int64()
+        self._chk(self._L.amls_hip_project(self._h, PRECISIONS[precision], C.byref(p)))
+        return p.value
+
+    def reduced_solve(self, want: int, vectors: bool = True) -> np.ndarray:
+        th = np.zeros(want, dtype=np.float64)
+        self._chk(self._L.amls_hip_reduced_solve(self._h, want, int(vectors), dptr(th)))
+        return th
+
+    def backtransform(self, want: int) -> np.ndarray:
+        Z = np.zeros((self.n, want), dtype=np.float64, order="F")
+        self._chk(self._L.amls_hip_backtransform(self._h, dptr(Z)))
+        return Z
+
+    def subspace_iter(self, steps: int, q: int, nev: int):
+        th = np.zeros(nev, dtype=np.float64)
+        Z = np.zeros((self.n, nev), dtype=np.float64, order="F")
+        self._chk(self._L.amls_hip_subspace_iter(self._h, steps, q, nev, dptr(th), dptr(Z)))
+        return th, Z
+
+    # ---- measurement -----------------------------------------------------
+    def profile(self, enable: bool = True):
+        self._chk(self._L.amls_hip_profile(self._h, int(enable)))
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        i = 0
+        while True:
+            name = C.create_string_buffer(64)
+            la, ms, fl, by = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+            st = self._L.amls_hip_kernel_stats(self._h, i, name, 64, C.byref(la), C.byref(ms),
+                                               C.byref(fl), C.byref(by))
+            if st != 0:
+                break
+            out[name.value.decode()] = {"launches": la.value, "ms": ms.value,
+                                        "flops": fl.value, "bytes": by.value}
+            i += 1
+        return out
+
+    def solve_numeric(self, opt: "AmlsOptions", vectors: bool = False):
+        """Numeric solve on the problem already set (inputs resident in HBM)."""
+        nev = opt.nev
+        theta = np.zeros(max(nev, 1))
+        Z = np.zeros((self.n, max(nev, 1)), order="F") if vectors else None
+        p = C.c_int64()
+        timing = np.zeros(8)
+        oc = opt.to_c()
+        self._chk(self._L.amls_hip_solve_numeric(self._h, C.byref(oc), dptr(theta), dptr(Z),
+                                                 C.byref(p), dptr(timing)))
+        return theta, Z, p.value, dict(zip(AmlsSolver.TIMING_KEYS, timing.tolist()))
+
+    # ---- inspection ------------------------------------------------------
+    def node_factor(self, p: int, n_p: int):
+        LD = np.zeros((n_p, n_p), order="F")
+        Mh = np.zeros((n_p, n_p), order="F")
+        self._chk(self._L.amls_hip_get_node_factor(self._h, p, dptr(LD), dptr(Mh)))
+        return LD, Mh
+
+    def local(self, p: int, n_p: int, k_p: int):
+        mu = np.zeros(max(k_p, 1))
+        Z = np.zeros((n_p, max(k_p, 1)), order="F")
+        self._chk(self._L.amls_hip_get_local(self._h, p, dptr(mu), dptr(Z)))
+        return mu[:k_p], Z[:, :k_p]
+
+    def reduced(self, ptot: int, nnodes: int):
+        Mt = np.zeros((ptot, ptot), order="F")
+        moff = np.zeros(nnodes, dtype=np.int64)
+        self._chk(self._L.amls_hip_get_reduced(self._h, dptr(Mt), iptr(moff)))
+        return Mt, moff
+
+    # ---- operator-granular ---------------------------------------------
+    def gemm(self, A, B, C0=None, alpha=1.0, beta=0.0, transA=False, transB=False,
+             precision="fp64"):
+        A = np.asfortranarray(A, dtype=np.float64)
+        B = np.asfortranarray(B, dtype=np.float64)
+        m = A.shape[1] if transA else A.shape[0]
+        k = A.shape[0] if transA else A.shape[1]
+        n = B.shape[0] if transB else B.shape[1]
+        Cm = np.zeros((m, n), order="F") if C0 is None else np.asfortranarray(C0, dtype=np.float64).copy(order="F")
+        self._chk(self._L.amls_hip_gemm(self._h, PRECISIONS[precision], int(transA), int(transB),
+                                        m, n, k, alpha, dptr(A), max(A.shape[0], 1), dptr(B),
+                                        max(B.shape[0], 1), beta, dptr(Cm), max(m, 1)))
+        return Cm
+
+    def ldlt_solve(self, A, B):
+        A = np.asfortranarray(A, dtype=np.float64)
+        B = np.asfortranarray(B, dtype=np.float64)
+        n = A.shape[0]
+        LD = np.zeros((n, n), order="F")
+        X = np.zeros_like(B, order="F")
+        self._chk(self._L.amls_hip_ldlt_solve(self._h, n, dptr(A), B.shape[1], dptr(B),
+                                              dptr(LD), dptr(X)))
+        return LD, X
+
+    def syevx_top(self, A, k: int):
+        A = np.asfortranarray(A, dtype=np.float64)
+        n = A.shape[0]
+        lam = np.zeros(max(k, 1))
+        V = np.zeros((n, max(k, 1)), order="F")
+        self._chk(self._L.amls_hip_syevx_top(self._h, n, dptr(A), k, dptr(lam), dptr(V)))
+        return lam[:k], V[:, :k]
+
+
+class AmlsSolver:
+    """AmlsSolver::solve (src/amls/solver.cpp:100-319) on one GPU."""
+
+    TIMING_KEYS = ("partition", "gauss", "condense", "reduced", "backtransform", "subspace",
+                   "total", "p0")
+
+    def __init__(self, device: int = 0):
+        self.ctx = HipContext(device)
+
+    def solve(self, pencil: Pencil, opt: AmlsOptions) -> Solution:
+        K, M = pencil.K, pencil.M
+        n = K.shape[0]
+        nev = min(opt.nev, n) if n > 0 else opt.nev
+        kp, ki, kx = _csc(K)
+        mp, mi, mx = _csc(M)
+        cc = None if pencil.coords is None else np.ascontiguousarray(pencil.coords, dtype=np.float64)
+        theta = np.zeros(max(nev, 1))
+        Z = np.zeros((n, max(nev, 1)), order="F") if opt.recoverVectors else None
+        p = C.c_int64()
+        timing = np.zeros(8)
+        oc = opt.to_c()
+        L = self.ctx._L
+        st = L.amls_hip_solve(self.ctx._h, n, iptr(kp), iptr(ki), dptr(kx), iptr(mp), iptr(mi),
+                              dptr(mx), dptr(cc), C.byref(oc), dptr(theta), dptr(Z), C.byref(p),
+                              dptr(timing))
+        self.ctx._chk(st)
+        m = min(nev, p.value) if opt.siSteps == 0 else nev
+        return Solution(theta=theta[:m].copy(), Z=None if Z is None else Z[:, :m].copy(order="F"),
+                        p=p.value, timing=dict(zip(self.TIMING_KEYS, timing.tolist())))

@@ -1,0 +1,2141 @@
+// Engine and C-ABI of libamls_hip.so.
+//
+// Host orchestration of the AMLS pipeline of AmlsSolver::solve
+// (src/amls/solver.cpp:100-319) over the batched HIP kernels.  The host
+// builds one descriptor blob per phase, uploads it once and then issues the
+// level-ordered launch sequence on the context's stream; the only device ->
+// host synchronisations are the ones the algorithm needs (the pass-A
+// eigenvalue lambda_hat, the per-node counts k_i of pass B, the reduced
+// eigenvalues, and the error word of the factorisations).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <functional>
+
+#include "../../include/amls_hip.h"
+
+namespace amls {
+
+using dev::Copy2dDesc;
+using dev::EigDesc;
+using dev::ExtAddDesc;
+using dev::FrontDesc;
+using dev::GatherDesc;
+using dev::GemmDesc;
+using dev::TileRef;
+using dev::TrsmDesc;
+using dev::TrsmWork;
+using dev::VecDesc;
+
+namespace {
+
+inline int cdiv(int64_t a, int64_t b) { return static_cast<int>((a + b - 1) / b); }
+
+using Clock = std::chrono::steady_clock;
+inline double secs(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double>(b - a).count();
+}
+
+GemmDesc gdesc() {
+    GemmDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.alpha1 = 1.0;
+    d.alpha2 = 1.0;
+    return d;
+}
+
+Copy2dDesc cdesc(const double* src, int64_t lds, double* dst, int64_t ldd, int64_t rows,
+                 int64_t cols, int mode, const double* rs = nullptr, const double* cs = nullptr,
+                 double alpha = 1.0) {
+    Copy2dDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.src = src;
+    d.dst = dst;
+    d.lds = lds;
+    d.ldd = ldd;
+    d.rows = static_cast<int32_t>(rows);
+    d.cols = static_cast<int32_t>(cols);
+    d.mode = mode;
+    d.rs = rs;
+    d.cs = cs;
+    d.alpha = alpha;
+    return d;
+}
+
+struct Copy2dBatch {
+    std::vector<Copy2dDesc> descs;
+    std::vector<int4> work;
+    void add(const Copy2dDesc& d) {
+        if (d.rows <= 0 || d.cols <= 0) return;
+        const int id = static_cast<int>(descs.size());
+        descs.push_back(d);
+        for (int i = 0; i < cdiv(d.rows, 64); ++i)
+            for (int j = 0; j < cdiv(d.cols, 64); ++j) work.push_back(make_int4(id, i, j, 0));
+    }
+};
+
+struct ExtBatch {
+    std::vector<ExtAddDesc> descs;
+    std::vector<int4> work;
+    void add(const ExtAddDesc& d) {
+        if (d.rows <= 0 || d.cols <= 0) return;
+        const int id = static_cast<int>(descs.size());
+        descs.push_back(d);
+        for (int i = 0; i < cdiv(d.rows, 64); ++i)
+            for (int j = 0; j < cdiv(d.cols, 64); ++j) {
+                if (d.symmetric && j > i) continue;
+                work.push_back(make_int4(id, i, j, 0));
+            }
+    }
+};
+
+struct TrsmBatch {
+    std::vector<TrsmDesc> descs;
+    std::vector<TrsmWork> work;
+    void add(const double* L, int64_t ldl, double* B, int64_t ldb, int64_t n, int64_t nrhs,
+             int trans) {
+        if (n <= 0 || nrhs <= 0) return;
+        TrsmDesc d;
+        std::memset(&d, 0, sizeof(d));
+        d.L = L;
+        d.B = B;
+        d.ldl = ldl;
+        d.ldb = ldb;
+        d.n = static_cast<int32_t>(n);
+        d.nrhs = static_cast<int32_t>(nrhs);
+        d.trans = trans;
+        const int id = static_cast<int>(descs.size());
+        descs.push_back(d);
+        for (int c = 0; c < cdiv(nrhs, 64); ++c) work.push_back({id, c, 0, 0});
+    }
+};
+
+struct GatherBatch {
+    std::vector<GatherDesc> descs;
+    std::vector<int4> work;
+    void add(const GatherDesc& d) {
+        if (d.rows <= 0 || d.cols <= 0) return;
+        const int id = static_cast<int>(descs.size());
+        descs.push_back(d);
+        for (int i = 0; i < cdiv(d.rows, 64); ++i)
+            for (int j = 0; j < cdiv(d.cols, 64); ++j) work.push_back(make_int4(id, i, j, 0));
+    }
+};
+
+// A sequence of launches whose descriptor arrays live in one blob.
+// algorithmic work of a GEMM batch: 2 m n k flops per product and
+// (m k + k n + 2 m n) words, counting only the written triangle for `lower`
+void gemmWork(const GemmBatch& b, bool fp32, double& flops, double& bytes) {
+    flops = bytes = 0.0;
+    const double w = fp32 ? 4.0 : 8.0;
+    for (const auto& d : b.descs) {
+        const double m = d.m, n = d.n;
+        double mn = m * n;
+        if (d.lower && m >= n) mn = m * n - n * (n - 1) / 2;
+        const double k = (d.A ? d.k : 0) + (d.A2 ? d.k2 : 0);
+        flops += 2.0 * mn * k;
+        bytes += (m * k + k * n + 2.0 * mn) * w;
+    }
+}
+
+struct Plan {
+    Blob* blob;
+    KernelStats* st;
+    std::vector<std::function<void(hipStream_t)>> steps;
+    Plan(Blob* b, KernelStats* k) : blob(b), st(k) {}
+
+    void timed(const std::string& name, double flops, double bytes,
+               std::function<void(hipStream_t)> f) {
+        KernelStats* k = st;
+        if (!k) {
+            steps.push_back(std::move(f));
+            return;
+        }
+        steps.push_back([=](hipStream_t s) { k->run(name, flops, bytes, s, [&] { f(s); }); });
+    }
+
+    void gemm(const GemmBatch& b, bool fp32) {
+        if (b.empty()) return;
+        const size_t od = blob->add(b.descs), ot = blob->add(b.tiles);
+        const int nt = static_cast<int>(b.tiles.size());
+        Blob* bl = blob;
+        double fl, by;
+        gemmWork(b, fp32, fl, by);
+        timed(fp32 ? "gemm_f32" : "gemm_f64", fl, by, [=](hipStream_t s) {
+            launch_gemm(bl->at<GemmDesc>(od), bl->at<TileRef>(ot), nt, fp32, s);
+        });
+    }
+    void copy(const Copy2dBatch& b) {
+        if (b.work.empty()) return;
+        const size_t od = blob->add(b.descs), ow = blob->add(b.work);
+        const int nw = static_cast<int>(b.work.size());
+        Blob* bl = blob;
+        steps.push_back([=](hipStream_t s) {
+            launch_copy2d(bl->at<Copy2dDesc>(od), bl->at<int4>(ow), nw, s);
+        });
+    }
+    void ext(const ExtBatch& b) {
+        if (b.work.empty()) return;
+        const size_t od = blob->add(b.descs), ow = blob->add(b.work);
+        const int nw = static_cast<int>(b.work.size());
+        Blob* bl = blob;
+        steps.push_back([=](hipStream_t s) {
+            launch_extadd(bl->at<ExtAddDesc>(od), bl->at<int4>(ow), nw, s);
+        });
+    }
+    void trsm(const TrsmBatch& b) {
+        if (b.work.empty()) return;
+        const size_t od = blob->add(b.descs), ow = blob->add(b.work);
+        const int nw = static_cast<int>(b.work.size());
+        Blob* bl = blob;
+        double fl = 0, by = 0;
+        for (const auto& d : b.descs) {
+            const double n = d.n, r = d.nrhs;
+            fl += n * (n - 1) * r;
+            by += (n * (n + 1) / 2 + 2 * n * r) * 8.0;
+        }
+        timed("trsm", fl, by, [=](hipStream_t s) {
+            launch_trsm(bl->at<TrsmDesc>(od), bl->at<TrsmWork>(ow), nw, s);
+        });
+    }
+    void gather(const GatherBatch& b) {
+        if (b.work.empty()) return;
+        const size_t od = blob->add(b.descs), ow = blob->add(b.work);
+        const int nw = static_cast<int>(b.work.size());
+        Blob* bl = blob;
+        steps.push_back([=](hipStream_t s) {
+            launch_gather(bl->at<GatherDesc>(od), bl->at<int4>(ow), nw, s);
+        });
+    }
+    void vec(const std::vector<VecDesc>& v) {
+        if (v.empty()) return;
+        const size_t od = blob->add(v);
+        const int nv = static_cast<int>(v.size());
+        Blob* bl = blob;
+        steps.push_back([=](hipStream_t s) { launch_vecop(bl->at<VecDesc>(od), nv, s); });
+    }
+    void fn(std::function<void(hipStream_t)> f) { steps.push_back(std::move(f)); }
+    void run(hipStream_t s) {
+        blob->upload(s);
+        for (auto& f : steps) f(s);
+        AMLS_HIP_CHECK(hipGetLastError());
+    }
+};
+
+// Tridiagonalisation schedule for a batch of matrices: launch i processes
+// every matrix with n_m > i; slices of 64 rows below the diagonal.
+struct SytrdPlan {
+    std::vector<int2> work;
+    std::vector<size_t> start;   // per step
+    std::vector<int> count;
+    int steps = 0, nmax = 0;
+    void build(const std::vector<int>& ns) {
+        for (int v : ns) nmax = std::max(nmax, v);
+        steps = nmax;
+        start.assign(steps, 0);
+        count.assign(steps, 0);
+        for (int i = 0; i < steps; ++i) {
+            start[i] = work.size();
+            for (int m = 0; m < static_cast<int>(ns.size()); ++m) {
+                const int n = ns[m];
+                if (n <= i) continue;
+                const int rows = n - i - 1;
+                const int sl = std::max(1, cdiv(rows, 64));
+                for (int y = 0; y < sl; ++y) work.push_back(make_int2(m, y));
+            }
+            count[i] = static_cast<int>(work.size() - start[i]);
+        }
+    }
+};
+
+void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<int64_t>& soff,
+              double* sv, double* sp) {
+    std::vector<int> ns;
+    for (const auto& d : descs) ns.push_back(d.n);
+    SytrdPlan sp_;
+    sp_.build(ns);
+    if (sp_.steps == 0) return;
+    const size_t od = plan.blob->add(descs), ow = plan.blob->add(sp_.work),
+                 oo = plan.blob->add(soff);
+    Blob* bl = plan.blob;
+    const int nmax = sp_.nmax;
+    const auto start = sp_.start;
+    const auto count = sp_.count;
+    // per step: each active matrix reads and writes its trailing block once
+    std::vector<double> stepBytes(start.size(), 0.0);
+    for (int i = 0; i < static_cast<int>(start.size()); ++i)
+        for (int v : ns)
+            if (v > i) stepBytes[i] += 2.0 * 8.0 * double(v - i - 1) * double(v - i - 1);
+    KernelStats* kst = plan.st;
+    plan.fn([=](hipStream_t s) {
+        for (int i = 0; i < static_cast<int>(start.size()); ++i) {
+            auto go = [&] {
+                launch_sytrd_step(bl->at<EigDesc>(od), bl->at<int2>(ow) + start[i], count[i], i,
+                                  sv, sp, bl->at<int64_t>(oo), nmax, s);
+            };
+            if (kst) kst->run("sytrd_step", 2.0 * stepBytes[i] / 8.0, stepBytes[i], s, go);
+            else go();
+        }
+    });
+}
+
+// Bisection, inverse iteration and back-transformation for descriptors with
+// lam/Y/work/k set.  woffs: per (matrix, j) scratch offsets (5 n each).
+void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
+    std::vector<int2> work;
+    std::vector<int64_t> woff;
+    for (int m = 0; m < static_cast<int>(descs.size()); ++m)
+        for (int j = 0; j < descs[m].k; ++j) {
+            work.push_back(make_int2(m, j));
+            woff.push_back(static_cast<int64_t>(j) * 5 * descs[m].n);
+        }
+    if (work.empty()) return;
+    const size_t od = plan.blob->add(descs), ow = plan.blob->add(work), oo = plan.blob->add(woff);
+    Blob* bl = plan.blob;
+    const int nw = static_cast<int>(work.size());
+    plan.fn([=](hipStream_t s) {
+        launch_bisect(bl->at<EigDesc>(od), bl->at<int2>(ow), nw, s);
+        if (vectors) {
+            launch_stein(bl->at<EigDesc>(od), bl->at<int2>(ow), nw, bl->at<int64_t>(oo), s);
+            launch_ormtr(bl->at<EigDesc>(od), bl->at<int2>(ow), nw, s);
+        }
+    });
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+DevArena::~DevArena() { release(); }
+
+void DevArena::commit() {
+    release();
+    // every take() rounds its request to 256 bytes; leave room for that
+    planned_ += 64 * 1024;
+    AMLS_HIP_CHECK(hipMalloc(&base_, planned_));
+    cap_ = planned_;
+    used_ = 0;
+}
+
+void DevArena::release() {
+    if (base_) (void)hipFree(base_);
+    base_ = nullptr;
+    cap_ = used_ = 0;
+}
+
+hipEvent_t KernelStats::get() {
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    AMLS_HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+void KernelStats::run(const std::string& name, double flops, double bytes, hipStream_t s,
+                      const std::function<void()>& launch) {
+    auto& en = byName[name];
+    en.launches += 1;
+    en.flops += flops;
+    en.bytes += bytes;
+    if (!enabled) {
+        launch();
+        return;
+    }
+    hipEvent_t a = get(), b = get();
+    AMLS_HIP_CHECK(hipEventRecord(a, s));
+    launch();
+    AMLS_HIP_CHECK(hipEventRecord(b, s));
+    pending.push_back({name, a, b});
+}
+
+void KernelStats::resolve() {
+    for (auto& p : pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess)
+            byName[p.name].ms += ms;
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+    }
+    pending.clear();
+}
+
+void KernelStats::reset() {
+    resolve();
+    byName.clear();
+}
+
+KernelStats::~KernelStats() {
+    for (auto& p : pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+}
+
+Blob::~Blob() {
+    if (dev_) (void)hipFree(dev_);
+}
+
+void Blob::upload(hipStream_t s) {
+    if (host_.empty()) return;
+    if (devcap_ < host_.size()) {
+        if (dev_) {
+            AMLS_HIP_CHECK(hipStreamSynchronize(s));
+            (void)hipFree(dev_);
+        }
+        AMLS_HIP_CHECK(hipMalloc(&dev_, host_.size()));
+        devcap_ = host_.size();
+    }
+    AMLS_HIP_CHECK(hipMemcpyAsync(dev_, host_.data(), host_.size(), hipMemcpyHostToDevice, s));
+}
+
+void GemmBatch::add(const GemmDesc& d) {
+    if (d.m <= 0 || d.n <= 0) return;
+    const int id = static_cast<int>(descs.size());
+    descs.push_back(d);
+    const int tm = cdiv(d.m, 64), tn = cdiv(d.n, 64);
+    for (int i = 0; i < tm; ++i)
+        for (int j = 0; j < tn; ++j) {
+            if (d.lower && j > i) continue;
+            tiles.push_back({id, i, j, 0});
+        }
+}
+
+void Ctx::sync() {
+    AMLS_HIP_CHECK(hipStreamSynchronize(stream));
+    kstats.resolve();
+}
+
+// ---------------------------------------------------------------------------
+// Analysis and upload
+// ---------------------------------------------------------------------------
+void Ctx::setProblem(int64_t nn, const int64_t* kp, const int64_t* ki, const double* kx,
+                     const int64_t* mp, const int64_t* mi, const double* mx, const double* coords,
+                     int nLevels, int64_t leafSize, double sh) {
+    if (nn <= 0) throw StatusError(AMLS_E_ARG, "empty matrix");
+    if (!kp || !ki || !kx || !mp || !mi || !mx) throw StatusError(AMLS_E_ARG, "null matrix array");
+    haveProblem = false;
+    geDone = specReady = selected = projected = reduced = recovered = false;
+    n = nn;
+    shift = sh;
+    Kp.assign(kp, kp + n + 1);
+    Ki.assign(ki, ki + Kp[n]);
+    Kx.assign(kx, kx + Kp[n]);
+    Mp.assign(mp, mp + n + 1);
+    Mi.assign(mi, mi + Mp[n]);
+    Mx.assign(mx, mx + Mp[n]);
+    for (int64_t e = 0; e < Kp[n]; ++e)
+        if (Ki[e] < 0 || Ki[e] >= n) throw StatusError(AMLS_E_ARG, "K row index out of range");
+    for (int64_t e = 0; e < Mp[n]; ++e)
+        if (Mi[e] < 0 || Mi[e] >= n) throw StatusError(AMLS_E_ARG, "M row index out of range");
+    g = graphFromCsc(n, Kp.data(), Ki.data(), Mp.data(), Mi.data());
+    part = recursiveBisection(g, coords, nLevels, leafSize);
+    S = analyze(part, g);
+    const int64_t N = S.nNodes;
+    owner.assign(static_cast<size_t>(n), 0);
+    for (int64_t p = 0; p < N; ++p)
+        for (int64_t j = S.off[p]; j < S.off[p] + S.size[p]; ++j) owner[j] = p;
+    fo.assign(N + 1, 0);
+    xo.assign(N + 1, 0);
+    wo.assign(N + 1, 0);
+    for (int64_t p = 0; p < N; ++p) {
+        fo[p + 1] = fo[p] + S.f(p) * S.f(p);
+        xo[p + 1] = xo[p] + S.size[p] * S.b(p);
+        wo[p + 1] = wo[p] + S.f(p) * 64;
+    }
+    ftot = fo[N];
+    xtot = xo[N];
+    wtot = wo[N];
+    haveProblem = true;
+    if (!gpu) return;
+
+    // assembly maps: lower entries (row >= col in the permuted ordering) of
+    // each matrix -> their slot in the column owner's front
+    auto buildMap = [&](const std::vector<int64_t>& cp, const std::vector<int64_t>& ri,
+                        std::vector<int64_t>& src, std::vector<int64_t>& tgt) {
+        src.clear();
+        tgt.clear();
+        for (int64_t c = 0; c < n; ++c) {
+            const int64_t cn = S.iperm[c];
+            const int64_t p = owner[cn];
+            const int64_t o = S.off[p], np = S.size[p], f = S.f(p);
+            const int64_t* B = S.bidx.data() + S.bptr[p];
+            const int64_t nb = S.b(p);
+            for (int64_t e = cp[c]; e < cp[c + 1]; ++e) {
+                const int64_t rn = S.iperm[ri[e]];
+                if (rn < cn) continue;
+                int64_t lr;
+                if (rn < o + np) {
+                    lr = rn - o;
+                } else {
+                    const int64_t* it = std::lower_bound(B, B + nb, rn);
+                    if (it == B + nb || *it != rn)
+                        throw StatusError(AMLS_E_INTERNAL, "entry outside its front");
+                    lr = np + (it - B);
+                }
+                src.push_back(e);
+                tgt.push_back(fo[p] + lr + (cn - o) * f);
+            }
+        }
+    };
+    std::vector<int64_t> srcK, tgtK, srcM, tgtM;
+    buildMap(Kp, Ki, srcK, tgtK);
+    buildMap(Mp, Mi, srcM, tgtM);
+    nasmK = static_cast<int64_t>(srcK.size());
+    nasmM = static_cast<int64_t>(srcM.size());
+    // permuted CSR (= CSC of the symmetric permuted matrix) of K and M
+    auto permCsr = [&](const std::vector<int64_t>& cp, const std::vector<int64_t>& ri,
+                       const std::vector<double>& vx, std::vector<int64_t>& pp,
+                       std::vector<int64_t>& pi, std::vector<double>& px) {
+        pp.assign(n + 1, 0);
+        for (int64_t c = 0; c < n; ++c) pp[S.iperm[c] + 1] = cp[c + 1] - cp[c];
+        for (int64_t j = 0; j < n; ++j) pp[j + 1] += pp[j];
+        pi.resize(cp[n]);
+        px.resize(cp[n]);
+        for (int64_t c = 0; c < n; ++c) {
+            int64_t w = pp[S.iperm[c]];
+            for (int64_t e = cp[c]; e < cp[c + 1]; ++e) {
+                pi[w] = S.iperm[ri[e]];
+                px[w] = vx[e];
+                ++w;
+            }
+        }
+    };
+    std::vector<int64_t> pkp, pki, pmp, pmi;
+    std::vector<double> pkx, pmx;
+    permCsr(Kp, Ki, Kx, pkp, pki, pkx);
+    permCsr(Mp, Mi, Mx, pmp, pmi, pmx);
+
+    probMem.reset(new DevArena());
+    auto& A = *probMem;
+    const int64_t nk = Kp[n], nm = Mp[n];
+    A.plan(nk * 8 * 2 + (n + 1) * 8);        // Kx, Ki, Kp
+    A.plan(nm * 8 * 2 + (n + 1) * 8);
+    A.plan(nasmK * 16 + nasmM * 16);
+    A.plan(n * 8);                           // perm
+    A.plan(S.bidx.size() * 8 * 2);           // bidx, rel
+    A.plan(nk * 16 + (n + 1) * 8 + nm * 16 + (n + 1) * 8);   // permuted CSR
+    A.commit();
+    dKx = A.take<double>(nk);
+    dKi = A.take<int64_t>(nk);
+    dKp = A.take<int64_t>(n + 1);
+    dMx = A.take<double>(nm);
+    dMi = A.take<int64_t>(nm);
+    dMp = A.take<int64_t>(n + 1);
+    dsrcK = A.take<int64_t>(nasmK);
+    dtgtK = A.take<int64_t>(nasmK);
+    dsrcM = A.take<int64_t>(nasmM);
+    dtgtM = A.take<int64_t>(nasmM);
+    dperm = A.take<int64_t>(n);
+    dbidx = A.take<int64_t>(S.bidx.size());
+    drel = A.take<int64_t>(S.rel.size());
+    dPKx = A.take<double>(nk);
+    dPKi = A.take<int64_t>(nk);
+    dPKp = A.take<int64_t>(n + 1);
+    dPMx = A.take<double>(nm);
+    dPMi = A.take<int64_t>(nm);
+    dPMp = A.take<int64_t>(n + 1);
+    auto up = [&](void* d, const void* h, size_t bytes) {
+        if (bytes) AMLS_HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
+    };
+    up(dKx, Kx.data(), nk * 8);
+    up(dKi, Ki.data(), nk * 8);
+    up(dKp, Kp.data(), (n + 1) * 8);
+    up(dMx, Mx.data(), nm * 8);
+    up(dMi, Mi.data(), nm * 8);
+    up(dMp, Mp.data(), (n + 1) * 8);
+    up(dsrcK, srcK.data(), nasmK * 8);
+    up(dtgtK, tgtK.data(), nasmK * 8);
+    up(dsrcM, srcM.data(), nasmM * 8);
+    up(dtgtM, tgtM.data(), nasmM * 8);
+    up(dperm, S.perm.data(), n * 8);
+    up(dbidx, S.bidx.data(), S.bidx.size() * 8);
+    up(drel, S.rel.data(), S.rel.size() * 8);
+    up(dPKx, pkx.data(), nk * 8);
+    up(dPKi, pki.data(), nk * 8);
+    up(dPKp, pkp.data(), (n + 1) * 8);
+    up(dPMx, pmx.data(), nm * 8);
+    up(dPMi, pmi.data(), nm * 8);
+    up(dPMp, pmp.data(), (n + 1) * 8);
+    sync();   // host vectors above go out of scope
+}
+
+// ---------------------------------------------------------------------------
+// Block Gaussian elimination (multifrontal form)
+// ---------------------------------------------------------------------------
+void Ctx::runGE(int prec) {
+    requireGpu();
+    sync();   // earlier phases may still use buffers released below
+    if (!haveProblem) throw StatusError(AMLS_E_STATE, "no problem set");
+    geDone = specReady = selected = projected = reduced = recovered = false;
+    gePrec = prec;
+    const int64_t N = S.nNodes;
+    geMem.reset(new DevArena());
+    auto& A = *geMem;
+    A.plan(ftot * 8);
+    A.plan(ftot * 8);
+    A.plan(xtot * 8);
+    A.plan(xtot * 8);
+    A.plan(wtot * 8);
+    A.plan(256);
+    A.commit();
+    dF = A.take<double>(ftot);
+    dG = A.take<double>(ftot);
+    dXb = A.take<double>(xtot);
+    dFm = A.take<double>(xtot);
+    dW = A.take<double>(wtot);
+    dErr = A.take<int>(1);
+    const int big = INT_MAX;
+    AMLS_HIP_CHECK(hipMemcpyAsync(dErr, &big, sizeof(int), hipMemcpyHostToDevice, stream));
+    launch_fill(dF, ftot, 0.0, stream);
+    launch_fill(dG, ftot, 0.0, stream);
+    launch_scatter(dsrcK, dtgtK, dKx, dF, nasmK, 1.0, false, stream);
+    launch_scatter(dsrcM, dtgtM, dMx, dG, nasmM, 1.0, false, stream);
+    if (shift != 0.0) launch_scatter(dsrcM, dtgtM, dMx, dF, nasmM, shift, true, stream);
+
+    geBlob.reset(new Blob());
+    Plan plan(geBlob.get(), &kstats);
+    const bool f32 = prec != 0;
+    for (int64_t l = 0; l <= S.maxLevel; ++l) {
+        const auto& nodes = S.byLevel[l];
+        // children's update blocks into this level's fronts (two launches:
+        // the two children of one parent write the same front)
+        for (int slot = 0; slot < 2; ++slot) {
+            ExtBatch eb;
+            for (int64_t p : nodes) {
+                const int64_t c = slot == 0 ? S.child0[p] : S.child1[p];
+                if (c < 0 || S.b(c) == 0) continue;
+                for (double* base : {dF, dG}) {
+                    ExtAddDesc d;
+                    std::memset(&d, 0, sizeof(d));
+                    d.src = base + fo[c] + S.size[c] * (1 + S.f(c));
+                    d.dst = base + fo[p];
+                    d.rowmap = drel + S.bptr[c];
+                    d.lds = S.f(c);
+                    d.ldd = S.f(p);
+                    d.rows = d.cols = static_cast<int32_t>(S.b(c));
+                    d.symmetric = 1;
+                    eb.add(d);
+                }
+            }
+            plan.ext(eb);
+        }
+        // partial LDL^T of every front of the level, 64-column panels
+        std::vector<FrontDesc> fds;
+        int64_t nmaxl = 0;
+        for (int64_t p : nodes) {
+            FrontDesc fd;
+            fd.F = dF + fo[p];
+            fd.W = dW + wo[p];
+            fd.f = S.f(p);
+            fd.n = static_cast<int32_t>(S.size[p]);
+            fd.node = static_cast<int32_t>(p);
+            fds.push_back(fd);
+            nmaxl = std::max(nmaxl, S.size[p]);
+        }
+        const size_t ofd = geBlob->add(fds);
+        for (int64_t s0 = 0; s0 < nmaxl; s0 += 64) {
+            std::vector<int32_t> list;
+            GemmBatch g64, gp;
+            for (size_t q = 0; q < nodes.size(); ++q) {
+                const int64_t p = nodes[q];
+                const int64_t np = S.size[p], f = S.f(p);
+                if (np <= s0) continue;
+                list.push_back(static_cast<int32_t>(q));
+                const int64_t jb = std::min<int64_t>(64, np - s0);
+                const int64_t r1 = s0 + jb;          // first trailing row/col
+                double* F = dF + fo[p];
+                double* W = dW + wo[p];
+                // trailing columns still inside the node (FP64: part of the
+                // diagonal-block factorisation), all rows below them
+                if (np > r1) {
+                    GemmDesc d = gdesc();
+                    d.A = F + r1 + s0 * f;
+                    d.lda = f;
+                    d.B = W;
+                    d.ldb = f;
+                    d.tb = 1;
+                    d.C = F + r1 + r1 * f;
+                    d.ldc = f;
+                    d.m = static_cast<int32_t>(f - r1);
+                    d.n = static_cast<int32_t>(np - r1);
+                    d.k = static_cast<int32_t>(jb);
+                    d.alpha1 = -1.0;
+                    d.beta = 1.0;
+                    d.lower = 1;
+                    g64.add(d);
+                }
+                // interface Schur complement block (rows/cols >= n_p)
+                if (f > np) {
+                    GemmDesc d = gdesc();
+                    d.A = F + np + s0 * f;
+                    d.lda = f;
+                    d.B = W + (np - r1);
+                    d.ldb = f;
+                    d.tb = 1;
+                    d.C = F + np + np * f;
+                    d.ldc = f;
+                    d.m = d.n = static_cast<int32_t>(f - np);
+                    d.k = static_cast<int32_t>(jb);
+                    d.alpha1 = -1.0;
+                    d.beta = 1.0;
+                    d.lower = 1;
+                    (f32 ? gp : g64).add(d);
+                }
+            }
+            std::vector<int2> rows;
+            for (int32_t q : list) {
+                const int64_t p = nodes[q];
+                const int64_t rem = S.f(p) - std::min<int64_t>(S.size[p], s0 + 64);
+                for (int c = 0; c < cdiv(rem, 256); ++c) rows.push_back(make_int2(q, c));
+            }
+            const size_t ol = geBlob->add(list), orw = geBlob->add(rows);
+            const int cnt = static_cast<int>(list.size()), nrw = static_cast<int>(rows.size());
+            Blob* bl = geBlob.get();
+            int* err = dErr;
+            const int s0i = static_cast<int>(s0);
+            double pf = 0.0, pb = 0.0;
+            for (int32_t q : list) {
+                const int64_t p = nodes[q];
+                const double jb = static_cast<double>(std::min<int64_t>(64, S.size[p] - s0));
+                const double rows = static_cast<double>(S.f(p) - s0 - jb);
+                pf += jb * jb * jb / 3.0 + rows * jb * jb;
+                pb += (jb * jb + 3.0 * rows * jb) * 8.0;
+            }
+            plan.timed("ldlt_panel", pf, pb, [=](hipStream_t s) {
+                launch_ldlt_panel(bl->at<FrontDesc>(ofd), bl->at<int32_t>(ol), cnt, s0i, err,
+                                  bl->at<int2>(orw), nrw, s);
+            });
+            plan.gemm(g64, false);
+            plan.gemm(gp, true);
+        }
+        // X = Khat^-1 K_iB, Fm = M_iB - Mhat X, and the M congruence update
+        Copy2dBatch cb;
+        TrsmBatch tb;
+        GemmBatch gFm, gG22;
+        for (int64_t p : nodes) {
+            const int64_t np = S.size[p], f = S.f(p), b = S.b(p);
+            if (b == 0) continue;
+            double* F = dF + fo[p];
+            double* G = dG + fo[p];
+            double* X = dXb + xo[p];
+            double* Fm = dFm + xo[p];
+            cb.add(cdesc(F + np, f, X, np, np, b, 1));       // X = L21^T
+            cb.add(cdesc(G + np, f, Fm, np, np, b, 1));      // Fm = M_Bi^T
+            cb.add(cdesc(G, f, G, f, np, np, 2));            // Mhat full in place
+            tb.add(F, f, X, np, np, b, 1);                   // X = L^-T L21^T
+            GemmDesc d = gdesc();                            // Fm -= Mhat X
+            d.A = G;
+            d.lda = f;
+            d.B = X;
+            d.ldb = np;
+            d.C = Fm;
+            d.ldc = np;
+            d.m = static_cast<int32_t>(np);
+            d.n = static_cast<int32_t>(b);
+            d.k = static_cast<int32_t>(np);
+            d.alpha1 = -1.0;
+            d.beta = 1.0;
+            gFm.add(d);
+            GemmDesc e = gdesc();                            // G22 -= X^T Fm + M_Bi X
+            e.A = X;
+            e.lda = np;
+            e.ta = 1;
+            e.B = Fm;
+            e.ldb = np;
+            e.k = static_cast<int32_t>(np);
+            e.alpha1 = -1.0;
+            e.A2 = G + np;
+            e.lda2 = f;
+            e.B2 = X;
+            e.ldb2 = np;
+            e.k2 = static_cast<int32_t>(np);
+            e.alpha2 = -1.0;
+            e.C = G + np + np * f;
+            e.ldc = f;
+            e.m = e.n = static_cast<int32_t>(b);
+            e.beta = 1.0;
+            e.lower = 1;
+            gG22.add(e);
+        }
+        plan.copy(cb);
+        plan.trsm(tb);
+        plan.gemm(gFm, f32);
+        plan.gemm(gG22, f32);
+    }
+    plan.run(stream);
+    int e = 0;
+    AMLS_HIP_CHECK(hipMemcpyAsync(&e, dErr, sizeof(int), hipMemcpyDeviceToHost, stream));
+    sync();
+    if (e != INT_MAX)
+        throw StatusError(AMLS_E_NOT_SPD,
+                          "LDL^T: non-positive or non-finite pivot in node " +
+                              std::to_string(S.treeId[e]) + " (post-order " + std::to_string(e) +
+                              ", n=" + std::to_string(S.size[e]) + ")");
+    geDone = true;
+}
+
+// ---------------------------------------------------------------------------
+// Local standard-form pencils and their tridiagonal reduction
+// ---------------------------------------------------------------------------
+void Ctx::spectralPrepare() {
+    requireGpu();
+    sync();   // earlier phases may still use buffers released below
+    if (!geDone) throw StatusError(AMLS_E_STATE, "spectral_prepare before ge");
+    specReady = selected = projected = reduced = recovered = false;
+    const int64_t N = S.nNodes;
+    co.assign(N + 1, 0);
+    vo.assign(N + 1, 0);
+    for (int64_t p = 0; p < N; ++p) {
+        co[p + 1] = co[p] + S.size[p] * S.size[p];
+        vo[p + 1] = vo[p] + S.size[p];
+    }
+    spMem.reset(new DevArena());
+    auto& A = *spMem;
+    A.plan(co[N] * 8 * 2);
+    A.plan(vo[N] * 8 * 4);
+    A.plan(vo[N] * 8 * 4);
+    A.plan((N + 1) * 8);
+    A.commit();
+    dC = A.take<double>(co[N]);
+    dT = A.take<double>(co[N]);
+    dD = A.take<double>(vo[N]);
+    dE = A.take<double>(vo[N]);
+    dTau = A.take<double>(vo[N]);
+    dDsq = A.take<double>(vo[N]);
+    dSv = A.take<double>(2 * vo[N]);
+    dSp = A.take<double>(2 * vo[N]);
+    spBlob.reset(new Blob());
+    Plan plan(spBlob.get(), &kstats);
+    Copy2dBatch c1, c2, c3, c4;
+    TrsmBatch t1, t2;
+    std::vector<VecDesc> vd;
+    std::vector<EigDesc> ed;
+    std::vector<int64_t> soff;
+    for (int64_t p = 0; p < N; ++p) {
+        const int64_t np = S.size[p], f = S.f(p);
+        double* F = dF + fo[p];
+        double* G = dG + fo[p];
+        double* C = dC + co[p];
+        double* T = dT + co[p];
+        c1.add(cdesc(G, f, C, np, np, np, 2));                 // C = Mhat
+        t1.add(F, f, C, np, np, np, 0);                        // C = L^-1 Mhat
+        c2.add(cdesc(C, np, T, np, np, np, 1));                // T = C^T
+        t2.add(F, f, T, np, np, np, 0);                        // T = L^-1 Mhat L^-T
+        VecDesc v;
+        std::memset(&v, 0, sizeof(v));
+        v.src = F;
+        v.dst = dDsq + vo[p];
+        v.ld = f;
+        v.n = static_cast<int32_t>(np);
+        v.op = 0;
+        vd.push_back(v);
+        c3.add(cdesc(T, np, T, np, np, np, 0, dDsq + vo[p], dDsq + vo[p]));
+        c4.add(cdesc(T, np, C, np, np, np, 2));                // exact symmetry
+        EigDesc e;
+        std::memset(&e, 0, sizeof(e));
+        e.A = C;
+        e.lda = np;
+        e.n = static_cast<int32_t>(np);
+        e.d = dD + vo[p];
+        e.e = dE + vo[p];
+        e.tau = dTau + vo[p];
+        e.kcap = static_cast<int32_t>(np);
+        e.node = static_cast<int32_t>(p);
+        ed.push_back(e);
+        soff.push_back(2 * vo[p]);
+    }
+    plan.copy(c1);
+    plan.trsm(t1);
+    plan.copy(c2);
+    plan.trsm(t2);
+    plan.vec(vd);
+    plan.copy(c3);
+    plan.copy(c4);
+    addSytrd(plan, ed, soff, dSv, dSp);
+    plan.run(stream);
+    sync();
+    specReady = true;
+}
+
+// ---------------------------------------------------------------------------
+// Local eigenpairs + truncation (both passes)
+// ---------------------------------------------------------------------------
+void Ctx::spectralSelect(int kind, double value) {
+    requireGpu();
+    sync();   // earlier phases may still use buffers released below
+    if (!specReady) throw StatusError(AMLS_E_STATE, "spectral before spectral_prepare");
+    selected = projected = reduced = recovered = false;
+    const int64_t N = S.nNodes;
+    k.assign(N, 0);
+    if (kind == 0) {
+        for (int64_t p = 0; p < N; ++p)
+            k[p] = value <= 0 ? S.size[p] : std::min<int64_t>(static_cast<int64_t>(value), S.size[p]);
+    } else {
+        // keep w <= lambda_cut  <=>  nu = 1/w >= 1/lambda_cut
+        std::vector<EigDesc> ed;
+        std::vector<double> thr;
+        for (int64_t p = 0; p < N; ++p) {
+            EigDesc e;
+            std::memset(&e, 0, sizeof(e));
+            e.d = dD + vo[p];
+            e.e = dE + vo[p];
+            e.n = static_cast<int32_t>(S.size[p]);
+            e.kcap = static_cast<int32_t>(S.size[p]);
+            ed.push_back(e);
+            thr.push_back(value > 0 ? 1.0 / value : INFINITY);
+        }
+        Blob cb;
+        const size_t od = cb.add(ed), ot = cb.add(thr);
+        std::vector<int32_t> cnt(N, 0);
+        const size_t oc = cb.add(cnt);
+        cb.upload(stream);
+        launch_eig_count(cb.at<EigDesc>(od), static_cast<int>(N), cb.at<double>(ot),
+                         const_cast<int32_t*>(cb.at<int32_t>(oc)), stream);
+        AMLS_HIP_CHECK(hipMemcpyAsync(cnt.data(), cb.at<int32_t>(oc), N * sizeof(int32_t),
+                                      hipMemcpyDeviceToHost, stream));
+        sync();
+        for (int64_t p = 0; p < N; ++p) k[p] = cnt[p];
+    }
+    moff.assign(N + 1, 0);
+    zo.assign(N + 1, 0);
+    int64_t wk = 0;
+    for (int64_t p = 0; p < N; ++p) {
+        moff[p + 1] = moff[p] + k[p];
+        zo[p + 1] = zo[p] + S.size[p] * k[p];
+        wk += 5 * S.size[p] * k[p];
+    }
+    ptot = moff[N];
+    passMem.reset(new DevArena());
+    auto& A = *passMem;
+    A.plan(ptot * 8 * 3);
+    A.plan(zo[N] * 8 * 2);
+    A.plan(wk * 8);
+    A.commit();
+    dLam = A.take<double>(ptot);
+    dMu = A.take<double>(ptot);
+    dSw = A.take<double>(ptot);
+    dY = A.take<double>(zo[N]);
+    dZ = A.take<double>(zo[N]);
+    double* dWk = A.take<double>(wk);
+    passBlob.reset(new Blob());
+    Plan plan(passBlob.get(), &kstats);
+    std::vector<EigDesc> ed;
+    std::vector<VecDesc> vd;
+    Copy2dBatch cz;
+    TrsmBatch tz;
+    int64_t wo2 = 0;
+    for (int64_t p = 0; p < N; ++p) {
+        const int64_t np = S.size[p], kp = k[p];
+        if (kp == 0) continue;
+        EigDesc e;
+        std::memset(&e, 0, sizeof(e));
+        e.A = dC + co[p];
+        e.lda = np;
+        e.n = static_cast<int32_t>(np);
+        e.d = dD + vo[p];
+        e.e = dE + vo[p];
+        e.tau = dTau + vo[p];
+        e.lam = dLam + moff[p];
+        e.Y = dY + zo[p];
+        e.work = dWk + wo2;
+        e.k = e.kcap = static_cast<int32_t>(kp);
+        e.node = static_cast<int32_t>(p);
+        ed.push_back(e);
+        wo2 += 5 * np * kp;
+        VecDesc v;
+        std::memset(&v, 0, sizeof(v));
+        v.src = dLam + moff[p];
+        v.dst = dMu + moff[p];
+        v.dst2 = dSw + moff[p];
+        v.n = static_cast<int32_t>(kp);
+        v.op = 1;
+        vd.push_back(v);
+        cz.add(cdesc(dY + zo[p], np, dZ + zo[p], np, np, kp, 0, dDsq + vo[p], dSw + moff[p]));
+        tz.add(dF + fo[p], S.f(p), dZ + zo[p], np, np, kp, 1);
+    }
+    addSelect(plan, ed, true);
+    plan.vec(vd);
+    plan.copy(cz);
+    plan.trsm(tz);
+    plan.run(stream);
+    selected = true;
+}
+
+// ---------------------------------------------------------------------------
+// Condensation: projection of M onto the retained modes (multifrontal)
+// ---------------------------------------------------------------------------
+void Ctx::project(int prec) {
+    requireGpu();
+    sync();   // earlier phases may still use buffers released below
+    if (!selected) throw StatusError(AMLS_E_STATE, "project before spectral");
+    projected = reduced = recovered = false;
+    const int64_t N = S.nNodes;
+    mstart.assign(N, 0);
+    ro.assign(N + 1, 0);
+    for (int64_t p = 0; p < N; ++p) {
+        mstart[p] = moff[S.subStart[p]];
+        const int64_t q = moff[p] - mstart[p];
+        ro[p + 1] = ro[p] + S.f(p) * (q + k[p]);
+    }
+    if (ptot <= 0) throw StatusError(AMLS_E_NOCONV, "reduced problem is empty (every node truncated to 0)");
+    // R fronts and Mtilde go to a second arena tied to the pass
+    auto* A = new DevArena();
+    std::unique_ptr<DevArena> hold(A);
+    A->plan(ro[N] * 8);
+    A->plan(ptot * ptot * 8);
+    A->commit();
+    dR = A->take<double>(ro[N]);
+    dMt = A->take<double>(ptot * ptot);
+    projMem.swap(hold);
+    passBlob2.reset(new Blob());
+    Plan plan(passBlob2.get(), &kstats);
+    const bool f32 = prec != 0;
+    plan.fn([this](hipStream_t s) {
+        launch_fill(dR, ro[S.nNodes], 0.0, s);
+        launch_fill(dMt, ptot * ptot, 0.0, s);
+    });
+    std::vector<VecDesc> vd(1);
+    std::memset(vd.data(), 0, sizeof(VecDesc));
+    vd[0].dst = dMt;
+    vd[0].ld = ptot;
+    vd[0].n = static_cast<int32_t>(ptot);
+    vd[0].op = 2;
+    plan.vec(vd);
+    for (int64_t l = 0; l <= S.maxLevel; ++l) {
+        ExtBatch eb;
+        GemmBatch gb;
+        for (int64_t p : S.byLevel[l]) {
+            const int64_t np = S.size[p], f = S.f(p), b = S.b(p), kp = k[p];
+            const int64_t q = moff[p] - mstart[p];
+            for (int64_t c : {S.child0[p], S.child1[p]}) {
+                if (c < 0 || S.b(c) == 0) continue;
+                const int64_t qc = moff[c] - mstart[c];
+                if (qc + k[c] == 0) continue;
+                ExtAddDesc d;
+                std::memset(&d, 0, sizeof(d));
+                d.src = dR + ro[c] + S.size[c];
+                d.lds = S.f(c);
+                d.dst = dR + ro[p];
+                d.ldd = f;
+                d.rowmap = drel + S.bptr[c];
+                d.rows = static_cast<int32_t>(S.b(c));
+                d.cols = static_cast<int32_t>(qc + k[c]);
+                d.coloff = static_cast<int32_t>(mstart[c] - mstart[p]);
+                eb.add(d);
+            }
+            double* R = dR + ro[p];
+            if (kp > 0 && q > 0) {            // Mtilde_{i, desc} = Z_i^T R_own
+                GemmDesc d = gdesc();
+                d.A = dZ + zo[p];
+                d.lda = np;
+                d.ta = 1;
+                d.B = R;
+                d.ldb = f;
+                d.k = static_cast<int32_t>(np);
+                d.C = dMt + moff[p] + mstart[p] * ptot;
+                d.ldc = ptot;
+                d.m = static_cast<int32_t>(kp);
+                d.n = static_cast<int32_t>(q);
+                gb.add(d);
+            }
+            if (b > 0 && q > 0) {             // R_B -= X^T R_own (mid update)
+                GemmDesc d = gdesc();
+                d.A = dXb + xo[p];
+                d.lda = np;
+                d.ta = 1;
+                d.B = R;
+                d.ldb = f;
+                d.k = static_cast<int32_t>(np);
+                d.alpha1 = -1.0;
+                d.beta = 1.0;
+                d.C = R + np;
+                d.ldc = f;
+                d.m = static_cast<int32_t>(b);
+                d.n = static_cast<int32_t>(q);
+                gb.add(d);
+            }
+            if (b > 0 && kp > 0) {            // R_B(:, own) = Fm^T Z_i (right projection)
+                GemmDesc d = gdesc();
+                d.A = dFm + xo[p];
+                d.lda = np;
+                d.ta = 1;
+                d.B = dZ + zo[p];
+                d.ldb = np;
+                d.k = static_cast<int32_t>(np);
+                d.C = R + np + q * f;
+                d.ldc = f;
+                d.m = static_cast<int32_t>(b);
+                d.n = static_cast<int32_t>(kp);
+                gb.add(d);
+            }
+        }
+        plan.ext(eb);
+        plan.gemm(gb, f32);
+    }
+    plan.run(stream);
+    projected = true;
+}
+
+// ---------------------------------------------------------------------------
+// Reduced problem (diag mu, Mtilde): lowest `want` eigenpairs through the
+// largest eigenvalues of S = mu^-1/2 Mtilde mu^-1/2
+// ---------------------------------------------------------------------------
+void Ctx::reducedSolve(int64_t w, bool vectors) {
+    requireGpu();
+    sync();   // earlier phases may still use buffers released below
+    if (!projected) throw StatusError(AMLS_E_STATE, "reduced_solve before project");
+    reduced = recovered = false;
+    if (w <= 0) throw StatusError(AMLS_E_ARG, "want must be positive");
+    w = std::min(w, ptot);
+    want = w;
+    const int64_t P = ptot;
+    redMem.reset(new DevArena());
+    auto& A = *redMem;
+    A.plan(P * P * 8);
+    A.plan(P * 8 * 8);
+    A.plan(w * 8 * 3);
+    A.plan(P * w * 8 * 2);
+    if (vectors) A.plan(w * 5 * P * 8);
+    A.commit();
+    double* dS = A.take<double>(P * P);
+    double* d = A.take<double>(P);
+    double* e = A.take<double>(P);
+    double* tau = A.take<double>(P);
+    double* murs = A.take<double>(P);
+    double* sv = A.take<double>(2 * P);
+    double* sp = A.take<double>(2 * P);
+    double* lam = A.take<double>(w);
+    double* th = A.take<double>(w);
+    double* sth = A.take<double>(w);
+    double* Y = A.take<double>(P * w);
+    dQ = A.take<double>(P * w);
+    double* wk = vectors ? A.take<double>(w * 5 * P) : nullptr;
+    redBlob.reset(new Blob());
+    Plan plan(redBlob.get(), &kstats);
+    std::vector<VecDesc> v1(1);
+    std::memset(v1.data(), 0, sizeof(VecDesc));
+    v1[0].src = dMu;
+    v1[0].dst = murs;
+    v1[0].n = static_cast<int32_t>(P);
+    v1[0].op = 3;
+    plan.vec(v1);
+    Copy2dBatch cs;
+    cs.add(cdesc(dMt, P, dS, P, P, P, 2, murs, murs));
+    plan.copy(cs);
+    EigDesc ed;
+    std::memset(&ed, 0, sizeof(ed));
+    ed.A = dS;
+    ed.lda = P;
+    ed.n = static_cast<int32_t>(P);
+    ed.d = d;
+    ed.e = e;
+    ed.tau = tau;
+    ed.lam = lam;
+    ed.Y = Y;
+    ed.work = wk;
+    ed.k = ed.kcap = static_cast<int32_t>(w);
+    addSytrd(plan, {ed}, {0}, sv, sp);
+    addSelect(plan, {ed}, vectors);
+    std::vector<VecDesc> v2(1);
+    std::memset(v2.data(), 0, sizeof(VecDesc));
+    v2[0].src = lam;
+    v2[0].dst = th;
+    v2[0].dst2 = sth;
+    v2[0].n = static_cast<int32_t>(w);
+    v2[0].op = 1;
+    plan.vec(v2);
+    if (vectors) {
+        Copy2dBatch cq;
+        cq.add(cdesc(Y, P, dQ, P, P, w, 0, murs, sth));
+        plan.copy(cq);
+    }
+    plan.run(stream);
+    theta.assign(w, 0.0);
+    AMLS_HIP_CHECK(hipMemcpyAsync(theta.data(), th, w * 8, hipMemcpyDeviceToHost, stream));
+    sync();
+    reduced = true;
+    redVectors = vectors;
+}
+
+// ---------------------------------------------------------------------------
+// Back-transformation z = L^-T S q, un-permutation, M-normalisation
+// ---------------------------------------------------------------------------
+void Ctx::backtransform() {
+    requireGpu();
+    sync();   // earlier phases may still use buffers released below
+    if (!reduced || !redVectors) throw StatusError(AMLS_E_STATE, "backtransform needs reduced vectors");
+    const int64_t N = S.nNodes, w = want;
+    std::vector<int64_t> zbo(N + 1, 0);
+    for (int64_t p = 0; p < N; ++p) zbo[p + 1] = zbo[p] + S.b(p) * w;
+    vecMem.reset(new DevArena());
+    auto& A = *vecMem;
+    A.plan(n * w * 8 * 3);
+    A.plan(zbo[N] * 8);
+    A.plan(w * 8);
+    A.commit();
+    dZfull = A.take<double>(n * w);
+    dXout = A.take<double>(n * w);
+    double* Ytmp = A.take<double>(n * w);
+    double* zB = A.take<double>(zbo[N]);
+    double* nrm = A.take<double>(w);
+    vecBlob.reset(new Blob());
+    Plan plan(vecBlob.get(), &kstats);
+    for (int64_t l = S.maxLevel; l >= 0; --l) {
+        GatherBatch gt;
+        GemmBatch gb;
+        for (int64_t p : S.byLevel[l]) {
+            const int64_t np = S.size[p], b = S.b(p), kp = k[p];
+            if (b > 0) {
+                GatherDesc d;
+                std::memset(&d, 0, sizeof(d));
+                d.src = dZfull;
+                d.lds = n;
+                d.dst = zB + zbo[p];
+                d.ldd = b;
+                d.rows_idx = dbidx + S.bptr[p];
+                d.rows = static_cast<int32_t>(b);
+                d.cols = static_cast<int32_t>(w);
+                gt.add(d);
+            }
+            GemmDesc d = gdesc();
+            if (kp > 0) {
+                d.A = dZ + zo[p];
+                d.lda = np;
+                d.B = dQ + moff[p];
+                d.ldb = ptot;
+                d.k = static_cast<int32_t>(kp);
+            }
+            if (b > 0) {
+                d.A2 = dXb + xo[p];
+                d.lda2 = np;
+                d.B2 = zB + zbo[p];
+                d.ldb2 = b;
+                d.k2 = static_cast<int32_t>(b);
+                d.alpha2 = -1.0;
+            }
+            d.C = dZfull + S.off[p];
+            d.ldc = n;
+            d.m = static_cast<int32_t>(np);
+            d.n = static_cast<int32_t>(w);
+            d.beta = 0.0;
+            gb.add(d);
+        }
+        plan.gather(gt);
+        plan.gemm(gb, false);
+    }
+    const int64_t nn = n;
+    int64_t* perm = dperm;
+    int64_t *mp = dMp, *mi = dMi;
+    double* mx = dMx;
+    double *zf = dZfull, *xo_ = dXout;
+    plan.fn([=](hipStream_t s) {
+        launch_permute_rows(nn, perm, zf, xo_, static_cast<int>(w), nn, s);
+        launch_spmm(nn, mp, mi, mx, xo_, Ytmp, static_cast<int>(w), nn, s);
+        launch_coldot(nn, xo_, Ytmp, static_cast<int>(w), nn, nrm, s);
+        launch_colscale_rsqrt(nn, xo_, static_cast<int>(w), nn, nrm, s);
+    });
+    plan.run(stream);
+    recovered = true;
+}
+
+// ---------------------------------------------------------------------------
+// Dense helpers on one matrix (appended to a plan)
+// ---------------------------------------------------------------------------
+namespace {
+
+// In-place LDL^T of the n x n lower triangle of A (lda), 64-column panels.
+void addDenseLdlt(Plan& plan, double* A, int64_t n, int64_t lda, double* W, int* err, int node) {
+    std::vector<FrontDesc> fd(1);
+    fd[0].F = A;
+    fd[0].W = W;
+    fd[0].f = lda;
+    fd[0].n = static_cast<int32_t>(n);
+    fd[0].node = node;
+    const size_t ofd = plan.blob->add(fd);
+    std::vector<int32_t> list{0};
+    const size_t ol = plan.blob->add(list);
+    Blob* bl = plan.blob;
+    for (int64_t s0 = 0; s0 < n; s0 += 64) {
+        const int64_t jb = std::min<int64_t>(64, n - s0), r1 = s0 + jb;
+        std::vector<int2> rows;
+        for (int c = 0; c < cdiv(lda - r1, 256); ++c) rows.push_back(make_int2(0, c));
+        const size_t orw = plan.blob->add(rows);
+        const int nrw = static_cast<int>(rows.size());
+        const int s0i = static_cast<int>(s0);
+        plan.fn([=](hipStream_t s) {
+            launch_ldlt_panel(bl->at<FrontDesc>(ofd), bl->at<int32_t>(ol), 1, s0i, err,
+                              bl->at<int2>(orw), nrw, s);
+        });
+        if (n > r1) {
+            GemmBatch gb;
+            GemmDesc d = gdesc();
+            d.A = A + r1 + s0 * lda;
+            d.lda = lda;
+            d.B = W;
+            d.ldb = lda;
+            d.tb = 1;
+            d.C = A + r1 + r1 * lda;
+            d.ldc = lda;
+            d.m = d.n = static_cast<int32_t>(n - r1);
+            d.k = static_cast<int32_t>(jb);
+            d.alpha1 = -1.0;
+            d.beta = 1.0;
+            d.lower = 1;
+            gb.add(d);
+            plan.gemm(gb, false);
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Subspace iteration with the AMLS block factors (AMLS-SIMb) and an FP64
+// Rayleigh-Ritz step after every iteration
+// ---------------------------------------------------------------------------
+void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& thetaOut) {
+    requireGpu();
+    sync();
+    if (!recovered) throw StatusError(AMLS_E_STATE, "subspace_iter needs recovered AMLS vectors");
+    if (steps <= 0) throw StatusError(AMLS_E_ARG, "steps must be positive");
+    q = want;                              // the recovered Ritz vectors
+    if (nev > q) throw StatusError(AMLS_E_ARG, "nev exceeds the subspace width");
+    const int64_t N = S.nNodes;
+    std::vector<int64_t> to(N + 1, 0), zbo(N + 1, 0);
+    for (int64_t p = 0; p < N; ++p) {
+        to[p + 1] = to[p] + S.f(p) * q;
+        zbo[p + 1] = zbo[p] + S.b(p) * q;
+    }
+    siMem.reset(new DevArena());
+    auto& A = *siMem;
+    A.plan(n * q * 8 * 4);                 // Q, R, Y, MY
+    A.plan(to[N] * 8);                     // fronts of the forward solve
+    A.plan(zbo[N] * 8);                    // gathered x_B
+    A.plan(q * q * 8 * 4);                 // Kc, Mc, C, T
+    A.plan(q * 64 * 8);                    // panel scratch
+    A.plan(q * 8 * 12);                    // eig vectors
+    A.plan(q * q * 8 * 2);                 // Yt, V
+    A.plan(q * 5 * q * 8);                 // stein scratch
+    A.plan(256);
+    A.commit();
+    double* Q = A.take<double>(n * q);
+    double* R = A.take<double>(n * q);
+    double* Yv = A.take<double>(n * q);
+    double* MY = A.take<double>(n * q);
+    double* T = A.take<double>(to[N]);
+    double* zB = A.take<double>(zbo[N]);
+    double* Kc = A.take<double>(q * q);
+    double* Mc = A.take<double>(q * q);
+    double* Cm = A.take<double>(q * q);
+    double* Tm = A.take<double>(q * q);
+    double* Wp = A.take<double>(q * 64);
+    double* d = A.take<double>(q);
+    double* e = A.take<double>(q);
+    double* tau = A.take<double>(q);
+    double* dsq = A.take<double>(q);
+    double* lam = A.take<double>(q);
+    double* th = A.take<double>(q);
+    double* sth = A.take<double>(q);
+    double* sv = A.take<double>(2 * q);
+    double* sp = A.take<double>(2 * q);
+    double* Yt = A.take<double>(q * q);
+    double* V = A.take<double>(q * q);
+    double* wk = A.take<double>(q * 5 * q);
+    int* err = A.take<int>(1);
+    const int big = INT_MAX;
+    AMLS_HIP_CHECK(hipMemcpyAsync(err, &big, sizeof(int), hipMemcpyHostToDevice, stream));
+    AMLS_HIP_CHECK(hipMemcpyAsync(Q, dZfull, n * q * 8, hipMemcpyDeviceToDevice, stream));
+    siBlob.reset(new Blob());
+    Plan plan(siBlob.get(), &kstats);
+    const int qi = static_cast<int>(q);
+    const int64_t nn = n;
+    for (int it = 0; it < steps; ++it) {
+        // R = M Q (permuted ordering)
+        const double nnzM = static_cast<double>(Mp[n]), nnzK = static_cast<double>(Kp[n]);
+        auto spmmBytes = [&](double nnz) { return nnz * 16.0 + (n + 1) * 8.0 + 2.0 * n * q * 8.0; };
+        plan.timed("spmm", 2.0 * nnzM * q, spmmBytes(nnzM), [=](hipStream_t s) {
+            launch_spmm(nn, dPMp, dPMi, dPMx, Q, R, qi, nn, s);
+        });
+        plan.fn([=](hipStream_t s) { launch_fill(T, to[S.nNodes], 0.0, s); });
+        // forward: fronts take their own rows of R
+        Copy2dBatch cown;
+        for (int64_t p = 0; p < N; ++p)
+            cown.add(cdesc(R + S.off[p], n, T + to[p], S.f(p), S.size[p], q, 0));
+        plan.copy(cown);
+        for (int64_t l = 0; l <= S.maxLevel; ++l) {
+            const auto& nodes = S.byLevel[l];
+            for (int slot = 0; slot < 2; ++slot) {
+                ExtBatch eb;
+                for (int64_t p : nodes) {
+                    const int64_t c = slot == 0 ? S.child0[p] : S.child1[p];
+                    if (c < 0 || S.b(c) == 0) continue;
+                    ExtAddDesc x;
+                    std::memset(&x, 0, sizeof(x));
+                    x.src = T + to[c] + S.size[c];
+                    x.lds = S.f(c);
+                    x.dst = T + to[p];
+                    x.ldd = S.f(p);
+                    x.rowmap = drel + S.bptr[c];
+                    x.rows = static_cast<int32_t>(S.b(c));
+                    x.cols = qi;
+                    eb.add(x);
+                }
+                plan.ext(eb);
+            }
+            // u_p = T_B - X^T y_p, then w_p = Khat^-1 y_p in place
+            GemmBatch gu;
+            TrsmBatch t1, t2;
+            Copy2dBatch sc;
+            for (int64_t p : nodes) {
+                const int64_t np = S.size[p], f = S.f(p), b = S.b(p);
+                double* Tp = T + to[p];
+                if (b > 0) {
+                    GemmDesc g = gdesc();
+                    g.A = dXb + xo[p];
+                    g.lda = np;
+                    g.ta = 1;
+                    g.B = Tp;
+                    g.ldb = f;
+                    g.k = static_cast<int32_t>(np);
+                    g.alpha1 = -1.0;
+                    g.beta = 1.0;
+                    g.C = Tp + np;
+                    g.ldc = f;
+                    g.m = static_cast<int32_t>(b);
+                    g.n = qi;
+                    gu.add(g);
+                }
+                t1.add(dF + fo[p], f, Tp, f, np, q, 0);
+                sc.add(cdesc(Tp, f, Tp, f, np, q, 0, dDsq + vo[p]));
+                t2.add(dF + fo[p], f, Tp, f, np, q, 1);
+            }
+            plan.gemm(gu, false);
+            plan.trsm(t1);
+            plan.copy(sc);
+            plan.copy(sc);
+            plan.trsm(t2);
+        }
+        // backward, top-down: x_p = w_p - X_p x_B
+        Copy2dBatch cw;
+        for (int64_t p = 0; p < N; ++p)
+            cw.add(cdesc(T + to[p], S.f(p), Yv + S.off[p], n, S.size[p], q, 0));
+        plan.copy(cw);
+        for (int64_t l = S.maxLevel; l >= 0; --l) {
+            GatherBatch gt;
+            GemmBatch gb;
+            for (int64_t p : S.byLevel[l]) {
+                const int64_t np = S.size[p], b = S.b(p);
+                if (b == 0) continue;
+                GatherDesc gd;
+                std::memset(&gd, 0, sizeof(gd));
+                gd.src = Yv;
+                gd.lds = n;
+                gd.dst = zB + zbo[p];
+                gd.ldd = b;
+                gd.rows_idx = dbidx + S.bptr[p];
+                gd.rows = static_cast<int32_t>(b);
+                gd.cols = qi;
+                gt.add(gd);
+                GemmDesc g = gdesc();
+                g.A = dXb + xo[p];
+                g.lda = np;
+                g.B = zB + zbo[p];
+                g.ldb = b;
+                g.k = static_cast<int32_t>(b);
+                g.alpha1 = -1.0;
+                g.beta = 1.0;
+                g.C = Yv + S.off[p];
+                g.ldc = n;
+                g.m = static_cast<int32_t>(np);
+                g.n = qi;
+                gb.add(g);
+            }
+            plan.gather(gt);
+            plan.gemm(gb, false);
+        }
+        // Rayleigh-Ritz with the original FP64 K and M: Kc = Y^T K Y,
+        // Mc = Y^T M Y (the block factors only steer the iteration; in the
+        // mixed form they carry FP32 Schur updates, so R^T Y is not used)
+        plan.timed("spmm", 2.0 * nnzM * q, spmmBytes(nnzM), [=](hipStream_t s) {
+            launch_spmm(nn, dPMp, dPMi, dPMx, Yv, MY, qi, nn, s);
+        });
+        plan.timed("spmm", 2.0 * nnzK * q, spmmBytes(nnzK), [=](hipStream_t s) {
+            launch_spmm(nn, dPKp, dPKi, dPKx, Yv, R, qi, nn, s);
+        });
+        GemmBatch gr;
+        GemmDesc g1 = gdesc();
+        g1.A = Yv;
+        g1.lda = n;
+        g1.ta = 1;
+        g1.B = R;
+        g1.ldb = n;
+        g1.k = static_cast<int32_t>(n);
+        g1.C = Kc;
+        g1.ldc = q;
+        g1.m = g1.n = qi;
+        g1.lower = 1;
+        gr.add(g1);
+        GemmDesc g2 = g1;
+        g2.A = Yv;
+        g2.B = MY;
+        g2.C = Mc;
+        gr.add(g2);
+        plan.gemm(gr, false);
+        // lowest q eigenpairs of (Kc, Mc): Kc = L D L^T,
+        // C = D^-1/2 L^-1 Mc L^-T D^-1/2, nu = 1/theta
+        addDenseLdlt(plan, Kc, q, q, Wp, err, -1);
+        Copy2dBatch c1, c2, c3, c4;
+        TrsmBatch u1, u2;
+        c1.add(cdesc(Mc, q, Cm, q, q, q, 2));
+        u1.add(Kc, q, Cm, q, q, q, 0);
+        c2.add(cdesc(Cm, q, Tm, q, q, q, 1));
+        u2.add(Kc, q, Tm, q, q, q, 0);
+        std::vector<VecDesc> vd(1);
+        std::memset(vd.data(), 0, sizeof(VecDesc));
+        vd[0].src = Kc;
+        vd[0].dst = dsq;
+        vd[0].ld = q;
+        vd[0].n = qi;
+        vd[0].op = 0;
+        c3.add(cdesc(Tm, q, Tm, q, q, q, 0, dsq, dsq));
+        c4.add(cdesc(Tm, q, Cm, q, q, q, 2));
+        plan.copy(c1);
+        plan.trsm(u1);
+        plan.copy(c2);
+        plan.trsm(u2);
+        plan.vec(vd);
+        plan.copy(c3);
+        plan.copy(c4);
+        EigDesc ed;
+        std::memset(&ed, 0, sizeof(ed));
+        ed.A = Cm;
+        ed.lda = q;
+        ed.n = qi;
+        ed.d = d;
+        ed.e = e;
+        ed.tau = tau;
+        ed.lam = lam;
+        ed.Y = Yt;
+        ed.work = wk;
+        ed.k = ed.kcap = qi;
+        addSytrd(plan, {ed}, {0}, sv, sp);
+        addSelect(plan, {ed}, true);
+        std::vector<VecDesc> v2(1);
+        std::memset(v2.data(), 0, sizeof(VecDesc));
+        v2[0].src = lam;
+        v2[0].dst = th;
+        v2[0].dst2 = sth;
+        v2[0].n = qi;
+        v2[0].op = 1;
+        plan.vec(v2);
+        Copy2dBatch cv;
+        TrsmBatch tv;
+        cv.add(cdesc(Yt, q, V, q, q, q, 0, dsq, sth));
+        tv.add(Kc, q, V, q, q, q, 1);
+        plan.copy(cv);
+        plan.trsm(tv);
+        // Q = Y V
+        GemmBatch gq;
+        GemmDesc g3 = gdesc();
+        g3.A = Yv;
+        g3.lda = n;
+        g3.B = V;
+        g3.ldb = q;
+        g3.k = qi;
+        g3.C = Q;
+        g3.ldc = n;
+        g3.m = static_cast<int32_t>(n);
+        g3.n = qi;
+        gq.add(g3);
+        plan.gemm(gq, false);
+    }
+    // un-permute and M-normalise the first nev columns into dXout
+    const int nv = static_cast<int>(nev);
+    int64_t* perm = dperm;
+    int64_t *mp = dMp, *mi = dMi;
+    double* mx = dMx;
+    double* xo_ = dXout;
+    double* nrm = dsq;
+    plan.fn([=](hipStream_t s) {
+        launch_permute_rows(nn, perm, Q, xo_, nv, nn, s);
+        launch_spmm(nn, mp, mi, mx, xo_, MY, nv, nn, s);
+        launch_coldot(nn, xo_, MY, nv, nn, nrm, s);
+        launch_colscale_rsqrt(nn, xo_, nv, nn, nrm, s);
+    });
+    plan.run(stream);
+    thetaOut.assign(nev, 0.0);
+    int ecode = 0;
+    AMLS_HIP_CHECK(hipMemcpyAsync(thetaOut.data(), th, nev * 8, hipMemcpyDeviceToHost, stream));
+    AMLS_HIP_CHECK(hipMemcpyAsync(&ecode, err, sizeof(int), hipMemcpyDeviceToHost, stream));
+    sync();
+    if (ecode != INT_MAX)
+        throw StatusError(AMLS_E_NOT_SPD, "subspace iteration: projected stiffness is not SPD");
+}
+
+}  // namespace amls
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+struct amls_ctx {
+    amls::Ctx c;
+};
+
+namespace {
+
+template <class F>
+amls_status guarded(amls_ctx* ctx, F&& f) {
+    if (!ctx) return AMLS_E_ARG;
+    try {
+        ctx->c.err.clear();
+        f(ctx->c);
+        return AMLS_OK;
+    } catch (const amls::StatusError& e) {
+        ctx->c.err = e.what();
+        return static_cast<amls_status>(e.code);
+    } catch (const std::bad_alloc&) {
+        ctx->c.err = "host allocation failed";
+        return AMLS_E_OOM;
+    } catch (const std::exception& e) {
+        ctx->c.err = e.what();
+        return AMLS_E_INTERNAL;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+amls_status amls_options_default(amls_options* o) {
+    if (!o) return AMLS_E_ARG;
+    o->nLevels = 6;
+    o->leafSize = 500;
+    o->thresholdFactor = 2.0;
+    o->fixedKeep = 0;
+    o->nev = 100;
+    o->denseProjectedMax = 5000;
+    o->shift = 0.0;
+    o->twoPass = 1;
+    o->numThreads = 0;
+    o->recoverVectors = 1;
+    o->verbosity = 1;
+    o->precision = 0;
+    o->siSteps = 0;
+    o->siVectors = 0;
+    return AMLS_OK;
+}
+
+amls_status amls_hip_create(int32_t device, uint32_t flags, amls_ctx** out) {
+    (void)flags;
+    if (!out) return AMLS_E_ARG;
+    *out = nullptr;
+    amls_ctx* ctx = new (std::nothrow) amls_ctx();
+    if (!ctx) return AMLS_E_OOM;
+    if (device >= 0) {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || device >= count) {
+            delete ctx;
+            return AMLS_E_HIP;
+        }
+        if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->c.stream) != hipSuccess) {
+            delete ctx;
+            return AMLS_E_HIP;
+        }
+        ctx->c.device = device;
+        ctx->c.gpu = true;
+    }
+    *out = ctx;
+    return AMLS_OK;
+}
+
+amls_status amls_hip_destroy(amls_ctx* ctx) {
+    if (!ctx) return AMLS_E_ARG;
+    if (ctx->c.gpu) {
+        (void)hipSetDevice(ctx->c.device);
+        (void)hipStreamSynchronize(ctx->c.stream);
+    }
+    hipStream_t s = ctx->c.stream;
+    const bool gpu = ctx->c.gpu;
+    delete ctx;
+    if (gpu && s) (void)hipStreamDestroy(s);
+    return AMLS_OK;
+}
+
+const char* amls_hip_last_error(const amls_ctx* ctx) {
+    return ctx ? ctx->c.err.c_str() : "null context";
+}
+
+amls_status amls_hip_set_problem(amls_ctx* ctx, int64_t n, const int64_t* Kp, const int64_t* Ki,
+                                 const double* Kx, const int64_t* Mp, const int64_t* Mi,
+                                 const double* Mx, const double* coords, int32_t nLevels,
+                                 int64_t leafSize, double shift) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (c.gpu) AMLS_HIP_CHECK(hipSetDevice(c.device));
+        c.setProblem(n, Kp, Ki, Kx, Mp, Mi, Mx, coords, nLevels, leafSize, shift);
+    });
+}
+
+amls_status amls_hip_tree_info(const amls_ctx* cctx, int64_t* nNodes, int64_t* maxLevel,
+                               int64_t* n) {
+    amls_ctx* ctx = const_cast<amls_ctx*>(cctx);
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (!c.haveProblem) throw amls::StatusError(AMLS_E_STATE, "no problem set");
+        if (nNodes) *nNodes = c.S.nNodes;
+        if (maxLevel) *maxLevel = c.S.maxLevel;
+        if (n) *n = c.n;
+    });
+}
+
+amls_status amls_hip_tree_get(const amls_ctx* cctx, int64_t* treeId, int64_t* offset,
+                              int64_t* size, int64_t* parent, int64_t* level, int64_t* bsize,
+                              int64_t* perm) {
+    amls_ctx* ctx = const_cast<amls_ctx*>(cctx);
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (!c.haveProblem) throw amls::StatusError(AMLS_E_STATE, "no problem set");
+        const auto& S = c.S;
+        for (int64_t p = 0; p < S.nNodes; ++p) {
+            if (treeId) treeId[p] = S.treeId[p];
+            if (offset) offset[p] = S.off[p];
+            if (size) size[p] = S.size[p];
+            if (parent) parent[p] = S.parent[p];
+            if (level) level[p] = S.level[p];
+            if (bsize) bsize[p] = S.b(p);
+        }
+        if (perm) std::copy(S.perm.begin(), S.perm.end(), perm);
+    });
+}
+
+amls_status amls_hip_ge(amls_ctx* ctx, int32_t precision) {
+    return guarded(ctx, [&](amls::Ctx& c) { c.runGE(precision); });
+}
+
+amls_status amls_hip_spectral_prepare(amls_ctx* ctx) {
+    return guarded(ctx, [&](amls::Ctx& c) { c.spectralPrepare(); });
+}
+
+amls_status amls_hip_spectral(amls_ctx* ctx, int32_t kind, double value, int64_t* k_out) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        c.spectralSelect(kind, value);
+        c.sync();
+        if (k_out) std::copy(c.k.begin(), c.k.end(), k_out);
+    });
+}
+
+amls_status amls_hip_project(amls_ctx* ctx, int32_t precision, int64_t* p_out) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        c.project(precision);
+        c.sync();
+        if (p_out) *p_out = c.ptot;
+    });
+}
+
+amls_status amls_hip_reduced_solve(amls_ctx* ctx, int64_t want, int32_t vectors, double* theta) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        c.reducedSolve(want, vectors != 0);
+        if (theta) std::copy(c.theta.begin(), c.theta.end(), theta);
+    });
+}
+
+amls_status amls_hip_backtransform(amls_ctx* ctx, double* Z) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        c.backtransform();
+        if (Z)
+            AMLS_HIP_CHECK(hipMemcpyAsync(Z, c.dXout, c.n * c.want * 8, hipMemcpyDeviceToHost,
+                                          c.stream));
+        c.sync();
+    });
+}
+
+amls_status amls_hip_subspace_iter(amls_ctx* ctx, int32_t steps, int64_t q, int64_t nev,
+                                   double* theta, double* Z) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        std::vector<double> th;
+        c.subspaceIter(steps, q, nev, th);
+        if (theta) std::copy(th.begin(), th.end(), theta);
+        if (Z)
+            AMLS_HIP_CHECK(hipMemcpyAsync(Z, c.dXout, c.n * nev * 8, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    });
+}
+
+}  // extern "C"
+
+namespace amls {
+
+// Numeric part of AmlsSolver::solve on the problem already set (values
+// resident on the device): GE, both truncation passes, the reduced solve,
+// vector recovery and the optional subspace iteration.
+static void solveNumeric(Ctx& c, amls_options o, double* theta, double* Z, int64_t* p_out,
+                         double* timing, Clock::time_point t0, Clock::time_point t1) {
+    if (o.nev <= 0) throw StatusError(AMLS_E_ARG, "nev must be positive");
+    c.requireGpu();
+    if (!c.haveProblem) throw StatusError(AMLS_E_STATE, "no problem set");
+    AMLS_HIP_CHECK(hipSetDevice(c.device));
+    o.nev = std::min<int64_t>(o.nev, c.n);
+    c.runGE(o.precision);
+    const auto t2 = Clock::now();
+    c.spectralPrepare();
+    int64_t p0 = 0;
+    if (o.fixedKeep > 0) {
+        c.spectralSelect(0, static_cast<double>(o.fixedKeep));
+        c.project(o.precision);
+    } else {
+        if (!o.twoPass) throw StatusError(AMLS_E_ARG, "twoPass=false requires fixedKeep > 0");
+        const int64_t N = c.S.nNodes;
+        const int64_t k0 = std::max<int64_t>(4, (3 * o.nev + N - 1) / N);
+        c.spectralSelect(0, static_cast<double>(k0));
+        c.project(o.precision);
+        p0 = c.ptot;
+        c.reducedSolve(std::min<int64_t>(o.nev, c.ptot), false);
+        if (static_cast<int64_t>(c.theta.size()) < o.nev)
+            throw StatusError(AMLS_E_NOCONV,
+                              "pass A reduced problem smaller than nev (p < nev); "
+                              "use fewer levels or a larger leafSize");
+        const double lambdaHat = c.theta[o.nev - 1];
+        c.spectralSelect(1, o.thresholdFactor * lambdaHat);
+        c.project(o.precision);
+    }
+    c.sync();
+    const auto t3 = Clock::now();
+    int64_t want = std::min<int64_t>(o.nev, c.ptot);
+    const bool vec = o.recoverVectors || o.siSteps > 0;
+    if (o.siSteps > 0) {
+        // subspace width: siVectors, else max(2 nev, nev + 8) (eq. 25 of the
+        // subspace-iteration paper), never more than p
+        const int64_t q = o.siVectors > 0 ? o.siVectors : std::max<int64_t>(2 * o.nev, o.nev + 8);
+        want = std::min<int64_t>(std::max<int64_t>(q, o.nev), c.ptot);
+        if (want < o.nev) throw StatusError(AMLS_E_NOCONV, "reduced problem smaller than nev");
+    }
+    c.reducedSolve(want, vec);
+    const auto t4 = Clock::now();
+    if (vec) {
+        c.backtransform();
+        c.sync();
+    }
+    const auto t5 = Clock::now();
+    std::vector<double> th = c.theta;
+    if (o.siSteps > 0) {
+        c.subspaceIter(o.siSteps, want, o.nev, th);
+        want = o.nev;
+    }
+    const auto t6 = Clock::now();
+    if (theta) std::copy(th.begin(), th.begin() + std::min<size_t>(th.size(), want), theta);
+    if (Z && vec)
+        AMLS_HIP_CHECK(hipMemcpyAsync(Z, c.dXout, c.n * want * 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    c.lastWant = want;
+    if (p_out) *p_out = c.ptot;
+    if (timing) {
+        timing[0] = secs(t0, t1);
+        timing[1] = secs(t1, t2);
+        timing[2] = secs(t2, t3);
+        timing[3] = secs(t3, t4);
+        timing[4] = secs(t4, t5);
+        timing[5] = secs(t5, t6);
+        timing[6] = secs(t0, Clock::now());
+        timing[7] = static_cast<double>(p0);
+    }
+}
+
+}  // namespace amls
+
+extern "C" {
+
+amls_status amls_hip_solve(amls_ctx* ctx, int64_t n, const int64_t* Kp, const int64_t* Ki,
+                           const double* Kx, const int64_t* Mp, const int64_t* Mi,
+                           const double* Mx, const double* coords, const amls_options* opt,
+                           double* theta, double* Z, int64_t* p_out, double* timing) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        using amls::StatusError;
+        if (!opt) throw StatusError(AMLS_E_ARG, "null options");
+        if (opt->nev <= 0) throw StatusError(AMLS_E_ARG, "nev must be positive");
+        if (n <= 0) throw StatusError(AMLS_E_ARG, "empty matrix");
+        c.requireGpu();
+        AMLS_HIP_CHECK(hipSetDevice(c.device));
+        const auto t0 = amls::Clock::now();
+        c.setProblem(n, Kp, Ki, Kx, Mp, Mi, Mx, coords, opt->nLevels, opt->leafSize, opt->shift);
+        const auto t1 = amls::Clock::now();
+        amls::solveNumeric(c, *opt, theta, Z, p_out, timing, t0, t1);
+    });
+}
+
+amls_status amls_hip_solve_numeric(amls_ctx* ctx, const amls_options* opt, double* theta,
+                                   double* Z, int64_t* p_out, double* timing) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (!opt) throw amls::StatusError(AMLS_E_ARG, "null options");
+        const auto t0 = amls::Clock::now();
+        amls::solveNumeric(c, *opt, theta, Z, p_out, timing, t0, t0);
+    });
+}
+
+amls_status amls_hip_profile(amls_ctx* ctx, int32_t enable) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        c.kstats.reset();
+        c.kstats.enabled = enable != 0;
+    });
+}
+
+amls_status amls_hip_kernel_stats(amls_ctx* ctx, int32_t index, char* name, int32_t name_len,
+                                  int64_t* launches, double* ms, double* flops, double* bytes) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (c.gpu) c.sync();
+        if (index < 0 || index >= static_cast<int32_t>(c.kstats.byName.size()))
+            throw amls::StatusError(AMLS_E_ARG, "kernel class index out of range");
+        auto it = c.kstats.byName.begin();
+        std::advance(it, index);
+        if (name && name_len > 0) {
+            std::strncpy(name, it->first.c_str(), static_cast<size_t>(name_len) - 1);
+            name[name_len - 1] = '\0';
+        }
+        if (launches) *launches = it->second.launches;
+        if (ms) *ms = it->second.ms;
+        if (flops) *flops = it->second.flops;
+        if (bytes) *bytes = it->second.bytes;
+    });
+}
+
+amls_status amls_hip_get_node_factor(amls_ctx* ctx, int64_t p, double* LD, double* Mhat) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        c.requireGpu();
+        if (!c.geDone) throw amls::StatusError(AMLS_E_STATE, "ge not run");
+        if (p < 0 || p >= c.S.nNodes) throw amls::StatusError(AMLS_E_ARG, "node out of range");
+        const int64_t np = c.S.size[p], f = c.S.f(p);
+        if (LD)
+            AMLS_HIP_CHECK(hipMemcpy2DAsync(LD, np * 8, c.dF + c.fo[p], f * 8, np * 8, np,
+                                            hipMemcpyDeviceToHost, c.stream));
+        if (Mhat)
+            AMLS_HIP_CHECK(hipMemcpy2DAsync(Mhat, np * 8, c.dG + c.fo[p], f * 8, np * 8, np,
+                                            hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    });
+}
+
+amls_status amls_hip_get_local(amls_ctx* ctx, int64_t p, double* mu, double* Z) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        c.requireGpu();
+        if (!c.selected) throw amls::StatusError(AMLS_E_STATE, "spectral not run");
+        if (p < 0 || p >= c.S.nNodes) throw amls::StatusError(AMLS_E_ARG, "node out of range");
+        const int64_t kp = c.k[p], np = c.S.size[p];
+        if (mu && kp)
+            AMLS_HIP_CHECK(hipMemcpyAsync(mu, c.dMu + c.moff[p], kp * 8, hipMemcpyDeviceToHost, c.stream));
+        if (Z && kp)
+            AMLS_HIP_CHECK(hipMemcpyAsync(Z, c.dZ + c.zo[p], np * kp * 8, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    });
+}
+
+amls_status amls_hip_get_reduced(amls_ctx* ctx, double* Mt, int64_t* moff) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        c.requireGpu();
+        if (!c.projected) throw amls::StatusError(AMLS_E_STATE, "project not run");
+        if (Mt)
+            AMLS_HIP_CHECK(hipMemcpyAsync(Mt, c.dMt, c.ptot * c.ptot * 8, hipMemcpyDeviceToHost, c.stream));
+        if (moff) std::copy(c.moff.begin(), c.moff.end() - 1, moff);
+        c.sync();
+    });
+}
+
+// ---- operator-granular entry points ----------------------------------------
+amls_status amls_hip_gemm(amls_ctx* ctx, int32_t precision, int32_t transA, int32_t transB,
+                          int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                          int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                          int64_t ldc) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        using amls::StatusError;
+        c.requireGpu();
+        if (m < 0 || n < 0 || k < 0) throw StatusError(AMLS_E_ARG, "negative size");
+        const int64_t ar = transA ? k : m, ac = transA ? m : k;
+        const int64_t br = transB ? n : k, bc = transB ? k : n;
+        if (lda < std::max<int64_t>(1, ar) || ldb < std::max<int64_t>(1, br) ||
+            ldc < std::max<int64_t>(1, m))
+            throw StatusError(AMLS_E_ARG, "leading dimension too small");
+        amls::DevArena mem;
+        mem.plan(lda * ac * 8);
+        mem.plan(ldb * bc * 8);
+        mem.plan(ldc * n * 8);
+        mem.commit();
+        double* dA = mem.take<double>(lda * ac);
+        double* dB = mem.take<double>(ldb * bc);
+        double* dC = mem.take<double>(ldc * n);
+        AMLS_HIP_CHECK(hipMemcpyAsync(dA, A, lda * ac * 8, hipMemcpyHostToDevice, c.stream));
+        AMLS_HIP_CHECK(hipMemcpyAsync(dB, B, ldb * bc * 8, hipMemcpyHostToDevice, c.stream));
+        AMLS_HIP_CHECK(hipMemcpyAsync(dC, C, ldc * n * 8, hipMemcpyHostToDevice, c.stream));
+        amls::Blob blob;
+        amls::Plan plan(&blob, &c.kstats);
+        amls::GemmBatch gb;
+        amls::dev::GemmDesc d = amls::gdesc();
+        d.A = dA;
+        d.lda = lda;
+        d.ta = transA;
+        d.B = dB;
+        d.ldb = ldb;
+        d.tb = transB;
+        d.C = dC;
+        d.ldc = ldc;
+        d.m = static_cast<int32_t>(m);
+        d.n = static_cast<int32_t>(n);
+        d.k = static_cast<int32_t>(k);
+        d.alpha1 = alpha;
+        d.beta = beta;
+        gb.add(d);
+        plan.gemm(gb, precision != 0);
+        plan.run(c.stream);
+        AMLS_HIP_CHECK(hipMemcpyAsync(C, dC, ldc * n * 8, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    });
+}
+
+amls_status amls_hip_ldlt_solve(amls_ctx* ctx, int64_t n, const double* A, int64_t nrhs,
+                                const double* B, double* LD, double* X) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        using amls::StatusError;
+        c.requireGpu();
+        if (n <= 0 || nrhs < 0) throw StatusError(AMLS_E_ARG, "bad size");
+        amls::DevArena mem;
+        mem.plan(n * n * 8);
+        mem.plan(n * 64 * 8);
+        mem.plan(n * std::max<int64_t>(nrhs, 1) * 8);
+        mem.plan(n * 8);
+        mem.plan(256);
+        mem.commit();
+        double* dA = mem.take<double>(n * n);
+        double* dW = mem.take<double>(n * 64);
+        double* dX = mem.take<double>(n * std::max<int64_t>(nrhs, 1));
+        double* dq = mem.take<double>(n);
+        int* derr = mem.take<int>(1);
+        const int big = INT_MAX;
+        AMLS_HIP_CHECK(hipMemcpyAsync(derr, &big, sizeof(int), hipMemcpyHostToDevice, c.stream));
+        AMLS_HIP_CHECK(hipMemcpyAsync(dA, A, n * n * 8, hipMemcpyHostToDevice, c.stream));
+        if (nrhs > 0)
+            AMLS_HIP_CHECK(hipMemcpyAsync(dX, B, n * nrhs * 8, hipMemcpyHostToDevice, c.stream));
+        amls::Blob blob;
+        amls::Plan plan(&blob, &c.kstats);
+        std::vector<amls::dev::FrontDesc> fd(1);
+        fd[0].F = dA;
+        fd[0].W = dW;
+        fd[0].f = n;
+        fd[0].n = static_cast<int32_t>(n);
+        fd[0].node = 0;
+        const size_t ofd = blob.add(fd);
+        std::vector<int32_t> list{0};
+        const size_t ol = blob.add(list);
+        amls::Blob* bl = &blob;
+        for (int64_t s0 = 0; s0 < n; s0 += 64) {
+            const int64_t jb = std::min<int64_t>(64, n - s0), r1 = s0 + jb;
+            const int s0i = static_cast<int>(s0);
+            std::vector<int2> rows;
+            for (int c = 0; c < amls::cdiv(n - r1, 256); ++c) rows.push_back(make_int2(0, c));
+            const size_t orw = blob.add(rows);
+            const int nrw = static_cast<int>(rows.size());
+            plan.fn([=](hipStream_t s) {
+                amls::launch_ldlt_panel(bl->at<amls::dev::FrontDesc>(ofd), bl->at<int32_t>(ol), 1, s0i,
+                                        derr, bl->at<int2>(orw), nrw, s);
+            });
+            if (n > r1) {
+                amls::GemmBatch gb;
+                amls::dev::GemmDesc d = amls::gdesc();
+                d.A = dA + r1 + s0 * n;
+                d.lda = n;
+                d.B = dW;
+                d.ldb = n;
+                d.tb = 1;
+                d.C = dA + r1 + r1 * n;
+                d.ldc = n;
+                d.m = d.n = static_cast<int32_t>(n - r1);
+                d.k = static_cast<int32_t>(jb);
+                d.alpha1 = -1.0;
+                d.beta = 1.0;
+                d.lower = 1;
+                gb.add(d);
+                plan.gemm(gb, false);
+            }
+        }
+        if (nrhs > 0) {
+            amls::TrsmBatch t1, t2;
+            t1.add(dA, n, dX, n, n, nrhs, 0);
+            t2.add(dA, n, dX, n, n, nrhs, 1);
+            std::vector<amls::dev::VecDesc> vd(1);
+            std::memset(vd.data(), 0, sizeof(amls::dev::VecDesc));
+            vd[0].src = dA;
+            vd[0].dst = dq;
+            vd[0].ld = n;
+            vd[0].n = static_cast<int32_t>(n);
+            vd[0].op = 0;
+            amls::Copy2dBatch sc;
+            sc.add(amls::cdesc(dX, n, dX, n, n, nrhs, 0, dq));
+            plan.trsm(t1);
+            plan.vec(vd);
+            plan.copy(sc);
+            plan.copy(sc);
+            plan.trsm(t2);
+        }
+        plan.run(c.stream);
+        int e = 0;
+        AMLS_HIP_CHECK(hipMemcpyAsync(&e, derr, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        if (LD) AMLS_HIP_CHECK(hipMemcpyAsync(LD, dA, n * n * 8, hipMemcpyDeviceToHost, c.stream));
+        if (X && nrhs > 0)
+            AMLS_HIP_CHECK(hipMemcpyAsync(X, dX, n * nrhs * 8, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        if (e != INT_MAX) throw StatusError(AMLS_E_NOT_SPD, "LDL^T: non-positive or non-finite pivot");
+    });
+}
+
+amls_status amls_hip_syevx_top(amls_ctx* ctx, int64_t n, const double* A, int64_t k, double* lam,
+                               double* V) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        using amls::StatusError;
+        c.requireGpu();
+        if (n <= 0 || k < 0 || k > n) throw StatusError(AMLS_E_ARG, "bad size");
+        amls::DevArena mem;
+        mem.plan(n * n * 8 * 2);
+        mem.plan(n * 8 * 7);
+        mem.plan(std::max<int64_t>(k, 1) * (n + 1) * 8 * 2);
+        mem.plan(std::max<int64_t>(k, 1) * 5 * n * 8);
+        mem.commit();
+        double* dIn = mem.take<double>(n * n);
+        double* dA = mem.take<double>(n * n);
+        double* d = mem.take<double>(n);
+        double* e = mem.take<double>(n);
+        double* tau = mem.take<double>(n);
+        double* sv = mem.take<double>(2 * n);
+        double* sp = mem.take<double>(2 * n);
+        double* dl = mem.take<double>(std::max<int64_t>(k, 1));
+        double* dY = mem.take<double>(std::max<int64_t>(k, 1) * n);
+        double* wk = mem.take<double>(std::max<int64_t>(k, 1) * 5 * n);
+        AMLS_HIP_CHECK(hipMemcpyAsync(dIn, A, n * n * 8, hipMemcpyHostToDevice, c.stream));
+        amls::Blob blob;
+        amls::Plan plan(&blob, &c.kstats);
+        amls::Copy2dBatch cb;
+        cb.add(amls::cdesc(dIn, n, dA, n, n, n, 2));
+        plan.copy(cb);
+        amls::dev::EigDesc ed;
+        std::memset(&ed, 0, sizeof(ed));
+        ed.A = dA;
+        ed.lda = n;
+        ed.n = static_cast<int32_t>(n);
+        ed.d = d;
+        ed.e = e;
+        ed.tau = tau;
+        ed.lam = dl;
+        ed.Y = dY;
+        ed.work = wk;
+        ed.k = ed.kcap = static_cast<int32_t>(k);
+        amls::addSytrd(plan, {ed}, {0}, sv, sp);
+        amls::addSelect(plan, {ed}, true);
+        plan.run(c.stream);
+        if (k > 0) {
+            AMLS_HIP_CHECK(hipMemcpyAsync(lam, dl, k * 8, hipMemcpyDeviceToHost, c.stream));
+            if (V) AMLS_HIP_CHECK(hipMemcpyAsync(V, dY, n * k * 8, hipMemcpyDeviceToHost, c.stream));
+        }
+        c.sync();
+    });
+}
+
+}  // extern "C"

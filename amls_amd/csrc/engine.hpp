@@ -1,0 +1,211 @@
+// Host-side engine of libamls_hip.so: owns the device memory of one problem
+// and sequences the batched kernels level by level.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "host/tree.hpp"
+#include "kernels/common.hpp"
+#include "kernels/launch.hpp"
+
+namespace amls {
+
+struct StatusError : std::runtime_error {
+    int code;
+    StatusError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define AMLS_HIP_CHECK(expr)                                                                    \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            throw ::amls::StatusError(e_ == hipErrorOutOfMemory ? 5 : 4,                        \
+                                      std::string("HIP error '") + hipGetErrorString(e_) +      \
+                                          "' at " #expr);                                       \
+    } while (0)
+
+// One hipMalloc per phase; sub-allocations aligned to 256 bytes.
+class DevArena {
+public:
+    DevArena() = default;
+    DevArena(const DevArena&) = delete;
+    DevArena& operator=(const DevArena&) = delete;
+    ~DevArena();
+    void plan(size_t bytes) { planned_ += (bytes + 255) & ~size_t(255); }
+    void commit();   // allocate exactly what was planned
+    template <class T> T* take(size_t count) {
+        const size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
+        if (used_ + bytes > cap_) throw StatusError(7, "device arena overflow");
+        T* p = reinterpret_cast<T*>(base_ + used_);
+        used_ += bytes;
+        return p;
+    }
+    void release();
+    size_t capacity() const { return cap_; }
+
+private:
+    char* base_ = nullptr;
+    size_t cap_ = 0, used_ = 0, planned_ = 0;
+};
+
+// Host staging of descriptor arrays, uploaded in one copy per phase.
+class Blob {
+public:
+    template <class T> size_t add(const std::vector<T>& v) {
+        size_t off = (host_.size() + 63) & ~size_t(63);
+        host_.resize(off + v.size() * sizeof(T));
+        if (!v.empty()) std::memcpy(host_.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
+    void upload(hipStream_t s);
+    template <class T> const T* at(size_t off) const {
+        return reinterpret_cast<const T*>(dev_ + off);
+    }
+    ~Blob();
+
+private:
+    std::vector<char> host_;
+    char* dev_ = nullptr;
+    size_t devcap_ = 0;
+};
+
+struct GemmBatch {
+    std::vector<dev::GemmDesc> descs;
+    std::vector<dev::TileRef> tiles;
+    void add(const dev::GemmDesc& d);
+    bool empty() const { return tiles.empty(); }
+};
+
+// Per kernel-class launch statistics: HIP events around every launch of the
+// class on the engine's stream (only while enabled), plus the algorithmic
+// flops and bytes of the launch computed from its descriptors.
+struct KernelStats {
+    bool enabled = false;
+    struct Entry {
+        int64_t launches = 0;
+        double ms = 0.0, flops = 0.0, bytes = 0.0;
+    };
+    std::map<std::string, Entry> byName;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get();
+    // record a launch; when enabled, `launch` is bracketed with events
+    void run(const std::string& name, double flops, double bytes, hipStream_t s,
+             const std::function<void()>& launch);
+    void resolve();   // after a stream sync
+    void reset();
+    ~KernelStats();
+};
+
+struct Ctx {
+    int device = -1;
+    bool gpu = false;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // ---- problem (host)
+    int64_t n = 0;
+    double shift = 0.0;
+    std::vector<int64_t> Kp, Ki, Mp, Mi;
+    std::vector<double> Kx, Mx;
+    Graph g;
+    PartitionResult part;
+    Symbolic S;
+    std::vector<int64_t> owner;
+    bool haveProblem = false;
+
+    // ---- problem (device)
+    std::unique_ptr<DevArena> probMem;
+    double *dKx = nullptr, *dMx = nullptr;
+    int64_t *dKp = nullptr, *dKi = nullptr, *dMp = nullptr, *dMi = nullptr;
+    int64_t *dsrcK = nullptr, *dtgtK = nullptr, *dsrcM = nullptr, *dtgtM = nullptr;
+    int64_t nasmK = 0, nasmM = 0;
+    int64_t *dperm = nullptr, *dbidx = nullptr, *drel = nullptr;
+    // permuted CSR of K and M (for the subspace iteration)
+    int64_t *dPKp = nullptr, *dPKi = nullptr, *dPMp = nullptr, *dPMi = nullptr;
+    double *dPKx = nullptr, *dPMx = nullptr;
+
+    // ---- elimination
+    std::unique_ptr<DevArena> geMem;
+    std::unique_ptr<Blob> geBlob;
+    std::vector<int64_t> fo, xo, wo;   // element offsets per post-order node
+    int64_t ftot = 0, xtot = 0, wtot = 0;
+    double *dF = nullptr, *dG = nullptr, *dXb = nullptr, *dFm = nullptr, *dW = nullptr;
+    int* dErr = nullptr;
+    bool geDone = false;
+    int gePrec = 0;
+
+    // ---- local spectra
+    std::unique_ptr<DevArena> spMem;
+    std::unique_ptr<Blob> spBlob;
+    std::vector<int64_t> co, vo;       // n_p^2 and n_p offsets
+    double *dC = nullptr, *dT = nullptr, *dD = nullptr, *dE = nullptr, *dTau = nullptr;
+    double *dSv = nullptr, *dSp = nullptr, *dDsq = nullptr;
+    int64_t* dSoff = nullptr;
+    bool specReady = false;
+
+    // ---- truncation pass
+    std::unique_ptr<DevArena> passMem;
+    std::unique_ptr<Blob> passBlob;
+    std::vector<int64_t> k, moff, mstart;   // per node
+    int64_t ptot = 0;
+    std::vector<int64_t> zo, ro;            // Z and R-front offsets
+    double *dLam = nullptr, *dY = nullptr, *dZ = nullptr, *dMu = nullptr, *dSw = nullptr;
+    double *dR = nullptr, *dMt = nullptr;
+    std::unique_ptr<DevArena> projMem;
+    std::unique_ptr<Blob> passBlob2;
+    bool selected = false, projected = false;
+
+    // ---- reduced problem
+    std::unique_ptr<DevArena> redMem;
+    std::unique_ptr<Blob> redBlob;
+    int64_t want = 0;
+    double* dQ = nullptr;
+    std::vector<double> theta;
+    bool reduced = false, redVectors = false;
+
+    // ---- recovered vectors (permuted order, n x want) and output
+    std::unique_ptr<DevArena> vecMem;
+    std::unique_ptr<Blob> vecBlob;
+    double *dZfull = nullptr, *dXout = nullptr;
+    bool recovered = false;
+
+    // ---- subspace iteration
+    std::unique_ptr<DevArena> siMem;
+    std::unique_ptr<Blob> siBlob;
+
+    std::map<std::string, double> timing;
+    int64_t lastWant = 0;
+    KernelStats kstats;
+
+    // phases
+    void setProblem(int64_t n, const int64_t* Kp, const int64_t* Ki, const double* Kx,
+                    const int64_t* Mp, const int64_t* Mi, const double* Mx, const double* coords,
+                    int nLevels, int64_t leafSize, double shift);
+    void runGE(int prec);
+    void spectralPrepare();
+    void spectralSelect(int kind, double value);
+    void project(int prec);
+    void reducedSolve(int64_t want, bool vectors);
+    void backtransform();
+    void subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& thetaOut);
+    void sync();
+    void requireGpu() const {
+        if (!gpu) throw StatusError(6, "context has no device");
+    }
+
+};
+
+}  // namespace amls

@@ -1,0 +1,477 @@
+// Separator tree, nested dissection and symbolic analysis (host side).
+#include "tree.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <numeric>
+#include <stdexcept>
+
+namespace amls {
+
+// ---------------------------------------------------------------------------
+// SeparatorTree (semantics of src/amls/separator_tree.cpp:9-107)
+// ---------------------------------------------------------------------------
+int64_t SeparatorTree::addNode(NodeRole role, std::vector<int64_t> indices) {
+    TreeNode nd;
+    nd.id = nNodes();
+    nd.role = role;
+    nd.indices = std::move(indices);
+    nodes_.push_back(std::move(nd));
+    return nodes_.back().id;
+}
+
+void SeparatorTree::setChildren(int64_t id, int64_t left, int64_t right) {
+    nodes_.at(static_cast<size_t>(id)).children = {left, right};
+}
+
+void SeparatorTree::setParent(int64_t id, int64_t parent) {
+    nodes_.at(static_cast<size_t>(id)).parent = parent;
+}
+
+int64_t SeparatorTree::root() const {
+    for (const auto& nd : nodes_)
+        if (nd.parent == -1) return nd.id;
+    return -1;
+}
+
+std::vector<int64_t> SeparatorTree::postOrder() const {
+    std::vector<int64_t> out;
+    out.reserve(nodes_.size());
+    if (nodes_.empty()) return out;
+    // iterative post-order: children left to right, then the node
+    std::vector<std::pair<int64_t, size_t>> st;
+    st.push_back({root(), 0});
+    while (!st.empty()) {
+        auto& top = st.back();
+        const TreeNode& nd = node(top.first);
+        if (top.second < nd.children.size()) {
+            const int64_t c = nd.children[top.second++];
+            st.push_back({c, 0});
+        } else {
+            out.push_back(top.first);
+            st.pop_back();
+        }
+    }
+    return out;
+}
+
+std::vector<int64_t> SeparatorTree::ancestors(int64_t id) const {
+    std::vector<int64_t> out;
+    for (int64_t p = node(id).parent; p != -1; p = node(p).parent) out.push_back(p);
+    return out;
+}
+
+std::vector<int64_t> SeparatorTree::descendants(int64_t id) const {
+    std::vector<int64_t> out, st(node(id).children.rbegin(), node(id).children.rend());
+    while (!st.empty()) {
+        const int64_t c = st.back();
+        st.pop_back();
+        out.push_back(c);
+        const auto& ch = node(c).children;
+        for (auto it = ch.rbegin(); it != ch.rend(); ++it) st.push_back(*it);
+    }
+    return out;
+}
+
+void SeparatorTree::relabelRoot() {
+    const int64_t r = root();
+    if (r >= 0 && !nodes_[static_cast<size_t>(r)].children.empty())
+        nodes_[static_cast<size_t>(r)].role = NodeRole::Root;
+}
+
+bool SeparatorTree::validate(int64_t nTotal, std::string* why) const {
+    auto fail = [&](const char* m) {
+        if (why) *why = m;
+        return false;
+    };
+    std::vector<char> seen(static_cast<size_t>(nTotal), 0);
+    int64_t cnt = 0, roots = 0;
+    for (const auto& nd : nodes_) {
+        for (int64_t v : nd.indices) {
+            if (v < 0 || v >= nTotal) return fail("index out of range");
+            if (seen[static_cast<size_t>(v)]) return fail("indices overlap");
+            seen[static_cast<size_t>(v)] = 1;
+            ++cnt;
+        }
+        if (nd.parent == -1) ++roots;
+        if (!nd.children.empty()) {
+            if (nd.children.size() != 2) return fail("separator without two children");
+            for (int64_t c : nd.children)
+                if (node(c).parent != nd.id) return fail("parent/child asymmetry");
+            if (nd.role == NodeRole::Leaf) return fail("leaf with children");
+        } else if (nd.role != NodeRole::Leaf) {
+            return fail("childless non-leaf");
+        }
+        if (nd.role == NodeRole::Root && nd.parent != -1) return fail("root with parent");
+    }
+    if (cnt != nTotal) return fail("indices do not cover 0..n-1");
+    if (roots != 1) return fail("not exactly one root");
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Graph of |K| + |M|
+// ---------------------------------------------------------------------------
+Graph graphFromCsc(int64_t n, const int64_t* cpK, const int64_t* riK,
+                   const int64_t* cpM, const int64_t* riM) {
+    Graph g;
+    g.n = n;
+    g.ptr.assign(static_cast<size_t>(n) + 1, 0);
+    std::vector<int64_t> mark(static_cast<size_t>(n), -1);
+    // union of both patterns, symmetrised, self loops dropped
+    std::vector<std::vector<int64_t>> tmp;  // avoided for large n: two passes
+    // pass 1: count
+    auto visit = [&](auto&& fn) {
+        for (int64_t c = 0; c < n; ++c) {
+            for (int64_t e = cpK[c]; e < cpK[c + 1]; ++e) fn(c, riK[e]);
+            if (cpM)
+                for (int64_t e = cpM[c]; e < cpM[c + 1]; ++e) fn(c, riM[e]);
+        }
+    };
+    std::vector<int64_t> deg(static_cast<size_t>(n), 0);
+    // symmetric patterns are the norm; symmetrise anyway by counting both
+    // directions and deduplicating per vertex afterwards
+    visit([&](int64_t c, int64_t r) {
+        if (r == c) return;
+        ++deg[static_cast<size_t>(c)];
+        ++deg[static_cast<size_t>(r)];
+    });
+    std::vector<int64_t> ptr(static_cast<size_t>(n) + 1, 0);
+    for (int64_t v = 0; v < n; ++v) ptr[v + 1] = ptr[v] + deg[v];
+    std::vector<int64_t> adj(static_cast<size_t>(ptr[n]));
+    std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+    visit([&](int64_t c, int64_t r) {
+        if (r == c) return;
+        adj[static_cast<size_t>(fill[c]++)] = r;
+        adj[static_cast<size_t>(fill[r]++)] = c;
+    });
+    // dedupe per vertex
+    g.adj.reserve(adj.size() / 2 + 16);
+    for (int64_t v = 0; v < n; ++v) {
+        for (int64_t e = ptr[v]; e < ptr[v + 1]; ++e) {
+            const int64_t w = adj[static_cast<size_t>(e)];
+            if (mark[w] != v) {
+                mark[w] = v;
+                g.adj.push_back(w);
+            }
+        }
+        std::sort(g.adj.begin() + g.ptr[v], g.adj.end());
+        g.ptr[v + 1] = static_cast<int64_t>(g.adj.size());
+    }
+    (void)tmp;
+    return g;
+}
+
+// ---------------------------------------------------------------------------
+// Nested dissection
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Bisector {
+    const Graph& g;
+    const double* coords;
+    std::vector<int64_t> stamp;   // membership stamp per vertex
+    std::vector<int64_t> dist;    // BFS level / side scratch
+    int64_t cur = 0;
+
+    Bisector(const Graph& gg, const double* c)
+        : g(gg), coords(c), stamp(static_cast<size_t>(gg.n), -1),
+          dist(static_cast<size_t>(gg.n), -1) {}
+
+    // side: 0 / 1 / 2 (= separator) for each vertex of verts, in order.
+    bool byPlane(const std::vector<int64_t>& verts, std::vector<int>& side) {
+        if (!coords) return false;
+        const int64_t m = static_cast<int64_t>(verts.size());
+        double lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) { lo[d] = 1e300; hi[d] = -1e300; }
+        for (int64_t v : verts)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::min(lo[d], coords[3 * v + d]);
+                hi[d] = std::max(hi[d], coords[3 * v + d]);
+            }
+        int axes[3] = {0, 1, 2};
+        std::sort(axes, axes + 3, [&](int a, int b) {
+            const double ea = hi[a] - lo[a], eb = hi[b] - lo[b];
+            return ea != eb ? ea > eb : a < b;
+        });
+        std::vector<double> vals(static_cast<size_t>(m));
+        for (int ai = 0; ai < 3; ++ai) {
+            const int d = axes[ai];
+            if (!(hi[d] > lo[d])) break;
+            for (int64_t t = 0; t < m; ++t) vals[t] = coords[3 * verts[t] + d];
+            std::vector<double> u(vals);
+            std::sort(u.begin(), u.end());
+            std::vector<double> uv;
+            std::vector<int64_t> cnt;
+            for (double x : u) {
+                if (uv.empty() || x != uv.back()) { uv.push_back(x); cnt.push_back(0); }
+                ++cnt.back();
+            }
+            const int64_t K = static_cast<int64_t>(uv.size());
+            if (K < 3) continue;
+            std::vector<int64_t> pre(static_cast<size_t>(K) + 1, 0);
+            for (int64_t j = 0; j < K; ++j) pre[j + 1] = pre[j] + cnt[j];
+            std::vector<int64_t> cand;
+            for (int64_t j = 1; j + 1 < K; ++j) cand.push_back(j);
+            auto bal = [&](int64_t j) {
+                const int64_t less = pre[j], greater = m - pre[j + 1];
+                return std::llabs(less - greater);
+            };
+            std::stable_sort(cand.begin(), cand.end(), [&](int64_t a, int64_t b) {
+                const int64_t ba = bal(a), bb = bal(b);
+                if (ba != bb) return ba < bb;
+                return cnt[a] < cnt[b];
+            });
+            if (cand.size() > 12) cand.resize(12);
+            ++cur;
+            for (int64_t t = 0; t < m; ++t) stamp[verts[t]] = cur;
+            for (int64_t j : cand) {
+                const double c = uv[j];
+                const int64_t less = pre[j], greater = m - pre[j + 1];
+                if (std::min(less, greater) < 2) continue;
+                bool ok = true;
+                for (int64_t t = 0; t < m && ok; ++t) {
+                    const int64_t v = verts[t];
+                    if (!(coords[3 * v + d] < c)) continue;
+                    for (int64_t e = g.ptr[v]; e < g.ptr[v + 1]; ++e) {
+                        const int64_t w = g.adj[e];
+                        if (stamp[w] == cur && coords[3 * w + d] > c) { ok = false; break; }
+                    }
+                }
+                if (!ok) continue;
+                side.assign(static_cast<size_t>(m), 0);
+                for (int64_t t = 0; t < m; ++t) {
+                    const double x = vals[t];
+                    side[t] = x < c ? 0 : (x > c ? 1 : 2);
+                }
+                return true;
+            }
+        }
+        return false;
+    }
+
+    // BFS level structure from a pseudo-peripheral vertex; separator = the
+    // level where the cumulative count first reaches half.
+    void byLevels(const std::vector<int64_t>& verts, std::vector<int>& side) {
+        const int64_t m = static_cast<int64_t>(verts.size());
+        ++cur;
+        const int64_t mine = cur;
+        for (int64_t t = 0; t < m; ++t) stamp[verts[t]] = mine;
+        auto bfs = [&](int64_t s, std::vector<int64_t>& order, int64_t& ecc) {
+            order.clear();
+            for (int64_t t = 0; t < m; ++t) dist[verts[t]] = -1;
+            dist[s] = 0;
+            order.push_back(s);
+            for (size_t h = 0; h < order.size(); ++h) {
+                const int64_t v = order[h];
+                for (int64_t e = g.ptr[v]; e < g.ptr[v + 1]; ++e) {
+                    const int64_t w = g.adj[e];
+                    if (stamp[w] == mine && dist[w] < 0) {
+                        dist[w] = dist[v] + 1;
+                        order.push_back(w);
+                    }
+                }
+            }
+            ecc = dist[order.back()];
+        };
+        auto degIn = [&](int64_t v) {
+            int64_t d = 0;
+            for (int64_t e = g.ptr[v]; e < g.ptr[v + 1]; ++e) d += stamp[g.adj[e]] == mine;
+            return d;
+        };
+        std::vector<int64_t> order;
+        int64_t s = verts[0], ecc = 0;
+        bfs(s, order, ecc);
+        for (int it = 0; it < 8; ++it) {
+            int64_t best = -1, bd = 0;
+            for (auto rit = order.rbegin(); rit != order.rend() && dist[*rit] == ecc; ++rit) {
+                const int64_t d = degIn(*rit);
+                if (best < 0 || d < bd || (d == bd && *rit < best)) { best = *rit; bd = d; }
+            }
+            std::vector<int64_t> o2;
+            int64_t e2 = 0;
+            bfs(best, o2, e2);
+            if (e2 <= ecc) { bfs(s, order, ecc); break; }
+            s = best;
+            order.swap(o2);
+            ecc = e2;
+        }
+        // cumulative level counts over the reached component
+        std::vector<int64_t> lc(static_cast<size_t>(ecc) + 1, 0);
+        for (int64_t v : order) ++lc[dist[v]];
+        int64_t acc = 0, mlev = 0;
+        const int64_t half = (m + 1) / 2;
+        for (int64_t l = 0; l <= ecc; ++l) {
+            acc += lc[l];
+            if (acc >= half) { mlev = l; break; }
+        }
+        side.assign(static_cast<size_t>(m), 1);
+        for (int64_t t = 0; t < m; ++t) {
+            const int64_t dv = dist[verts[t]];
+            if (dv < 0) side[t] = 1;                  // other component
+            else side[t] = dv < mlev ? 0 : (dv == mlev ? 2 : 1);
+        }
+    }
+};
+
+}  // namespace
+
+PartitionResult recursiveBisection(const Graph& g, const double* coords,
+                                   int nLevels, int64_t leafSize) {
+    PartitionResult res;
+    const int64_t n = g.n;
+    if (n == 0) return res;
+    Bisector bis(g, coords);
+    std::function<int64_t(std::vector<int64_t>&, int)> recurse =
+        [&](std::vector<int64_t>& verts, int depth) -> int64_t {
+        const int64_t m = static_cast<int64_t>(verts.size());
+        if (m <= leafSize || depth >= nLevels || m < 4)
+            return res.tree.addNode(NodeRole::Leaf, verts);
+        std::vector<int> side;
+        if (!bis.byPlane(verts, side)) bis.byLevels(verts, side);
+        std::vector<int64_t> s0, s1, sep;
+        for (int64_t t = 0; t < m; ++t) {
+            if (side[t] == 2) sep.push_back(verts[t]);
+            else if (side[t] == 0) s0.push_back(verts[t]);
+            else s1.push_back(verts[t]);
+        }
+        const int64_t minSide = static_cast<int64_t>(std::min(s0.size(), s1.size()));
+        if (sep.empty() || minSide < 2)
+            return res.tree.addNode(NodeRole::Leaf, verts);
+        const int64_t sepId = res.tree.addNode(NodeRole::Separator, std::move(sep));
+        std::vector<int64_t>().swap(verts);
+        const int64_t l = recurse(s0, depth + 1);
+        const int64_t r = recurse(s1, depth + 1);
+        res.tree.setChildren(sepId, l, r);
+        res.tree.setParent(l, sepId);
+        res.tree.setParent(r, sepId);
+        return sepId;
+    };
+    std::vector<int64_t> all(static_cast<size_t>(n));
+    std::iota(all.begin(), all.end(), 0);
+    recurse(all, 0);
+    res.tree.relabelRoot();
+    for (int64_t id : res.tree.postOrder()) {
+        const auto& idx = res.tree.node(id).indices;
+        res.perm.insert(res.perm.end(), idx.begin(), idx.end());
+    }
+    std::string why;
+    if (!res.tree.validate(n, &why))
+        throw std::runtime_error("partition: tree validation failed: " + why);
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// Symbolic analysis
+// ---------------------------------------------------------------------------
+Symbolic analyze(const PartitionResult& part, const Graph& g) {
+    Symbolic S;
+    const SeparatorTree& T = part.tree;
+    S.n = g.n;
+    S.nNodes = T.nNodes();
+    const int64_t N = S.nNodes;
+    const std::vector<int64_t> order = T.postOrder();
+    std::vector<int64_t> posOf(static_cast<size_t>(N));
+    for (int64_t p = 0; p < N; ++p) posOf[order[p]] = p;
+    S.treeId = order;
+    S.off.resize(N);
+    S.size.resize(N);
+    S.parent.assign(N, -1);
+    S.child0.assign(N, -1);
+    S.child1.assign(N, -1);
+    S.level.assign(N, 0);
+    S.subStart.resize(N);
+    int64_t acc = 0;
+    for (int64_t p = 0; p < N; ++p) {
+        const TreeNode& nd = T.node(order[p]);
+        S.off[p] = acc;
+        S.size[p] = static_cast<int64_t>(nd.indices.size());
+        acc += S.size[p];
+        if (nd.parent >= 0) S.parent[p] = posOf[nd.parent];
+        if (nd.children.size() == 2) {
+            S.child0[p] = posOf[nd.children[0]];
+            S.child1[p] = posOf[nd.children[1]];
+            S.level[p] = std::max(S.level[S.child0[p]], S.level[S.child1[p]]) + 1;
+            S.subStart[p] = std::min(S.subStart[S.child0[p]], S.subStart[S.child1[p]]);
+        } else {
+            S.subStart[p] = p;
+        }
+        S.maxLevel = std::max(S.maxLevel, S.level[p]);
+    }
+    if (acc != S.n) throw std::runtime_error("analyze: tree does not cover n");
+    S.perm = part.perm;
+    S.iperm.assign(static_cast<size_t>(S.n), 0);
+    for (int64_t j = 0; j < S.n; ++j) S.iperm[S.perm[j]] = j;
+    std::vector<int64_t> owner(static_cast<size_t>(S.n));
+    for (int64_t p = 0; p < N; ++p)
+        for (int64_t j = S.off[p]; j < S.off[p] + S.size[p]; ++j) owner[j] = p;
+
+    // separator property: every edge joins a node with itself, an ancestor
+    // or a descendant (subtree of q = positions [subStart[q], q])
+    auto inSub = [&](int64_t p, int64_t q) { return S.subStart[q] <= p && p <= q; };
+    for (int64_t v = 0; v < S.n; ++v) {
+        const int64_t pv = owner[S.iperm[v]];
+        for (int64_t e = g.ptr[v]; e < g.ptr[v + 1]; ++e) {
+            const int64_t pw = owner[S.iperm[g.adj[e]]];
+            if (pv != pw && !inSub(pv, pw) && !inSub(pw, pv))
+                throw std::runtime_error("analyze: partition is not a vertex-separator tree");
+        }
+    }
+
+    // boundaries, post-order (children first)
+    S.bptr.assign(static_cast<size_t>(N) + 1, 0);
+    std::vector<std::vector<int64_t>> B(static_cast<size_t>(N));
+    std::vector<int64_t> mark(static_cast<size_t>(S.n), -1);
+    for (int64_t p = 0; p < N; ++p) {
+        const int64_t hiEnd = S.off[p] + S.size[p];
+        auto& out = B[p];
+        auto add = [&](int64_t j) {
+            if (j >= hiEnd && mark[j] != p) {
+                mark[j] = p;
+                out.push_back(j);
+            }
+        };
+        for (int64_t j = S.off[p]; j < hiEnd; ++j) {
+            const int64_t v = S.perm[j];
+            for (int64_t e = g.ptr[v]; e < g.ptr[v + 1]; ++e) add(S.iperm[g.adj[e]]);
+        }
+        for (int64_t c : {S.child0[p], S.child1[p]})
+            if (c >= 0)
+                for (int64_t j : B[c]) add(j);
+        std::sort(out.begin(), out.end());
+    }
+    for (int64_t p = 0; p < N; ++p) S.bptr[p + 1] = S.bptr[p] + static_cast<int64_t>(B[p].size());
+    S.bidx.resize(static_cast<size_t>(S.bptr[N]));
+    for (int64_t p = 0; p < N; ++p) std::copy(B[p].begin(), B[p].end(), S.bidx.begin() + S.bptr[p]);
+    // relative indices of each child's boundary inside the parent's front
+    S.relptr = S.bptr;
+    S.rel.assign(S.bidx.size(), -1);
+    for (int64_t p = 0; p < N; ++p) {
+        const int64_t q = S.parent[p];
+        if (q < 0) {
+            if (S.b(p) != 0) throw std::runtime_error("analyze: root has a boundary");
+            continue;
+        }
+        const auto& Bq = B[q];
+        for (int64_t t = S.bptr[p]; t < S.bptr[p + 1]; ++t) {
+            const int64_t j = S.bidx[t];
+            int64_t r;
+            if (j < S.off[q]) throw std::runtime_error("analyze: boundary below parent");
+            if (j < S.off[q] + S.size[q]) {
+                r = j - S.off[q];
+            } else {
+                auto it = std::lower_bound(Bq.begin(), Bq.end(), j);
+                if (it == Bq.end() || *it != j)
+                    throw std::runtime_error("analyze: child boundary not in parent front");
+                r = S.size[q] + (it - Bq.begin());
+            }
+            S.rel[t] = r;
+        }
+    }
+    S.byLevel.assign(static_cast<size_t>(S.maxLevel) + 1, {});
+    for (int64_t p = 0; p < N; ++p) S.byLevel[S.level[p]].push_back(p);
+    return S;
+}
+
+}  // namespace amls

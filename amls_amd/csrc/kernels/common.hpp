@@ -1,0 +1,124 @@
+// Device-side descriptors and helpers shared by the HIP kernels (gfx950).
+//
+// Every batched kernel takes an array of small descriptors that point into
+// the context's device arena; the host builds the arrays once per phase and
+// uploads them, so a launch carries only (descriptor pointer, work list).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace amls {
+namespace dev {
+
+// C = alpha1 * op(A) op(B) + alpha2 * op(A2) op(B2) + beta * C
+// op(A) is m x k, op(B) is k x n; ta/tb = 1 means the stored matrix is the
+// transpose.  lower = 1 writes only elements with row >= col (after the
+// row/col offsets below are removed: the test is on tile coordinates).
+struct GemmDesc {
+    const double* A;
+    const double* B;
+    const double* A2;
+    const double* B2;
+    double* C;
+    int64_t lda, ldb, lda2, ldb2, ldc;
+    int32_t m, n, k, k2;
+    int32_t ta, tb, ta2, tb2;
+    double alpha1, alpha2, beta;
+    int32_t lower;
+    int32_t pad;
+};
+
+// One 64 x 64 output tile of descriptor `desc`.
+struct TileRef {
+    int32_t desc, tm, tn, pad;
+};
+
+// Left triangular solve in place: op(L) X = B with L unit lower triangular
+// (strictly lower part of an LDL^T factor, leading dimension ldl).
+// trans = 0: L X = B (forward), trans = 1: L^T X = B (backward).
+struct TrsmDesc {
+    const double* L;
+    double* B;
+    int64_t ldl, ldb;
+    int32_t n, nrhs;
+    int32_t trans, pad;
+};
+
+struct TrsmWork {
+    int32_t desc, cb, pad0, pad1;   // cb = 64-column block of the right-hand side
+};
+
+// Partial LDL^T of a front: columns [0, n) of the f x f lower-stored front F
+// (leading dimension f) are eliminated; the trailing (f-n) x (f-n) block
+// becomes the Schur complement.  W is an f x 64 scratch panel.
+struct FrontDesc {
+    double* F;
+    double* W;
+    int64_t f;
+    int32_t n, node;
+};
+
+// dst[rowmap[s] (or s), colmap[t] (or coloff + t)] += src[s, t]
+struct ExtAddDesc {
+    const double* src;
+    double* dst;
+    const int64_t* rowmap;   // null: identity
+    int64_t lds, ldd;
+    int32_t rows, cols;
+    int32_t coloff;          // used when symmetric == 0
+    int32_t symmetric;       // 1: square src, colmap = rowmap, lower part only
+};
+
+// Symmetric tridiagonal eigen-problem bookkeeping for one dense matrix.
+struct EigDesc {
+    double* A;        // n x n, leading dimension lda (reflectors on exit)
+    double* d;        // n
+    double* e;        // n  (e[i] = T(i+1, i))
+    double* tau;      // n
+    double* lam;      // selected eigenvalues (descending), capacity kcap
+    double* Y;        // n x kcap eigenvectors (ld n)
+    double* work;     // per-eigenvalue scratch: 5 * n doubles per cluster thread
+    int64_t lda;
+    int32_t n, k;     // k = number selected
+    int32_t node, kcap;
+};
+
+// dst(i,j) = alpha * rs[i] * cs[j] * S(i,j); mode 0: S = src, 1: S = src^T,
+// 2: S = symmetric completion of src's lower triangle.
+struct Copy2dDesc {
+    const double* src;
+    double* dst;
+    const double* rs;
+    const double* cs;
+    int64_t lds, ldd;
+    int32_t rows, cols;
+    int32_t mode, pad;
+    double alpha;
+};
+
+// scatter = 0: dst(r, c) = src(rows_idx[r], c); 1: dst(rows_idx[r], c) = src(r, c)
+struct GatherDesc {
+    const double* src;
+    double* dst;
+    const int64_t* rows_idx;
+    int64_t lds, ldd;
+    int32_t rows, cols;
+    int32_t scatter, pad;
+};
+
+// Small per-matrix vector operations (one block per descriptor):
+//   op 0: dst[i] = 1 / sqrt(src[i + i*ld])       (D^{-1/2} from an LDL^T diagonal)
+//   op 1: dst[i] = 1 / src[i], dst2[i] = sqrt(1 / src[i])
+//   op 2: dst[i + i*ld] = 1
+//   op 3: dst[i] = 1 / sqrt(src[i])
+//   op 4: dst[i] = sqrt(1 / src[i])
+struct VecDesc {
+    const double* src;
+    double* dst;
+    double* dst2;
+    int64_t ld;
+    int32_t n, op;
+};
+
+}  // namespace dev
+}  // namespace amls

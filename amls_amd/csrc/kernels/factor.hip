@@ -1,0 +1,251 @@
+// Front assembly, partial LDL^T of fronts, extend-add and batched copies.
+//
+// Replaces the per-node factorisation of the reference
+// (GaussianElimination::factorNode_, src/amls/gaussian_elimination.cpp:458-508,
+// with Eigen SimplicialLDLT / LDLT in src/linalg/sparse_direct.cpp:10-63,
+// including the "D > 0 and finite" check) and the block scatter of its Schur /
+// congruence updates into ancestor blocks (:678-781), restated as a
+// multifrontal extend-add so no two workgroups write one element.
+#include "common.hpp"
+#include "launch.hpp"
+
+#include <climits>
+
+namespace amls {
+namespace dev {
+
+constexpr int NB = 64;
+
+// One workgroup per front with n > s0: unblocked LDL^T of the diagonal block
+// of panel s0 in LDS; L11 (strict lower) and D (diagonal) are written back.
+__global__ void __launch_bounds__(256) ldlt_diag_kernel(const FrontDesc* __restrict__ descs,
+                                                        const int32_t* __restrict__ list,
+                                                        int s0, int* __restrict__ err) {
+    __shared__ double sD[NB * (NB + 1)];
+    const FrontDesc d = descs[list[blockIdx.x]];
+    const int64_t f = d.f;
+    const int jb = min(NB, d.n - s0);
+    double* F = d.F;
+    const int t = threadIdx.x;
+    for (int e = t; e < NB * NB; e += 256) {
+        const int i = e & 63, j = e >> 6;
+        double v = 0.0;
+        if (i < jb && j < jb && i >= j) v = F[(s0 + i) + (int64_t)(s0 + j) * f];
+        sD[i * (NB + 1) + j] = v;
+    }
+    __syncthreads();
+    for (int j = 0; j < jb; ++j) {
+        const double dj = sD[j * (NB + 1) + j];
+        if (t == 0 && (!(dj > 0.0) || !(dj < INFINITY))) atomicMin(err, d.node);
+        const int m = jb - j - 1;
+        for (int e = t; e < m * m; e += 256) {
+            const int i = j + 1 + e / m, k = j + 1 + e % m;
+            if (k <= i) sD[i * (NB + 1) + k] -= sD[i * (NB + 1) + j] * sD[k * (NB + 1) + j] / dj;
+        }
+        __syncthreads();
+        for (int i = j + 1 + t; i < jb; i += 256) sD[i * (NB + 1) + j] /= dj;
+        __syncthreads();
+    }
+    for (int e = t; e < jb * jb; e += 256) {
+        const int i = e % jb, j = e / jb;
+        if (i >= j) F[(s0 + i) + (int64_t)(s0 + j) * f] = sD[i * (NB + 1) + j];
+    }
+}
+
+// Rows below the diagonal block of panel s0, one row per thread:
+// w L11^T = a (forward substitution in registers), L = w D^-1, W = w.
+// Work item (x = front index, y = chunk of 256 rows).
+__global__ void __launch_bounds__(256) ldlt_rows_kernel(const FrontDesc* __restrict__ descs,
+                                                        const int2* __restrict__ work, int s0) {
+    __shared__ double sL[NB * NB];      // sL[j * NB + k] = L11(k, j), k > j, else 0
+    __shared__ double sdinv[NB];
+    const int2 wk = work[blockIdx.x];
+    const FrontDesc d = descs[wk.x];
+    const int64_t f = d.f;
+    const int jb = min(NB, d.n - s0);
+    const double* F = d.F;
+    const int t = threadIdx.x;
+    for (int e = t; e < NB * NB; e += 256) {
+        const int k = e & 63, j = e >> 6;
+        sL[j * NB + k] = (k > j && k < jb && j < jb) ? F[(s0 + k) + (int64_t)(s0 + j) * f] : 0.0;
+    }
+    if (t < NB) sdinv[t] = t < jb ? 1.0 / F[(s0 + t) + (int64_t)(s0 + t) * f] : 0.0;
+    __syncthreads();
+    const int64_t rb = s0 + jb;
+    const int64_t i = rb + (int64_t)wk.y * 256 + t;
+    if (i >= f) return;
+    // left-looking in blocks of 16 columns: earlier w values are re-read from
+    // W (written by this thread), keeping only 16 row values in registers
+    double* W = d.W + (i - rb);
+    double* Fi = d.F + i;
+    for (int J = 0; J * 16 < jb; ++J) {
+        double a[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int col = J * 16 + q;
+            a[q] = col < jb ? Fi[(int64_t)(s0 + col) * f] : 0.0;
+        }
+        for (int j = 0; j < J * 16; ++j) {
+            const double wj = W[(int64_t)j * f];
+            const double* lj = sL + j * NB + J * 16;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) a[q] -= wj * lj[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const double* lq = sL + (J * 16 + q) * NB + J * 16;
+#pragma unroll
+            for (int r = q + 1; r < 16; ++r) a[r] -= a[q] * lq[r];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int col = J * 16 + q;
+            if (col < jb) {
+                W[(int64_t)col * f] = a[q];
+                Fi[(int64_t)(s0 + col) * f] = a[q] * sdinv[col];
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) extadd_kernel(const ExtAddDesc* __restrict__ descs,
+                                                     const int4* __restrict__ work) {
+    const int4 w = work[blockIdx.x];
+    const ExtAddDesc d = descs[w.x];
+    const int t = threadIdx.x;
+    const int s = w.y * 64 + (t & 63);
+    if (s >= d.rows) return;
+    const int64_t ds = d.rowmap ? d.rowmap[s] : s;
+    for (int q = 0; q < 16; ++q) {
+        const int c = w.z * 64 + (t >> 6) + 4 * q;
+        if (c >= d.cols) break;
+        if (d.symmetric) {
+            if (c > s) continue;
+            const int64_t dc = d.rowmap ? d.rowmap[c] : c;
+            d.dst[ds + dc * d.ldd] += d.src[s + (int64_t)c * d.lds];
+        } else {
+            d.dst[ds + (int64_t)(d.coloff + c) * d.ldd] += d.src[s + (int64_t)c * d.lds];
+        }
+    }
+}
+
+// dst[tgt[e]] (+)= scale * vals[src[e]]
+__global__ void scatter_kernel(const int64_t* __restrict__ src, const int64_t* __restrict__ tgt,
+                               const double* __restrict__ vals, double* __restrict__ dst,
+                               int64_t count, double scale, int accumulate) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    const double v = scale * vals[src[e]];
+    if (accumulate) dst[tgt[e]] += v;
+    else dst[tgt[e]] = v;
+}
+
+__global__ void fill_kernel(double* __restrict__ p, int64_t count, double v) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+         e += (int64_t)gridDim.x * blockDim.x)
+        p[e] = v;
+}
+
+// dst(i,j) = alpha * rs[i] * cs[j] * S(i,j), S = src, src^T or the symmetric
+// completion of src's lower triangle.
+__global__ void __launch_bounds__(256) copy2d_kernel(const Copy2dDesc* __restrict__ descs,
+                                                     const int4* __restrict__ work) {
+    const int4 w = work[blockIdx.x];
+    const Copy2dDesc d = descs[w.x];
+    const int t = threadIdx.x;
+    const int i = w.y * 64 + (t & 63);
+    if (i >= d.rows) return;
+    for (int q = 0; q < 16; ++q) {
+        const int j = w.z * 64 + (t >> 6) + 4 * q;
+        if (j >= d.cols) break;
+        double v;
+        if (d.mode == 1) v = d.src[j + (int64_t)i * d.lds];
+        else if (d.mode == 2) v = i >= j ? d.src[i + (int64_t)j * d.lds] : d.src[j + (int64_t)i * d.lds];
+        else v = d.src[i + (int64_t)j * d.lds];
+        double s = d.alpha;
+        if (d.rs) s *= d.rs[i];
+        if (d.cs) s *= d.cs[j];
+        d.dst[i + (int64_t)j * d.ldd] = s * v;
+    }
+}
+
+// dst(r, c) = src(rows[r], c)   (gather rows of a column-major block)
+__global__ void gather_rows_kernel(const GatherDesc* __restrict__ descs, const int4* __restrict__ work) {
+    const int4 w = work[blockIdx.x];
+    const GatherDesc d = descs[w.x];
+    const int t = threadIdx.x;
+    const int r = w.y * 64 + (t & 63);
+    if (r >= d.rows) return;
+    const int64_t sr = d.rows_idx[r];
+    for (int q = 0; q < 16; ++q) {
+        const int c = w.z * 64 + (t >> 6) + 4 * q;
+        if (c >= d.cols) break;
+        if (d.scatter) d.dst[sr + (int64_t)c * d.ldd] = d.src[r + (int64_t)c * d.lds];
+        else d.dst[r + (int64_t)c * d.ldd] = d.src[sr + (int64_t)c * d.lds];
+    }
+}
+
+__global__ void vecop_kernel(const VecDesc* __restrict__ descs) {
+    const VecDesc d = descs[blockIdx.x];
+    for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+        switch (d.op) {
+            case 0: d.dst[i] = 1.0 / sqrt(d.src[i + (int64_t)i * d.ld]); break;
+            case 1: {
+                const double r = 1.0 / d.src[i];
+                d.dst[i] = r;
+                d.dst2[i] = sqrt(r);
+                break;
+            }
+            case 2: d.dst[i + (int64_t)i * d.ld] = 1.0; break;
+            case 3: d.dst[i] = 1.0 / sqrt(d.src[i]); break;
+            default: d.dst[i] = sqrt(1.0 / d.src[i]); break;
+        }
+    }
+}
+
+}  // namespace dev
+
+void launch_vecop(const dev::VecDesc* d, int count, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(dev::vecop_kernel, dim3(count), dim3(256), 0, s, d);
+}
+
+void launch_ldlt_panel(const dev::FrontDesc* d, const int32_t* list, int count, int s0, int* err,
+                       const int2* rows, int nrows, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(dev::ldlt_diag_kernel, dim3(count), dim3(256), 0, s, d, list, s0, err);
+    if (nrows > 0)
+        hipLaunchKernelGGL(dev::ldlt_rows_kernel, dim3(nrows), dim3(256), 0, s, d, rows, s0);
+}
+
+void launch_extadd(const dev::ExtAddDesc* d, const int4* work, int nwork, hipStream_t s) {
+    if (nwork <= 0) return;
+    hipLaunchKernelGGL(dev::extadd_kernel, dim3(nwork), dim3(256), 0, s, d, work);
+}
+
+void launch_scatter(const int64_t* src, const int64_t* tgt, const double* vals, double* dst,
+                    int64_t count, double scale, bool accumulate, hipStream_t s) {
+    if (count <= 0) return;
+    const int64_t blocks = (count + 255) / 256;
+    hipLaunchKernelGGL(dev::scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src, tgt, vals,
+                       dst, count, scale, accumulate ? 1 : 0);
+}
+
+void launch_fill(double* p, int64_t count, double v, hipStream_t s) {
+    if (count <= 0) return;
+    int64_t blocks = (count + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(dev::fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, count, v);
+}
+
+void launch_copy2d(const dev::Copy2dDesc* d, const int4* work, int nwork, hipStream_t s) {
+    if (nwork <= 0) return;
+    hipLaunchKernelGGL(dev::copy2d_kernel, dim3(nwork), dim3(256), 0, s, d, work);
+}
+
+void launch_gather(const dev::GatherDesc* d, const int4* work, int nwork, hipStream_t s) {
+    if (nwork <= 0) return;
+    hipLaunchKernelGGL(dev::gather_rows_kernel, dim3(nwork), dim3(256), 0, s, d, work);
+}
+
+}  // namespace amls

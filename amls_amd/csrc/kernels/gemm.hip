@@ -1,0 +1,266 @@
+// Batched variable-size GEMM and triangular solves on CDNA4 matrix cores.
+//
+// Tile: 64 x 64 outputs per 256-thread workgroup (four wave64s, each a 32 x 32
+// quarter made of 2 x 2 MFMA 16x16 blocks).  Operands are staged through LDS
+// in 16-deep k-chunks.  The FP64 path uses v_mfma_f64_16x16x4f64; the mixed
+// path converts the FP64 operands to FP32 while staging and runs
+// v_mfma_f32_16x16x4f32 with FP32 accumulation.
+//
+// These kernels carry the GEMM-shaped work of the reference's ColCompOps
+// (src/linalg/colcomp.hpp:124-171, OpenBLAS backend :174-423) and of its
+// Eigen triangular solves (src/linalg/sparse_direct.cpp:26-31, :56-61).
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace amls {
+namespace dev {
+
+template <typename T> struct Mfma;
+// Operand layout (both types): lane l supplies A(l % 16, l / 16) and
+// B(l / 16, l % 16) of the 16x4 / 4x16 blocks.  Result layout differs: the
+// FP64 instruction interleaves rows (row = l/16 + 4*item), the FP32 one
+// stacks them (row = 4*(l/16) + item); column = l % 16 for both.  Checked on
+// gfx950 with random integer operands.
+template <> struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int item) { return (lane >> 4) + 4 * item; }
+};
+template <> struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int item) { return 4 * (lane >> 4) + item; }
+};
+
+constexpr int TM = 64;      // tile rows / cols
+constexpr int KC = 16;      // k-chunk staged in LDS
+constexpr int LDSP = 80;    // padded LDS row (elements)
+
+template <typename T>
+struct TileAcc {
+    typename Mfma<T>::v4 c[2][2];
+    __device__ void zero() {
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int q = 0; q < 4; ++q) c[i][j][q] = T(0);
+    }
+};
+
+// Accumulate op(A)[0:mv, kb:ke] * op(B)[kb:ke, 0:nv] (scaled by `scale`)
+// into the tile accumulator.  A, B point at the tile's first row / column
+// (see the element formulas below); out-of-range elements read as zero.
+template <typename T>
+__device__ void tile_mma(TileAcc<T>& acc, const double* __restrict__ A, int64_t lda,
+                         int ta, int mv, const double* __restrict__ B, int64_t ldb,
+                         int tb, int nv, int kb, int ke, double scale, T* sA, T* sB) {
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int w = t >> 6;
+    const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
+    for (int k0 = kb; k0 < ke; k0 += KC) {
+        // ---- stage A chunk: sA[kk][r] = op(A)(r, k0 + kk)
+        if (ta == 0) {
+            const int r = t & 63;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int kk = (t >> 6) + 4 * j;
+                const int kg = k0 + kk;
+                double v = 0.0;
+                if (r < mv && kg < ke) v = A[r + (int64_t)kg * lda] * scale;
+                sA[kk * LDSP + r] = T(v);
+            }
+        } else {
+            const int kk = t & 15;
+            const int kg = k0 + kk;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int r = (t >> 4) + 16 * j;
+                double v = 0.0;
+                if (r < mv && kg < ke) v = A[kg + (int64_t)r * lda] * scale;
+                sA[kk * LDSP + r] = T(v);
+            }
+        }
+        // ---- stage B chunk: sB[kk][c] = op(B)(k0 + kk, c)
+        if (tb == 0) {
+            const int kk = t & 15;
+            const int kg = k0 + kk;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int c = (t >> 4) + 16 * j;
+                double v = 0.0;
+                if (c < nv && kg < ke) v = B[kg + (int64_t)c * ldb];
+                sB[kk * LDSP + c] = T(v);
+            }
+        } else {
+            const int c = t & 63;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int kk = (t >> 6) + 4 * j;
+                const int kg = k0 + kk;
+                double v = 0.0;
+                if (c < nv && kg < ke) v = B[c + (int64_t)kg * ldb];
+                sB[kk * LDSP + c] = T(v);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < KC / 4; ++ks) {
+            const int kr = ks * 4 + (lane >> 4);
+            const T a0 = sA[kr * LDSP + wr + (lane & 15)];
+            const T a1 = sA[kr * LDSP + wr + 16 + (lane & 15)];
+            const T b0 = sB[kr * LDSP + wc + (lane & 15)];
+            const T b1 = sB[kr * LDSP + wc + 16 + (lane & 15)];
+            acc.c[0][0] = Mfma<T>::mma(a0, b0, acc.c[0][0]);
+            acc.c[0][1] = Mfma<T>::mma(a0, b1, acc.c[0][1]);
+            acc.c[1][0] = Mfma<T>::mma(a1, b0, acc.c[1][0]);
+            acc.c[1][1] = Mfma<T>::mma(a1, b1, acc.c[1][1]);
+        }
+        __syncthreads();
+    }
+}
+
+// Visit the tile's elements held by this lane: fn(row_in_tile, col_in_tile, value).
+template <typename T, typename Fn>
+__device__ __forceinline__ void tile_foreach(const TileAcc<T>& acc, Fn fn) {
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int w = t >> 6;
+    const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                fn(wr + i * 16 + Mfma<T>::row(lane, q), wc + j * 16 + (lane & 15),
+                   (double)acc.c[i][j][q]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) gemm_batched_kernel(const GemmDesc* __restrict__ descs,
+                                                           const TileRef* __restrict__ tiles) {
+    __shared__ T sA[KC * LDSP];
+    __shared__ T sB[KC * LDSP];
+    const TileRef tr = tiles[blockIdx.x];
+    const GemmDesc d = descs[tr.desc];
+    const int r0 = tr.tm * TM, c0 = tr.tn * TM;
+    const int mv = min(TM, d.m - r0), nv = min(TM, d.n - c0);
+    TileAcc<T> acc;
+    acc.zero();
+    if (d.k > 0 && d.A) {
+        const double* A = d.ta ? d.A + (int64_t)r0 * d.lda : d.A + r0;
+        const double* B = d.tb ? d.B + c0 : d.B + (int64_t)c0 * d.ldb;
+        tile_mma<T>(acc, A, d.lda, d.ta, mv, B, d.ldb, d.tb, nv, 0, d.k, d.alpha1, sA, sB);
+    }
+    if (d.k2 > 0 && d.A2) {
+        const double* A = d.ta2 ? d.A2 + (int64_t)r0 * d.lda2 : d.A2 + r0;
+        const double* B = d.tb2 ? d.B2 + c0 : d.B2 + (int64_t)c0 * d.ldb2;
+        tile_mma<T>(acc, A, d.lda2, d.ta2, mv, B, d.ldb2, d.tb2, nv, 0, d.k2, d.alpha2, sA, sB);
+    }
+    const double beta = d.beta;
+    const int lower = d.lower;
+    double* C = d.C;
+    const int64_t ldc = d.ldc;
+    tile_foreach<T>(acc, [&](int r, int c, double v) {
+        const int gr = r0 + r, gc = c0 + c;
+        if (r < mv && c < nv && (!lower || gr >= gc)) {
+            double* p = C + gr + (int64_t)gc * ldc;
+            *p = beta == 0.0 ? v : v + beta * *p;
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solve, one workgroup per (descriptor, 64-column RHS block).
+// Left-looking over 64-row blocks: the off-diagonal part is an MFMA tile
+// product against already-solved rows, the diagonal block is solved in LDS.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) trsm_unit_kernel(const TrsmDesc* __restrict__ descs,
+                                                        const TrsmWork* __restrict__ work) {
+    __shared__ double sA[KC * LDSP];
+    __shared__ double sB[KC * LDSP];
+    __shared__ double sT[TM * (TM + 1)];
+    __shared__ double sL[TM * (TM + 1)];
+    const TrsmWork wk = work[blockIdx.x];
+    const TrsmDesc d = descs[wk.desc];
+    const int n = d.n;
+    const int c0 = wk.cb * TM;
+    const int nv = min(TM, d.nrhs - c0);
+    const int nb = (n + TM - 1) / TM;
+    const int t = threadIdx.x;
+    double* X = d.B + (int64_t)c0 * d.ldb;
+    for (int it = 0; it < nb; ++it) {
+        const int rb = d.trans ? nb - 1 - it : it;
+        const int r0 = rb * TM;
+        const int mv = min(TM, n - r0);
+        TileAcc<double> acc;
+        acc.zero();
+        if (!d.trans) {
+            if (r0 > 0)
+                tile_mma<double>(acc, d.L + r0, d.ldl, 0, mv, X, d.ldb, 0, nv, 0, r0, 1.0, sA, sB);
+        } else {
+            const int kb = r0 + TM;
+            if (kb < n)
+                tile_mma<double>(acc, d.L + (int64_t)r0 * d.ldl, d.ldl, 1, mv, X, d.ldb, 0, nv,
+                                 kb, n, 1.0, sA, sB);
+        }
+        // T = B_tile - acc
+        for (int e = t; e < TM * TM; e += 256) {
+            const int r = e & 63, c = e >> 6;
+            sT[r * (TM + 1) + c] = (r < mv && c < nv) ? X[r0 + r + (int64_t)c * d.ldb] : 0.0;
+            // diagonal block of L: sL[i][j] = L(r0+i, r0+j)
+            const int i = e & 63, j = e >> 6;
+            sL[i * (TM + 1) + j] =
+                (i < mv && j < mv) ? d.L[r0 + i + (int64_t)(r0 + j) * d.ldl] : 0.0;
+        }
+        __syncthreads();
+        tile_foreach<double>(acc, [&](int r, int c, double v) {
+            sT[r * (TM + 1) + c] -= v;
+        });
+        __syncthreads();
+        const int c = t & 63;
+        if (!d.trans) {
+            for (int j = 0; j < mv - 1; ++j) {
+                const double xj = sT[j * (TM + 1) + c];
+                for (int i = j + 1 + (t >> 6); i < mv; i += 4)
+                    sT[i * (TM + 1) + c] -= sL[i * (TM + 1) + j] * xj;
+                __syncthreads();
+            }
+        } else {
+            // L^T upper unit: x_j -= sum_{i > j} L(i, j) x_i, backward
+            for (int j = mv - 1; j > 0; --j) {
+                const double xj = sT[j * (TM + 1) + c];
+                for (int i = (t >> 6); i < j; i += 4)
+                    sT[i * (TM + 1) + c] -= sL[j * (TM + 1) + i] * xj;
+                __syncthreads();
+            }
+        }
+        for (int e = t; e < TM * TM; e += 256) {
+            const int r = e & 63, cc = e >> 6;
+            if (r < mv && cc < nv) X[r0 + r + (int64_t)cc * d.ldb] = sT[r * (TM + 1) + cc];
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace dev
+
+void launch_gemm(const dev::GemmDesc* d, const dev::TileRef* tiles, int ntiles, bool fp32,
+                 hipStream_t s) {
+    if (ntiles <= 0) return;
+    if (fp32)
+        hipLaunchKernelGGL(dev::gemm_batched_kernel<float>, dim3(ntiles), dim3(256), 0, s, d, tiles);
+    else
+        hipLaunchKernelGGL(dev::gemm_batched_kernel<double>, dim3(ntiles), dim3(256), 0, s, d, tiles);
+}
+
+void launch_trsm(const dev::TrsmDesc* d, const dev::TrsmWork* w, int nwork, hipStream_t s) {
+    if (nwork <= 0) return;
+    hipLaunchKernelGGL(dev::trsm_unit_kernel, dim3(nwork), dim3(256), 0, s, d, w);
+}
+
+}  // namespace amls

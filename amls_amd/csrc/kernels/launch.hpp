@@ -1,0 +1,45 @@
+// Host-callable launchers for the batched HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace amls {
+
+void launch_gemm(const dev::GemmDesc* d, const dev::TileRef* tiles, int ntiles, bool fp32,
+                 hipStream_t s);
+void launch_trsm(const dev::TrsmDesc* d, const dev::TrsmWork* w, int nwork, hipStream_t s);
+// Panel s0 of the fronts in `list` (indices into d): diagonal block, then the
+// rows below it in chunks of 256 (rows = (front index, chunk) work items).
+void launch_ldlt_panel(const dev::FrontDesc* d, const int32_t* list, int count, int s0, int* err,
+                       const int2* rows, int nrows, hipStream_t s);
+void launch_extadd(const dev::ExtAddDesc* d, const int4* work, int nwork, hipStream_t s);
+void launch_scatter(const int64_t* src, const int64_t* tgt, const double* vals, double* dst,
+                    int64_t count, double scale, bool accumulate, hipStream_t s);
+void launch_fill(double* p, int64_t count, double v, hipStream_t s);
+void launch_vecop(const dev::VecDesc* d, int count, hipStream_t s);
+void launch_copy2d(const dev::Copy2dDesc* d, const int4* work, int nwork, hipStream_t s);
+void launch_gather(const dev::GatherDesc* d, const int4* work, int nwork, hipStream_t s);
+void launch_sytrd_step(const dev::EigDesc* d, const int2* work, int nwork, int i, double* sv,
+                       double* sp, const int64_t* soff, int nmax, hipStream_t s);
+void launch_eig_count(const dev::EigDesc* d, int nmat, const double* thr, int32_t* counts,
+                      hipStream_t s);
+void launch_bisect(const dev::EigDesc* d, const int2* work, int nwork, hipStream_t s);
+void launch_stein(const dev::EigDesc* d, const int2* work, int nwork, const int64_t* woff,
+                  hipStream_t s);
+void launch_ormtr(const dev::EigDesc* d, const int2* work, int nwork, hipStream_t s);
+// y(:, c) = A x(:, c) for CSR A (n x n), column-major x, y with leading dimension n
+void launch_spmm(int64_t n, const int64_t* rowptr, const int64_t* colidx, const double* vals,
+                 const double* x, double* y, int ncols, int64_t ld, hipStream_t s);
+// out[c] = sum_r x(r, c) * y(r, c)
+void launch_coldot(int64_t n, const double* x, const double* y, int ncols, int64_t ld, double* out,
+                   hipStream_t s);
+// x(r, c) *= 1 / sqrt(max(nrm[c], tiny))
+void launch_colscale_rsqrt(int64_t n, double* x, int ncols, int64_t ld, const double* nrm,
+                           hipStream_t s);
+// dst(perm[j], c) = src(j, c)  (un-permute rows)
+void launch_permute_rows(int64_t n, const int64_t* perm, const double* src, double* dst, int ncols,
+                         int64_t ld, hipStream_t s);
+
+}  // namespace amls

@@ -1,0 +1,105 @@
+// CSR products and column reductions (HBM-bound).
+//
+// Replaces the M-normalisation SpMVs of AmlsSolver::solve
+// (src/amls/solver.cpp:307-310) and provides the SpMM used by the subspace
+// iteration and the residual checks.
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace amls {
+namespace dev {
+
+// One wave per row; lanes stride the row's nonzeros and the columns of X are
+// processed in chunks of 8 so each nonzero is read once per chunk.
+__global__ void __launch_bounds__(256) spmm_kernel(int64_t n, const int64_t* __restrict__ rowptr,
+                                                   const int64_t* __restrict__ colidx,
+                                                   const double* __restrict__ vals,
+                                                   const double* __restrict__ x,
+                                                   double* __restrict__ y, int ncols, int64_t ld) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const int64_t b = rowptr[row], e = rowptr[row + 1];
+    for (int c0 = 0; c0 < ncols; c0 += 8) {
+        double acc[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] = 0.0;
+        for (int64_t k = b + lane; k < e; k += 64) {
+            const double a = vals[k];
+            const int64_t col = colidx[k];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (c0 + q < ncols) acc[q] += a * x[col + (int64_t)(c0 + q) * ld];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double v = acc[q];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0 && c0 + q < ncols) y[row + (int64_t)(c0 + q) * ld] = v;
+        }
+    }
+}
+
+// One block per column.
+__global__ void __launch_bounds__(256) coldot_kernel(int64_t n, const double* __restrict__ x,
+                                                     const double* __restrict__ y, int64_t ld,
+                                                     double* __restrict__ out) {
+    __shared__ double red[4];
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int64_t r = threadIdx.x; r < n; r += 256) s += x[r + c * ld] * y[r + c * ld];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[c] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void colscale_kernel(int64_t n, double* __restrict__ x, int64_t ld,
+                                const double* __restrict__ nrm) {
+    const int c = blockIdx.y;
+    const double s = 1.0 / sqrt(fmax(nrm[c], 1e-300));
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (int64_t)gridDim.x * 256)
+        x[r + c * ld] *= s;
+}
+
+__global__ void permute_rows_kernel(int64_t n, const int64_t* __restrict__ perm,
+                                    const double* __restrict__ src, double* __restrict__ dst,
+                                    int64_t ld) {
+    const int c = blockIdx.y;
+    for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256)
+        dst[perm[j] + c * ld] = src[j + c * ld];
+}
+
+}  // namespace dev
+
+void launch_spmm(int64_t n, const int64_t* rowptr, const int64_t* colidx, const double* vals,
+                 const double* x, double* y, int ncols, int64_t ld, hipStream_t s) {
+    if (n <= 0 || ncols <= 0) return;
+    hipLaunchKernelGGL(dev::spmm_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, rowptr,
+                       colidx, vals, x, y, ncols, ld);
+}
+
+void launch_coldot(int64_t n, const double* x, const double* y, int ncols, int64_t ld, double* out,
+                   hipStream_t s) {
+    if (ncols <= 0) return;
+    hipLaunchKernelGGL(dev::coldot_kernel, dim3(ncols), dim3(256), 0, s, n, x, y, ld, out);
+}
+
+void launch_colscale_rsqrt(int64_t n, double* x, int ncols, int64_t ld, const double* nrm,
+                           hipStream_t s) {
+    if (ncols <= 0 || n <= 0) return;
+    unsigned bx = (unsigned)((n + 255) / 256);
+    if (bx > 64) bx = 64;
+    hipLaunchKernelGGL(dev::colscale_kernel, dim3(bx, ncols), dim3(256), 0, s, n, x, ld, nrm);
+}
+
+void launch_permute_rows(int64_t n, const int64_t* perm, const double* src, double* dst, int ncols,
+                         int64_t ld, hipStream_t s) {
+    if (ncols <= 0 || n <= 0) return;
+    unsigned bx = (unsigned)((n + 255) / 256);
+    if (bx > 64) bx = 64;
+    hipLaunchKernelGGL(dev::permute_rows_kernel, dim3(bx, ncols), dim3(256), 0, s, n, perm, src, dst,
+                       ld);
+}
+
+}  // namespace amls

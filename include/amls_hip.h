@@ -1,0 +1,223 @@
+/*
+ * amls_hip.h - C-ABI of libamls_hip.so, the MI355X (gfx950) implementation of
+ * the AMLS numeric hot path: block Gaussian elimination down a nested-
+ * dissection tree, local eigenproblems, condensation, the reduced eigen-
+ * problem, back-transformation and M-normalisation, plus an FP64 subspace-
+ * iteration Rayleigh-Ritz stage.
+ *
+ * Conventions (those of the reference's C++ interfaces):
+ *   - matrices are symmetric and given in CSC with int64 column pointers and
+ *     row indices, both triangles stored (Eigen::SparseMatrix<double,
+ *     ColMajor, int64_t>, src/core/types.hpp:11-21);
+ *   - dense blocks are column-major double;
+ *   - perm[new] = old (src/partition/partitioner.hpp:8-21);
+ *   - every entry point returns an amls_status; the text of the last failure
+ *     is available from amls_hip_last_error().  The C++ reference throws
+ *     std::invalid_argument / std::runtime_error instead
+ *     (src/amls/solver.cpp:107-111, 148-149, 254-256;
+ *     src/linalg/sparse_direct.cpp:13-22).
+ *   - a context is externally synchronised: one host thread at a time.
+ */
+#ifndef AMLS_HIP_H
+#define AMLS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum amls_status {
+    AMLS_OK = 0,
+    AMLS_E_ARG = 1,      /* invalid argument (nev <= 0, empty matrix, ...) */
+    AMLS_E_NOT_SPD = 2,  /* non-positive or non-finite pivot in a node LDL^T */
+    AMLS_E_NOCONV = 3,   /* eigen-solve could not deliver what was asked */
+    AMLS_E_HIP = 4,      /* HIP runtime error */
+    AMLS_E_OOM = 5,      /* device allocation failed */
+    AMLS_E_STATE = 6,    /* call out of order, or no device in this context */
+    AMLS_E_INTERNAL = 7
+} amls_status;
+
+typedef struct amls_ctx amls_ctx;
+
+/*
+ * Options of AmlsSolver::solve.  The first eleven fields are AmlsOptions of
+ * src/amls/solver.hpp:18-30 with the same meaning and defaults; the rest
+ * select the GPU forms.
+ */
+typedef struct amls_options {
+    int32_t nLevels;            /* 6 */
+    int64_t leafSize;           /* 500 */
+    double thresholdFactor;     /* 2.0  (eta, lambda_cut = eta * lambda_hat) */
+    int64_t fixedKeep;          /* 0: two-pass threshold truncation */
+    int64_t nev;                /* 100 */
+    int64_t denseProjectedMax;  /* 5000 (diagnostic only, as in the reference) */
+    double shift;               /* 0.0: K <- K + shift*M */
+    int32_t twoPass;            /* 1 */
+    int32_t numThreads;         /* 0 (host threads; the GPU path ignores it) */
+    int32_t recoverVectors;     /* 1 */
+    int32_t verbosity;          /* 1 */
+    int32_t precision;          /* 0: all FP64; 1: FP32 MFMA for the interface
+                                   Schur, M-congruence and projection GEMMs */
+    int32_t siSteps;            /* subspace-iteration Rayleigh-Ritz steps (0 = plain AMLS) */
+    int64_t siVectors;          /* subspace width q (0: max(2*nev, nev+8), capped at p) */
+} amls_options;
+
+/* Fills *opt with the defaults above. */
+amls_status amls_options_default(amls_options* opt);
+
+/* device < 0 creates a host-only context: partition and symbolic analysis
+ * work, every GPU phase returns AMLS_E_STATE. */
+amls_status amls_hip_create(int32_t device, uint32_t flags, amls_ctx** out);
+amls_status amls_hip_destroy(amls_ctx* ctx);
+const char* amls_hip_last_error(const amls_ctx* ctx);
+
+/*
+ * Analysis + upload.  Partitions |K|+|M| (nested dissection, nLevels,
+ * leafSize; coords = optional n x 3 row-major dof coordinates that enable
+ * plane separators), permutes, runs the symbolic analysis and copies the
+ * values to the device.  Replaces step 1 of AmlsSolver::solve
+ * (src/amls/solver.cpp:124-134: MetisPartitioner::recursiveBisection +
+ * permuteSymmetric) and GaussianElimination::precompute_/symbolicFill_
+ * (src/amls/gaussian_elimination.cpp:183-270).  shift: K <- K + shift*M
+ * (src/amls/solver.cpp:114-122).
+ */
+amls_status amls_hip_set_problem(amls_ctx* ctx, int64_t n, const int64_t* Kcolptr,
+                                 const int64_t* Krowidx, const double* Kval,
+                                 const int64_t* Mcolptr, const int64_t* Mrowidx,
+                                 const double* Mval, const double* coords, int32_t nLevels,
+                                 int64_t leafSize, double shift);
+
+/* Tree in post-order positions p = 0..nNodes-1 (the elimination order of
+ * SeparatorTree::postOrder, src/amls/separator_tree.cpp:33-38). */
+amls_status amls_hip_tree_info(const amls_ctx* ctx, int64_t* nNodes, int64_t* maxLevel,
+                               int64_t* n);
+/* Any output pointer may be null.  treeId = preorder id of the node at p,
+ * offset/size = its range in the permuted ordering, parent = parent position
+ * (-1 at the root), level = 0 for leaves, boundary sizes b_p, perm[new] = old. */
+amls_status amls_hip_tree_get(const amls_ctx* ctx, int64_t* treeId, int64_t* offset,
+                              int64_t* size, int64_t* parent, int64_t* level,
+                              int64_t* bsize, int64_t* perm);
+
+/*
+ * Block Gaussian elimination of every tree level (leaves first): per node a
+ * partial LDL^T of its front, X = Khat^-1 K_iB, the K Schur complement and
+ * the M congruence update, passed to the parent by extend-add.  Replaces
+ * GaussianElimination::run/runLevel/runDag (src/amls/gaussian_elimination.cpp
+ * :458-866).  precision: 0 = FP64, 1 = mixed.
+ */
+amls_status amls_hip_ge(amls_ctx* ctx, int32_t precision);
+
+/* Standard-form local pencils C_i = (L sqrt(D))^-1 Mhat (L sqrt(D))^-T and
+ * their tridiagonal reduction, for all nodes (after amls_hip_ge). */
+amls_status amls_hip_spectral_prepare(amls_ctx* ctx);
+
+/*
+ * Local eigenpairs and truncation for every node: kind 0 = FixedCount(value)
+ * (value <= 0 keeps all), kind 1 = Threshold(lambda_cut = value).  k_out
+ * (nNodes, post-order) receives k_i.  Replaces
+ * SubspaceCondensation::spectralLevel + EigenpairSolver + the truncation
+ * policies (src/amls/subspace_condensation.cpp:45-100,
+ * src/amls/eigenpair_solver.hpp:32-63, src/amls/spectral_truncation.hpp:35-88).
+ */
+amls_status amls_hip_spectral(amls_ctx* ctx, int32_t kind, double value, int64_t* k_out);
+
+/* Projection of M onto the retained modes, all levels; returns p.
+ * Replaces SubspaceCondensation::projectLevel/projectNode_/finalize
+ * (src/amls/subspace_condensation.cpp:102-261). */
+amls_status amls_hip_project(amls_ctx* ctx, int32_t precision, int64_t* p_out);
+
+/* Lowest `want` eigenpairs of (diag mu, Mtilde); theta (want) to the host.
+ * vectors = 0 computes eigenvalues only.  Replaces solveProjected
+ * (src/amls/solver.cpp:145-161). */
+amls_status amls_hip_reduced_solve(amls_ctx* ctx, int64_t want, int32_t vectors,
+                                   double* theta);
+
+/* z = L^-T S q, un-permute, M-normalise; Z (n x want, original ordering,
+ * column-major) to the host if non-null.  Replaces
+ * SubspaceCondensation::backtransform and the normalisation loop
+ * (src/amls/subspace_condensation.cpp:263-319, src/amls/solver.cpp:302-311). */
+amls_status amls_hip_backtransform(amls_ctx* ctx, double* Z);
+
+/*
+ * Subspace iteration with the AMLS block factors as the inverse
+ * (Q <- K^-1 M Q through L, Khat, L^T), FP64 B-orthogonal Rayleigh-Ritz with
+ * the original K and M after every step.  q columns start from the lowest q
+ * Ritz vectors of the last reduced solve.  theta (nev) and Z (n x nev) to the
+ * host if non-null.  No counterpart in the reference's sources (README.md:14
+ * claims it); algorithm AMLS-SIMb of papers/Improving_eigenpairs_of_automated_
+ * multilevel_substructuring_with_subspace_iterations.md:384-427.
+ */
+amls_status amls_hip_subspace_iter(amls_ctx* ctx, int32_t steps, int64_t q, int64_t nev,
+                                   double* theta, double* Z);
+
+/*
+ * The whole pipeline of AmlsSolver::solve (src/amls/solver.cpp:100-319),
+ * including partition (coords may be null), two-pass truncation, the reduced
+ * solve, vector recovery and the optional subspace iteration.
+ * theta: nev (ascending); Z: n x nev or null; p_out: reduced size;
+ * timing: 8 doubles {partition, gauss, condense, reduced, backtransform,
+ * subspace, total, p0}.
+ */
+amls_status amls_hip_solve(amls_ctx* ctx, int64_t n, const int64_t* Kcolptr,
+                           const int64_t* Krowidx, const double* Kval, const int64_t* Mcolptr,
+                           const int64_t* Mrowidx, const double* Mval, const double* coords,
+                           const amls_options* opt, double* theta, double* Z, int64_t* p_out,
+                           double* timing);
+
+/*
+ * The numeric part of amls_hip_solve on the problem of the last
+ * amls_hip_set_problem (values already in device memory): everything after
+ * the partition.  theta / Z / p_out / timing as in amls_hip_solve (timing[0]
+ * = 0); Z may be null to keep the vectors in device memory only.  Lets a
+ * caller time the solve with its inputs resident in HBM, or re-solve after
+ * changing options.
+ */
+amls_status amls_hip_solve_numeric(amls_ctx* ctx, const amls_options* opt, double* theta,
+                                   double* Z, int64_t* p_out, double* timing);
+
+/* ---- measurement -------------------------------------------------------
+ * enable != 0 brackets every launch on the context's stream with HIP events
+ * (grouped by kernel class) and clears the counters; enable = 0 stops the
+ * event timing and clears the counters.  Launch counts and the algorithmic
+ * flops / bytes of each class (from the launch descriptors: 2mnk and
+ * (mk + kn + 2mn) words per GEMM, n(n-1) r for a unit triangular solve, two
+ * passes over the trailing block per tridiagonalisation step, 2 nnz r for a
+ * CSR product) are always counted.  No counterpart in the reference. */
+amls_status amls_hip_profile(amls_ctx* ctx, int32_t enable);
+/* Class `index` (0-based, sorted by name); AMLS_E_ARG past the last one. */
+amls_status amls_hip_kernel_stats(amls_ctx* ctx, int32_t index, char* name, int32_t name_len,
+                                  int64_t* launches, double* ms, double* flops, double* bytes);
+
+/* ---- inspection (tests) ---------------------------------------------- */
+/* Node p after GE: the factored K block (L strict lower, D diagonal, n_p x n_p)
+ * and the final Mhat_pp (lower triangle valid). */
+amls_status amls_hip_get_node_factor(amls_ctx* ctx, int64_t p, double* LD, double* Mhat);
+/* Node p after amls_hip_spectral: mu (k_p) and Z (n_p x k_p). */
+amls_status amls_hip_get_local(amls_ctx* ctx, int64_t p, double* mu, double* Z);
+/* After amls_hip_project: Mtilde (p x p, lower triangle valid, modes in
+ * post-order node order) and the mode offset of each node (nNodes). */
+amls_status amls_hip_get_reduced(amls_ctx* ctx, double* Mtilde, int64_t* moff);
+
+/* ---- operator-granular entry points (host buffers, one matrix) ------- */
+/* C = alpha op(A) op(B) + beta C (column-major; trans 0/1; precision 0/1).
+ * The GEMM of OpenBlasColCompOps::dgemm (src/linalg/colcomp.hpp:355-364). */
+amls_status amls_hip_gemm(amls_ctx* ctx, int32_t precision, int32_t transA, int32_t transB,
+                          int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                          int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                          int64_t ldc);
+/* A = L D L^T (A n x n SPD, lower used) and X = A^-1 B (n x nrhs).  LD
+ * receives L (strict lower) and D (diagonal).  DenseLDLTSolver::factorize +
+ * solveWithFactor (src/linalg/sparse_direct.cpp:38-63). */
+amls_status amls_hip_ldlt_solve(amls_ctx* ctx, int64_t n, const double* A, int64_t nrhs,
+                                const double* B, double* LD, double* X);
+/* Largest k eigenpairs of a symmetric A (n x n, lower used): lam descending,
+ * V n x k orthonormal. */
+amls_status amls_hip_syevx_top(amls_ctx* ctx, int64_t n, const double* A, int64_t k,
+                               double* lam, double* V);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMLS_HIP_H */

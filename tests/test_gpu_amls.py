@@ -1,0 +1,186 @@
+"""Phase-by-phase and end-to-end parity of the GPU AMLS path with the CPU
+oracle on the same separator tree (libamls_hip's nested dissection)."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from amls_amd import AmlsOptions, AmlsSolver, HipContext, Pencil, cases
+from oracle import amls_oracle as O
+from tests.helpers import grid_coords, tree_of
+
+pytestmark = pytest.mark.gpu
+
+
+def small_hex(seed=11, dims=(4, 4, 5), order=1):
+    rng = np.random.default_rng(seed)
+    ne = dims[0] * dims[1] * dims[2]
+    return cases.hex_elasticity(order, *dims, kscale=rng.lognormal(0, 0.3, ne),
+                                rscale=rng.uniform(0.5, 2.0, ne))
+
+
+def gpu_phases(K, M, coords, nLevels, leafSize, kind, value, want, prec="fp64"):
+    ctx = HipContext(0)
+    ctx.set_problem(K, M, coords, nLevels, leafSize)
+    ctx.ge(prec)
+    ctx.spectral_prepare()
+    k = ctx.spectral(kind, value)
+    p = ctx.project(prec)
+    th = ctx.reduced_solve(want, True)
+    Z = ctx.backtransform(min(want, p))
+    return ctx, k, p, th, Z
+
+
+def test_full_keep_exactness_grid(blas1):
+    """tests/unit/test_amls.cpp:110-150: keeping every local mode reproduces
+    the closed-form spectrum of a 10x10 grid, back-transform residual < 1e-10."""
+    K, M = cases.grid_graph_2d(10, 10)
+    ctx, k, p, th, Z = gpu_phases(K, M, grid_coords(10, 10), 2, 20, "fixed", 0, 10)
+    assert p == 100
+    ref = cases.grid_graph_2d_spectrum(10, 10)[:10]
+    assert np.all(np.abs(th - ref) <= 1e-10 * np.maximum(1.0, np.abs(ref)))
+    Kd = K.toarray()
+    for j in range(3):
+        z = Z[:, j]
+        r = Kd @ z - th[j] * (M @ z)
+        assert np.linalg.norm(r) < 1e-10 * np.linalg.norm(Kd @ z)
+
+
+def test_ge_blocks_match_oracle(blas1):
+    pb = small_hex()
+    tree, t = tree_of(pb.K, pb.M, pb.coords, 3, 40)
+    ctx = HipContext(0)
+    ctx.set_problem(pb.K, pb.M, pb.coords, 3, 40)
+    ctx.ge("fp64")
+    perm = tree.perm
+    ge = O.BlockGE(pb.K.tocsr()[perm][:, perm], pb.M.tocsr()[perm][:, perm], tree).run()
+    for p in range(tree.N):
+        n_p = int(tree.size[p])
+        LD, Mh = ctx.node_factor(p, n_p)
+        L = np.tril(LD, -1) + np.eye(n_p)
+        Kh = L @ np.diag(np.diag(LD)) @ L.T
+        Mh = np.tril(Mh) + np.tril(Mh, -1).T
+        sk = np.abs(ge.Khat[p]).max()
+        sm = np.abs(ge.Mhat[p]).max()
+        assert np.abs(Kh - ge.Khat[p]).max() <= 1e-11 * sk, p
+        assert np.abs(Mh - ge.Mhat[p]).max() <= 1e-11 * sm, p
+
+
+def test_local_spectra_match_oracle(blas1):
+    pb = small_hex(seed=12)
+    tree, t = tree_of(pb.K, pb.M, pb.coords, 3, 40)
+    ctx = HipContext(0)
+    ctx.set_problem(pb.K, pb.M, pb.coords, 3, 40)
+    ctx.ge("fp64")
+    ctx.spectral_prepare()
+    kfix = 7
+    k = ctx.spectral("fixed", kfix)
+    perm = tree.perm
+    ge = O.BlockGE(pb.K.tocsr()[perm][:, perm], pb.M.tocsr()[perm][:, perm], tree).run()
+    for p in range(tree.N):
+        n_p = int(tree.size[p])
+        assert k[p] == min(kfix, n_p)
+        mu, Z = ctx.local(p, n_p, int(k[p]))
+        w, V = sla.eigh(ge.Khat[p], ge.Mhat[p], subset_by_index=[0, k[p] - 1])
+        assert np.abs(mu - w).max() <= 1e-10 * w.max(), p
+        # M-orthonormal, eigen-residual, and the same subspace as LAPACK's
+        Mh, Kh = ge.Mhat[p], ge.Khat[p]
+        assert np.abs(Z.T @ Mh @ Z - np.eye(k[p])).max() <= 1e-9
+        assert np.linalg.norm(Kh @ Z - Mh @ Z * mu) <= 1e-9 * np.linalg.norm(Kh @ Z)
+        C = np.abs(Z.T @ Mh @ V)
+        assert np.abs(np.linalg.svd(C, compute_uv=False) - 1).max() <= 1e-8
+
+
+def test_threshold_counts_and_reduced(blas1):
+    pb = small_hex(seed=13)
+    tree, t = tree_of(pb.K, pb.M, pb.coords, 3, 40)
+    perm = tree.perm
+    ge = O.BlockGE(pb.K.tocsr()[perm][:, perm], pb.M.tocsr()[perm][:, perm], tree).run()
+    cut = 0.6
+    cond = O.Condensation(ge).run("threshold", cut)
+    ctx, k, p, th, Z = gpu_phases(pb.K, pb.M, pb.coords, 3, 40, "threshold", cut, 12)
+    assert np.array_equal(k, cond.k)
+    assert p == cond.p
+    ref = cond.solve_projected(12, vectors=False)
+    assert np.abs(th - ref).max() <= 1e-10 * ref.max()
+    # M-tilde blocks: same up to the sign of each local mode
+    Mt, moff = ctx.reduced(p, tree.N)
+    Mt = np.tril(Mt) + np.tril(Mt, -1).T
+    Kp, Mp = cond.assemble()
+    # map post-order mode layout -> oracle (node-id) layout, aligning signs
+    idx = np.zeros(p, dtype=np.int64)
+    sgn = np.ones(p)
+    for q in range(tree.N):
+        kq = int(k[q])
+        if kq == 0:
+            continue
+        _, Zg = ctx.local(q, int(tree.size[q]), kq)
+        s = np.sign(np.einsum("ij,ij->j", Zg, ge.Mhat[q] @ cond.Z[q]))
+        idx[moff[q]:moff[q] + kq] = cond.projOffset[q] + np.arange(kq)
+        sgn[moff[q]:moff[q] + kq] = s
+    Mo = Mp[np.ix_(idx, idx)] * np.outer(sgn, sgn)
+    assert np.abs(Mt - Mo).max() <= 1e-10
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_solve_matches_oracle_hex8(blas1, prec):
+    pb = small_hex(seed=21, dims=(5, 5, 6))
+    tree, t = tree_of(pb.K, pb.M, pb.coords, 3, 60)
+    nev = 12
+    ref = O.amls_solve(pb.K, pb.M, tree, nev=nev)
+    sol = AmlsSolver(0).solve(Pencil(pb.K, pb.M, pb.coords),
+                              AmlsOptions(nev=nev, nLevels=3, leafSize=60, precision=prec,
+                                          verbosity=0))
+    assert sol.p == ref.p
+    rel = np.abs(sol.theta - ref.theta) / ref.theta
+    # FP64: same arithmetic up to rounding.  Mixed: FP32 Schur / congruence /
+    # projection GEMMs perturb the reduced pencil at ~1e-7 relative.
+    assert rel.max() <= (1e-10 if prec == "fp64" else 5e-5)
+    Mz = pb.M @ sol.Z
+    assert np.abs(np.einsum("ij,ij->j", sol.Z, Mz) - 1).max() <= 1e-10
+    if prec == "fp64":
+        # same vectors up to sign
+        s = np.sign(np.einsum("ij,ij->j", sol.Z, pb.M @ ref.Z))
+        assert np.abs(sol.Z * s - ref.Z).max() <= 1e-7 * np.abs(ref.Z).max()
+
+
+def test_solve_hex8_10_matches_oracle(blas1):
+    """The 8-node 10x10x10 workload (n = 3,630, 3 levels, nev 20)."""
+    pb = cases.case_hex8_10()
+    tree, t = tree_of(pb.K, pb.M, pb.coords, 3, 500)
+    ref = O.amls_solve(pb.K, pb.M, tree, nev=20)
+    sol = AmlsSolver(0).solve(Pencil(pb.K, pb.M, pb.coords),
+                              AmlsOptions(nev=20, nLevels=3, verbosity=0))
+    assert sol.p == ref.p
+    assert np.abs(sol.theta - ref.theta).max() <= 1e-10 * ref.theta.max()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_subspace_iteration_matches_oracle(blas1, prec):
+    """AMLS-SIMb with a Rayleigh-Ritz step per iteration: same iteration as
+    the oracle's restatement, and the Ritz values approach the FP64 ground
+    truth from above (Courant-Fischer)."""
+    pb = small_hex(seed=31, dims=(5, 5, 6))
+    tree, t = tree_of(pb.K, pb.M, pb.coords, 3, 60)
+    nev, q, steps = 12, 30, 2
+    ref = O.amls_solve(pb.K, pb.M, tree, nev=nev)
+    _, Q = ref.cond.solve_projected(q)
+    z = ref.cond.backtransform(Q)
+    X = np.zeros_like(z)
+    X[tree.perm] = z
+    t_ref, _ = O.subspace_iteration(pb.K, pb.M, ref.ge, X, steps, nev, tree.perm)
+    sol = AmlsSolver(0).solve(Pencil(pb.K, pb.M, pb.coords),
+                              AmlsOptions(nev=nev, nLevels=3, leafSize=60, precision=prec,
+                                          siSteps=steps, siVectors=q, verbosity=0))
+    exact = O.reference_eigs(pb.K, pb.M, nev)
+    assert sol.theta.shape == (nev,)
+    assert np.all(sol.theta >= exact * (1 - 1e-12))
+    rel = np.abs(sol.theta - t_ref) / t_ref
+    assert rel.max() <= (1e-9 if prec == "fp64" else 1e-6)
+    assert O.rel_err(sol.theta, exact).max() < O.rel_err(ref.theta, exact).max()
+    assert np.abs(np.einsum("ij,ij->j", sol.Z, pb.M @ sol.Z) - 1).max() <= 1e-10
+
+    def resid(th, Z):
+        Kz = pb.K @ Z
+        return np.linalg.norm(Kz - (pb.M @ Z) * th, axis=0) / np.linalg.norm(Kz, axis=0)
+    t_or, Z_or = O.subspace_iteration(pb.K, pb.M, ref.ge, X, steps, nev, tree.perm)
+    assert np.all(resid(sol.theta, sol.Z) <= resid(t_or, Z_or) * (1 + 1e-3) + 1e-12)
