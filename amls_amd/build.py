@@ -29,6 +29,8 @@ HIP_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
+    # hardware FP64 atomics on coarse-grained (hipMalloc) memory
+    "-munsafe-fp-atomics",
     # no packed FP32 instructions from the compiler
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]

@@ -227,13 +227,15 @@ struct Plan {
 };
 
 // Tridiagonalisation schedule for a batch of matrices: launch i processes
-// every matrix with n_m > i; slices of 64 rows below the diagonal.
+// every matrix with n_m > i, one work item per 64 x 64 tile of the trailing
+// lower triangle [i+1, n_m)^2 (a single finishing item at i = n_m - 1).
 struct SytrdPlan {
-    std::vector<int2> work;
+    std::vector<int4> work;
     std::vector<size_t> start;   // per step
     std::vector<int> count;
-    int steps = 0, nmax = 0;
+    int steps = 0;
     void build(const std::vector<int>& ns) {
+        int nmax = 0;
         for (int v : ns) nmax = std::max(nmax, v);
         steps = nmax;
         start.assign(steps, 0);
@@ -243,39 +245,51 @@ struct SytrdPlan {
             for (int m = 0; m < static_cast<int>(ns.size()); ++m) {
                 const int n = ns[m];
                 if (n <= i) continue;
-                const int rows = n - i - 1;
-                const int sl = std::max(1, cdiv(rows, 64));
-                for (int y = 0; y < sl; ++y) work.push_back(make_int2(m, y));
+                if (n - i - 1 <= 0) {
+                    work.push_back(make_int4(m, -1, -1, 1));
+                    continue;
+                }
+                const int t0 = (i + 1) / 64, t1 = (n - 1) / 64;
+                for (int R = t0; R <= t1; ++R)
+                    for (int C = t0; C <= R; ++C)
+                        work.push_back(make_int4(m, R, C, (R == t0 && C == t0) ? 1 : 0));
             }
             count[i] = static_cast<int>(work.size() - start[i]);
         }
     }
 };
 
+// soff[m]: per-matrix offset in units of its n (sv holds 2 n and sp 3 n
+// doubles per matrix from 2 soff / 3 soff on)
 void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<int64_t>& soff,
               double* sv, double* sp) {
     std::vector<int> ns;
-    for (const auto& d : descs) ns.push_back(d.n);
+    int64_t spLen = 0;
+    for (size_t m = 0; m < descs.size(); ++m) {
+        ns.push_back(descs[m].n);
+        spLen = std::max<int64_t>(spLen, 3 * (soff[m] + descs[m].n));
+    }
     SytrdPlan sp_;
     sp_.build(ns);
     if (sp_.steps == 0) return;
     const size_t od = plan.blob->add(descs), ow = plan.blob->add(sp_.work),
                  oo = plan.blob->add(soff);
     Blob* bl = plan.blob;
-    const int nmax = sp_.nmax;
     const auto start = sp_.start;
     const auto count = sp_.count;
-    // per step: each active matrix reads and writes its trailing block once
+    // algorithmic traffic per step: each active matrix reads and writes the
+    // lower triangle of its trailing block once
     std::vector<double> stepBytes(start.size(), 0.0);
     for (int i = 0; i < static_cast<int>(start.size()); ++i)
         for (int v : ns)
-            if (v > i) stepBytes[i] += 2.0 * 8.0 * double(v - i - 1) * double(v - i - 1);
+            if (v > i) stepBytes[i] += 8.0 * double(v - i - 1) * double(v - i);
     KernelStats* kst = plan.st;
     plan.fn([=](hipStream_t s) {
+        launch_fill(sp, spLen, 0.0, s);
         for (int i = 0; i < static_cast<int>(start.size()); ++i) {
             auto go = [&] {
-                launch_sytrd_step(bl->at<EigDesc>(od), bl->at<int2>(ow) + start[i], count[i], i,
-                                  sv, sp, bl->at<int64_t>(oo), nmax, s);
+                launch_sytrd_step(bl->at<EigDesc>(od), bl->at<int4>(ow) + start[i], count[i], i,
+                                  sv, sp, bl->at<int64_t>(oo), s);
             };
             if (kst) kst->run("sytrd_step", 2.0 * stepBytes[i] / 8.0, stepBytes[i], s, go);
             else go();
@@ -283,25 +297,65 @@ void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<i
     });
 }
 
+// Y = A X for column-major X, Y (n x ncols, ld n): transpose X into the
+// row-major scratch t1, CSR product into t2, transpose back into Y.
+void addSpmm(Plan& plan, int64_t n, const int64_t* rp, const int64_t* ci, const double* vx,
+             double nnz, const double* X, double* Y, int64_t ncols, double* t1, double* t2) {
+    Copy2dBatch to, back;
+    to.add(cdesc(X, n, t1, ncols, ncols, n, 1));     // t1 (ncols x n col-major) = X^T
+    back.add(cdesc(t2, ncols, Y, n, n, ncols, 1));   // Y = t2^T
+    plan.copy(to);
+    const int nc = static_cast<int>(ncols);
+    plan.timed("spmm", 2.0 * nnz * ncols, nnz * 16.0 + (n + 1) * 8.0 + 2.0 * n * ncols * 8.0,
+               [=](hipStream_t s) { launch_spmm_rm(n, rp, ci, vx, t1, t2, nc, s); });
+    plan.copy(back);
+}
+
 // Bisection, inverse iteration and back-transformation for descriptors with
-// lam/Y/work/k set.  woffs: per (matrix, j) scratch offsets (5 n each).
+// lam/Y/work/k set (work: 6 n k doubles per matrix, see stein_kernel).
 void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
-    std::vector<int2> work;
+    std::vector<int2> work, swork;
+    std::vector<int4> owork;
     std::vector<int64_t> woff;
-    for (int m = 0; m < static_cast<int>(descs.size()); ++m)
+    int lds = 0, nmax = 1;
+    for (int m = 0; m < static_cast<int>(descs.size()); ++m) {
         for (int j = 0; j < descs[m].k; ++j) {
             work.push_back(make_int2(m, j));
-            woff.push_back(static_cast<int64_t>(j) * 5 * descs[m].n);
+            swork.push_back(make_int2(m, j));
+            woff.push_back(0);
         }
+        if (descs[m].k > 0) {
+            nmax = std::max(nmax, descs[m].n);
+            while (swork.size() % 64) swork.push_back(make_int2(m, -1));   // pad to the block
+        }
+        // LDS-resident column blocks of at most 8 columns and 64 KiB
+        const int n = descs[m].n;
+        const int per = std::max(1, std::min(8, 8000 / std::max(n, 1)));
+        for (int c = 0; c < descs[m].k; c += per) {
+            const int nc = std::min(per, descs[m].k - c);
+            owork.push_back(make_int4(m, c, nc, 0));
+            lds = std::max(lds, n * nc);
+        }
+    }
     if (work.empty()) return;
-    const size_t od = plan.blob->add(descs), ow = plan.blob->add(work), oo = plan.blob->add(woff);
+    woff.resize(std::max(woff.size(), swork.size()), 0);
+    const size_t od = plan.blob->add(descs), ow = plan.blob->add(work), oo = plan.blob->add(woff),
+                 oq = plan.blob->add(owork), os = plan.blob->add(swork);
     Blob* bl = plan.blob;
-    const int nw = static_cast<int>(work.size());
+    const int nw = static_cast<int>(work.size()), nq = static_cast<int>(owork.size());
+    const int ns = static_cast<int>(swork.size());
+    KernelStats* kst = plan.st;
     plan.fn([=](hipStream_t s) {
-        launch_bisect(bl->at<EigDesc>(od), bl->at<int2>(ow), nw, s);
+        auto run = [&](const char* name, const std::function<void()>& f) {
+            if (kst) kst->run(name, 0.0, 0.0, s, f);
+            else f();
+        };
+        run("bisect", [&] { launch_bisect(bl->at<EigDesc>(od), bl->at<int2>(ow), nw, s); });
         if (vectors) {
-            launch_stein(bl->at<EigDesc>(od), bl->at<int2>(ow), nw, bl->at<int64_t>(oo), s);
-            launch_ormtr(bl->at<EigDesc>(od), bl->at<int2>(ow), nw, s);
+            run("stein", [&] {
+                launch_stein(bl->at<EigDesc>(od), bl->at<int2>(os), ns, bl->at<int64_t>(oo), nmax, s);
+            });
+            run("ormtr", [&] { launch_ormtr(bl->at<EigDesc>(od), bl->at<int4>(oq), nq, lds, s); });
         }
     });
 }
@@ -452,6 +506,12 @@ void Ctx::setProblem(int64_t nn, const int64_t* kp, const int64_t* ki, const dou
     ftot = fo[N];
     xtot = xo[N];
     wtot = wo[N];
+    co.assign(N + 1, 0);
+    vo.assign(N + 1, 0);
+    for (int64_t p = 0; p < N; ++p) {
+        co[p + 1] = co[p] + S.size[p] * S.size[p];
+        vo[p + 1] = vo[p] + S.size[p];
+    }
     haveProblem = true;
     if (!gpu) return;
 
@@ -583,6 +643,8 @@ void Ctx::runGE(int prec) {
     A.plan(xtot * 8);
     A.plan(xtot * 8);
     A.plan(wtot * 8);
+    A.plan(co[N] * 8);
+    A.plan(vo[N] * 8 * 2);
     A.plan(256);
     A.commit();
     dF = A.take<double>(ftot);
@@ -590,11 +652,15 @@ void Ctx::runGE(int prec) {
     dXb = A.take<double>(xtot);
     dFm = A.take<double>(xtot);
     dW = A.take<double>(wtot);
+    dLinv = A.take<double>(co[N]);
+    dDsq = A.take<double>(vo[N]);
+    dDinv = A.take<double>(vo[N]);
     dErr = A.take<int>(1);
     const int big = INT_MAX;
     AMLS_HIP_CHECK(hipMemcpyAsync(dErr, &big, sizeof(int), hipMemcpyHostToDevice, stream));
     launch_fill(dF, ftot, 0.0, stream);
     launch_fill(dG, ftot, 0.0, stream);
+    launch_fill(dLinv, co[N], 0.0, stream);
     launch_scatter(dsrcK, dtgtK, dKx, dF, nasmK, 1.0, false, stream);
     launch_scatter(dsrcM, dtgtM, dMx, dG, nasmM, 1.0, false, stream);
     if (shift != 0.0) launch_scatter(dsrcM, dtgtM, dMx, dF, nasmM, shift, true, stream);
@@ -715,21 +781,61 @@ void Ctx::runGE(int prec) {
             plan.gemm(g64, false);
             plan.gemm(gp, true);
         }
-        // X = Khat^-1 K_iB, Fm = M_iB - Mhat X, and the M congruence update
+        // L^-1 of every node (forward solve on the identity), D^-1/2, D^-1;
+        // X = Khat^-1 K_iB = L^-T L21^T, Fm = M_iB - Mhat X, M congruence
+        std::vector<VecDesc> vdiag;
+        TrsmBatch tinv;
+        GemmBatch gX;
+        for (int64_t p : nodes) {
+            const int64_t np = S.size[p], f = S.f(p), b = S.b(p);
+            double* F = dF + fo[p];
+            double* Li = dLinv + co[p];
+            VecDesc v;
+            std::memset(&v, 0, sizeof(v));
+            v.n = static_cast<int32_t>(np);
+            v.dst = Li;
+            v.ld = np;
+            v.op = 2;
+            vdiag.push_back(v);
+            v.src = F;
+            v.ld = f;
+            v.dst = dDsq + vo[p];
+            v.op = 0;
+            vdiag.push_back(v);
+            v.dst = dDinv + vo[p];
+            v.op = 5;
+            vdiag.push_back(v);
+            tinv.add(F, f, Li, np, np, np, 0);
+            tinv.descs.back().rhsLower = 1;
+            if (b > 0) {
+                GemmDesc g = gdesc();
+                g.A = Li;
+                g.lda = np;
+                g.ta = 1;
+                g.B = F + np;
+                g.ldb = f;
+                g.tb = 1;
+                g.k = static_cast<int32_t>(np);
+                g.C = dXb + xo[p];
+                g.ldc = np;
+                g.m = static_cast<int32_t>(np);
+                g.n = static_cast<int32_t>(b);
+                gX.add(g);
+            }
+        }
+        plan.vec(vdiag);
+        plan.trsm(tinv);
+        plan.gemm(gX, false);
         Copy2dBatch cb;
-        TrsmBatch tb;
         GemmBatch gFm, gG22;
         for (int64_t p : nodes) {
             const int64_t np = S.size[p], f = S.f(p), b = S.b(p);
             if (b == 0) continue;
-            double* F = dF + fo[p];
             double* G = dG + fo[p];
             double* X = dXb + xo[p];
             double* Fm = dFm + xo[p];
-            cb.add(cdesc(F + np, f, X, np, np, b, 1));       // X = L21^T
             cb.add(cdesc(G + np, f, Fm, np, np, b, 1));      // Fm = M_Bi^T
             cb.add(cdesc(G, f, G, f, np, np, 2));            // Mhat full in place
-            tb.add(F, f, X, np, np, b, 1);                   // X = L^-T L21^T
             GemmDesc d = gdesc();                            // Fm -= Mhat X
             d.A = G;
             d.lda = f;
@@ -765,7 +871,6 @@ void Ctx::runGE(int prec) {
             gG22.add(e);
         }
         plan.copy(cb);
-        plan.trsm(tb);
         plan.gemm(gFm, f32);
         plan.gemm(gG22, f32);
     }
@@ -790,54 +895,56 @@ void Ctx::spectralPrepare() {
     if (!geDone) throw StatusError(AMLS_E_STATE, "spectral_prepare before ge");
     specReady = selected = projected = reduced = recovered = false;
     const int64_t N = S.nNodes;
-    co.assign(N + 1, 0);
-    vo.assign(N + 1, 0);
-    for (int64_t p = 0; p < N; ++p) {
-        co[p + 1] = co[p] + S.size[p] * S.size[p];
-        vo[p + 1] = vo[p] + S.size[p];
-    }
     spMem.reset(new DevArena());
     auto& A = *spMem;
     A.plan(co[N] * 8 * 2);
-    A.plan(vo[N] * 8 * 4);
-    A.plan(vo[N] * 8 * 4);
-    A.plan((N + 1) * 8);
+    A.plan(vo[N] * 8 * 3);
+    A.plan(vo[N] * 8 * 5);
     A.commit();
     dC = A.take<double>(co[N]);
     dT = A.take<double>(co[N]);
     dD = A.take<double>(vo[N]);
     dE = A.take<double>(vo[N]);
     dTau = A.take<double>(vo[N]);
-    dDsq = A.take<double>(vo[N]);
     dSv = A.take<double>(2 * vo[N]);
-    dSp = A.take<double>(2 * vo[N]);
+    dSp = A.take<double>(3 * vo[N]);
     spBlob.reset(new Blob());
     Plan plan(spBlob.get(), &kstats);
-    Copy2dBatch c1, c2, c3, c4;
-    TrsmBatch t1, t2;
-    std::vector<VecDesc> vd;
+    // C_i = D^-1/2 L^-1 Mhat L^-T D^-1/2 with the explicit L^-1 (two GEMMs)
+    Copy2dBatch c1, c3, c4;
+    GemmBatch g1, g2;
     std::vector<EigDesc> ed;
     std::vector<int64_t> soff;
     for (int64_t p = 0; p < N; ++p) {
         const int64_t np = S.size[p], f = S.f(p);
-        double* F = dF + fo[p];
         double* G = dG + fo[p];
         double* C = dC + co[p];
         double* T = dT + co[p];
+        double* Li = dLinv + co[p];
         c1.add(cdesc(G, f, C, np, np, np, 2));                 // C = Mhat
-        t1.add(F, f, C, np, np, np, 0);                        // C = L^-1 Mhat
-        c2.add(cdesc(C, np, T, np, np, np, 1));                // T = C^T
-        t2.add(F, f, T, np, np, np, 0);                        // T = L^-1 Mhat L^-T
-        VecDesc v;
-        std::memset(&v, 0, sizeof(v));
-        v.src = F;
-        v.dst = dDsq + vo[p];
-        v.ld = f;
-        v.n = static_cast<int32_t>(np);
-        v.op = 0;
-        vd.push_back(v);
-        c3.add(cdesc(T, np, T, np, np, np, 0, dDsq + vo[p], dDsq + vo[p]));
-        c4.add(cdesc(T, np, C, np, np, np, 2));                // exact symmetry
+        GemmDesc a = gdesc();                                  // T = L^-1 Mhat
+        a.A = Li;
+        a.lda = np;
+        a.B = C;
+        a.ldb = np;
+        a.k = static_cast<int32_t>(np);
+        a.C = T;
+        a.ldc = np;
+        a.m = a.n = static_cast<int32_t>(np);
+        g1.add(a);
+        GemmDesc b = gdesc();                                  // C = T L^-T
+        b.A = T;
+        b.lda = np;
+        b.B = Li;
+        b.ldb = np;
+        b.tb = 1;
+        b.k = static_cast<int32_t>(np);
+        b.C = C;
+        b.ldc = np;
+        b.m = b.n = static_cast<int32_t>(np);
+        g2.add(b);
+        c3.add(cdesc(C, np, C, np, np, np, 0, dDsq + vo[p], dDsq + vo[p]));
+        c4.add(cdesc(C, np, C, np, np, np, 2));                // exact symmetry
         EigDesc e;
         std::memset(&e, 0, sizeof(e));
         e.A = C;
@@ -849,13 +956,11 @@ void Ctx::spectralPrepare() {
         e.kcap = static_cast<int32_t>(np);
         e.node = static_cast<int32_t>(p);
         ed.push_back(e);
-        soff.push_back(2 * vo[p]);
+        soff.push_back(vo[p]);
     }
     plan.copy(c1);
-    plan.trsm(t1);
-    plan.copy(c2);
-    plan.trsm(t2);
-    plan.vec(vd);
+    plan.gemm(g1, false);
+    plan.gemm(g2, false);
     plan.copy(c3);
     plan.copy(c4);
     addSytrd(plan, ed, soff, dSv, dSp);
@@ -909,7 +1014,7 @@ void Ctx::spectralSelect(int kind, double value) {
     for (int64_t p = 0; p < N; ++p) {
         moff[p + 1] = moff[p] + k[p];
         zo[p + 1] = zo[p] + S.size[p] * k[p];
-        wk += 5 * S.size[p] * k[p];
+        wk += 6 * S.size[p] * k[p];
     }
     ptot = moff[N];
     passMem.reset(new DevArena());
@@ -929,7 +1034,7 @@ void Ctx::spectralSelect(int kind, double value) {
     std::vector<EigDesc> ed;
     std::vector<VecDesc> vd;
     Copy2dBatch cz;
-    TrsmBatch tz;
+    GemmBatch gz;
     int64_t wo2 = 0;
     for (int64_t p = 0; p < N; ++p) {
         const int64_t np = S.size[p], kp = k[p];
@@ -948,7 +1053,7 @@ void Ctx::spectralSelect(int kind, double value) {
         e.k = e.kcap = static_cast<int32_t>(kp);
         e.node = static_cast<int32_t>(p);
         ed.push_back(e);
-        wo2 += 5 * np * kp;
+        wo2 += 6 * np * kp;
         VecDesc v;
         std::memset(&v, 0, sizeof(v));
         v.src = dLam + moff[p];
@@ -957,13 +1062,24 @@ void Ctx::spectralSelect(int kind, double value) {
         v.n = static_cast<int32_t>(kp);
         v.op = 1;
         vd.push_back(v);
-        cz.add(cdesc(dY + zo[p], np, dZ + zo[p], np, np, kp, 0, dDsq + vo[p], dSw + moff[p]));
-        tz.add(dF + fo[p], S.f(p), dZ + zo[p], np, np, kp, 1);
+        cz.add(cdesc(dY + zo[p], np, dY + zo[p], np, np, kp, 0, dDsq + vo[p], dSw + moff[p]));
+        GemmDesc g = gdesc();
+        g.A = dLinv + co[p];
+        g.lda = np;
+        g.ta = 1;
+        g.B = dY + zo[p];
+        g.ldb = np;
+        g.k = static_cast<int32_t>(np);
+        g.C = dZ + zo[p];
+        g.ldc = np;
+        g.m = static_cast<int32_t>(np);
+        g.n = static_cast<int32_t>(kp);
+        gz.add(g);
     }
     addSelect(plan, ed, true);
     plan.vec(vd);
     plan.copy(cz);
-    plan.trsm(tz);
+    plan.gemm(gz, false);
     plan.run(stream);
     selected = true;
 }
@@ -1099,10 +1215,10 @@ void Ctx::reducedSolve(int64_t w, bool vectors) {
     redMem.reset(new DevArena());
     auto& A = *redMem;
     A.plan(P * P * 8);
-    A.plan(P * 8 * 8);
+    A.plan(P * 8 * 9);
     A.plan(w * 8 * 3);
     A.plan(P * w * 8 * 2);
-    if (vectors) A.plan(w * 5 * P * 8);
+    if (vectors) A.plan(w * 6 * P * 8);
     A.commit();
     double* dS = A.take<double>(P * P);
     double* d = A.take<double>(P);
@@ -1110,13 +1226,13 @@ void Ctx::reducedSolve(int64_t w, bool vectors) {
     double* tau = A.take<double>(P);
     double* murs = A.take<double>(P);
     double* sv = A.take<double>(2 * P);
-    double* sp = A.take<double>(2 * P);
+    double* sp = A.take<double>(3 * P);
     double* lam = A.take<double>(w);
     double* th = A.take<double>(w);
     double* sth = A.take<double>(w);
     double* Y = A.take<double>(P * w);
     dQ = A.take<double>(P * w);
-    double* wk = vectors ? A.take<double>(w * 5 * P) : nullptr;
+    double* wk = vectors ? A.take<double>(w * 6 * P) : nullptr;
     redBlob.reset(new Blob());
     Plan plan(redBlob.get(), &kstats);
     std::vector<VecDesc> v1(1);
@@ -1176,13 +1292,15 @@ void Ctx::backtransform() {
     for (int64_t p = 0; p < N; ++p) zbo[p + 1] = zbo[p] + S.b(p) * w;
     vecMem.reset(new DevArena());
     auto& A = *vecMem;
-    A.plan(n * w * 8 * 3);
+    A.plan(n * w * 8 * 5);
     A.plan(zbo[N] * 8);
     A.plan(w * 8);
     A.commit();
     dZfull = A.take<double>(n * w);
     dXout = A.take<double>(n * w);
     double* Ytmp = A.take<double>(n * w);
+    double* Tr1 = A.take<double>(n * w);
+    double* Tr2 = A.take<double>(n * w);
     double* zB = A.take<double>(zbo[N]);
     double* nrm = A.take<double>(w);
     vecBlob.reset(new Blob());
@@ -1235,9 +1353,9 @@ void Ctx::backtransform() {
     int64_t *mp = dMp, *mi = dMi;
     double* mx = dMx;
     double *zf = dZfull, *xo_ = dXout;
+    plan.fn([=](hipStream_t s) { launch_permute_rows(nn, perm, zf, xo_, static_cast<int>(w), nn, s); });
+    addSpmm(plan, n, mp, mi, mx, static_cast<double>(Mp[n]), xo_, Ytmp, w, Tr1, Tr2);
     plan.fn([=](hipStream_t s) {
-        launch_permute_rows(nn, perm, zf, xo_, static_cast<int>(w), nn, s);
-        launch_spmm(nn, mp, mi, mx, xo_, Ytmp, static_cast<int>(w), nn, s);
         launch_coldot(nn, xo_, Ytmp, static_cast<int>(w), nn, nrm, s);
         launch_colscale_rsqrt(nn, xo_, static_cast<int>(w), nn, nrm, s);
     });
@@ -1297,44 +1415,69 @@ void addDenseLdlt(Plan& plan, double* A, int64_t n, int64_t lda, double* W, int*
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Subspace iteration with the AMLS block factors (AMLS-SIMb) and an FP64
-// Rayleigh-Ritz step after every iteration
+// Subspace iteration with the AMLS block factors (AMLS-SIMb): Q <- K^-1 M Q
+// through L, Khat and L^T; FP64 M-orthonormalisation between steps; one FP64
+// Rayleigh-Ritz step with the original K and M at the end.
+//
+// fromBasis = true starts from the whole AMLS subspace, Q0 = L^-T blockdiag(Z)
+// (q = p, no reduced eigenvectors needed); otherwise from the q recovered
+// Ritz vectors of the last reduced solve (dZfull).
 // ---------------------------------------------------------------------------
-void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& thetaOut) {
+void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& thetaOut,
+                       bool fromBasis) {
     requireGpu();
     sync();
-    if (!recovered) throw StatusError(AMLS_E_STATE, "subspace_iter needs recovered AMLS vectors");
     if (steps <= 0) throw StatusError(AMLS_E_ARG, "steps must be positive");
-    q = want;                              // the recovered Ritz vectors
+    if (fromBasis) {
+        if (!projected) throw StatusError(AMLS_E_STATE, "subspace_iter needs the AMLS basis");
+        q = ptot;
+    } else {
+        if (!recovered) throw StatusError(AMLS_E_STATE, "subspace_iter needs recovered AMLS vectors");
+        q = want;
+    }
     if (nev > q) throw StatusError(AMLS_E_ARG, "nev exceeds the subspace width");
     const int64_t N = S.nNodes;
     std::vector<int64_t> to(N + 1, 0), zbo(N + 1, 0);
+    int64_t zbLevel = 0;
     for (int64_t p = 0; p < N; ++p) {
         to[p + 1] = to[p] + S.f(p) * q;
         zbo[p + 1] = zbo[p] + S.b(p) * q;
     }
+    // gathered x_B blocks are only live within a level: one buffer per level
+    std::vector<int64_t> zbl(N, 0);
+    for (int64_t l = 0; l <= S.maxLevel; ++l) {
+        int64_t acc = 0;
+        for (int64_t p : S.byLevel[l]) {
+            zbl[p] = acc;
+            acc += S.b(p) * q;
+        }
+        zbLevel = std::max(zbLevel, acc);
+    }
     siMem.reset(new DevArena());
     auto& A = *siMem;
-    A.plan(n * q * 8 * 4);                 // Q, R, Y, MY
+    A.plan(n * q * 8 * 6);                 // Q, R, Y, MY, two transposes
     A.plan(to[N] * 8);                     // fronts of the forward solve
-    A.plan(zbo[N] * 8);                    // gathered x_B
-    A.plan(q * q * 8 * 4);                 // Kc, Mc, C, T
+    A.plan(zbLevel * 8);                   // gathered x_B of one level
+    A.plan(q * q * 8 * 5);                 // Kc, Mc, C, T, W
     A.plan(q * 64 * 8);                    // panel scratch
-    A.plan(q * 8 * 12);                    // eig vectors
-    A.plan(q * q * 8 * 2);                 // Yt, V
-    A.plan(q * 5 * q * 8);                 // stein scratch
+    A.plan(q * 8 * 13);                    // eig vectors
+    A.plan(q * nev * 8 * 2);               // Yt, V
+    A.plan(nev * 6 * q * 8);               // inverse-iteration scratch
     A.plan(256);
     A.commit();
     double* Q = A.take<double>(n * q);
     double* R = A.take<double>(n * q);
     double* Yv = A.take<double>(n * q);
     double* MY = A.take<double>(n * q);
+    double* Tr1 = A.take<double>(n * q);
+    double* Tr2 = A.take<double>(n * q);
     double* T = A.take<double>(to[N]);
-    double* zB = A.take<double>(zbo[N]);
+    double* zB = A.take<double>(zbLevel);
     double* Kc = A.take<double>(q * q);
     double* Mc = A.take<double>(q * q);
     double* Cm = A.take<double>(q * q);
     double* Tm = A.take<double>(q * q);
+    double* Wq = A.take<double>(q * q);
     double* Wp = A.take<double>(q * 64);
     double* d = A.take<double>(q);
     double* e = A.take<double>(q);
@@ -1344,27 +1487,75 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     double* th = A.take<double>(q);
     double* sth = A.take<double>(q);
     double* sv = A.take<double>(2 * q);
-    double* sp = A.take<double>(2 * q);
-    double* Yt = A.take<double>(q * q);
-    double* V = A.take<double>(q * q);
-    double* wk = A.take<double>(q * 5 * q);
+    double* sp = A.take<double>(3 * q);
+    double* Yt = A.take<double>(q * nev);
+    double* V = A.take<double>(q * nev);
+    double* wk = A.take<double>(nev * 6 * q);
     int* err = A.take<int>(1);
     const int big = INT_MAX;
     AMLS_HIP_CHECK(hipMemcpyAsync(err, &big, sizeof(int), hipMemcpyHostToDevice, stream));
-    AMLS_HIP_CHECK(hipMemcpyAsync(Q, dZfull, n * q * 8, hipMemcpyDeviceToDevice, stream));
     siBlob.reset(new Blob());
     Plan plan(siBlob.get(), &kstats);
     const int qi = static_cast<int>(q);
     const int64_t nn = n;
+    const double nnzM = static_cast<double>(Mp[n]), nnzK = static_cast<double>(Kp[n]);
+
+    // top-down application of L^-T: rows of node p become
+    // Q(p) = Q(p) - X_p Q(B_p), ancestors first
+    auto addBacksolve = [&](double* X) {
+        for (int64_t l = S.maxLevel; l >= 0; --l) {
+            GatherBatch gt;
+            GemmBatch gb;
+            for (int64_t p : S.byLevel[l]) {
+                const int64_t np = S.size[p], b = S.b(p);
+                if (b == 0) continue;
+                GatherDesc gd;
+                std::memset(&gd, 0, sizeof(gd));
+                gd.src = X;
+                gd.lds = n;
+                gd.dst = zB + zbl[p];
+                gd.ldd = b;
+                gd.rows_idx = dbidx + S.bptr[p];
+                gd.rows = static_cast<int32_t>(b);
+                gd.cols = qi;
+                gt.add(gd);
+                GemmDesc g = gdesc();
+                g.A = dXb + xo[p];
+                g.lda = np;
+                g.B = zB + zbl[p];
+                g.ldb = b;
+                g.k = static_cast<int32_t>(b);
+                g.alpha1 = -1.0;
+                g.beta = 1.0;
+                g.C = X + S.off[p];
+                g.ldc = n;
+                g.m = static_cast<int32_t>(np);
+                g.n = qi;
+                gb.add(g);
+            }
+            plan.gather(gt);
+            plan.gemm(gb, false);
+        }
+    };
+
+    if (fromBasis) {
+        // Q0 = L^-T blockdiag(Z_p): Z_p in its rows and mode columns, then the
+        // top-down solve
+        plan.fn([=](hipStream_t s) { launch_fill(Q, nn * q, 0.0, s); });
+        Copy2dBatch cz;
+        for (int64_t p = 0; p < N; ++p)
+            cz.add(cdesc(dZ + zo[p], S.size[p], Q + S.off[p] + moff[p] * n, n, S.size[p], k[p], 0));
+        plan.copy(cz);
+        addBacksolve(Q);
+    } else {
+        AMLS_HIP_CHECK(hipMemcpyAsync(Q, dZfull, n * q * 8, hipMemcpyDeviceToDevice, stream));
+    }
+    addSpmm(plan, n, dPMp, dPMi, dPMx, nnzM, Q, R, q, Tr1, Tr2);     // R = M Q
+
     for (int it = 0; it < steps; ++it) {
-        // R = M Q (permuted ordering)
-        const double nnzM = static_cast<double>(Mp[n]), nnzK = static_cast<double>(Kp[n]);
-        auto spmmBytes = [&](double nnz) { return nnz * 16.0 + (n + 1) * 8.0 + 2.0 * n * q * 8.0; };
-        plan.timed("spmm", 2.0 * nnzM * q, spmmBytes(nnzM), [=](hipStream_t s) {
-            launch_spmm(nn, dPMp, dPMi, dPMx, Q, R, qi, nn, s);
-        });
+        const bool last = it + 1 == steps;
+        // ---- forward: fronts take their own rows of R, children's updates
         plan.fn([=](hipStream_t s) { launch_fill(T, to[S.nNodes], 0.0, s); });
-        // forward: fronts take their own rows of R
         Copy2dBatch cown;
         for (int64_t p = 0; p < N; ++p)
             cown.add(cdesc(R + S.off[p], n, T + to[p], S.f(p), S.size[p], q, 0));
@@ -1389,9 +1580,9 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                 }
                 plan.ext(eb);
             }
-            // u_p = T_B - X^T y_p, then w_p = Khat^-1 y_p in place
-            GemmBatch gu;
-            TrsmBatch t1, t2;
+            // u_p = T_B - X^T y_p, then w_p = L^-T D^-1 L^-1 y_p in place
+            // (explicit L^-1: two GEMMs through the scratch Tr1)
+            GemmBatch gu, gl1, gl2;
             Copy2dBatch sc;
             for (int64_t p : nodes) {
                 const int64_t np = S.size[p], f = S.f(p), b = S.b(p);
@@ -1412,63 +1603,107 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                     g.n = qi;
                     gu.add(g);
                 }
-                t1.add(dF + fo[p], f, Tp, f, np, q, 0);
-                sc.add(cdesc(Tp, f, Tp, f, np, q, 0, dDsq + vo[p]));
-                t2.add(dF + fo[p], f, Tp, f, np, q, 1);
+                GemmDesc a = gdesc();
+                a.A = dLinv + co[p];
+                a.lda = np;
+                a.B = Tp;
+                a.ldb = f;
+                a.k = static_cast<int32_t>(np);
+                a.C = Tr1 + S.off[p];
+                a.ldc = n;
+                a.m = static_cast<int32_t>(np);
+                a.n = qi;
+                gl1.add(a);
+                sc.add(cdesc(Tr1 + S.off[p], n, Tr1 + S.off[p], n, np, q, 0, dDinv + vo[p]));
+                GemmDesc b2 = gdesc();
+                b2.A = dLinv + co[p];
+                b2.lda = np;
+                b2.ta = 1;
+                b2.B = Tr1 + S.off[p];
+                b2.ldb = n;
+                b2.k = static_cast<int32_t>(np);
+                b2.C = Tp;
+                b2.ldc = f;
+                b2.m = static_cast<int32_t>(np);
+                b2.n = qi;
+                gl2.add(b2);
             }
             plan.gemm(gu, false);
-            plan.trsm(t1);
+            plan.gemm(gl1, false);
             plan.copy(sc);
-            plan.copy(sc);
-            plan.trsm(t2);
+            plan.gemm(gl2, false);
         }
-        // backward, top-down: x_p = w_p - X_p x_B
+        // ---- backward: Y = w, then Y(p) -= X_p Y(B_p) top-down
         Copy2dBatch cw;
         for (int64_t p = 0; p < N; ++p)
             cw.add(cdesc(T + to[p], S.f(p), Yv + S.off[p], n, S.size[p], q, 0));
         plan.copy(cw);
-        for (int64_t l = S.maxLevel; l >= 0; --l) {
-            GatherBatch gt;
-            GemmBatch gb;
-            for (int64_t p : S.byLevel[l]) {
-                const int64_t np = S.size[p], b = S.b(p);
-                if (b == 0) continue;
-                GatherDesc gd;
-                std::memset(&gd, 0, sizeof(gd));
-                gd.src = Yv;
-                gd.lds = n;
-                gd.dst = zB + zbo[p];
-                gd.ldd = b;
-                gd.rows_idx = dbidx + S.bptr[p];
-                gd.rows = static_cast<int32_t>(b);
-                gd.cols = qi;
-                gt.add(gd);
-                GemmDesc g = gdesc();
-                g.A = dXb + xo[p];
-                g.lda = np;
-                g.B = zB + zbo[p];
-                g.ldb = b;
-                g.k = static_cast<int32_t>(b);
-                g.alpha1 = -1.0;
-                g.beta = 1.0;
-                g.C = Yv + S.off[p];
-                g.ldc = n;
-                g.m = static_cast<int32_t>(np);
-                g.n = qi;
-                gb.add(g);
-            }
-            plan.gather(gt);
-            plan.gemm(gb, false);
+        addBacksolve(Yv);
+        addSpmm(plan, n, dPMp, dPMi, dPMx, nnzM, Yv, MY, q, Tr1, Tr2);   // MY = M Y
+        if (!last) {
+            // M-orthonormalise: Y^T M Y = L D L^T, W = L^-T D^-1/2,
+            // Q = Y W and M Q = (M Y) W
+            GemmBatch gg;
+            GemmDesc g = gdesc();
+            g.A = Yv;
+            g.lda = n;
+            g.ta = 1;
+            g.B = MY;
+            g.ldb = n;
+            g.k = static_cast<int32_t>(n);
+            g.C = Mc;
+            g.ldc = q;
+            g.m = g.n = qi;
+            g.lower = 1;
+            gg.add(g);
+            plan.gemm(gg, false);
+            addDenseLdlt(plan, Mc, q, q, Wp, err, -2);
+            std::vector<VecDesc> vd(1);
+            std::memset(vd.data(), 0, sizeof(VecDesc));
+            vd[0].src = Mc;
+            vd[0].dst = dsq;
+            vd[0].ld = q;
+            vd[0].n = qi;
+            vd[0].op = 0;
+            plan.vec(vd);
+            plan.fn([=](hipStream_t s) { launch_fill(Wq, q * q, 0.0, s); });
+            std::vector<VecDesc> vdg(1);
+            std::memset(vdg.data(), 0, sizeof(VecDesc));
+            vdg[0].dst = Wq;
+            vdg[0].ld = q;
+            vdg[0].n = qi;
+            vdg[0].op = 2;                             // W = I
+            plan.vec(vdg);
+            Copy2dBatch cd;
+            cd.add(cdesc(Wq, q, Wq, q, q, q, 0, dsq));  // W = D^-1/2
+            plan.copy(cd);
+            TrsmBatch tw;
+            tw.add(Mc, q, Wq, q, q, q, 1);              // W = L^-T D^-1/2
+            plan.trsm(tw);
+            GemmBatch gq;
+            GemmDesc a = gdesc();
+            a.A = Yv;
+            a.lda = n;
+            a.B = Wq;
+            a.ldb = q;
+            a.k = qi;
+            a.C = Q;
+            a.ldc = n;
+            a.m = static_cast<int32_t>(n);
+            a.n = qi;
+            gq.add(a);
+            GemmDesc b = a;
+            b.A = MY;
+            b.C = R;
+            gq.add(b);
+            plan.gemm(gq, false);
         }
-        // Rayleigh-Ritz with the original FP64 K and M: Kc = Y^T K Y,
-        // Mc = Y^T M Y (the block factors only steer the iteration; in the
-        // mixed form they carry FP32 Schur updates, so R^T Y is not used)
-        plan.timed("spmm", 2.0 * nnzM * q, spmmBytes(nnzM), [=](hipStream_t s) {
-            launch_spmm(nn, dPMp, dPMi, dPMx, Yv, MY, qi, nn, s);
-        });
-        plan.timed("spmm", 2.0 * nnzK * q, spmmBytes(nnzK), [=](hipStream_t s) {
-            launch_spmm(nn, dPKp, dPKi, dPKx, Yv, R, qi, nn, s);
-        });
+    }
+    // ---- final Rayleigh-Ritz with the original FP64 K and M: Kc = Y^T K Y,
+    // Mc = Y^T M Y; lowest nev pairs of (Kc, Mc) through Kc = L D L^T,
+    // C = D^-1/2 L^-1 Mc L^-T D^-1/2, nu = 1/theta
+    addSpmm(plan, n, dPKp, dPKi, dPKx, nnzK, Yv, R, q, Tr1, Tr2);
+    {
         GemmBatch gr;
         GemmDesc g1 = gdesc();
         g1.A = Yv;
@@ -1483,89 +1718,95 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
         g1.lower = 1;
         gr.add(g1);
         GemmDesc g2 = g1;
-        g2.A = Yv;
         g2.B = MY;
         g2.C = Mc;
         gr.add(g2);
         plan.gemm(gr, false);
-        // lowest q eigenpairs of (Kc, Mc): Kc = L D L^T,
-        // C = D^-1/2 L^-1 Mc L^-T D^-1/2, nu = 1/theta
-        addDenseLdlt(plan, Kc, q, q, Wp, err, -1);
-        Copy2dBatch c1, c2, c3, c4;
-        TrsmBatch u1, u2;
-        c1.add(cdesc(Mc, q, Cm, q, q, q, 2));
-        u1.add(Kc, q, Cm, q, q, q, 0);
-        c2.add(cdesc(Cm, q, Tm, q, q, q, 1));
-        u2.add(Kc, q, Tm, q, q, q, 0);
-        std::vector<VecDesc> vd(1);
-        std::memset(vd.data(), 0, sizeof(VecDesc));
-        vd[0].src = Kc;
-        vd[0].dst = dsq;
-        vd[0].ld = q;
-        vd[0].n = qi;
-        vd[0].op = 0;
-        c3.add(cdesc(Tm, q, Tm, q, q, q, 0, dsq, dsq));
-        c4.add(cdesc(Tm, q, Cm, q, q, q, 2));
-        plan.copy(c1);
-        plan.trsm(u1);
-        plan.copy(c2);
-        plan.trsm(u2);
-        plan.vec(vd);
-        plan.copy(c3);
-        plan.copy(c4);
-        EigDesc ed;
-        std::memset(&ed, 0, sizeof(ed));
-        ed.A = Cm;
-        ed.lda = q;
-        ed.n = qi;
-        ed.d = d;
-        ed.e = e;
-        ed.tau = tau;
-        ed.lam = lam;
-        ed.Y = Yt;
-        ed.work = wk;
-        ed.k = ed.kcap = qi;
-        addSytrd(plan, {ed}, {0}, sv, sp);
-        addSelect(plan, {ed}, true);
-        std::vector<VecDesc> v2(1);
-        std::memset(v2.data(), 0, sizeof(VecDesc));
-        v2[0].src = lam;
-        v2[0].dst = th;
-        v2[0].dst2 = sth;
-        v2[0].n = qi;
-        v2[0].op = 1;
-        plan.vec(v2);
-        Copy2dBatch cv;
-        TrsmBatch tv;
-        cv.add(cdesc(Yt, q, V, q, q, q, 0, dsq, sth));
-        tv.add(Kc, q, V, q, q, q, 1);
-        plan.copy(cv);
-        plan.trsm(tv);
-        // Q = Y V
-        GemmBatch gq;
-        GemmDesc g3 = gdesc();
-        g3.A = Yv;
-        g3.lda = n;
-        g3.B = V;
-        g3.ldb = q;
-        g3.k = qi;
-        g3.C = Q;
-        g3.ldc = n;
-        g3.m = static_cast<int32_t>(n);
-        g3.n = qi;
-        gq.add(g3);
-        plan.gemm(gq, false);
     }
-    // un-permute and M-normalise the first nev columns into dXout
+    addDenseLdlt(plan, Kc, q, q, Wp, err, -1);
+    Copy2dBatch c1, c2, c3, c4;
+    TrsmBatch u1, u2;
+    c1.add(cdesc(Mc, q, Cm, q, q, q, 2));
+    u1.add(Kc, q, Cm, q, q, q, 0);
+    c2.add(cdesc(Cm, q, Tm, q, q, q, 1));
+    u2.add(Kc, q, Tm, q, q, q, 0);
+    std::vector<VecDesc> vd(1);
+    std::memset(vd.data(), 0, sizeof(VecDesc));
+    vd[0].src = Kc;
+    vd[0].dst = dsq;
+    vd[0].ld = q;
+    vd[0].n = qi;
+    vd[0].op = 0;
+    c3.add(cdesc(Tm, q, Tm, q, q, q, 0, dsq, dsq));
+    c4.add(cdesc(Tm, q, Cm, q, q, q, 2));
+    plan.copy(c1);
+    plan.trsm(u1);
+    plan.copy(c2);
+    plan.trsm(u2);
+    plan.vec(vd);
+    plan.copy(c3);
+    plan.copy(c4);
+    EigDesc ed;
+    std::memset(&ed, 0, sizeof(ed));
+    ed.A = Cm;
+    ed.lda = q;
+    ed.n = qi;
+    ed.d = d;
+    ed.e = e;
+    ed.tau = tau;
+    ed.lam = lam;
+    ed.Y = Yt;
+    ed.work = wk;
+    ed.k = ed.kcap = static_cast<int32_t>(nev);
+    addSytrd(plan, {ed}, {0}, sv, sp);
+    addSelect(plan, {ed}, true);
     const int nv = static_cast<int>(nev);
+    std::vector<VecDesc> v2(1);
+    std::memset(v2.data(), 0, sizeof(VecDesc));
+    v2[0].src = lam;
+    v2[0].dst = th;
+    v2[0].dst2 = sth;
+    v2[0].n = nv;
+    v2[0].op = 1;
+    plan.vec(v2);
+    Copy2dBatch cv;
+    TrsmBatch tv;
+    cv.add(cdesc(Yt, q, V, q, q, nev, 0, dsq, sth));
+    tv.add(Kc, q, V, q, q, nev, 1);
+    plan.copy(cv);
+    plan.trsm(tv);
+    GemmBatch gx;
+    GemmDesc g3 = gdesc();
+    g3.A = Yv;
+    g3.lda = n;
+    g3.B = V;
+    g3.ldb = q;
+    g3.k = qi;
+    g3.C = Q;
+    g3.ldc = n;
+    g3.m = static_cast<int32_t>(n);
+    g3.n = nv;
+    gx.add(g3);
+    plan.gemm(gx, false);
+    // un-permute and M-normalise into the output block
+    // the subspace-iteration result has its own output block (dXout may
+    // point into the recovery arena of an earlier, narrower solve)
+    if (!outMem || lastOutCols < nev) {
+        outMem.reset(new DevArena());
+        outMem->plan(n * nev * 8);
+        outMem->commit();
+        outBuf = outMem->take<double>(n * nev);
+        lastOutCols = nev;
+    }
+    dXout = outBuf;
     int64_t* perm = dperm;
     int64_t *mp = dMp, *mi = dMi;
     double* mx = dMx;
     double* xo_ = dXout;
     double* nrm = dsq;
+    plan.fn([=](hipStream_t s) { launch_permute_rows(nn, perm, Q, xo_, nv, nn, s); });
+    addSpmm(plan, n, mp, mi, mx, static_cast<double>(Mp[n]), xo_, MY, nev, Tr1, Tr2);
     plan.fn([=](hipStream_t s) {
-        launch_permute_rows(nn, perm, Q, xo_, nv, nn, s);
-        launch_spmm(nn, mp, mi, mx, xo_, MY, nv, nn, s);
         launch_coldot(nn, xo_, MY, nv, nn, nrm, s);
         launch_colscale_rsqrt(nn, xo_, nv, nn, nrm, s);
     });
@@ -1576,7 +1817,9 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     AMLS_HIP_CHECK(hipMemcpyAsync(&ecode, err, sizeof(int), hipMemcpyDeviceToHost, stream));
     sync();
     if (ecode != INT_MAX)
-        throw StatusError(AMLS_E_NOT_SPD, "subspace iteration: projected stiffness is not SPD");
+        throw StatusError(AMLS_E_NOT_SPD, ecode == -2
+                                              ? "subspace iteration: Y^T M Y is not SPD"
+                                              : "subspace iteration: projected stiffness is not SPD");
 }
 
 }  // namespace amls
@@ -1757,7 +2000,7 @@ amls_status amls_hip_subspace_iter(amls_ctx* ctx, int32_t steps, int64_t q, int6
                                    double* theta, double* Z) {
     return guarded(ctx, [&](amls::Ctx& c) {
         std::vector<double> th;
-        c.subspaceIter(steps, q, nev, th);
+        c.subspaceIter(steps, q, nev, th, false);
         if (theta) std::copy(th.begin(), th.end(), theta);
         if (Z)
             AMLS_HIP_CHECK(hipMemcpyAsync(Z, c.dXout, c.n * nev * 8, hipMemcpyDeviceToHost, c.stream));
@@ -1805,25 +2048,33 @@ static void solveNumeric(Ctx& c, amls_options o, double* theta, double* Z, int64
     c.sync();
     const auto t3 = Clock::now();
     int64_t want = std::min<int64_t>(o.nev, c.ptot);
-    const bool vec = o.recoverVectors || o.siSteps > 0;
+    bool vec = o.recoverVectors != 0;
+    bool fromBasis = false;
     if (o.siSteps > 0) {
         // subspace width: siVectors, else max(2 nev, nev + 8) (eq. 25 of the
-        // subspace-iteration paper), never more than p
+        // subspace-iteration paper), never more than p.  q = p iterates on
+        // the whole AMLS subspace, which needs no reduced eigenvectors.
         const int64_t q = o.siVectors > 0 ? o.siVectors : std::max<int64_t>(2 * o.nev, o.nev + 8);
         want = std::min<int64_t>(std::max<int64_t>(q, o.nev), c.ptot);
         if (want < o.nev) throw StatusError(AMLS_E_NOCONV, "reduced problem smaller than nev");
+        fromBasis = want == c.ptot;
+        vec = !fromBasis;
     }
-    c.reducedSolve(want, vec);
+    std::vector<double> th;
+    if (!fromBasis) {
+        c.reducedSolve(want, vec);
+        th = c.theta;
+    }
     const auto t4 = Clock::now();
     if (vec) {
         c.backtransform();
         c.sync();
     }
     const auto t5 = Clock::now();
-    std::vector<double> th = c.theta;
     if (o.siSteps > 0) {
-        c.subspaceIter(o.siSteps, want, o.nev, th);
+        c.subspaceIter(o.siSteps, want, o.nev, th, fromBasis);
         want = o.nev;
+        vec = true;
     }
     const auto t6 = Clock::now();
     if (theta) std::copy(th.begin(), th.begin() + std::min<size_t>(th.size(), want), theta);
@@ -2095,9 +2346,9 @@ amls_status amls_hip_syevx_top(amls_ctx* ctx, int64_t n, const double* A, int64_
         if (n <= 0 || k < 0 || k > n) throw StatusError(AMLS_E_ARG, "bad size");
         amls::DevArena mem;
         mem.plan(n * n * 8 * 2);
-        mem.plan(n * 8 * 7);
+        mem.plan(n * 8 * 8);
         mem.plan(std::max<int64_t>(k, 1) * (n + 1) * 8 * 2);
-        mem.plan(std::max<int64_t>(k, 1) * 5 * n * 8);
+        mem.plan(std::max<int64_t>(k, 1) * 6 * n * 8);
         mem.commit();
         double* dIn = mem.take<double>(n * n);
         double* dA = mem.take<double>(n * n);
@@ -2105,10 +2356,10 @@ amls_status amls_hip_syevx_top(amls_ctx* ctx, int64_t n, const double* A, int64_
         double* e = mem.take<double>(n);
         double* tau = mem.take<double>(n);
         double* sv = mem.take<double>(2 * n);
-        double* sp = mem.take<double>(2 * n);
+        double* sp = mem.take<double>(3 * n);
         double* dl = mem.take<double>(std::max<int64_t>(k, 1));
         double* dY = mem.take<double>(std::max<int64_t>(k, 1) * n);
-        double* wk = mem.take<double>(std::max<int64_t>(k, 1) * 5 * n);
+        double* wk = mem.take<double>(std::max<int64_t>(k, 1) * 6 * n);
         AMLS_HIP_CHECK(hipMemcpyAsync(dIn, A, n * n * 8, hipMemcpyHostToDevice, c.stream));
         amls::Blob blob;
         amls::Plan plan(&blob, &c.kstats);

@@ -143,6 +143,7 @@ struct Ctx {
     std::vector<int64_t> fo, xo, wo;   // element offsets per post-order node
     int64_t ftot = 0, xtot = 0, wtot = 0;
     double *dF = nullptr, *dG = nullptr, *dXb = nullptr, *dFm = nullptr, *dW = nullptr;
+    double *dLinv = nullptr, *dDinv = nullptr;   // explicit L^-1 per node, D^-1
     int* dErr = nullptr;
     bool geDone = false;
     int gePrec = 0;
@@ -183,7 +184,9 @@ struct Ctx {
     bool recovered = false;
 
     // ---- subspace iteration
-    std::unique_ptr<DevArena> siMem;
+    std::unique_ptr<DevArena> siMem, outMem;
+    int64_t lastOutCols = 0;
+    double* outBuf = nullptr;
     std::unique_ptr<Blob> siBlob;
 
     std::map<std::string, double> timing;
@@ -200,7 +203,8 @@ struct Ctx {
     void project(int prec);
     void reducedSolve(int64_t want, bool vectors);
     void backtransform();
-    void subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& thetaOut);
+    void subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& thetaOut,
+                      bool fromBasis);
     void sync();
     void requireGpu() const {
         if (!gpu) throw StatusError(6, "context has no device");

@@ -41,7 +41,9 @@ struct TrsmDesc {
     double* B;
     int64_t ldl, ldb;
     int32_t n, nrhs;
-    int32_t trans, pad;
+    int32_t trans;
+    int32_t rhsLower;   // forward solve with a lower-triangular RHS (e.g. I): rows above
+                        // each 64-column block start zero and stay zero
 };
 
 struct TrsmWork {
@@ -77,7 +79,7 @@ struct EigDesc {
     double* tau;      // n
     double* lam;      // selected eigenvalues (descending), capacity kcap
     double* Y;        // n x kcap eigenvectors (ld n)
-    double* work;     // per-eigenvalue scratch: 5 * n doubles per cluster thread
+    double* work;     // inverse-iteration scratch: 6 * n * k doubles (interleaved)
     int64_t lda;
     int32_t n, k;     // k = number selected
     int32_t node, kcap;
@@ -112,6 +114,7 @@ struct GatherDesc {
 //   op 2: dst[i + i*ld] = 1
 //   op 3: dst[i] = 1 / sqrt(src[i])
 //   op 4: dst[i] = sqrt(1 / src[i])
+//   op 5: dst[i] = 1 / src[i + i*ld]             (D^{-1} from an LDL^T diagonal)
 struct VecDesc {
     const double* src;
     double* dst;

@@ -30,112 +30,127 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
     return red[0] + red[1] + red[2] + red[3];
 }
 
-// Step i of the lower Householder reduction.  Matrices are stored full
-// (both triangles kept current on the trailing block).  Per-matrix scratch
-// (2n each, double-buffered by step parity): v of the previous step and
-// p = A v.  Launch i reads the (i-1) buffers and writes the i buffers, so no
-// workgroup overwrites a value another one of the same launch still reads.
-// Work item (x = matrix, y = row slice of RS rows starting at i+1).
-__global__ void __launch_bounds__(256) sytrd_step_kernel(const EigDesc* __restrict__ descs,
-                                                         const int2* __restrict__ work, int i,
+// Step i of the lower Householder reduction, one 64 x 64 tile of the
+// trailing lower triangle [i+1, n)^2 per workgroup (absolute tile grid).
+// Only the lower triangle is read or written.  Fused per tile: apply the
+// rank-2 update of step i-1 and accumulate p_i = A v_i (row sums and, by
+// symmetry, column sums) with FP64 atomics.  Per-matrix scratch:
+//   sv (2n): v_{i-1} / v_i by step parity,
+//   sp (3n): p_{i-1} is read, p_i accumulated, p_{i+1} cleared (step mod 3).
+// Every workgroup recomputes the scalars it needs (p.v, the Householder
+// vector of column i) from the shared vectors, so one launch per step
+// suffices.  Work item (x = matrix, y = tile row, z = tile col, w = 1 for the
+// matrix's leader, which writes d, e, tau, v_i and the previous reflector).
+__global__ void __launch_bounds__(256) sytrd_tile_kernel(const EigDesc* __restrict__ descs,
+                                                         const int4* __restrict__ work, int i,
                                                          double* __restrict__ sv_all,
                                                          double* __restrict__ sp_all,
                                                          const int64_t* __restrict__ soff) {
-    extern __shared__ double lds[];
+    __shared__ double tile[64 * 65];
+    __shared__ double vr[64], wr[64], xr[64], vc[64], wc[64], xc[64];
     __shared__ double red[8];
-    __shared__ double pacc[4 * RS];
-    const int2 wk = work[blockIdx.x];
+    const int4 wk = work[blockIdx.x];
     const EigDesc d = descs[wk.x];
     const int n = d.n;
     const int64_t lda = d.lda;
     double* A = d.A;
-    const double* svr = sv_all + soff[wk.x] + (int64_t)((i + 1) & 1) * n;   // step i-1
-    const double* spr = sp_all + soff[wk.x] + (int64_t)((i + 1) & 1) * n;
-    double* svw = sv_all + soff[wk.x] + (int64_t)(i & 1) * n;               // step i
-    double* spw = sp_all + soff[wk.x] + (int64_t)(i & 1) * n;
     const int t = threadIdx.x;
-    double* v = lds;            // indexed by global row/col
-    double* w = lds + n;
-    double* vprev = lds + 2 * n;
-    // ---- w_{i-1} = tau p - 1/2 tau^2 (p.v) v, recomputed redundantly per block
+    const double* svr = sv_all + 2 * soff[wk.x] + (int64_t)((i + 1) & 1) * n;   // v_{i-1}
+    double* svw = sv_all + 2 * soff[wk.x] + (int64_t)(i & 1) * n;               // v_i
+    const double* spr = sp_all + 3 * soff[wk.x] + (int64_t)((i + 2) % 3) * n;   // p_{i-1}
+    double* spw = sp_all + 3 * soff[wk.x] + (int64_t)(i % 3) * n;               // p_i
+    double* spz = sp_all + 3 * soff[wk.x] + (int64_t)((i + 1) % 3) * n;         // p_{i+1}
     const bool hasPrev = i > 0;
-    const double tauPrev = hasPrev ? d.tau[i - 1] : 0.0;
+    const double tauP = hasPrev ? d.tau[i - 1] : 0.0;
+    // w_{i-1} = tau p + alpha v with alpha = -1/2 tau^2 (p.v)
+    double alphaP = 0.0;
     if (hasPrev) {
         double part = 0.0;
         for (int r = i + t; r < n; r += 256) part += spr[r] * svr[r];
-        const double pv = block_sum(part, red);
-        const double alpha = -0.5 * tauPrev * tauPrev * pv;
-        for (int r = i + t; r < n; r += 256) {
-            vprev[r] = svr[r];
-            w[r] = tauPrev * spr[r] + alpha * svr[r];
-        }
+        alphaP = -0.5 * tauP * tauP * block_sum(part, red);
     }
-    __syncthreads();
+    auto wprev = [&](int r) { return hasPrev ? tauP * spr[r] + alphaP * svr[r] : 0.0; };
+    auto vprev = [&](int r) { return hasPrev ? svr[r] : 0.0; };
+    const double vi = vprev(i), wi = wprev(i);
     const int m = n - i - 1;
     if (m <= 0) {
-        if (wk.y == 0 && t == 0) {
-            double a = A[i + (int64_t)i * lda];
-            if (hasPrev) a -= 2.0 * vprev[i] * w[i];
-            d.d[i] = a;
-        }
+        if (wk.w && t == 0) d.d[i] = A[i + (int64_t)i * lda] - 2.0 * vi * wi;
         return;
     }
-    // ---- column i after the previous update; Householder of A(i+1:n, i)
+    // column i below the diagonal after update i-1
+    auto xcol = [&](int r) {
+        return A[r + (int64_t)i * lda] - (vprev(r) * wi + wprev(r) * vi);
+    };
     double part = 0.0;
-    for (int r = i + 1 + t; r < n; r += 256) {
-        double a = A[r + (int64_t)i * lda];
-        if (hasPrev) a -= vprev[r] * w[i] + w[r] * vprev[i];
-        v[r] = a;
-        if (r > i + 1) part += a * a;
+    for (int r = i + 2 + t; r < n; r += 256) {
+        const double x = xcol(r);
+        part += x * x;
     }
     const double xnorm2 = block_sum(part, red);
-    double aii = A[i + (int64_t)i * lda];
-    if (hasPrev) aii -= 2.0 * vprev[i] * w[i];
-    const double alpha = v[i + 1];
+    const double alpha = xcol(i + 1);
     double tau = 0.0, beta = alpha, scal = 0.0;
     if (xnorm2 > 0.0) {
         beta = -copysign(sqrt(alpha * alpha + xnorm2), alpha);
         tau = (beta - alpha) / beta;
         scal = 1.0 / (alpha - beta);
     }
-    __syncthreads();
-    for (int r = i + 1 + t; r < n; r += 256) v[r] = (r == i + 1) ? 1.0 : v[r] * scal;
-    __syncthreads();
-    if (wk.y == 0 && t == 0) {
-        d.d[i] = aii;
-        d.e[i] = beta;
-        d.tau[i] = tau;
-    }
-    // ---- fused: apply the previous update to my rows and form p = A v;
-    // the previous reflector is stored below the subdiagonal of column i-1
-    const int r0 = i + 1 + wk.y * RS;
-    const int rl = t & 63, cg = t >> 6;
-    const int r = r0 + rl;
-    double acc = 0.0;
-    if (r < n) {
-        if (hasPrev && cg == 0) A[r + (int64_t)(i - 1) * lda] = vprev[r];
-        const double vr = hasPrev ? vprev[r] : 0.0, wr = hasPrev ? w[r] : 0.0;
-        for (int c = i + 1 + cg; c < n; c += 4) {
-            double* pa = A + r + (int64_t)c * lda;
-            double a = *pa;
-            if (hasPrev) {
-                a -= vr * w[c] + wr * vprev[c];
-                *pa = a;
-            }
-            acc += a * v[c];
+    auto vcur = [&](int r) { return r == i + 1 ? 1.0 : xcol(r) * scal; };
+    if (wk.w) {
+        if (t == 0) {
+            d.d[i] = A[i + (int64_t)i * lda] - 2.0 * vi * wi;
+            d.e[i] = beta;
+            d.tau[i] = tau;
+            svw[i] = 0.0;
+        }
+        for (int r = i + 1 + t; r < n; r += 256) {
+            svw[r] = vcur(r);
+            spz[r] = 0.0;
+            if (hasPrev) A[r + (int64_t)(i - 1) * lda] = svr[r];   // reflector of step i-1
         }
     }
-    pacc[cg * RS + rl] = acc;
-    __syncthreads();
-    if (t < RS && r0 + t < n) {
-        const double pr = pacc[t] + pacc[RS + t] + pacc[2 * RS + t] + pacc[3 * RS + t];
-        spw[r0 + t] = pr;
-        svw[r0 + t] = v[r0 + t];
+    if (wk.y < 0) return;
+    const int r0 = wk.y * 64, c0 = wk.z * 64;
+    if (t < 64) {
+        const int r = r0 + t;
+        const bool ok = r > i && r < n;
+        vr[t] = ok ? vprev(r) : 0.0;
+        wr[t] = ok ? wprev(r) : 0.0;
+        xr[t] = ok ? vcur(r) : 0.0;
+    } else if (t < 128) {
+        const int c = c0 + t - 64;
+        const bool ok = c > i && c < n;
+        vc[t - 64] = ok ? vprev(c) : 0.0;
+        wc[t - 64] = ok ? wprev(c) : 0.0;
+        xc[t - 64] = ok ? vcur(c) : 0.0;
     }
-    // slice 0 also records v(i) = 0 so the next step's window [i+1, n) is clean
-    if (wk.y == 0 && t == 0) {
-        svw[i] = 0.0;
-        spw[i] = 0.0;
+    __syncthreads();
+    for (int e = t; e < 64 * 64; e += 256) {
+        const int rr = e & 63, cc = e >> 6;
+        const int r = r0 + rr, c = c0 + cc;
+        double a = 0.0;
+        if (r < n && c > i && r >= c) {
+            double* pa = A + r + (int64_t)c * lda;
+            a = *pa;
+            if (hasPrev) {
+                a -= vr[rr] * wc[cc] + wr[rr] * vc[cc];
+                *pa = a;
+            }
+        }
+        tile[rr * 65 + cc] = a;
+    }
+    __syncthreads();
+    if (t < 64) {
+        const int r = r0 + t;
+        double s = 0.0;
+        for (int cc = 0; cc < 64; ++cc) s += tile[t * 65 + cc] * xc[cc];
+        if (r > i && r < n) atomicAdd(spw + r, s);
+    } else if (t < 128) {
+        const int cc = t - 64;
+        const int c = c0 + cc;
+        double s = 0.0;
+        for (int rr = 0; rr < 64; ++rr)
+            if (r0 + rr != c) s += tile[rr * 65 + cc] * xr[rr];
+        if (c > i && c < n) atomicAdd(spw + c, s);
     }
 }
 
@@ -189,164 +204,262 @@ __global__ void eig_count_kernel(const EigDesc* __restrict__ descs, int nmat,
     counts[m] = k;
 }
 
-// lam[j] = (j+1)-th largest eigenvalue, j < k; work item (x = matrix, y = j)
-__global__ void bisect_kernel(const EigDesc* __restrict__ descs, const int2* __restrict__ work,
-                              int nwork) {
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= nwork) return;
-    const int2 wk = work[id];
-    const EigDesc d = descs[wk.x];
-    const int n = d.n;
-    const int j = wk.y;
-    double lo, hi, pivmin, tn;
-    gersh(d.d, d.e, n, lo, hi, pivmin, tn);
-    const int a = n - 1 - j;    // ascending index
-    for (int it = 0; it < 256; ++it) {
-        const double mid = 0.5 * (lo + hi);
-        const double tol = 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
-        if (hi - lo <= tol || mid <= lo || mid >= hi) break;
-        if (sturm_count(d.d, d.e, n, mid, pivmin) > a) hi = mid;
-        else lo = mid;
-    }
-    d.lam[j] = 0.5 * (lo + hi);
-}
-
-// Inverse iteration.  Thread per eigenvalue; a thread whose eigenvalue starts
-// a cluster (gap to the previous one above ctol*|T|) computes the whole
-// cluster in order, orthogonalising each vector against the earlier ones.
-__global__ void stein_kernel(const EigDesc* __restrict__ descs, const int2* __restrict__ work,
-                             int nwork, const int64_t* __restrict__ woff) {
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= nwork) return;
-    const int2 wk = work[id];
-    const EigDesc d = descs[wk.x];
-    const int n = d.n;
-    const int j0 = wk.y;
-    double lo, hi, pivmin, tn;
-    gersh(d.d, d.e, n, lo, hi, pivmin, tn);
-    const double ctol = 1e-7 * tn;
-    if (j0 > 0 && fabs(d.lam[j0 - 1] - d.lam[j0]) <= ctol) return;   // not a leader
-    int j1 = j0 + 1;
-    while (j1 < d.k && fabs(d.lam[j1 - 1] - d.lam[j1]) <= ctol) ++j1;
-    double* ud = d.work + woff[id];
-    double* u1 = ud + n;
-    double* u2 = u1 + n;
-    double* ml = u2 + n;
-    double* pv = ml + n;
-    const double tiny = DBL_EPSILON * fmax(tn, DBL_MIN);
-    for (int j = j0; j < j1; ++j) {
-        // small perturbation inside a cluster keeps the factorisations distinct
-        const double lam = d.lam[j] + (j - j0) * 10.0 * DBL_EPSILON * tn;
-        // ---- LU with partial pivoting of T - lam I
-        ud[0] = d.d[0] - lam;
-        u1[0] = n > 1 ? d.e[0] : 0.0;
-        u2[0] = 0.0;
-        for (int k = 0; k + 1 < n; ++k) {
-            const double sub = d.e[k];
-            const double dg = d.d[k + 1] - lam;
-            const double sp = k + 2 < n ? d.e[k + 1] : 0.0;
-            if (fabs(ud[k]) >= fabs(sub)) {
-                double p = ud[k];
-                if (fabs(p) < tiny) p = copysign(tiny, p == 0.0 ? 1.0 : p);
-                ud[k] = p;
-                const double l = sub / p;
-                ml[k] = l;
-                pv[k] = 0.0;
-                ud[k + 1] = dg - l * u1[k];
-                u1[k + 1] = sp - l * u2[k];
-                u2[k + 1] = 0.0;
-            } else {
-                const double l = ud[k] / sub;
-                const double o1 = u1[k], o2 = u2[k];
-                ml[k] = l;
-                pv[k] = 1.0;
-                ud[k] = sub;
-                u1[k] = dg;
-                u2[k] = sp;
-                ud[k + 1] = o1 - l * dg;
-                u1[k + 1] = o2 - l * sp;
-                u2[k + 1] = 0.0;
-            }
-        }
-        if (fabs(ud[n - 1]) < tiny) ud[n - 1] = copysign(tiny, ud[n - 1] == 0.0 ? 1.0 : ud[n - 1]);
-        double* x = d.Y + (int64_t)j * n;
-        // deterministic start vector
-        uint32_t s = 2463534242u ^ (uint32_t)(j * 2654435761u) ^ (uint32_t)(wk.x * 40503u);
-        for (int r = 0; r < n; ++r) {
-            s ^= s << 13; s ^= s >> 17; s ^= s << 5;
-            x[r] = 1.0 + (double)(s & 0xffff) * (1.0 / 65536.0);
-        }
-        for (int it = 0; it < 4; ++it) {
-            // forward: apply row interchanges / multipliers
-            for (int k = 0; k + 1 < n; ++k) {
-                if (pv[k] != 0.0) {
-                    const double tmp = x[k];
-                    x[k] = x[k + 1];
-                    x[k + 1] = tmp;
-                }
-                x[k + 1] -= ml[k] * x[k];
-            }
-            // back substitution with U (bandwidth 3)
-            x[n - 1] = x[n - 1] / ud[n - 1];
-            if (n > 1) x[n - 2] = (x[n - 2] - u1[n - 2] * x[n - 1]) / ud[n - 2];
-            for (int k = n - 3; k >= 0; --k)
-                x[k] = (x[k] - u1[k] * x[k + 1] - u2[k] * x[k + 2]) / ud[k];
-            // orthogonalise against earlier members of the cluster
-            for (int q = j0; q < j; ++q) {
-                const double* y = d.Y + (int64_t)q * n;
-                double dot = 0.0;
-                for (int r = 0; r < n; ++r) dot += x[r] * y[r];
-                for (int r = 0; r < n; ++r) x[r] -= dot * y[r];
-            }
-            double nrm = 0.0, amax = 0.0;
-            for (int r = 0; r < n; ++r) nrm += x[r] * x[r];
-            nrm = sqrt(nrm);
-            double sgn = 1.0;
-            for (int r = 0; r < n; ++r)
-                if (fabs(x[r]) > amax) { amax = fabs(x[r]); sgn = x[r] < 0.0 ? -1.0 : 1.0; }
-            const double sc = sgn / nrm;
-            for (int r = 0; r < n; ++r) x[r] *= sc;
-        }
-    }
-}
-
-// y := Q y for the reflectors stored by the tridiagonalisation; one wave per
-// (matrix, column).
-__global__ void __launch_bounds__(256) ormtr_kernel(const EigDesc* __restrict__ descs,
-                                                    const int2* __restrict__ work, int nwork) {
+// lam[j] = (j+1)-th largest eigenvalue, j < k; one wave per work item
+// (x = matrix, y = j).  Multisection: each round the 64 lanes evaluate the
+// Sturm count at 64 interior points of the current bracket, which shrinks the
+// bracket 65-fold per round (instead of 2-fold for plain bisection).
+__global__ void __launch_bounds__(256) bisect_kernel(const EigDesc* __restrict__ descs,
+                                                     const int2* __restrict__ work, int nwork) {
     const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wid >= nwork) return;
     const int2 wk = work[wid];
     const EigDesc d = descs[wk.x];
     const int n = d.n;
-    double* y = d.Y + (int64_t)wk.y * n;
+    const int j = wk.y;
+    // Gershgorin bounds, lane-parallel
+    double lo = 1e308, hi = -1e308, emax = 0.0;
+    for (int r = lane; r < n; r += 64) {
+        const double el = r > 0 ? fabs(d.e[r - 1]) : 0.0;
+        const double er = r + 1 < n ? fabs(d.e[r]) : 0.0;
+        lo = fmin(lo, d.d[r] - el - er);
+        hi = fmax(hi, d.d[r] + el + er);
+        if (r + 1 < n) emax = fmax(emax, d.e[r] * d.e[r]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, o, 64));
+        hi = fmax(hi, __shfl_xor(hi, o, 64));
+        emax = fmax(emax, __shfl_xor(emax, o, 64));
+    }
+    const double tn = fmax(fabs(lo), fabs(hi));
+    lo -= 2.0 * DBL_EPSILON * tn + 1e-300;
+    hi += 2.0 * DBL_EPSILON * tn + 1e-300;
+    const double pivmin = DBL_MIN * fmax(1.0, emax);
+    const int a = n - 1 - j;    // ascending index
+    for (int round = 0; round < 16; ++round) {
+        const double tol = 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
+        if (hi - lo <= tol) break;
+        const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
+        const int c = sturm_count(d.d, d.e, n, x, pivmin);
+        // counts grow with x: lanes with c <= a form a prefix
+        const unsigned long long mask = __ballot(c <= a);
+        const int np = __popcll(mask);
+        const double nlo = np > 0 ? lo + (hi - lo) * (double)np / 65.0 : lo;
+        const double nhi = np < 64 ? lo + (hi - lo) * (double)(np + 1) / 65.0 : hi;
+        if (!(nhi < hi || nlo > lo)) break;   // no progress at this resolution
+        lo = nlo;
+        hi = nhi;
+    }
+    if (lane == 0) d.lam[j] = 0.5 * (lo + hi);
+}
+
+// Inverse iteration.  Thread per eigenvalue; a thread whose eigenvalue starts
+// a cluster (gap to the previous one above ctol*|T|) computes the whole
+// cluster in order, orthogonalising each vector against the earlier ones.
+// Scratch of matrix m is interleaved by eigenvalue, element (r, j) of array
+// q at work[(q n + r) k + j], so a wave's accesses are contiguous.
+__global__ void __launch_bounds__(64) stein_kernel(const EigDesc* __restrict__ descs,
+                                                   const int2* __restrict__ work, int nwork,
+                                                   const int64_t* __restrict__ woff,
+                                                   int useLds) {
+    // every 64-item block belongs to one matrix (the host pads per matrix);
+    // its tridiagonal is staged in LDS when it fits (useLds)
+    extern __shared__ double sde[];
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    const int2 wk0 = work[blockIdx.x * blockDim.x];
+    const EigDesc d = descs[wk0.x];
+    const int n = d.n;
+    const double* dd = d.d;
+    const double* ee = d.e;
+    if (useLds) {
+        for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            sde[r] = d.d[r];
+            sde[n + r] = r + 1 < n ? d.e[r] : 0.0;
+        }
+        __syncthreads();
+        dd = sde;
+        ee = sde + n;
+    }
+    if (id >= nwork) return;
+    const int2 wk = work[id];
+    if (wk.y < 0) return;
+    const int K = d.k;
+    const int j0 = wk.y;
+    double lo, hi, pivmin, tn;
+    gersh(dd, ee, n, lo, hi, pivmin, tn);
+    const double ctol = 1e-7 * tn;
+    if (j0 > 0 && fabs(d.lam[j0 - 1] - d.lam[j0]) <= ctol) return;   // not a leader
+    int j1 = j0 + 1;
+    while (j1 < K && fabs(d.lam[j1 - 1] - d.lam[j1]) <= ctol) ++j1;
+    (void)woff;
+    double* base = d.work + j0;
+    const int64_t sK = K;
+    double* ud = base;
+    double* u1 = base + (int64_t)n * sK;
+    double* u2 = base + 2 * (int64_t)n * sK;
+    double* ml = base + 3 * (int64_t)n * sK;
+    double* pv = base + 4 * (int64_t)n * sK;
+    double* x = base + 5 * (int64_t)n * sK;
+#define S_(arr, r) arr[(int64_t)(r) * sK]
+    const double tiny = DBL_EPSILON * fmax(tn, DBL_MIN);
+    for (int j = j0; j < j1; ++j) {
+        const double lam = d.lam[j] + (j - j0) * 10.0 * DBL_EPSILON * tn;
+        // LU with partial pivoting of T - lam I, kept in registers along the
+        // recurrence and stored for the solves
+        double udk = dd[0] - lam;
+        double u1k = n > 1 ? ee[0] : 0.0;
+        double u2k = 0.0;
+        for (int k = 0; k + 1 < n; ++k) {
+            const double sub = ee[k];
+            const double dg = dd[k + 1] - lam;
+            const double sp = k + 2 < n ? ee[k + 1] : 0.0;
+            double nud, nu1;
+            if (fabs(udk) >= fabs(sub)) {
+                double p = udk;
+                if (fabs(p) < tiny) p = copysign(tiny, p == 0.0 ? 1.0 : p);
+                const double l = sub / p;
+                S_(ud, k) = p;
+                S_(u1, k) = u1k;
+                S_(u2, k) = u2k;
+                S_(ml, k) = l;
+                S_(pv, k) = 0.0;
+                nud = dg - l * u1k;
+                nu1 = sp - l * u2k;
+            } else {
+                const double l = udk / sub;
+                S_(ud, k) = sub;
+                S_(u1, k) = dg;
+                S_(u2, k) = sp;
+                S_(ml, k) = l;
+                S_(pv, k) = 1.0;
+                nud = u1k - l * dg;
+                nu1 = u2k - l * sp;
+            }
+            udk = nud;
+            u1k = nu1;
+            u2k = 0.0;
+        }
+        if (fabs(udk) < tiny) udk = copysign(tiny, udk == 0.0 ? 1.0 : udk);
+        S_(ud, n - 1) = udk;
+        S_(u1, n - 1) = 0.0;
+        S_(u2, n - 1) = 0.0;
+        // deterministic start vector
+        uint32_t s = 2463534242u ^ (uint32_t)(j * 2654435761u) ^ (uint32_t)(wk.x * 40503u);
+        for (int r = 0; r < n; ++r) {
+            s ^= s << 13; s ^= s >> 17; s ^= s << 5;
+            S_(x, r) = 1.0 + (double)(s & 0xffff) * (1.0 / 65536.0);
+        }
+        for (int it = 0; it < 4; ++it) {
+            // forward: row interchanges and multipliers
+            double xk = S_(x, 0);
+            for (int k = 0; k + 1 < n; ++k) {
+                double xn = S_(x, k + 1);
+                if (S_(pv, k) != 0.0) {
+                    const double tmp = xk;
+                    xk = xn;
+                    xn = tmp;
+                }
+                S_(x, k) = xk;
+                xk = xn - S_(ml, k) * xk;
+            }
+            S_(x, n - 1) = xk;
+            // back substitution with U (bandwidth 3)
+            double x1 = S_(x, n - 1) / S_(ud, n - 1);
+            S_(x, n - 1) = x1;
+            double x2 = 0.0;
+            if (n > 1) {
+                const double v = (S_(x, n - 2) - S_(u1, n - 2) * x1) / S_(ud, n - 2);
+                S_(x, n - 2) = v;
+                x2 = x1;
+                x1 = v;
+            }
+            for (int k = n - 3; k >= 0; --k) {
+                const double v = (S_(x, k) - S_(u1, k) * x1 - S_(u2, k) * x2) / S_(ud, k);
+                S_(x, k) = v;
+                x2 = x1;
+                x1 = v;
+            }
+            // orthogonalise against earlier members of the cluster
+            for (int q = j0; q < j; ++q) {
+                const double* y = d.Y + (int64_t)q * n;
+                double dot = 0.0;
+                for (int r = 0; r < n; ++r) dot += S_(x, r) * y[r];
+                for (int r = 0; r < n; ++r) S_(x, r) -= dot * y[r];
+            }
+            double nrm = 0.0, amax = 0.0, sgn = 1.0;
+            for (int r = 0; r < n; ++r) {
+                const double v = S_(x, r);
+                nrm += v * v;
+                if (fabs(v) > amax) { amax = fabs(v); sgn = v < 0.0 ? -1.0 : 1.0; }
+            }
+            const double sc = sgn / sqrt(nrm);
+            for (int r = 0; r < n; ++r) S_(x, r) *= sc;
+        }
+        double* y = d.Y + (int64_t)j * n;
+        for (int r = 0; r < n; ++r) y[r] = S_(x, r);
+    }
+#undef S_
+}
+
+// Y := Q Y for the reflectors stored by the tridiagonalisation.  One
+// 256-thread workgroup per (matrix, block of nc <= 8 columns); the block of Y
+// lives in LDS for the whole sweep over the reflectors, so each reflector
+// costs one read of its vector from memory and LDS traffic for the rest.
+// Work item (x = matrix, y = first column, z = column count).
+__global__ void __launch_bounds__(256) ormtr_kernel(const EigDesc* __restrict__ descs,
+                                                    const int4* __restrict__ work) {
+    extern __shared__ double sY[];       // n x nc, column-major (ld n)
+    __shared__ double part[4][8];
+    __shared__ double coef[8];
+    const int4 wk = work[blockIdx.x];
+    const EigDesc d = descs[wk.x];
+    const int n = d.n;
+    const int c0 = wk.y, nc = wk.z;
+    double* Y = d.Y + (int64_t)c0 * n;
     const double* A = d.A;
     const int64_t lda = d.lda;
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    for (int e = t; e < n * nc; e += 256) sY[e] = Y[e];
+    __syncthreads();
     for (int i = n - 2; i >= 0; --i) {
         const double tau = d.tau[i];
         if (tau == 0.0) continue;
-        double s = 0.0;
-        for (int r = i + 1 + lane; r < n; r += 64) {
+        double s[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) s[c] = 0.0;
+        for (int r = i + 1 + t; r < n; r += 256) {
             const double vr = r == i + 1 ? 1.0 : A[r + (int64_t)i * lda];
-            s += vr * y[r];
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (c < nc) s[c] += vr * sY[r + c * n];
         }
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-        s *= tau;
-        for (int r = i + 1 + lane; r < n; r += 64) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            double v = s[c];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) part[w][c] = v;
+        }
+        __syncthreads();
+        if (t < 8) coef[t] = tau * (part[0][t] + part[1][t] + part[2][t] + part[3][t]);
+        __syncthreads();
+        for (int r = i + 1 + t; r < n; r += 256) {
             const double vr = r == i + 1 ? 1.0 : A[r + (int64_t)i * lda];
-            y[r] -= s * vr;
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (c < nc) sY[r + c * n] -= coef[c] * vr;
         }
+        __syncthreads();
     }
+    for (int e = t; e < n * nc; e += 256) Y[e] = sY[e];
 }
 
 }  // namespace dev
 
-void launch_sytrd_step(const dev::EigDesc* d, const int2* work, int nwork, int i, double* sv,
-                       double* sp, const int64_t* soff, int nmax, hipStream_t s) {
+void launch_sytrd_step(const dev::EigDesc* d, const int4* work, int nwork, int i, double* sv,
+                       double* sp, const int64_t* soff, hipStream_t s) {
     if (nwork <= 0) return;
-    const size_t lds = (size_t)3 * nmax * sizeof(double);
-    hipLaunchKernelGGL(dev::sytrd_step_kernel, dim3(nwork), dim3(256), lds, s, d, work, i, sv, sp,
+    hipLaunchKernelGGL(dev::sytrd_tile_kernel, dim3(nwork), dim3(256), 0, s, d, work, i, sv, sp,
                        soff);
 }
 
@@ -359,19 +472,21 @@ void launch_eig_count(const dev::EigDesc* d, int nmat, const double* thr, int32_
 
 void launch_bisect(const dev::EigDesc* d, const int2* work, int nwork, hipStream_t s) {
     if (nwork <= 0) return;
-    hipLaunchKernelGGL(dev::bisect_kernel, dim3((nwork + 63) / 64), dim3(64), 0, s, d, work, nwork);
+    hipLaunchKernelGGL(dev::bisect_kernel, dim3((nwork + 3) / 4), dim3(256), 0, s, d, work, nwork);
 }
 
 void launch_stein(const dev::EigDesc* d, const int2* work, int nwork, const int64_t* woff,
-                  hipStream_t s) {
+                  int nmax, hipStream_t s) {
     if (nwork <= 0) return;
-    hipLaunchKernelGGL(dev::stein_kernel, dim3((nwork + 63) / 64), dim3(64), 0, s, d, work, nwork,
-                       woff);
+    const size_t lds = (size_t)2 * nmax * sizeof(double);
+    const int useLds = lds <= 60000 ? 1 : 0;
+    hipLaunchKernelGGL(dev::stein_kernel, dim3((nwork + 63) / 64), dim3(64), useLds ? lds : 0, s, d,
+                       work, nwork, woff, useLds);
 }
 
-void launch_ormtr(const dev::EigDesc* d, const int2* work, int nwork, hipStream_t s) {
+void launch_ormtr(const dev::EigDesc* d, const int4* work, int nwork, int ldsDoubles, hipStream_t s) {
     if (nwork <= 0) return;
-    hipLaunchKernelGGL(dev::ormtr_kernel, dim3((nwork + 3) / 4), dim3(256), 0, s, d, work, nwork);
+    hipLaunchKernelGGL(dev::ormtr_kernel, dim3(nwork), dim3(256), (size_t)ldsDoubles * 8, s, d, work);
 }
 
 }  // namespace amls

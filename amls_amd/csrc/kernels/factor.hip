@@ -198,6 +198,7 @@ __global__ void vecop_kernel(const VecDesc* __restrict__ descs) {
             }
             case 2: d.dst[i + (int64_t)i * d.ld] = 1.0; break;
             case 3: d.dst[i] = 1.0 / sqrt(d.src[i]); break;
+            case 5: d.dst[i] = 1.0 / d.src[i + (int64_t)i * d.ld]; break;
             default: d.dst[i] = sqrt(1.0 / d.src[i]); break;
         }
     }

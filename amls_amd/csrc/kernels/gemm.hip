@@ -193,15 +193,19 @@ __global__ void __launch_bounds__(256) trsm_unit_kernel(const TrsmDesc* __restri
     const int nb = (n + TM - 1) / TM;
     const int t = threadIdx.x;
     double* X = d.B + (int64_t)c0 * d.ldb;
-    for (int it = 0; it < nb; ++it) {
+    // a lower-triangular right-hand side is zero above the block's first
+    // column, so the forward solve can start at that row block
+    const int first = (d.rhsLower && !d.trans) ? wk.cb : 0;
+    for (int it = first; it < nb; ++it) {
         const int rb = d.trans ? nb - 1 - it : it;
         const int r0 = rb * TM;
         const int mv = min(TM, n - r0);
         TileAcc<double> acc;
         acc.zero();
         if (!d.trans) {
-            if (r0 > 0)
-                tile_mma<double>(acc, d.L + r0, d.ldl, 0, mv, X, d.ldb, 0, nv, 0, r0, 1.0, sA, sB);
+            const int kb = first * TM;
+            if (r0 > kb)
+                tile_mma<double>(acc, d.L + r0, d.ldl, 0, mv, X, d.ldb, 0, nv, kb, r0, 1.0, sA, sB);
         } else {
             const int kb = r0 + TM;
             if (kb < n)

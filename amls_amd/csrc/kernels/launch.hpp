@@ -21,17 +21,24 @@ void launch_fill(double* p, int64_t count, double v, hipStream_t s);
 void launch_vecop(const dev::VecDesc* d, int count, hipStream_t s);
 void launch_copy2d(const dev::Copy2dDesc* d, const int4* work, int nwork, hipStream_t s);
 void launch_gather(const dev::GatherDesc* d, const int4* work, int nwork, hipStream_t s);
-void launch_sytrd_step(const dev::EigDesc* d, const int2* work, int nwork, int i, double* sv,
-                       double* sp, const int64_t* soff, int nmax, hipStream_t s);
+// Step i of the tiled tridiagonalisation; sv holds 2 soff-strided vectors
+// of n per matrix and sp 3 (soff[m] = element offset / stride unit n_m).
+void launch_sytrd_step(const dev::EigDesc* d, const int4* work, int nwork, int i, double* sv,
+                       double* sp, const int64_t* soff, hipStream_t s);
 void launch_eig_count(const dev::EigDesc* d, int nmat, const double* thr, int32_t* counts,
                       hipStream_t s);
 void launch_bisect(const dev::EigDesc* d, const int2* work, int nwork, hipStream_t s);
+// work: (matrix, j) padded per matrix to multiples of 64 with j = -1
 void launch_stein(const dev::EigDesc* d, const int2* work, int nwork, const int64_t* woff,
-                  hipStream_t s);
-void launch_ormtr(const dev::EigDesc* d, const int2* work, int nwork, hipStream_t s);
+                  int nmax, hipStream_t s);
+// work: (matrix, first column, column count <= 8); ldsDoubles >= n * count
+void launch_ormtr(const dev::EigDesc* d, const int4* work, int nwork, int ldsDoubles, hipStream_t s);
 // y(:, c) = A x(:, c) for CSR A (n x n), column-major x, y with leading dimension n
 void launch_spmm(int64_t n, const int64_t* rowptr, const int64_t* colidx, const double* vals,
                  const double* x, double* y, int ncols, int64_t ld, hipStream_t s);
+// Y = A X with X, Y row-major (n x ncols, leading dimension ncols)
+void launch_spmm_rm(int64_t n, const int64_t* rowptr, const int64_t* colidx, const double* vals,
+                    const double* x, double* y, int ncols, hipStream_t s);
 // out[c] = sum_r x(r, c) * y(r, c)
 void launch_coldot(int64_t n, const double* x, const double* y, int ncols, int64_t ld, double* out,
                    hipStream_t s);

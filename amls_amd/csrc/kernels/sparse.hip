@@ -40,6 +40,30 @@ __global__ void __launch_bounds__(256) spmm_kernel(int64_t n, const int64_t* __r
     }
 }
 
+// Row-major operands: Y(r, :) = sum_e val[e] X(col[e], :), one wave per
+// (row, 64-column chunk), lanes over contiguous columns so every nonzero is
+// one coalesced 512-byte read of X.  Chunks are the slow grid dimension, so
+// the whole GPU streams through one 64-column slab of X at a time.
+__global__ void __launch_bounds__(256) spmm_rm_kernel(int64_t n, const int64_t* __restrict__ rowptr,
+                                                      const int64_t* __restrict__ colidx,
+                                                      const double* __restrict__ vals,
+                                                      const double* __restrict__ x,
+                                                      double* __restrict__ y, int ncols) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const int c = blockIdx.y * 64 + lane;
+    const bool on = c < ncols;
+    const int64_t b = rowptr[row], e = rowptr[row + 1];
+    double acc = 0.0;
+    for (int64_t k = b; k < e; ++k) {
+        const double a = vals[k];
+        const int64_t col = colidx[k];
+        if (on) acc += a * x[col * ncols + c];
+    }
+    if (on) y[row * ncols + c] = acc;
+}
+
 // One block per column.
 __global__ void __launch_bounds__(256) coldot_kernel(int64_t n, const double* __restrict__ x,
                                                      const double* __restrict__ y, int64_t ld,
@@ -77,6 +101,13 @@ void launch_spmm(int64_t n, const int64_t* rowptr, const int64_t* colidx, const 
     if (n <= 0 || ncols <= 0) return;
     hipLaunchKernelGGL(dev::spmm_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, rowptr,
                        colidx, vals, x, y, ncols, ld);
+}
+
+void launch_spmm_rm(int64_t n, const int64_t* rowptr, const int64_t* colidx, const double* vals,
+                    const double* x, double* y, int ncols, hipStream_t s) {
+    if (n <= 0 || ncols <= 0) return;
+    hipLaunchKernelGGL(dev::spmm_rm_kernel, dim3((unsigned)((n + 3) / 4), (unsigned)((ncols + 63) / 64)),
+                       dim3(256), 0, s, n, rowptr, colidx, vals, x, y, ncols);
 }
 
 void launch_coldot(int64_t n, const double* x, const double* y, int ncols, int64_t ld, double* out,

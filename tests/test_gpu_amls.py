@@ -154,15 +154,20 @@ def test_solve_hex8_10_matches_oracle(blas1):
     assert np.abs(sol.theta - ref.theta).max() <= 1e-10 * ref.theta.max()
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_subspace_iteration_matches_oracle(blas1, prec):
-    """AMLS-SIMb with a Rayleigh-Ritz step per iteration: same iteration as
-    the oracle's restatement, and the Ritz values approach the FP64 ground
-    truth from above (Courant-Fischer)."""
+@pytest.mark.parametrize("prec,qmode", [("fp64", "ritz"), ("mixed", "ritz"),
+                                        ("fp64", "basis"), ("mixed", "basis")])
+def test_subspace_iteration_matches_oracle(blas1, prec, qmode):
+    """AMLS-SIMb: the GPU M-orthonormalises between steps and does one FP64
+    Rayleigh-Ritz at the end, the oracle does Rayleigh-Ritz after every step;
+    both iterate the same subspace, so the final Ritz values agree, and they
+    approach the FP64 ground truth from above (Courant-Fischer).  'ritz'
+    starts from the q lowest AMLS Ritz vectors, 'basis' from the whole AMLS
+    subspace (q = p)."""
     pb = small_hex(seed=31, dims=(5, 5, 6))
     tree, t = tree_of(pb.K, pb.M, pb.coords, 3, 60)
-    nev, q, steps = 12, 30, 2
+    nev, steps = 12, 2
     ref = O.amls_solve(pb.K, pb.M, tree, nev=nev)
+    q = 30 if qmode == "ritz" else ref.p
     _, Q = ref.cond.solve_projected(q)
     z = ref.cond.backtransform(Q)
     X = np.zeros_like(z)
@@ -170,7 +175,8 @@ def test_subspace_iteration_matches_oracle(blas1, prec):
     t_ref, _ = O.subspace_iteration(pb.K, pb.M, ref.ge, X, steps, nev, tree.perm)
     sol = AmlsSolver(0).solve(Pencil(pb.K, pb.M, pb.coords),
                               AmlsOptions(nev=nev, nLevels=3, leafSize=60, precision=prec,
-                                          siSteps=steps, siVectors=q, verbosity=0))
+                                          siSteps=steps, siVectors=q if qmode == "ritz" else 10**6,
+                                          verbosity=0))
     exact = O.reference_eigs(pb.K, pb.M, nev)
     assert sol.theta.shape == (nev,)
     assert np.all(sol.theta >= exact * (1 - 1e-12))
@@ -183,4 +189,7 @@ def test_subspace_iteration_matches_oracle(blas1, prec):
         Kz = pb.K @ Z
         return np.linalg.norm(Kz - (pb.M @ Z) * th, axis=0) / np.linalg.norm(Kz, axis=0)
     t_or, Z_or = O.subspace_iteration(pb.K, pb.M, ref.ge, X, steps, nev, tree.perm)
-    assert np.all(resid(sol.theta, sol.Z) <= resid(t_or, Z_or) * (1 + 1e-3) + 1e-12)
+    # FP64: the same iteration; mixed: the FP32-perturbed factors steer the
+    # iteration slightly differently (Rayleigh-Ritz stays exact FP64)
+    slack = 1e-3 if prec == "fp64" else 0.1
+    assert np.all(resid(sol.theta, sol.Z) <= resid(t_or, Z_or) * (1 + slack) + 1e-12)
