@@ -279,21 +279,22 @@ void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<i
     const auto count = sp_.count;
     // algorithmic traffic per step: each active matrix reads and writes the
     // lower triangle of its trailing block once
-    std::vector<double> stepBytes(start.size(), 0.0);
+    double bytes = 0.0;
     for (int i = 0; i < static_cast<int>(start.size()); ++i)
         for (int v : ns)
-            if (v > i) stepBytes[i] += 8.0 * double(v - i - 1) * double(v - i);
+            if (v > i) bytes += 8.0 * double(v - i - 1) * double(v - i);
     KernelStats* kst = plan.st;
+    const int64_t nsteps = static_cast<int64_t>(start.size());
     plan.fn([=](hipStream_t s) {
         launch_fill(sp, spLen, 0.0, s);
-        for (int i = 0; i < static_cast<int>(start.size()); ++i) {
-            auto go = [&] {
+        // the steps run back to back; one event pair brackets the loop
+        auto loop = [&] {
+            for (int i = 0; i < static_cast<int>(start.size()); ++i)
                 launch_sytrd_step(bl->at<EigDesc>(od), bl->at<int4>(ow) + start[i], count[i], i,
                                   sv, sp, bl->at<int64_t>(oo), s);
-            };
-            if (kst) kst->run("sytrd_step", 2.0 * stepBytes[i] / 8.0, stepBytes[i], s, go);
-            else go();
-        }
+        };
+        if (kst) kst->run("sytrd_step", 2.0 * bytes / 8.0, bytes, s, loop, nsteps);
+        else loop();
     });
 }
 
@@ -392,9 +393,9 @@ hipEvent_t KernelStats::get() {
 }
 
 void KernelStats::run(const std::string& name, double flops, double bytes, hipStream_t s,
-                      const std::function<void()>& launch) {
+                      const std::function<void()>& launch, int64_t count) {
     auto& en = byName[name];
-    en.launches += 1;
+    en.launches += count;
     en.flops += flops;
     en.bytes += bytes;
     if (!enabled) {

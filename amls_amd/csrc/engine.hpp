@@ -102,8 +102,10 @@ struct KernelStats {
     std::vector<hipEvent_t> pool;
     hipEvent_t get();
     // record a launch; when enabled, `launch` is bracketed with events
+    // `launch` may issue `count` back-to-back launches of the same kernel
+    // (one event pair brackets them all)
     void run(const std::string& name, double flops, double bytes, hipStream_t s,
-             const std::function<void()>& launch);
+             const std::function<void()>& launch, int64_t count = 1);
     void resolve();   // after a stream sync
     void reset();
     ~KernelStats();

@@ -61,7 +61,22 @@ def quality(ctx, pb, opt, ref):
     return float(rel.max()), float(res.max()), p
 
 
+def measured_traffic(name):
+    """HBM bytes per dispatch of a kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by a plain copy kernel's
+    reading), or None when that kernel has no entry."""
+    path = os.path.join(ROOT, "profiles", "traffic_per_dispatch.json")
+    try:
+        with open(path) as f:
+            ent = json.load(f).get(name)
+    except OSError:
+        return None
+    return None if ent is None else float(ent["bytes_per_dispatch"])
+
+
 def roofline(stats):
+    """Dominant kernel by HIP-event time over the profiled solves: algorithmic
+    bytes (or flops) per launch over its average launch duration."""
     dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
     name, st = dom
     if st["ms"] <= 0:
@@ -73,7 +88,7 @@ def roofline(stats):
         peak, unit, bound = HBM_PEAK_GBS, "GB/s", "hbm"
         achieved = st["bytes"] / (st["ms"] * 1e-3) / 1e9
     return {"kernel": name, "bound": bound, "achieved": round(achieved, 3), "peak": peak,
-            "unit": unit, "frac": round(achieved / peak, 5), "traffic": None,
+            "unit": unit, "frac": round(achieved / peak, 5), "traffic": measured_traffic(name),
             "launches": st["launches"], "avg_launch_us": round(1e3 * st["ms"] / st["launches"], 3),
             "algorithmic_per_launch": (st["flops"] if name in PEAK else st["bytes"]) / st["launches"]}
 
@@ -89,7 +104,20 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-table", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy 10 times (the reference kernel "
+                         "for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8,
+                          "launches": 10}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)

@@ -141,12 +141,16 @@ amls_status amls_hip_backtransform(amls_ctx* ctx, double* Z);
 
 /*
  * Subspace iteration with the AMLS block factors as the inverse
- * (Q <- K^-1 M Q through L, Khat, L^T), FP64 B-orthogonal Rayleigh-Ritz with
- * the original K and M after every step.  q columns start from the lowest q
- * Ritz vectors of the last reduced solve.  theta (nev) and Z (n x nev) to the
- * host if non-null.  No counterpart in the reference's sources (README.md:14
- * claims it); algorithm AMLS-SIMb of papers/Improving_eigenpairs_of_automated_
- * multilevel_substructuring_with_subspace_iterations.md:384-427.
+ * (Q <- K^-1 M Q through L, Khat, L^T): `steps` applications, an FP64
+ * M-orthonormalisation between them, then one FP64 Rayleigh-Ritz with the
+ * original K and M that keeps the lowest nev pairs.  The q start columns are
+ * the lowest q Ritz vectors of the last reduced solve.  theta (nev) and
+ * Z (n x nev, M-normalised) to the host if non-null.  No counterpart in the
+ * reference's sources (README.md:14 claims it); algorithm AMLS-SIMb of
+ * papers/Improving_eigenpairs_of_automated_multilevel_substructuring_with_
+ * subspace_iterations.md:384-427.  amls_hip_solve / amls_hip_solve_numeric
+ * with siSteps > 0 start from the whole AMLS subspace instead when
+ * siVectors >= p (q = p, no reduced solve).
  */
 amls_status amls_hip_subspace_iter(amls_ctx* ctx, int32_t steps, int64_t q, int64_t nev,
                                    double* theta, double* Z);

@@ -34,9 +34,11 @@ factorisation of an SPD matrix, different rounding); the SIL local solver
 approximates (it is only selected for nodes with density < 0.15, which the
 hexahedral workloads do not have).
 
-Parity pins for this module live in tests/test_oracle.py (closed-form
-spectra of the reference's unit / regression tests, full-keep exactness,
-scipy eigsh/eigh fixtures).
+Parity pins for this module live in tests/test_oracle.py: the closed-form
+spectra and structural properties of the reference's unit and regression
+tests, full-keep exactness, brute-force dense Schur complements and
+congruences for every node block, the committed golden eigenvalue fixtures,
+and planted mistakes (the `plant` arguments) that those checks must reject.
 """
 from __future__ import annotations
 
@@ -132,9 +134,15 @@ class BlockGE:
     def solve(self, p: int, B: np.ndarray) -> np.ndarray:
         return sla.cho_solve(self.fac[p], B, check_finite=False)
 
-    def run(self):
+    def run(self, plant: Optional[str] = None):
         """gaussian_elimination.cpp:458-826 in post-order (the serial
-        run()/runLevel order; the DAG order computes the same sums)."""
+        run()/runLevel order; the DAG order computes the same sums).
+
+        plant: a deliberate mistake for the tests that show the parity checks
+        catch it -- "drop_wx" (no +K WX term in the M congruence),
+        "schur_sign" (K Schur update added instead of subtracted),
+        "coup_transpose" (the (K_lo,i Y_hi)^T coupling term untransposed on a
+        square block, dropped otherwise)."""
         tr = self.tree
         for p in range(tr.N):
             Kh = self.Kd[p]
@@ -171,7 +179,8 @@ class BlockGE:
             for a in anc:                               # schurKDiagTask_ (:678-693)
                 r = rK[a]
                 if r.size:
-                    self.Kd[a][ix(r, r)] -= self.Kc[(a, p)][r] @ X[a][:, r]
+                    upd = self.Kc[(a, p)][r] @ X[a][:, r]
+                    self.Kd[a][ix(r, r)] += upd if plant == "schur_sign" else -upd
             for u in range(len(anc)):                   # schurKCoupTask_ (:695-715)
                 for v in range(u + 1, len(anc)):
                     lo, hi = anc[u], anc[v]
@@ -185,7 +194,7 @@ class BlockGE:
                     T1 = B[r] @ Y[a][:, rm]
                     self.Md[a][ix(r, rm)] -= T1
                     self.Md[a][ix(rm, r)] -= T1.T
-                if r.size:
+                if r.size and plant != "drop_wx":
                     self.Md[a][ix(r, r)] += B[r] @ WX[a][:, r]
             for u in range(len(anc)):                   # congCoupTask_ (:751-781)
                 for v in range(u + 1, len(anc)):
@@ -196,7 +205,11 @@ class BlockGE:
                     if rh.size and rml.size:
                         tgt[ix(rh, rml)] -= self.Kc[(hi, p)][rh] @ Y[lo][:, rml]
                     if rl.size and rmh.size:
-                        tgt[ix(rmh, rl)] -= (self.Kc[(lo, p)][rl] @ Y[hi][:, rmh]).T
+                        T2 = self.Kc[(lo, p)][rl] @ Y[hi][:, rmh]
+                        if plant != "coup_transpose":
+                            tgt[ix(rmh, rl)] -= T2.T
+                        elif T2.shape[0] == T2.shape[1]:
+                            tgt[ix(rmh, rl)] -= T2
                     if rh.size and rl.size:
                         tgt[ix(rh, rl)] += self.Kc[(hi, p)][rh] @ WX[lo][:, rl]
         return self
@@ -236,7 +249,10 @@ class Condensation:
     def __init__(self, ge: BlockGE):
         self.ge = ge
 
-    def run(self, kind: str, value: float, pinned_k: Optional[np.ndarray] = None):
+    def run(self, kind: str, value: float, pinned_k: Optional[np.ndarray] = None,
+            plant: Optional[str] = None):
+        """plant: "drop_mid" skips the mid update M_a,kk -= K_a,i Khat^-1 M_i,kk
+        (a deliberate mistake for the tests of the parity checks)."""
         ge, tr = self.ge, self.ge.tree
         N = tr.N
         self.k = np.zeros(N, dtype=np.int64)
@@ -267,7 +283,7 @@ class Condensation:
             ki = int(self.k[i])
             rowKK = rows.get(i, [])
             if anc:
-                if rowKK:
+                if rowKK and plant != "drop_mid":
                     RHS = np.hstack([mbar[(i, kk)] for kk in rowKK])
                     Vs = ge.solve(i, RHS)
                     off = 0
