@@ -212,6 +212,24 @@ class HipContext:
                                                  C.byref(p), dptr(timing)))
         return theta, Z, p.value, dict(zip(AmlsSolver.TIMING_KEYS, timing.tolist()))
 
+    def adjoint(self, conn, Ke0, Me0, theta, Z=None):
+        """Element sensitivities (grad_s, grad_r), each nelem x nev:
+        d theta_j / d s_e = x_je^T Ke0 x_je and d theta_j / d r_e =
+        -theta_j x_je^T Me0 x_je.  Z None uses the eigenvectors of the last
+        solve kept in device memory."""
+        conn = np.ascontiguousarray(conn, dtype=np.int64)
+        nelem, npe = conn.shape
+        K0 = np.asfortranarray(Ke0, dtype=np.float64)
+        M0 = np.asfortranarray(Me0, dtype=np.float64)
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        nev = th.size
+        Zc = None if Z is None else np.asfortranarray(Z[:, :nev], dtype=np.float64)
+        gs = np.zeros((nelem, nev), order="F")
+        gr = np.zeros((nelem, nev), order="F")
+        self._chk(self._L.amls_hip_adjoint(self._h, nelem, npe, iptr(conn), dptr(K0), dptr(M0),
+                                           nev, dptr(th), dptr(Zc), dptr(gs), dptr(gr)))
+        return gs, gr
+
     # ---- inspection ------------------------------------------------------
     def node_factor(self, p: int, n_p: int):
         LD = np.zeros((n_p, n_p), order="F")
@@ -290,6 +308,7 @@ class AmlsSolver:
                               dptr(mx), dptr(cc), C.byref(oc), dptr(theta), dptr(Z), C.byref(p),
                               dptr(timing))
         self.ctx._chk(st)
+        self.ctx.n = n
         m = min(nev, p.value) if opt.siSteps == 0 else nev
         return Solution(theta=theta[:m].copy(), Z=None if Z is None else Z[:, :m].copy(order="F"),
                         p=p.value, timing=dict(zip(self.TIMING_KEYS, timing.tolist())))

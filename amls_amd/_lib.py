@@ -25,7 +25,7 @@ EXPORTS = [
     "amls_hip_reduced_solve", "amls_hip_backtransform", "amls_hip_subspace_iter",
     "amls_hip_solve", "amls_hip_solve_numeric", "amls_hip_get_node_factor", "amls_hip_get_local",
     "amls_hip_get_reduced", "amls_hip_gemm", "amls_hip_ldlt_solve", "amls_hip_syevx_top",
-    "amls_hip_profile", "amls_hip_kernel_stats",
+    "amls_hip_profile", "amls_hip_kernel_stats", "amls_hip_adjoint",
 ]
 
 
@@ -90,6 +90,7 @@ def lib() -> C.CDLL:
                             C.POINTER(AmlsOptionsC), PD, PD, PI64, PD], C.c_int),
         "amls_hip_solve_numeric": ([P, C.POINTER(AmlsOptionsC), PD, PD, PI64, PD], C.c_int),
         "amls_hip_profile": ([P, I32], C.c_int),
+        "amls_hip_adjoint": ([P, I64, I32, PI64, PD, PD, I64, PD, PD, PD, PD], C.c_int),
         "amls_hip_kernel_stats": ([P, I32, C.c_char_p, I32, PI64, PD, PD, PD], C.c_int),
         "amls_hip_get_node_factor": ([P, I64, PD, PD], C.c_int),
         "amls_hip_get_local": ([P, I64, PD, PD], C.c_int),

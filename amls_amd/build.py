@@ -20,6 +20,7 @@ SOURCES = [
     CSRC / "kernels" / "factor.hip",
     CSRC / "kernels" / "eig.hip",
     CSRC / "kernels" / "sparse.hip",
+    CSRC / "kernels" / "adjoint.hip",
     CSRC / "host" / "tree.cpp",
     CSRC / "engine.cpp",
 ]

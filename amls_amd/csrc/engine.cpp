@@ -1994,6 +1994,8 @@ amls_status amls_hip_backtransform(amls_ctx* ctx, double* Z) {
             AMLS_HIP_CHECK(hipMemcpyAsync(Z, c.dXout, c.n * c.want * 8, hipMemcpyDeviceToHost,
                                           c.stream));
         c.sync();
+        c.lastWant = c.want;
+        c.lastVectors = true;
     });
 }
 
@@ -2002,6 +2004,8 @@ amls_status amls_hip_subspace_iter(amls_ctx* ctx, int32_t steps, int64_t q, int6
     return guarded(ctx, [&](amls::Ctx& c) {
         std::vector<double> th;
         c.subspaceIter(steps, q, nev, th, false);
+        c.lastWant = nev;
+        c.lastVectors = true;
         if (theta) std::copy(th.begin(), th.end(), theta);
         if (Z)
             AMLS_HIP_CHECK(hipMemcpyAsync(Z, c.dXout, c.n * nev * 8, hipMemcpyDeviceToHost, c.stream));
@@ -2083,6 +2087,7 @@ static void solveNumeric(Ctx& c, amls_options o, double* theta, double* Z, int64
         AMLS_HIP_CHECK(hipMemcpyAsync(Z, c.dXout, c.n * want * 8, hipMemcpyDeviceToHost, c.stream));
     c.sync();
     c.lastWant = want;
+    c.lastVectors = vec;
     if (p_out) *p_out = c.ptot;
     if (timing) {
         timing[0] = secs(t0, t1);
@@ -2124,6 +2129,66 @@ amls_status amls_hip_solve_numeric(amls_ctx* ctx, const amls_options* opt, doubl
         if (!opt) throw amls::StatusError(AMLS_E_ARG, "null options");
         const auto t0 = amls::Clock::now();
         amls::solveNumeric(c, *opt, theta, Z, p_out, timing, t0, t0);
+    });
+}
+
+amls_status amls_hip_adjoint(amls_ctx* ctx, int64_t nelem, int32_t npe, const int64_t* conn,
+                             const double* Ke0, const double* Me0, int64_t nev,
+                             const double* theta, const double* Z, double* grad_s,
+                             double* grad_r) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        using amls::StatusError;
+        c.requireGpu();
+        if (nelem <= 0 || nev <= 0 || !conn || !Ke0 || !Me0 || !theta)
+            throw StatusError(AMLS_E_ARG, "adjoint: empty element set or null argument");
+        if (npe <= 0 || npe > amls::adjoint_max_npe())
+            throw StatusError(AMLS_E_ARG, "adjoint: dofs per element must be in 1.." +
+                                              std::to_string(amls::adjoint_max_npe()));
+        if (c.n <= 0) throw StatusError(AMLS_E_STATE, "adjoint: no problem set");
+        if (!Z && (!c.lastVectors || nev > c.lastWant))
+            throw StatusError(AMLS_E_STATE, "adjoint: no device eigenvectors of the last solve "
+                                            "(pass Z or solve with vectors first)");
+        for (int64_t i = 0; i < nelem * npe; ++i)
+            if (conn[i] < -1 || conn[i] >= c.n)
+                throw StatusError(AMLS_E_ARG, "adjoint: dof index out of range in conn");
+        AMLS_HIP_CHECK(hipSetDevice(c.device));
+        const size_t nz = Z ? static_cast<size_t>(c.n * nev) : 0;
+        amls::DevArena A;
+        A.plan(nelem * npe * 8);
+        A.plan(2 * npe * npe * 8);
+        A.plan(nev * 8);
+        A.plan(nz * 8);
+        A.plan(2 * nelem * nev * 8);
+        A.commit();
+        int64_t* dconn = A.take<int64_t>(nelem * npe);
+        double* dK0 = A.take<double>(npe * npe);
+        double* dM0 = A.take<double>(npe * npe);
+        double* dth = A.take<double>(nev);
+        double* dZ = Z ? A.take<double>(nz) : c.dXout;
+        double* dgs = A.take<double>(nelem * nev);
+        double* dgr = A.take<double>(nelem * nev);
+        auto up = [&](void* d, const void* h, size_t b) {
+            AMLS_HIP_CHECK(hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, c.stream));
+        };
+        up(dconn, conn, nelem * npe * 8);
+        up(dK0, Ke0, npe * npe * 8);
+        up(dM0, Me0, npe * npe * 8);
+        up(dth, theta, nev * 8);
+        if (Z) up(dZ, Z, nz * 8);
+        c.kstats.run("adjoint", 4.0 * (npe * npe + npe) * nelem * nev,
+                     8.0 * (nelem * npe * (1 + nev) + 2 * nelem * nev), c.stream, [&] {
+                         amls::launch_adjoint(c.n, nelem, npe, dconn, dK0, dM0,
+                                              static_cast<int>(nev), dth, dZ,
+                                              grad_s ? dgs : nullptr, grad_r ? dgr : nullptr,
+                                              c.stream);
+                     });
+        if (grad_s)
+            AMLS_HIP_CHECK(hipMemcpyAsync(grad_s, dgs, nelem * nev * 8, hipMemcpyDeviceToHost,
+                                          c.stream));
+        if (grad_r)
+            AMLS_HIP_CHECK(hipMemcpyAsync(grad_r, dgr, nelem * nev * 8, hipMemcpyDeviceToHost,
+                                          c.stream));
+        c.sync();
     });
 }
 

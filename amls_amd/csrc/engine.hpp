@@ -193,6 +193,7 @@ struct Ctx {
 
     std::map<std::string, double> timing;
     int64_t lastWant = 0;
+    bool lastVectors = false;    // dXout holds lastWant M-normalised eigenvectors
     KernelStats kstats;
 
     // phases

@@ -49,4 +49,13 @@ void launch_colscale_rsqrt(int64_t n, double* x, int ncols, int64_t ld, const do
 void launch_permute_rows(int64_t n, const int64_t* perm, const double* src, double* dst, int ncols,
                          int64_t ld, hipStream_t s);
 
+// Element sensitivities of eigenvalues: gs(e, j) = x_je^T K0 x_je and
+// gr(e, j) = -theta_j x_je^T M0 x_je (column-major nelem x nev; either may be
+// null); conn is nelem x npe row-major, -1 for a constrained dof;
+// npe <= adjoint_max_npe().
+int adjoint_max_npe();
+void launch_adjoint(int64_t n, int64_t nelem, int npe, const int64_t* conn, const double* K0,
+                    const double* M0, int nev, const double* theta, const double* Z, double* gs,
+                    double* gr, hipStream_t s);
+
 }  // namespace amls

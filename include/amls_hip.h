@@ -180,6 +180,26 @@ amls_status amls_hip_solve(amls_ctx* ctx, int64_t n, const int64_t* Kcolptr,
 amls_status amls_hip_solve_numeric(amls_ctx* ctx, const amls_options* opt, double* theta,
                                    double* Z, int64_t* p_out, double* timing);
 
+/*
+ * Eigenvalue sensitivities with respect to per-element stiffness and density
+ * scales, for pencils assembled as K = sum_e s_e P_e^T Ke0 P_e and
+ * M = sum_e r_e P_e^T Me0 P_e from one reference element pair:
+ *   grad_s(e, j) = d theta_j / d s_e =          x_je^T Ke0 x_je
+ *   grad_r(e, j) = d theta_j / d r_e = -theta_j x_je^T Me0 x_je
+ * for M-normalised eigenvectors x_j gathered on element e's dofs.
+ * conn: nelem x npe row-major dof indices in the original ordering (-1 for a
+ * constrained dof); Ke0, Me0: npe x npe column-major, symmetric; theta: nev.
+ * Z: n x nev column-major, or null to use the eigenvectors of the last solve
+ * still in device memory (amls_hip_solve / _solve_numeric with vectors,
+ * amls_hip_backtransform or amls_hip_subspace_iter).  grad_s, grad_r:
+ * nelem x nev column-major host buffers, either may be null.  npe <= 96.
+ * No counterpart in the reference's sources (the adjoint is new).
+ */
+amls_status amls_hip_adjoint(amls_ctx* ctx, int64_t nelem, int32_t npe, const int64_t* conn,
+                             const double* Ke0, const double* Me0, int64_t nev,
+                             const double* theta, const double* Z, double* grad_s,
+                             double* grad_r);
+
 /* ---- measurement -------------------------------------------------------
  * enable != 0 brackets every launch on the context's stream with HIP events
  * (grouped by kernel class) and clears the counters; enable = 0 stops the

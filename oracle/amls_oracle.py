@@ -507,3 +507,23 @@ def residual(K, M, theta: float, z: np.ndarray) -> float:
     num = np.linalg.norm(Kz - theta * Mz)
     den = np.linalg.norm(Kz) + abs(theta) * np.linalg.norm(Mz)
     return num / den if den > 0 else 0.0
+
+
+# --------------------------------------------------------------------------
+# Eigenvalue adjoint (new; no counterpart in the reference)
+# --------------------------------------------------------------------------
+
+def eig_adjoint(conn: np.ndarray, Ke0: np.ndarray, Me0: np.ndarray, theta: np.ndarray,
+                Z: np.ndarray):
+    """d theta_j / d s_e = x_je^T Ke0 x_je, d theta_j / d r_e =
+    -theta_j x_je^T Me0 x_je for K = sum_e s_e P_e^T Ke0 P_e,
+    M = sum_e r_e P_e^T Me0 P_e and M-normalised x_j (the first-order
+    perturbation of a simple eigenvalue, d theta = x^T (dK - theta dM) x).
+    Returns (grad_s, grad_r), nelem x nev."""
+    conn = np.asarray(conn)
+    nev = len(theta)
+    Zp = np.vstack([Z[:, :nev], np.zeros((1, nev))])      # row -1 -> zeros
+    X = Zp[np.where(conn >= 0, conn, Z.shape[0])]           # nelem x npe x nev
+    gs = np.einsum("eij,ik,ekj->ej", X, Ke0, X)
+    gr = -np.einsum("eij,ik,ekj->ej", X, Me0, X) * np.asarray(theta)[None, :]
+    return gs, gr

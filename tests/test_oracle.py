@@ -256,3 +256,28 @@ def test_subspace_iteration_bounds(blas1):
         assert np.all(th <= prev * (1 + 1e-12))
         prev = th
     assert O.rel_err(prev, exact).max() < 0.1 * O.rel_err(ref.theta, exact).max()
+
+
+def test_adjoint_oracle_fd_and_identities(blas1):
+    """The oracle's eigenvalue adjoint against central differences of dense
+    eigh (h = 1e-6 relative) and the exact identities sum_e s_e dtheta/ds_e =
+    theta, sum_e r_e dtheta/dr_e = -theta."""
+    rng = np.random.default_rng(3)
+    dims, ne, nev = (2, 2, 3), 12, 5
+    ks, rs = rng.lognormal(0, 0.3, ne), rng.uniform(0.5, 2, ne)
+    pb = cases.hex_elasticity(1, *dims, kscale=ks, rscale=rs)
+    w, V = sla.eigh(pb.K.toarray(), pb.M.toarray())
+    gs, gr = O.eig_adjoint(pb.conn, pb.Ke0, pb.Me0, w[:nev], V)
+    assert np.abs((ks[:, None] * gs).sum(0) - w[:nev]).max() <= 1e-12 * w[nev - 1]
+    assert np.abs((rs[:, None] * gr).sum(0) + w[:nev]).max() <= 1e-12 * w[nev - 1]
+    h = 1e-6
+    for which, e in [(0, 0), (0, 7), (1, 3), (1, 11)]:
+        vals = []
+        for sgn in (1, -1):
+            k2, r2 = ks.copy(), rs.copy()
+            (k2 if which == 0 else r2)[e] *= 1 + sgn * h
+            p2 = cases.hex_elasticity(1, *dims, kscale=k2, rscale=r2)
+            vals.append(sla.eigh(p2.K.toarray(), p2.M.toarray(), eigvals_only=True)[:nev])
+        fd = (vals[0] - vals[1]) / (2 * h * (ks[e] if which == 0 else rs[e]))
+        g = gs[e] if which == 0 else gr[e]
+        assert np.abs(fd - g).max() <= 1e-6 * w[nev - 1]
