@@ -92,6 +92,27 @@ def _csc(A):
             np.ascontiguousarray(A.data, dtype=np.float64))
 
 
+def _batch_args(pencils):
+    """ctypes arguments (count, n[], K colptr/rowidx/val[], M ..., coords[])."""
+    arrs = [(_csc(pc.K), _csc(pc.M),
+             None if pc.coords is None else np.ascontiguousarray(pc.coords, dtype=np.float64))
+            for pc in pencils]
+    cnt = len(arrs)
+    ns = np.array([pc.K.shape[0] for pc in pencils], dtype=np.int64)
+
+    def ptrs(get):
+        out = (C.c_void_p * cnt)()
+        for i, a in enumerate(arrs):
+            v = get(a)
+            out[i] = None if v is None else v.ctypes.data
+        return C.cast(out, C.POINTER(C.c_void_p)), out
+
+    cols = [ptrs(lambda a, i=i, j=j: a[i][j]) for i in (0, 1) for j in range(3)]
+    cc = ptrs(lambda a: a[2])
+    keep = (arrs, ns, cols, cc)
+    return (cnt, iptr(ns), *[c[0] for c in cols], cc[0], keep)
+
+
 class HipContext:
     """One amls_ctx.  device < 0: host-only (partition and analysis only)."""
 
@@ -131,6 +152,23 @@ class HipContext:
                                                iptr(mp), iptr(mi), dptr(mx), dptr(cc),
                                                nLevels, leafSize, shift))
         self.n = K.shape[0]
+
+    def set_problem_batch(self, pencils, nLevels: int = 6, leafSize: int = 500,
+                          shift: float = 0.0):
+        """Independent pencils solved together (one forest, level-batched)."""
+        args = _batch_args(pencils)
+        self._keep = args
+        self._chk(self._L.amls_hip_set_problem_batch(self._h, *args[:9], nLevels, leafSize,
+                                                     shift))
+        self.n = int(sum(pc.K.shape[0] for pc in pencils))
+
+    def batch_info(self):
+        cnt = C.c_int64()
+        self._chk(self._L.amls_hip_batch_info(self._h, C.byref(cnt), None, None))
+        rows = np.zeros(cnt.value + 1, dtype=np.int64)
+        nodes = np.zeros(cnt.value + 1, dtype=np.int64)
+        self._chk(self._L.amls_hip_batch_info(self._h, None, iptr(rows), iptr(nodes)))
+        return rows, nodes
 
     def tree(self) -> dict:
         nn, ml, n = C.c_int64(), C.c_int64(), C.c_int64()
@@ -312,3 +350,31 @@ class AmlsSolver:
         m = min(nev, p.value) if opt.siSteps == 0 else nev
         return Solution(theta=theta[:m].copy(), Z=None if Z is None else Z[:, :m].copy(order="F"),
                         p=p.value, timing=dict(zip(self.TIMING_KEYS, timing.tolist())))
+
+    def solve_batch(self, pencils, opt: AmlsOptions):
+        """Independent pencils in one batched solve; one Solution per pencil
+        (timing is that of the whole batch)."""
+        cnt = len(pencils)
+        ns = [pc.K.shape[0] for pc in pencils]
+        nev = min([opt.nev] + ns)
+        ntot = int(sum(ns))
+        theta = np.zeros(cnt * opt.nev)
+        vec = opt.recoverVectors or opt.siSteps > 0
+        Z = np.zeros((ntot, nev), order="F") if vec else None
+        p = np.zeros(cnt, dtype=np.int64)
+        timing = np.zeros(8)
+        oc = opt.to_c()
+        args = _batch_args(pencils)
+        self.ctx._chk(self.ctx._L.amls_hip_solve_batch(self.ctx._h, *args[:9], C.byref(oc),
+                                                       dptr(theta), dptr(Z), iptr(p),
+                                                       dptr(timing)))
+        self.ctx.n = ntot
+        tm = dict(zip(self.TIMING_KEYS, timing.tolist()))
+        out, r0 = [], 0
+        for s_, n_s in enumerate(ns):
+            m = min(nev, int(p.min())) if opt.siSteps == 0 else nev
+            out.append(Solution(theta=theta[s_ * opt.nev:s_ * opt.nev + m].copy(),
+                                Z=None if Z is None else Z[r0:r0 + n_s, :m].copy(order="F"),
+                                p=int(p[s_]), timing=tm))
+            r0 += n_s
+        return out

@@ -26,6 +26,7 @@ EXPORTS = [
     "amls_hip_solve", "amls_hip_solve_numeric", "amls_hip_get_node_factor", "amls_hip_get_local",
     "amls_hip_get_reduced", "amls_hip_gemm", "amls_hip_ldlt_solve", "amls_hip_syevx_top",
     "amls_hip_profile", "amls_hip_kernel_stats", "amls_hip_adjoint",
+    "amls_hip_set_problem_batch", "amls_hip_batch_info", "amls_hip_solve_batch",
 ]
 
 
@@ -62,6 +63,7 @@ P = C.c_void_p
 I32, I64, D = C.c_int32, C.c_int64, C.c_double
 PD = C.POINTER(C.c_double)
 PI64 = C.POINTER(C.c_int64)
+PP = C.POINTER(C.c_void_p)      # array of per-problem pointers
 
 
 def lib() -> C.CDLL:
@@ -91,6 +93,11 @@ def lib() -> C.CDLL:
         "amls_hip_solve_numeric": ([P, C.POINTER(AmlsOptionsC), PD, PD, PI64, PD], C.c_int),
         "amls_hip_profile": ([P, I32], C.c_int),
         "amls_hip_adjoint": ([P, I64, I32, PI64, PD, PD, I64, PD, PD, PD, PD], C.c_int),
+        "amls_hip_set_problem_batch": ([P, I64, PI64, PP, PP, PP, PP, PP, PP, PP, I32, I64, D],
+                                       C.c_int),
+        "amls_hip_batch_info": ([P, PI64, PI64, PI64], C.c_int),
+        "amls_hip_solve_batch": ([P, I64, PI64, PP, PP, PP, PP, PP, PP, PP,
+                                  C.POINTER(AmlsOptionsC), PD, PD, PI64, PD], C.c_int),
         "amls_hip_kernel_stats": ([P, I32, C.c_char_p, I32, PI64, PD, PD, PD], C.c_int),
         "amls_hip_get_node_factor": ([P, I64, PD, PD], C.c_int),
         "amls_hip_get_local": ([P, I64, PD, PD], C.c_int),

@@ -470,28 +470,62 @@ void Ctx::sync() {
 // ---------------------------------------------------------------------------
 // Analysis and upload
 // ---------------------------------------------------------------------------
-void Ctx::setProblem(int64_t nn, const int64_t* kp, const int64_t* ki, const double* kx,
-                     const int64_t* mp, const int64_t* mi, const double* mx, const double* coords,
-                     int nLevels, int64_t leafSize, double sh) {
-    if (nn <= 0) throw StatusError(AMLS_E_ARG, "empty matrix");
-    if (!kp || !ki || !kx || !mp || !mi || !mx) throw StatusError(AMLS_E_ARG, "null matrix array");
+void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t leafSize,
+                      double sh) {
+    if (probs.empty()) throw StatusError(AMLS_E_ARG, "no problem given");
     haveProblem = false;
     geDone = specReady = selected = projected = reduced = recovered = false;
-    n = nn;
+    lastVectors = false;
     shift = sh;
-    Kp.assign(kp, kp + n + 1);
-    Ki.assign(ki, ki + Kp[n]);
-    Kx.assign(kx, kx + Kp[n]);
-    Mp.assign(mp, mp + n + 1);
-    Mi.assign(mi, mi + Mp[n]);
-    Mx.assign(mx, mx + Mp[n]);
-    for (int64_t e = 0; e < Kp[n]; ++e)
-        if (Ki[e] < 0 || Ki[e] >= n) throw StatusError(AMLS_E_ARG, "K row index out of range");
-    for (int64_t e = 0; e < Mp[n]; ++e)
-        if (Mi[e] < 0 || Mi[e] >= n) throw StatusError(AMLS_E_ARG, "M row index out of range");
-    g = graphFromCsc(n, Kp.data(), Ki.data(), Mp.data(), Mi.data());
-    part = recursiveBisection(g, coords, nLevels, leafSize);
-    S = analyze(part, g);
+    nseg = static_cast<int64_t>(probs.size());
+    segRow.assign(nseg + 1, 0);
+    segNode.assign(nseg + 1, 0);
+    for (int64_t s = 0; s < nseg; ++s) {
+        const ProblemIn& P = probs[s];
+        if (P.n <= 0) throw StatusError(AMLS_E_ARG, "empty matrix");
+        if (!P.Kp || !P.Ki || !P.Kx || !P.Mp || !P.Mi || !P.Mx)
+            throw StatusError(AMLS_E_ARG, "null matrix array");
+        segRow[s + 1] = segRow[s] + P.n;
+    }
+    n = segRow[nseg];
+    // block-diagonal CSC of all problems
+    Kp.assign(1, 0);
+    Mp.assign(1, 0);
+    Ki.clear();
+    Kx.clear();
+    Mi.clear();
+    Mx.clear();
+    std::vector<Symbolic> parts;
+    for (int64_t s = 0; s < nseg; ++s) {
+        const ProblemIn& P = probs[s];
+        const int64_t nn = P.n, base = segRow[s];
+        for (int64_t e = 0; e < P.Kp[nn]; ++e)
+            if (P.Ki[e] < 0 || P.Ki[e] >= nn) throw StatusError(AMLS_E_ARG, "K row index out of range");
+        for (int64_t e = 0; e < P.Mp[nn]; ++e)
+            if (P.Mi[e] < 0 || P.Mi[e] >= nn) throw StatusError(AMLS_E_ARG, "M row index out of range");
+        const int64_t k0 = static_cast<int64_t>(Ki.size()), m0 = static_cast<int64_t>(Mi.size());
+        for (int64_t c = 0; c < nn; ++c) {
+            Kp.push_back(k0 + P.Kp[c + 1]);
+            Mp.push_back(m0 + P.Mp[c + 1]);
+        }
+        for (int64_t e = 0; e < P.Kp[nn]; ++e) {
+            Ki.push_back(P.Ki[e] + base);
+            Kx.push_back(P.Kx[e]);
+        }
+        for (int64_t e = 0; e < P.Mp[nn]; ++e) {
+            Mi.push_back(P.Mi[e] + base);
+            Mx.push_back(P.Mx[e]);
+        }
+        const Graph gs = graphFromCsc(nn, P.Kp, P.Ki, P.Mp, P.Mi);
+        const PartitionResult ps = recursiveBisection(gs, P.coords, nLevels, leafSize);
+        parts.push_back(analyze(ps, gs));
+        segNode[s + 1] = segNode[s] + parts.back().nNodes;
+    }
+    S = mergeSymbolic(parts);
+    std::vector<Symbolic>().swap(parts);
+    nodeSeg.assign(S.nNodes, 0);
+    for (int64_t s = 0; s < nseg; ++s)
+        for (int64_t p = segNode[s]; p < segNode[s + 1]; ++p) nodeSeg[p] = s;
     const int64_t N = S.nNodes;
     owner.assign(static_cast<size_t>(n), 0);
     for (int64_t p = 0; p < N; ++p)
@@ -582,6 +616,7 @@ void Ctx::setProblem(int64_t nn, const int64_t* kp, const int64_t* ki, const dou
     A.plan(n * 8);                           // perm
     A.plan(S.bidx.size() * 8 * 2);           // bidx, rel
     A.plan(nk * 16 + (n + 1) * 8 + nm * 16 + (n + 1) * 8);   // permuted CSR
+    A.plan((nseg + 1) * 8);
     A.commit();
     dKx = A.take<double>(nk);
     dKi = A.take<int64_t>(nk);
@@ -602,6 +637,7 @@ void Ctx::setProblem(int64_t nn, const int64_t* kp, const int64_t* ki, const dou
     dPMx = A.take<double>(nm);
     dPMi = A.take<int64_t>(nm);
     dPMp = A.take<int64_t>(n + 1);
+    dSegRow = A.take<int64_t>(nseg + 1);
     auto up = [&](void* d, const void* h, size_t bytes) {
         if (bytes) AMLS_HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
     };
@@ -624,6 +660,7 @@ void Ctx::setProblem(int64_t nn, const int64_t* kp, const int64_t* ki, const dou
     up(dPMx, pmx.data(), nm * 8);
     up(dPMi, pmi.data(), nm * 8);
     up(dPMp, pmp.data(), (n + 1) * 8);
+    up(dSegRow, segRow.data(), (nseg + 1) * 8);
     sync();   // host vectors above go out of scope
 }
 
@@ -973,16 +1010,20 @@ void Ctx::spectralPrepare() {
 // ---------------------------------------------------------------------------
 // Local eigenpairs + truncation (both passes)
 // ---------------------------------------------------------------------------
-void Ctx::spectralSelect(int kind, double value) {
+void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
     requireGpu();
     sync();   // earlier phases may still use buffers released below
     if (!specReady) throw StatusError(AMLS_E_STATE, "spectral before spectral_prepare");
+    if (static_cast<int64_t>(valueSeg.size()) != nseg)
+        throw StatusError(AMLS_E_INTERNAL, "one truncation value per problem expected");
     selected = projected = reduced = recovered = false;
     const int64_t N = S.nNodes;
     k.assign(N, 0);
     if (kind == 0) {
-        for (int64_t p = 0; p < N; ++p)
+        for (int64_t p = 0; p < N; ++p) {
+            const double value = valueSeg[nodeSeg[p]];
             k[p] = value <= 0 ? S.size[p] : std::min<int64_t>(static_cast<int64_t>(value), S.size[p]);
+        }
     } else {
         // keep w <= lambda_cut  <=>  nu = 1/w >= 1/lambda_cut
         std::vector<EigDesc> ed;
@@ -995,6 +1036,7 @@ void Ctx::spectralSelect(int kind, double value) {
             e.n = static_cast<int32_t>(S.size[p]);
             e.kcap = static_cast<int32_t>(S.size[p]);
             ed.push_back(e);
+            const double value = valueSeg[nodeSeg[p]];
             thr.push_back(value > 0 ? 1.0 / value : INFINITY);
         }
         Blob cb;
@@ -1018,6 +1060,15 @@ void Ctx::spectralSelect(int kind, double value) {
         wk += 6 * S.size[p] * k[p];
     }
     ptot = moff[N];
+    segMoff.assign(nseg + 1, 0);
+    segP.assign(nseg, 0);
+    mto.assign(nseg + 1, 0);
+    for (int64_t sg = 0; sg < nseg; ++sg) {
+        segMoff[sg] = moff[segNode[sg]];
+        segMoff[sg + 1] = moff[segNode[sg + 1]];
+        segP[sg] = segMoff[sg + 1] - segMoff[sg];
+        mto[sg + 1] = mto[sg] + segP[sg] * segP[sg];
+    }
     passMem.reset(new DevArena());
     auto& A = *passMem;
     A.plan(ptot * 8 * 3);
@@ -1101,29 +1152,34 @@ void Ctx::project(int prec) {
         const int64_t q = moff[p] - mstart[p];
         ro[p + 1] = ro[p] + S.f(p) * (q + k[p]);
     }
-    if (ptot <= 0) throw StatusError(AMLS_E_NOCONV, "reduced problem is empty (every node truncated to 0)");
-    // R fronts and Mtilde go to a second arena tied to the pass
+    for (int64_t sg = 0; sg < nseg; ++sg)
+        if (segP[sg] <= 0)
+            throw StatusError(AMLS_E_NOCONV, "reduced problem is empty (every node truncated to 0)");
+    // R fronts and Mtilde (one p_s x p_s block per problem) go to a second
+    // arena tied to the pass
     auto* A = new DevArena();
     std::unique_ptr<DevArena> hold(A);
     A->plan(ro[N] * 8);
-    A->plan(ptot * ptot * 8);
+    A->plan(mto[nseg] * 8);
     A->commit();
     dR = A->take<double>(ro[N]);
-    dMt = A->take<double>(ptot * ptot);
+    dMt = A->take<double>(mto[nseg]);
     projMem.swap(hold);
     passBlob2.reset(new Blob());
     Plan plan(passBlob2.get(), &kstats);
     const bool f32 = prec != 0;
     plan.fn([this](hipStream_t s) {
         launch_fill(dR, ro[S.nNodes], 0.0, s);
-        launch_fill(dMt, ptot * ptot, 0.0, s);
+        launch_fill(dMt, mto[nseg], 0.0, s);
     });
-    std::vector<VecDesc> vd(1);
-    std::memset(vd.data(), 0, sizeof(VecDesc));
-    vd[0].dst = dMt;
-    vd[0].ld = ptot;
-    vd[0].n = static_cast<int32_t>(ptot);
-    vd[0].op = 2;
+    std::vector<VecDesc> vd(nseg);
+    std::memset(vd.data(), 0, sizeof(VecDesc) * nseg);
+    for (int64_t sg = 0; sg < nseg; ++sg) {
+        vd[sg].dst = dMt + mto[sg];
+        vd[sg].ld = segP[sg];
+        vd[sg].n = static_cast<int32_t>(segP[sg]);
+        vd[sg].op = 2;
+    }
     plan.vec(vd);
     for (int64_t l = 0; l <= S.maxLevel; ++l) {
         ExtBatch eb;
@@ -1156,8 +1212,9 @@ void Ctx::project(int prec) {
                 d.B = R;
                 d.ldb = f;
                 d.k = static_cast<int32_t>(np);
-                d.C = dMt + moff[p] + mstart[p] * ptot;
-                d.ldc = ptot;
+                const int64_t sg = nodeSeg[p], P = segP[sg];
+                d.C = dMt + mto[sg] + (moff[p] - segMoff[sg]) + (mstart[p] - segMoff[sg]) * P;
+                d.ldc = P;
                 d.m = static_cast<int32_t>(kp);
                 d.n = static_cast<int32_t>(q);
                 gb.add(d);
@@ -1210,27 +1267,28 @@ void Ctx::reducedSolve(int64_t w, bool vectors) {
     if (!projected) throw StatusError(AMLS_E_STATE, "reduced_solve before project");
     reduced = recovered = false;
     if (w <= 0) throw StatusError(AMLS_E_ARG, "want must be positive");
-    w = std::min(w, ptot);
+    for (int64_t sg = 0; sg < nseg; ++sg) w = std::min(w, segP[sg]);
     want = w;
-    const int64_t P = ptot;
+    // one matrix per problem: S_s = mu^-1/2 Mtilde_s mu^-1/2 (p_s x p_s)
+    const int64_t P = ptot, NS = nseg;
     redMem.reset(new DevArena());
     auto& A = *redMem;
-    A.plan(P * P * 8);
+    A.plan(mto[NS] * 8);
     A.plan(P * 8 * 9);
-    A.plan(w * 8 * 3);
+    A.plan(NS * w * 8 * 3);
     A.plan(P * w * 8 * 2);
     if (vectors) A.plan(w * 6 * P * 8);
     A.commit();
-    double* dS = A.take<double>(P * P);
+    double* dS = A.take<double>(mto[NS]);
     double* d = A.take<double>(P);
     double* e = A.take<double>(P);
     double* tau = A.take<double>(P);
     double* murs = A.take<double>(P);
     double* sv = A.take<double>(2 * P);
     double* sp = A.take<double>(3 * P);
-    double* lam = A.take<double>(w);
-    double* th = A.take<double>(w);
-    double* sth = A.take<double>(w);
+    double* lam = A.take<double>(NS * w);
+    double* th = A.take<double>(NS * w);
+    double* sth = A.take<double>(NS * w);
     double* Y = A.take<double>(P * w);
     dQ = A.take<double>(P * w);
     double* wk = vectors ? A.take<double>(w * 6 * P) : nullptr;
@@ -1243,39 +1301,47 @@ void Ctx::reducedSolve(int64_t w, bool vectors) {
     v1[0].n = static_cast<int32_t>(P);
     v1[0].op = 3;
     plan.vec(v1);
-    Copy2dBatch cs;
-    cs.add(cdesc(dMt, P, dS, P, P, P, 2, murs, murs));
-    plan.copy(cs);
-    EigDesc ed;
-    std::memset(&ed, 0, sizeof(ed));
-    ed.A = dS;
-    ed.lda = P;
-    ed.n = static_cast<int32_t>(P);
-    ed.d = d;
-    ed.e = e;
-    ed.tau = tau;
-    ed.lam = lam;
-    ed.Y = Y;
-    ed.work = wk;
-    ed.k = ed.kcap = static_cast<int32_t>(w);
-    addSytrd(plan, {ed}, {0}, sv, sp);
-    addSelect(plan, {ed}, vectors);
-    std::vector<VecDesc> v2(1);
-    std::memset(v2.data(), 0, sizeof(VecDesc));
-    v2[0].src = lam;
-    v2[0].dst = th;
-    v2[0].dst2 = sth;
-    v2[0].n = static_cast<int32_t>(w);
-    v2[0].op = 1;
-    plan.vec(v2);
-    if (vectors) {
-        Copy2dBatch cq;
-        cq.add(cdesc(Y, P, dQ, P, P, w, 0, murs, sth));
-        plan.copy(cq);
+    Copy2dBatch cs, cq;
+    std::vector<EigDesc> eds;
+    std::vector<VecDesc> v2;
+    std::vector<int64_t> soff;
+    for (int64_t sg = 0; sg < NS; ++sg) {
+        const int64_t ps = segP[sg], m0 = segMoff[sg];
+        cs.add(cdesc(dMt + mto[sg], ps, dS + mto[sg], ps, ps, ps, 2, murs + m0, murs + m0));
+        EigDesc ed;
+        std::memset(&ed, 0, sizeof(ed));
+        ed.A = dS + mto[sg];
+        ed.lda = ps;
+        ed.n = static_cast<int32_t>(ps);
+        ed.d = d + m0;
+        ed.e = e + m0;
+        ed.tau = tau + m0;
+        ed.lam = lam + sg * w;
+        ed.Y = Y + m0 * w;
+        ed.work = vectors ? wk + m0 * w * 6 : nullptr;
+        ed.k = ed.kcap = static_cast<int32_t>(w);
+        ed.node = static_cast<int32_t>(sg);
+        eds.push_back(ed);
+        soff.push_back(m0);
+        VecDesc v;
+        std::memset(&v, 0, sizeof(v));
+        v.src = lam + sg * w;
+        v.dst = th + sg * w;
+        v.dst2 = sth + sg * w;
+        v.n = static_cast<int32_t>(w);
+        v.op = 1;
+        v2.push_back(v);
+        // problem s's reduced vectors are rows [m0, m0 + p_s) of Q (P x w)
+        cq.add(cdesc(Y + m0 * w, ps, dQ + m0, P, ps, w, 0, murs + m0, sth + sg * w));
     }
+    plan.copy(cs);
+    addSytrd(plan, eds, soff, sv, sp);
+    addSelect(plan, eds, vectors);
+    plan.vec(v2);
+    if (vectors) plan.copy(cq);
     plan.run(stream);
-    theta.assign(w, 0.0);
-    AMLS_HIP_CHECK(hipMemcpyAsync(theta.data(), th, w * 8, hipMemcpyDeviceToHost, stream));
+    theta.assign(NS * w, 0.0);
+    AMLS_HIP_CHECK(hipMemcpyAsync(theta.data(), th, NS * w * 8, hipMemcpyDeviceToHost, stream));
     sync();
     reduced = true;
     redVectors = vectors;
@@ -1295,7 +1361,7 @@ void Ctx::backtransform() {
     auto& A = *vecMem;
     A.plan(n * w * 8 * 5);
     A.plan(zbo[N] * 8);
-    A.plan(w * 8);
+    A.plan(nseg * w * 8);
     A.commit();
     dZfull = A.take<double>(n * w);
     dXout = A.take<double>(n * w);
@@ -1303,7 +1369,7 @@ void Ctx::backtransform() {
     double* Tr1 = A.take<double>(n * w);
     double* Tr2 = A.take<double>(n * w);
     double* zB = A.take<double>(zbo[N]);
-    double* nrm = A.take<double>(w);
+    double* nrm = A.take<double>(nseg * w);
     vecBlob.reset(new Blob());
     Plan plan(vecBlob.get(), &kstats);
     for (int64_t l = S.maxLevel; l >= 0; --l) {
@@ -1356,9 +1422,10 @@ void Ctx::backtransform() {
     double *zf = dZfull, *xo_ = dXout;
     plan.fn([=](hipStream_t s) { launch_permute_rows(nn, perm, zf, xo_, static_cast<int>(w), nn, s); });
     addSpmm(plan, n, mp, mi, mx, static_cast<double>(Mp[n]), xo_, Ytmp, w, Tr1, Tr2);
+    const int64_t* seg = dSegRow;
+    const int ns = static_cast<int>(nseg);
     plan.fn([=](hipStream_t s) {
-        launch_coldot(nn, xo_, Ytmp, static_cast<int>(w), nn, nrm, s);
-        launch_colscale_rsqrt(nn, xo_, static_cast<int>(w), nn, nrm, s);
+        launch_colnorm_seg(seg, ns, xo_, Ytmp, static_cast<int>(w), nn, nrm, s);
     });
     plan.run(stream);
     recovered = true;
@@ -1369,45 +1436,55 @@ void Ctx::backtransform() {
 // ---------------------------------------------------------------------------
 namespace {
 
-// In-place LDL^T of the n x n lower triangle of A (lda), 64-column panels.
-void addDenseLdlt(Plan& plan, double* A, int64_t n, int64_t lda, double* W, int* err, int node) {
-    std::vector<FrontDesc> fd(1);
-    fd[0].F = A;
-    fd[0].W = W;
-    fd[0].f = lda;
-    fd[0].n = static_cast<int32_t>(n);
-    fd[0].node = node;
+// In-place LDL^T of the n x n lower triangles of the matrices As[m] (all of
+// order n, leading dimension lda), 64-column panels; Ws[m]: lda x 64 scratch.
+void addDenseLdlt(Plan& plan, const std::vector<double*>& As, int64_t n, int64_t lda,
+                  const std::vector<double*>& Ws, int* err, int node) {
+    const int nm = static_cast<int>(As.size());
+    std::vector<FrontDesc> fd(nm);
+    std::vector<int32_t> list(nm);
+    for (int m = 0; m < nm; ++m) {
+        std::memset(&fd[m], 0, sizeof(FrontDesc));
+        fd[m].F = As[m];
+        fd[m].W = Ws[m];
+        fd[m].f = lda;
+        fd[m].n = static_cast<int32_t>(n);
+        fd[m].node = node;
+        list[m] = m;
+    }
     const size_t ofd = plan.blob->add(fd);
-    std::vector<int32_t> list{0};
     const size_t ol = plan.blob->add(list);
     Blob* bl = plan.blob;
     for (int64_t s0 = 0; s0 < n; s0 += 64) {
         const int64_t jb = std::min<int64_t>(64, n - s0), r1 = s0 + jb;
         std::vector<int2> rows;
-        for (int c = 0; c < cdiv(lda - r1, 256); ++c) rows.push_back(make_int2(0, c));
+        for (int m = 0; m < nm; ++m)
+            for (int c = 0; c < cdiv(lda - r1, 256); ++c) rows.push_back(make_int2(m, c));
         const size_t orw = plan.blob->add(rows);
         const int nrw = static_cast<int>(rows.size());
         const int s0i = static_cast<int>(s0);
         plan.fn([=](hipStream_t s) {
-            launch_ldlt_panel(bl->at<FrontDesc>(ofd), bl->at<int32_t>(ol), 1, s0i, err,
+            launch_ldlt_panel(bl->at<FrontDesc>(ofd), bl->at<int32_t>(ol), nm, s0i, err,
                               bl->at<int2>(orw), nrw, s);
         });
         if (n > r1) {
             GemmBatch gb;
-            GemmDesc d = gdesc();
-            d.A = A + r1 + s0 * lda;
-            d.lda = lda;
-            d.B = W;
-            d.ldb = lda;
-            d.tb = 1;
-            d.C = A + r1 + r1 * lda;
-            d.ldc = lda;
-            d.m = d.n = static_cast<int32_t>(n - r1);
-            d.k = static_cast<int32_t>(jb);
-            d.alpha1 = -1.0;
-            d.beta = 1.0;
-            d.lower = 1;
-            gb.add(d);
+            for (int m = 0; m < nm; ++m) {
+                GemmDesc d = gdesc();
+                d.A = As[m] + r1 + s0 * lda;
+                d.lda = lda;
+                d.B = Ws[m];
+                d.ldb = lda;
+                d.tb = 1;
+                d.C = As[m] + r1 + r1 * lda;
+                d.ldc = lda;
+                d.m = d.n = static_cast<int32_t>(n - r1);
+                d.k = static_cast<int32_t>(jb);
+                d.alpha1 = -1.0;
+                d.beta = 1.0;
+                d.lower = 1;
+                gb.add(d);
+            }
             plan.gemm(gb, false);
         }
     }
@@ -1431,6 +1508,9 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     if (steps <= 0) throw StatusError(AMLS_E_ARG, "steps must be positive");
     if (fromBasis) {
         if (!projected) throw StatusError(AMLS_E_STATE, "subspace_iter needs the AMLS basis");
+        if (nseg != 1)
+            throw StatusError(AMLS_E_ARG, "the whole-basis start needs one problem "
+                                          "(a batch iterates the lowest Ritz vectors)");
         q = ptot;
     } else {
         if (!recovered) throw StatusError(AMLS_E_STATE, "subspace_iter needs recovered AMLS vectors");
@@ -1459,11 +1539,12 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     A.plan(n * q * 8 * 6);                 // Q, R, Y, MY, two transposes
     A.plan(to[N] * 8);                     // fronts of the forward solve
     A.plan(zbLevel * 8);                   // gathered x_B of one level
-    A.plan(q * q * 8 * 5);                 // Kc, Mc, C, T, W
-    A.plan(q * 64 * 8);                    // panel scratch
-    A.plan(q * 8 * 13);                    // eig vectors
-    A.plan(q * nev * 8 * 2);               // Yt, V
-    A.plan(nev * 6 * q * 8);               // inverse-iteration scratch
+    const int64_t NS = nseg;               // one q x q problem per segment
+    A.plan(NS * q * q * 8 * 5);            // Kc, Mc, C, T, W
+    A.plan(NS * q * 64 * 8);               // panel scratch
+    A.plan(NS * q * 8 * 13);               // eig vectors
+    A.plan(NS * q * nev * 8 * 2);          // Yt, V
+    A.plan(NS * nev * 6 * q * 8);          // inverse-iteration scratch
     A.plan(256);
     A.commit();
     double* Q = A.take<double>(n * q);
@@ -1474,24 +1555,33 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     double* Tr2 = A.take<double>(n * q);
     double* T = A.take<double>(to[N]);
     double* zB = A.take<double>(zbLevel);
-    double* Kc = A.take<double>(q * q);
-    double* Mc = A.take<double>(q * q);
-    double* Cm = A.take<double>(q * q);
-    double* Tm = A.take<double>(q * q);
-    double* Wq = A.take<double>(q * q);
-    double* Wp = A.take<double>(q * 64);
-    double* d = A.take<double>(q);
-    double* e = A.take<double>(q);
-    double* tau = A.take<double>(q);
-    double* dsq = A.take<double>(q);
-    double* lam = A.take<double>(q);
-    double* th = A.take<double>(q);
-    double* sth = A.take<double>(q);
-    double* sv = A.take<double>(2 * q);
-    double* sp = A.take<double>(3 * q);
-    double* Yt = A.take<double>(q * nev);
-    double* V = A.take<double>(q * nev);
-    double* wk = A.take<double>(nev * 6 * q);
+    double* Kc = A.take<double>(NS * q * q);
+    double* Mc = A.take<double>(NS * q * q);
+    double* Cm = A.take<double>(NS * q * q);
+    double* Tm = A.take<double>(NS * q * q);
+    double* Wq = A.take<double>(NS * q * q);
+    double* Wp = A.take<double>(NS * q * 64);
+    double* d = A.take<double>(NS * q);
+    double* e = A.take<double>(NS * q);
+    double* tau = A.take<double>(NS * q);
+    double* dsq = A.take<double>(NS * q);
+    double* lam = A.take<double>(NS * q);
+    double* th = A.take<double>(NS * q);
+    double* sth = A.take<double>(NS * q);
+    double* sv = A.take<double>(2 * NS * q);
+    double* sp = A.take<double>(3 * NS * q);
+    double* Yt = A.take<double>(NS * q * nev);
+    double* V = A.take<double>(NS * q * nev);
+    double* wk = A.take<double>(NS * nev * 6 * q);
+    const int64_t qq = q * q;
+    std::vector<double*> McS, KcS, WpS;
+    for (int64_t sg = 0; sg < NS; ++sg) {
+        McS.push_back(Mc + sg * qq);
+        KcS.push_back(Kc + sg * qq);
+        WpS.push_back(Wp + sg * q * 64);
+    }
+    // rows of problem s: [r0, r0 + ns) in the permuted ordering
+    auto rowsOf = [&](int64_t sg) { return std::make_pair(segRow[sg], segRow[sg + 1] - segRow[sg]); };
     int* err = A.take<int>(1);
     const int big = INT_MAX;
     AMLS_HIP_CHECK(hipMemcpyAsync(err, &big, sizeof(int), hipMemcpyHostToDevice, stream));
@@ -1642,104 +1732,155 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
         addBacksolve(Yv);
         addSpmm(plan, n, dPMp, dPMi, dPMx, nnzM, Yv, MY, q, Tr1, Tr2);   // MY = M Y
         if (!last) {
-            // M-orthonormalise: Y^T M Y = L D L^T, W = L^-T D^-1/2,
-            // Q = Y W and M Q = (M Y) W
+            // M-orthonormalise each problem's block: Y^T M Y = L D L^T,
+            // W = L^-T D^-1/2, Q = Y W and M Q = (M Y) W
             GemmBatch gg;
-            GemmDesc g = gdesc();
-            g.A = Yv;
-            g.lda = n;
-            g.ta = 1;
-            g.B = MY;
-            g.ldb = n;
-            g.k = static_cast<int32_t>(n);
-            g.C = Mc;
-            g.ldc = q;
-            g.m = g.n = qi;
-            g.lower = 1;
-            gg.add(g);
+            for (int64_t sg = 0; sg < NS; ++sg) {
+                const auto [r0, ns] = rowsOf(sg);
+                GemmDesc g = gdesc();
+                g.A = Yv + r0;
+                g.lda = n;
+                g.ta = 1;
+                g.B = MY + r0;
+                g.ldb = n;
+                g.k = static_cast<int32_t>(ns);
+                g.C = McS[sg];
+                g.ldc = q;
+                g.m = g.n = qi;
+                g.lower = 1;
+                gg.add(g);
+            }
             plan.gemm(gg, false);
-            addDenseLdlt(plan, Mc, q, q, Wp, err, -2);
-            std::vector<VecDesc> vd(1);
-            std::memset(vd.data(), 0, sizeof(VecDesc));
-            vd[0].src = Mc;
-            vd[0].dst = dsq;
-            vd[0].ld = q;
-            vd[0].n = qi;
-            vd[0].op = 0;
-            plan.vec(vd);
-            plan.fn([=](hipStream_t s) { launch_fill(Wq, q * q, 0.0, s); });
-            std::vector<VecDesc> vdg(1);
-            std::memset(vdg.data(), 0, sizeof(VecDesc));
-            vdg[0].dst = Wq;
-            vdg[0].ld = q;
-            vdg[0].n = qi;
-            vdg[0].op = 2;                             // W = I
-            plan.vec(vdg);
+            addDenseLdlt(plan, McS, q, q, WpS, err, -2);
+            std::vector<VecDesc> vd(NS), vdg(NS);
             Copy2dBatch cd;
-            cd.add(cdesc(Wq, q, Wq, q, q, q, 0, dsq));  // W = D^-1/2
-            plan.copy(cd);
             TrsmBatch tw;
-            tw.add(Mc, q, Wq, q, q, q, 1);              // W = L^-T D^-1/2
-            plan.trsm(tw);
             GemmBatch gq;
-            GemmDesc a = gdesc();
-            a.A = Yv;
-            a.lda = n;
-            a.B = Wq;
-            a.ldb = q;
-            a.k = qi;
-            a.C = Q;
-            a.ldc = n;
-            a.m = static_cast<int32_t>(n);
-            a.n = qi;
-            gq.add(a);
-            GemmDesc b = a;
-            b.A = MY;
-            b.C = R;
-            gq.add(b);
+            for (int64_t sg = 0; sg < NS; ++sg) {
+                const auto [r0, ns] = rowsOf(sg);
+                std::memset(&vd[sg], 0, sizeof(VecDesc));
+                vd[sg].src = McS[sg];
+                vd[sg].dst = dsq + sg * q;
+                vd[sg].ld = q;
+                vd[sg].n = qi;
+                vd[sg].op = 0;
+                std::memset(&vdg[sg], 0, sizeof(VecDesc));
+                vdg[sg].dst = Wq + sg * qq;
+                vdg[sg].ld = q;
+                vdg[sg].n = qi;
+                vdg[sg].op = 2;                         // W = I
+                cd.add(cdesc(Wq + sg * qq, q, Wq + sg * qq, q, q, q, 0, dsq + sg * q));
+                tw.add(McS[sg], q, Wq + sg * qq, q, q, q, 1);   // W = L^-T D^-1/2
+                GemmDesc a = gdesc();
+                a.A = Yv + r0;
+                a.lda = n;
+                a.B = Wq + sg * qq;
+                a.ldb = q;
+                a.k = qi;
+                a.C = Q + r0;
+                a.ldc = n;
+                a.m = static_cast<int32_t>(ns);
+                a.n = qi;
+                gq.add(a);
+                GemmDesc b = a;
+                b.A = MY + r0;
+                b.C = R + r0;
+                gq.add(b);
+            }
+            plan.vec(vd);
+            plan.fn([=](hipStream_t s) { launch_fill(Wq, NS * q * q, 0.0, s); });
+            plan.vec(vdg);
+            plan.copy(cd);                              // W = D^-1/2
+            plan.trsm(tw);
             plan.gemm(gq, false);
         }
     }
-    // ---- final Rayleigh-Ritz with the original FP64 K and M: Kc = Y^T K Y,
-    // Mc = Y^T M Y; lowest nev pairs of (Kc, Mc) through Kc = L D L^T,
-    // C = D^-1/2 L^-1 Mc L^-T D^-1/2, nu = 1/theta
+    // ---- final Rayleigh-Ritz with the original FP64 K and M, per problem:
+    // Kc = Y^T K Y, Mc = Y^T M Y; lowest nev pairs of (Kc, Mc) through
+    // Kc = L D L^T, C = D^-1/2 L^-1 Mc L^-T D^-1/2, nu = 1/theta
     addSpmm(plan, n, dPKp, dPKi, dPKx, nnzK, Yv, R, q, Tr1, Tr2);
     {
         GemmBatch gr;
-        GemmDesc g1 = gdesc();
-        g1.A = Yv;
-        g1.lda = n;
-        g1.ta = 1;
-        g1.B = R;
-        g1.ldb = n;
-        g1.k = static_cast<int32_t>(n);
-        g1.C = Kc;
-        g1.ldc = q;
-        g1.m = g1.n = qi;
-        g1.lower = 1;
-        gr.add(g1);
-        GemmDesc g2 = g1;
-        g2.B = MY;
-        g2.C = Mc;
-        gr.add(g2);
+        for (int64_t sg = 0; sg < NS; ++sg) {
+            const auto [r0, ns] = rowsOf(sg);
+            GemmDesc g1 = gdesc();
+            g1.A = Yv + r0;
+            g1.lda = n;
+            g1.ta = 1;
+            g1.B = R + r0;
+            g1.ldb = n;
+            g1.k = static_cast<int32_t>(ns);
+            g1.C = KcS[sg];
+            g1.ldc = q;
+            g1.m = g1.n = qi;
+            g1.lower = 1;
+            gr.add(g1);
+            GemmDesc g2 = g1;
+            g2.B = MY + r0;
+            g2.C = McS[sg];
+            gr.add(g2);
+        }
         plan.gemm(gr, false);
     }
-    addDenseLdlt(plan, Kc, q, q, Wp, err, -1);
-    Copy2dBatch c1, c2, c3, c4;
-    TrsmBatch u1, u2;
-    c1.add(cdesc(Mc, q, Cm, q, q, q, 2));
-    u1.add(Kc, q, Cm, q, q, q, 0);
-    c2.add(cdesc(Cm, q, Tm, q, q, q, 1));
-    u2.add(Kc, q, Tm, q, q, q, 0);
-    std::vector<VecDesc> vd(1);
-    std::memset(vd.data(), 0, sizeof(VecDesc));
-    vd[0].src = Kc;
-    vd[0].dst = dsq;
-    vd[0].ld = q;
-    vd[0].n = qi;
-    vd[0].op = 0;
-    c3.add(cdesc(Tm, q, Tm, q, q, q, 0, dsq, dsq));
-    c4.add(cdesc(Tm, q, Cm, q, q, q, 2));
+    addDenseLdlt(plan, KcS, q, q, WpS, err, -1);
+    const int nv = static_cast<int>(nev);
+    Copy2dBatch c1, c2, c3, c4, cv;
+    TrsmBatch u1, u2, tv;
+    std::vector<VecDesc> vd(NS), v2(NS);
+    std::vector<EigDesc> eds;
+    std::vector<int64_t> soff;
+    GemmBatch gx;
+    for (int64_t sg = 0; sg < NS; ++sg) {
+        const auto [r0, ns] = rowsOf(sg);
+        double *kc = KcS[sg], *mc = McS[sg], *cm = Cm + sg * qq, *tm = Tm + sg * qq;
+        double* ds = dsq + sg * q;
+        c1.add(cdesc(mc, q, cm, q, q, q, 2));
+        u1.add(kc, q, cm, q, q, q, 0);
+        c2.add(cdesc(cm, q, tm, q, q, q, 1));
+        u2.add(kc, q, tm, q, q, q, 0);
+        std::memset(&vd[sg], 0, sizeof(VecDesc));
+        vd[sg].src = kc;
+        vd[sg].dst = ds;
+        vd[sg].ld = q;
+        vd[sg].n = qi;
+        vd[sg].op = 0;
+        c3.add(cdesc(tm, q, tm, q, q, q, 0, ds, ds));
+        c4.add(cdesc(tm, q, cm, q, q, q, 2));
+        EigDesc ed;
+        std::memset(&ed, 0, sizeof(ed));
+        ed.A = cm;
+        ed.lda = q;
+        ed.n = qi;
+        ed.d = d + sg * q;
+        ed.e = e + sg * q;
+        ed.tau = tau + sg * q;
+        ed.lam = lam + sg * q;
+        ed.Y = Yt + sg * q * nev;
+        ed.work = wk + sg * nev * 6 * q;
+        ed.k = ed.kcap = nv;
+        ed.node = static_cast<int32_t>(sg);
+        eds.push_back(ed);
+        soff.push_back(sg * q);
+        std::memset(&v2[sg], 0, sizeof(VecDesc));
+        v2[sg].src = lam + sg * q;
+        v2[sg].dst = th + sg * nev;
+        v2[sg].dst2 = sth + sg * nev;
+        v2[sg].n = nv;
+        v2[sg].op = 1;
+        cv.add(cdesc(Yt + sg * q * nev, q, V + sg * q * nev, q, q, nev, 0, ds, sth + sg * nev));
+        tv.add(kc, q, V + sg * q * nev, q, q, nev, 1);
+        GemmDesc g3 = gdesc();
+        g3.A = Yv + r0;
+        g3.lda = n;
+        g3.B = V + sg * q * nev;
+        g3.ldb = q;
+        g3.k = qi;
+        g3.C = Q + r0;
+        g3.ldc = n;
+        g3.m = static_cast<int32_t>(ns);
+        g3.n = nv;
+        gx.add(g3);
+    }
     plan.copy(c1);
     plan.trsm(u1);
     plan.copy(c2);
@@ -1747,47 +1888,11 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     plan.vec(vd);
     plan.copy(c3);
     plan.copy(c4);
-    EigDesc ed;
-    std::memset(&ed, 0, sizeof(ed));
-    ed.A = Cm;
-    ed.lda = q;
-    ed.n = qi;
-    ed.d = d;
-    ed.e = e;
-    ed.tau = tau;
-    ed.lam = lam;
-    ed.Y = Yt;
-    ed.work = wk;
-    ed.k = ed.kcap = static_cast<int32_t>(nev);
-    addSytrd(plan, {ed}, {0}, sv, sp);
-    addSelect(plan, {ed}, true);
-    const int nv = static_cast<int>(nev);
-    std::vector<VecDesc> v2(1);
-    std::memset(v2.data(), 0, sizeof(VecDesc));
-    v2[0].src = lam;
-    v2[0].dst = th;
-    v2[0].dst2 = sth;
-    v2[0].n = nv;
-    v2[0].op = 1;
+    addSytrd(plan, eds, soff, sv, sp);
+    addSelect(plan, eds, true);
     plan.vec(v2);
-    Copy2dBatch cv;
-    TrsmBatch tv;
-    cv.add(cdesc(Yt, q, V, q, q, nev, 0, dsq, sth));
-    tv.add(Kc, q, V, q, q, nev, 1);
     plan.copy(cv);
     plan.trsm(tv);
-    GemmBatch gx;
-    GemmDesc g3 = gdesc();
-    g3.A = Yv;
-    g3.lda = n;
-    g3.B = V;
-    g3.ldb = q;
-    g3.k = qi;
-    g3.C = Q;
-    g3.ldc = n;
-    g3.m = static_cast<int32_t>(n);
-    g3.n = nv;
-    gx.add(g3);
     plan.gemm(gx, false);
     // un-permute and M-normalise into the output block
     // the subspace-iteration result has its own output block (dXout may
@@ -1804,17 +1909,16 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     int64_t *mp = dMp, *mi = dMi;
     double* mx = dMx;
     double* xo_ = dXout;
-    double* nrm = dsq;
+    double* nrm = dsq;                     // NS x nev <= NS q
+    const int64_t* seg = dSegRow;
+    const int nsg = static_cast<int>(NS);
     plan.fn([=](hipStream_t s) { launch_permute_rows(nn, perm, Q, xo_, nv, nn, s); });
     addSpmm(plan, n, mp, mi, mx, static_cast<double>(Mp[n]), xo_, MY, nev, Tr1, Tr2);
-    plan.fn([=](hipStream_t s) {
-        launch_coldot(nn, xo_, MY, nv, nn, nrm, s);
-        launch_colscale_rsqrt(nn, xo_, nv, nn, nrm, s);
-    });
+    plan.fn([=](hipStream_t s) { launch_colnorm_seg(seg, nsg, xo_, MY, nv, nn, nrm, s); });
     plan.run(stream);
-    thetaOut.assign(nev, 0.0);
+    thetaOut.assign(NS * nev, 0.0);
     int ecode = 0;
-    AMLS_HIP_CHECK(hipMemcpyAsync(thetaOut.data(), th, nev * 8, hipMemcpyDeviceToHost, stream));
+    AMLS_HIP_CHECK(hipMemcpyAsync(thetaOut.data(), th, NS * nev * 8, hipMemcpyDeviceToHost, stream));
     AMLS_HIP_CHECK(hipMemcpyAsync(&ecode, err, sizeof(int), hipMemcpyDeviceToHost, stream));
     sync();
     if (ecode != INT_MAX)
@@ -1926,6 +2030,45 @@ amls_status amls_hip_set_problem(amls_ctx* ctx, int64_t n, const int64_t* Kp, co
     });
 }
 
+namespace {
+std::vector<amls::Ctx::ProblemIn> batchInputs(int64_t count, const int64_t* n,
+                                              const int64_t* const* Kp, const int64_t* const* Ki,
+                                              const double* const* Kx, const int64_t* const* Mp,
+                                              const int64_t* const* Mi, const double* const* Mx,
+                                              const double* const* coords) {
+    if (count <= 0 || !n || !Kp || !Ki || !Kx || !Mp || !Mi || !Mx)
+        throw amls::StatusError(AMLS_E_ARG, "empty batch or null array");
+    std::vector<amls::Ctx::ProblemIn> v;
+    for (int64_t s = 0; s < count; ++s)
+        v.push_back({n[s], Kp[s], Ki[s], Kx[s], Mp[s], Mi[s], Mx[s], coords ? coords[s] : nullptr});
+    return v;
+}
+}  // namespace
+
+amls_status amls_hip_set_problem_batch(amls_ctx* ctx, int64_t count, const int64_t* n,
+                                       const int64_t* const* Kp, const int64_t* const* Ki,
+                                       const double* const* Kx, const int64_t* const* Mp,
+                                       const int64_t* const* Mi, const double* const* Mx,
+                                       const double* const* coords, int32_t nLevels,
+                                       int64_t leafSize, double shift) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (c.gpu) AMLS_HIP_CHECK(hipSetDevice(c.device));
+        c.setProblems(batchInputs(count, n, Kp, Ki, Kx, Mp, Mi, Mx, coords), nLevels, leafSize,
+                      shift);
+    });
+}
+
+amls_status amls_hip_batch_info(const amls_ctx* cctx, int64_t* count, int64_t* rowStart,
+                                int64_t* nodeStart) {
+    amls_ctx* ctx = const_cast<amls_ctx*>(cctx);
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (!c.haveProblem) throw amls::StatusError(AMLS_E_STATE, "no problem set");
+        if (count) *count = c.nseg;
+        if (rowStart) std::copy(c.segRow.begin(), c.segRow.end(), rowStart);
+        if (nodeStart) std::copy(c.segNode.begin(), c.segNode.end(), nodeStart);
+    });
+}
+
 amls_status amls_hip_tree_info(const amls_ctx* cctx, int64_t* nNodes, int64_t* maxLevel,
                                int64_t* n) {
     amls_ctx* ctx = const_cast<amls_ctx*>(cctx);
@@ -2026,7 +2169,11 @@ static void solveNumeric(Ctx& c, amls_options o, double* theta, double* Z, int64
     c.requireGpu();
     if (!c.haveProblem) throw StatusError(AMLS_E_STATE, "no problem set");
     AMLS_HIP_CHECK(hipSetDevice(c.device));
-    o.nev = std::min<int64_t>(o.nev, c.n);
+    const int64_t NS = c.nseg;
+    const int64_t nevOut = o.nev;   // stride of theta per problem
+    for (int64_t sg = 0; sg < NS; ++sg)
+        o.nev = std::min<int64_t>(o.nev, c.segRow[sg + 1] - c.segRow[sg]);
+    auto minP = [&] { return *std::min_element(c.segP.begin(), c.segP.end()); };
     c.runGE(o.precision);
     const auto t2 = Clock::now();
     c.spectralPrepare();
@@ -2036,33 +2183,40 @@ static void solveNumeric(Ctx& c, amls_options o, double* theta, double* Z, int64
         c.project(o.precision);
     } else {
         if (!o.twoPass) throw StatusError(AMLS_E_ARG, "twoPass=false requires fixedKeep > 0");
-        const int64_t N = c.S.nNodes;
-        const int64_t k0 = std::max<int64_t>(4, (3 * o.nev + N - 1) / N);
-        c.spectralSelect(0, static_cast<double>(k0));
+        // pass A per problem: k0 = max(4, ceil(3 nev / N_s)) modes per node;
+        // lambda_hat = the nev-th eigenvalue of its reduced problem
+        std::vector<double> k0(NS), cut(NS);
+        for (int64_t sg = 0; sg < NS; ++sg) {
+            const int64_t Ns = c.segNode[sg + 1] - c.segNode[sg];
+            k0[sg] = static_cast<double>(std::max<int64_t>(4, (3 * o.nev + Ns - 1) / Ns));
+        }
+        c.spectralSelect(0, k0);
         c.project(o.precision);
         p0 = c.ptot;
-        c.reducedSolve(std::min<int64_t>(o.nev, c.ptot), false);
-        if (static_cast<int64_t>(c.theta.size()) < o.nev)
+        c.reducedSolve(std::min<int64_t>(o.nev, minP()), false);
+        if (c.want < o.nev)
             throw StatusError(AMLS_E_NOCONV,
                               "pass A reduced problem smaller than nev (p < nev); "
                               "use fewer levels or a larger leafSize");
-        const double lambdaHat = c.theta[o.nev - 1];
-        c.spectralSelect(1, o.thresholdFactor * lambdaHat);
+        for (int64_t sg = 0; sg < NS; ++sg)
+            cut[sg] = o.thresholdFactor * c.theta[sg * c.want + o.nev - 1];
+        c.spectralSelect(1, cut);
         c.project(o.precision);
     }
     c.sync();
     const auto t3 = Clock::now();
-    int64_t want = std::min<int64_t>(o.nev, c.ptot);
+    int64_t want = std::min<int64_t>(o.nev, minP());
     bool vec = o.recoverVectors != 0;
     bool fromBasis = false;
     if (o.siSteps > 0) {
         // subspace width: siVectors, else max(2 nev, nev + 8) (eq. 25 of the
         // subspace-iteration paper), never more than p.  q = p iterates on
-        // the whole AMLS subspace, which needs no reduced eigenvectors.
+        // the whole AMLS subspace, which needs no reduced eigenvectors (one
+        // problem only: a batch iterates its lowest Ritz vectors).
         const int64_t q = o.siVectors > 0 ? o.siVectors : std::max<int64_t>(2 * o.nev, o.nev + 8);
-        want = std::min<int64_t>(std::max<int64_t>(q, o.nev), c.ptot);
+        want = std::min<int64_t>(std::max<int64_t>(q, o.nev), minP());
         if (want < o.nev) throw StatusError(AMLS_E_NOCONV, "reduced problem smaller than nev");
-        fromBasis = want == c.ptot;
+        fromBasis = NS == 1 && want == c.ptot;
         vec = !fromBasis;
     }
     std::vector<double> th;
@@ -2076,19 +2230,27 @@ static void solveNumeric(Ctx& c, amls_options o, double* theta, double* Z, int64
         c.sync();
     }
     const auto t5 = Clock::now();
+    int64_t thStride = want;
     if (o.siSteps > 0) {
         c.subspaceIter(o.siSteps, want, o.nev, th, fromBasis);
         want = o.nev;
+        thStride = o.nev;
         vec = true;
     }
     const auto t6 = Clock::now();
-    if (theta) std::copy(th.begin(), th.begin() + std::min<size_t>(th.size(), want), theta);
+    if (theta) {
+        const int64_t m = std::min(want, nevOut);
+        for (int64_t sg = 0; sg < NS; ++sg)
+            std::copy(th.begin() + sg * thStride, th.begin() + sg * thStride + m,
+                      theta + sg * nevOut);
+    }
     if (Z && vec)
         AMLS_HIP_CHECK(hipMemcpyAsync(Z, c.dXout, c.n * want * 8, hipMemcpyDeviceToHost, c.stream));
     c.sync();
     c.lastWant = want;
     c.lastVectors = vec;
-    if (p_out) *p_out = c.ptot;
+    if (p_out)
+        for (int64_t sg = 0; sg < NS; ++sg) p_out[sg] = c.segP[sg];
     if (timing) {
         timing[0] = secs(t0, t1);
         timing[1] = secs(t1, t2);
@@ -2118,6 +2280,26 @@ amls_status amls_hip_solve(amls_ctx* ctx, int64_t n, const int64_t* Kp, const in
         AMLS_HIP_CHECK(hipSetDevice(c.device));
         const auto t0 = amls::Clock::now();
         c.setProblem(n, Kp, Ki, Kx, Mp, Mi, Mx, coords, opt->nLevels, opt->leafSize, opt->shift);
+        const auto t1 = amls::Clock::now();
+        amls::solveNumeric(c, *opt, theta, Z, p_out, timing, t0, t1);
+    });
+}
+
+amls_status amls_hip_solve_batch(amls_ctx* ctx, int64_t count, const int64_t* n,
+                                 const int64_t* const* Kp, const int64_t* const* Ki,
+                                 const double* const* Kx, const int64_t* const* Mp,
+                                 const int64_t* const* Mi, const double* const* Mx,
+                                 const double* const* coords, const amls_options* opt,
+                                 double* theta, double* Z, int64_t* p_out, double* timing) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        using amls::StatusError;
+        if (!opt) throw StatusError(AMLS_E_ARG, "null options");
+        if (opt->nev <= 0) throw StatusError(AMLS_E_ARG, "nev must be positive");
+        c.requireGpu();
+        AMLS_HIP_CHECK(hipSetDevice(c.device));
+        const auto t0 = amls::Clock::now();
+        c.setProblems(batchInputs(count, n, Kp, Ki, Kx, Mp, Mi, Mx, coords), opt->nLevels,
+                      opt->leafSize, opt->shift);
         const auto t1 = amls::Clock::now();
         amls::solveNumeric(c, *opt, theta, Z, p_out, timing, t0, t1);
     });
@@ -2252,8 +2434,15 @@ amls_status amls_hip_get_reduced(amls_ctx* ctx, double* Mt, int64_t* moff) {
     return guarded(ctx, [&](amls::Ctx& c) {
         c.requireGpu();
         if (!c.projected) throw amls::StatusError(AMLS_E_STATE, "project not run");
-        if (Mt)
-            AMLS_HIP_CHECK(hipMemcpyAsync(Mt, c.dMt, c.ptot * c.ptot * 8, hipMemcpyDeviceToHost, c.stream));
+        if (Mt) {
+            // one p_s x p_s block per problem, on the diagonal of the p x p output
+            std::fill(Mt, Mt + c.ptot * c.ptot, 0.0);
+            for (int64_t sg = 0; sg < c.nseg; ++sg) {
+                const int64_t P = c.segP[sg], m0 = c.segMoff[sg];
+                AMLS_HIP_CHECK(hipMemcpy2DAsync(Mt + m0 + m0 * c.ptot, c.ptot * 8, c.dMt + c.mto[sg],
+                                                P * 8, P * 8, P, hipMemcpyDeviceToHost, c.stream));
+            }
+        }
         if (moff) std::copy(c.moff.begin(), c.moff.end() - 1, moff);
         c.sync();
     });

@@ -122,11 +122,14 @@ struct Ctx {
     double shift = 0.0;
     std::vector<int64_t> Kp, Ki, Mp, Mi;
     std::vector<double> Kx, Mx;
-    Graph g;
-    PartitionResult part;
     Symbolic S;
     std::vector<int64_t> owner;
     bool haveProblem = false;
+    // independent problems solved together (a forest): problem s owns rows
+    // [segRow[s], segRow[s+1]) in both orderings and post-order nodes
+    // [segNode[s], segNode[s+1]); one problem is the case nseg = 1
+    int64_t nseg = 1;
+    std::vector<int64_t> segRow, segNode, nodeSeg;
 
     // ---- problem (device)
     std::unique_ptr<DevArena> probMem;
@@ -135,6 +138,7 @@ struct Ctx {
     int64_t *dsrcK = nullptr, *dtgtK = nullptr, *dsrcM = nullptr, *dtgtM = nullptr;
     int64_t nasmK = 0, nasmM = 0;
     int64_t *dperm = nullptr, *dbidx = nullptr, *drel = nullptr;
+    int64_t* dSegRow = nullptr;   // segRow on the device
     // permuted CSR of K and M (for the subspace iteration)
     int64_t *dPKp = nullptr, *dPKi = nullptr, *dPMp = nullptr, *dPMi = nullptr;
     double *dPKx = nullptr, *dPMx = nullptr;
@@ -164,6 +168,7 @@ struct Ctx {
     std::unique_ptr<Blob> passBlob;
     std::vector<int64_t> k, moff, mstart;   // per node
     int64_t ptot = 0;
+    std::vector<int64_t> segMoff, segP, mto;   // per problem: first mode, p, Mtilde offset
     std::vector<int64_t> zo, ro;            // Z and R-front offsets
     double *dLam = nullptr, *dY = nullptr, *dZ = nullptr, *dMu = nullptr, *dSw = nullptr;
     double *dR = nullptr, *dMt = nullptr;
@@ -176,7 +181,7 @@ struct Ctx {
     std::unique_ptr<Blob> redBlob;
     int64_t want = 0;
     double* dQ = nullptr;
-    std::vector<double> theta;
+    std::vector<double> theta;              // nseg x want, problem-major
     bool reduced = false, redVectors = false;
 
     // ---- recovered vectors (permuted order, n x want) and output
@@ -197,12 +202,29 @@ struct Ctx {
     KernelStats kstats;
 
     // phases
+    struct ProblemIn {
+        int64_t n;
+        const int64_t *Kp, *Ki;
+        const double* Kx;
+        const int64_t *Mp, *Mi;
+        const double* Mx;
+        const double* coords;
+    };
+    void setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t leafSize,
+                     double shift);
     void setProblem(int64_t n, const int64_t* Kp, const int64_t* Ki, const double* Kx,
                     const int64_t* Mp, const int64_t* Mi, const double* Mx, const double* coords,
-                    int nLevels, int64_t leafSize, double shift);
+                    int nLevels, int64_t leafSize, double shift) {
+        setProblems({ProblemIn{n, Kp, Ki, Kx, Mp, Mi, Mx, coords}}, nLevels, leafSize, shift);
+    }
     void runGE(int prec);
     void spectralPrepare();
-    void spectralSelect(int kind, double value);
+    // kind 0: keep min(value, n_i) modes per node (value <= 0 keeps all);
+    // kind 1: keep modes with w <= value; one value per problem
+    void spectralSelect(int kind, const std::vector<double>& valuePerSeg);
+    void spectralSelect(int kind, double value) {
+        spectralSelect(kind, std::vector<double>(static_cast<size_t>(nseg), value));
+    }
     void project(int prec);
     void reducedSolve(int64_t want, bool vectors);
     void backtransform();

@@ -474,4 +474,39 @@ Symbolic analyze(const PartitionResult& part, const Graph& g) {
     return S;
 }
 
+Symbolic mergeSymbolic(const std::vector<Symbolic>& parts) {
+    if (parts.size() == 1) return parts[0];
+    Symbolic S;
+    int64_t rowBase = 0, nodeBase = 0, bBase = 0;
+    for (const Symbolic& P : parts) {
+        S.maxLevel = std::max(S.maxLevel, P.maxLevel);
+        for (int64_t p = 0; p < P.nNodes; ++p) {
+            S.treeId.push_back(P.treeId[p] + nodeBase);
+            S.off.push_back(P.off[p] + rowBase);
+            S.size.push_back(P.size[p]);
+            S.parent.push_back(P.parent[p] < 0 ? -1 : P.parent[p] + nodeBase);
+            S.child0.push_back(P.child0[p] < 0 ? -1 : P.child0[p] + nodeBase);
+            S.child1.push_back(P.child1[p] < 0 ? -1 : P.child1[p] + nodeBase);
+            S.level.push_back(P.level[p]);
+            S.subStart.push_back(P.subStart[p] + nodeBase);
+            S.bptr.push_back(P.bptr[p] + bBase);
+        }
+        for (int64_t j : P.bidx) S.bidx.push_back(j + rowBase);
+        S.rel.insert(S.rel.end(), P.rel.begin(), P.rel.end());
+        for (int64_t j = 0; j < P.n; ++j) S.perm.push_back(P.perm[j] + rowBase);
+        rowBase += P.n;
+        nodeBase += P.nNodes;
+        bBase += static_cast<int64_t>(P.bidx.size());
+    }
+    S.bptr.push_back(bBase);
+    S.n = rowBase;
+    S.nNodes = nodeBase;
+    S.relptr = S.bptr;
+    S.iperm.assign(static_cast<size_t>(S.n), 0);
+    for (int64_t j = 0; j < S.n; ++j) S.iperm[S.perm[j]] = j;
+    S.byLevel.assign(static_cast<size_t>(S.maxLevel) + 1, {});
+    for (int64_t p = 0; p < S.nNodes; ++p) S.byLevel[S.level[p]].push_back(p);
+    return S;
+}
+
 }  // namespace amls

@@ -98,4 +98,11 @@ struct Symbolic {
 
 Symbolic analyze(const PartitionResult& part, const Graph& g);
 
+// Forest of independent problems: the symbolic structures of parts[0..m)
+// concatenated in order (part i's rows and nodes follow those of part i-1,
+// both in the original and in the permuted ordering).  Levels keep their
+// meaning (0 = leaves), so nodes of equal height in different problems share
+// a level; every part's root has parent -1.
+Symbolic mergeSymbolic(const std::vector<Symbolic>& parts);
+
 }  // namespace amls

@@ -78,6 +78,32 @@ __global__ void __launch_bounds__(256) coldot_kernel(int64_t n, const double* __
     if (threadIdx.x == 0) out[c] = red[0] + red[1] + red[2] + red[3];
 }
 
+// Per problem segment: out[s + c nseg] = sum over rows [seg[s], seg[s+1]) of
+// x(r, c) y(r, c).  One block per (segment, column).
+__global__ void __launch_bounds__(256) coldot_seg_kernel(const int64_t* __restrict__ seg, int nseg,
+                                                         const double* __restrict__ x,
+                                                         const double* __restrict__ y, int64_t ld,
+                                                         double* __restrict__ out) {
+    __shared__ double red[4];
+    const int sg = blockIdx.x, c = blockIdx.y;
+    const int64_t r0 = seg[sg], r1 = seg[sg + 1];
+    double s = 0.0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) s += x[r + c * ld] * y[r + c * ld];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[sg + (int64_t)c * nseg] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void __launch_bounds__(256) colscale_seg_kernel(const int64_t* __restrict__ seg,
+                                                           int nseg, double* __restrict__ x,
+                                                           int64_t ld,
+                                                           const double* __restrict__ nrm) {
+    const int sg = blockIdx.x, c = blockIdx.y;
+    const double sc = 1.0 / sqrt(fmax(nrm[sg + (int64_t)c * nseg], 1e-300));
+    for (int64_t r = seg[sg] + threadIdx.x; r < seg[sg + 1]; r += 256) x[r + c * ld] *= sc;
+}
+
 __global__ void colscale_kernel(int64_t n, double* __restrict__ x, int64_t ld,
                                 const double* __restrict__ nrm) {
     const int c = blockIdx.y;
@@ -114,6 +140,15 @@ void launch_coldot(int64_t n, const double* x, const double* y, int ncols, int64
                    hipStream_t s) {
     if (ncols <= 0) return;
     hipLaunchKernelGGL(dev::coldot_kernel, dim3(ncols), dim3(256), 0, s, n, x, y, ld, out);
+}
+
+void launch_colnorm_seg(const int64_t* seg, int nseg, double* x, const double* y, int ncols,
+                        int64_t ld, double* nrm, hipStream_t s) {
+    if (ncols <= 0 || nseg <= 0) return;
+    hipLaunchKernelGGL(dev::coldot_seg_kernel, dim3(nseg, ncols), dim3(256), 0, s, seg, nseg, x, y,
+                       ld, nrm);
+    hipLaunchKernelGGL(dev::colscale_seg_kernel, dim3(nseg, ncols), dim3(256), 0, s, seg, nseg, x,
+                       ld, nrm);
 }
 
 void launch_colscale_rsqrt(int64_t n, double* x, int ncols, int64_t ld, const double* nrm,

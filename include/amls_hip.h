@@ -88,6 +88,29 @@ amls_status amls_hip_set_problem(amls_ctx* ctx, int64_t n, const int64_t* Kcolpt
                                  const double* Mval, const double* coords, int32_t nLevels,
                                  int64_t leafSize, double shift);
 
+/*
+ * A batch of independent pencils solved together (config C4: many small
+ * problems).  Problem s is given by the s-th entry of each pointer array;
+ * coords may be null, or hold null entries.  Each problem is partitioned on
+ * its own; the trees form one forest whose nodes of equal height share every
+ * level-batched launch, and the reduced problems, the subspace iteration's
+ * dense steps and the normalisation run once per problem, batched.  Rows of
+ * problem s occupy [rowStart[s], rowStart[s+1]) of the concatenated ordering.
+ * The reference solves one pencil per AmlsSolver::solve call; this entry has
+ * no counterpart there.
+ */
+amls_status amls_hip_set_problem_batch(amls_ctx* ctx, int64_t count, const int64_t* n,
+                                       const int64_t* const* Kcolptr,
+                                       const int64_t* const* Krowidx, const double* const* Kval,
+                                       const int64_t* const* Mcolptr,
+                                       const int64_t* const* Mrowidx, const double* const* Mval,
+                                       const double* const* coords, int32_t nLevels,
+                                       int64_t leafSize, double shift);
+/* count problems; rowStart (count+1) and nodeStart (count+1, post-order
+ * positions of the forest) may be null. */
+amls_status amls_hip_batch_info(const amls_ctx* ctx, int64_t* count, int64_t* rowStart,
+                                int64_t* nodeStart);
+
 /* Tree in post-order positions p = 0..nNodes-1 (the elimination order of
  * SeparatorTree::postOrder, src/amls/separator_tree.cpp:33-38). */
 amls_status amls_hip_tree_info(const amls_ctx* ctx, int64_t* nNodes, int64_t* maxLevel,
@@ -179,6 +202,26 @@ amls_status amls_hip_solve(amls_ctx* ctx, int64_t n, const int64_t* Kcolptr,
  */
 amls_status amls_hip_solve_numeric(amls_ctx* ctx, const amls_options* opt, double* theta,
                                    double* Z, int64_t* p_out, double* timing);
+
+/*
+ * amls_hip_set_problem_batch + amls_hip_solve_numeric in one call.  For a
+ * batch of `count` problems (here and in amls_hip_solve_numeric after
+ * amls_hip_set_problem_batch): theta holds count x nev values, problem-major
+ * (theta[s * nev + j]); Z is (sum n_s) x nev, column-major, the rows of
+ * problem s at rowStart[s]; p_out receives count reduced sizes; timing as
+ * above (timing[7] = total pass-A size).  The subspace iteration of a batch
+ * starts from the lowest q Ritz vectors of every problem (siVectors is capped
+ * at the smallest p_s).  In a batch, amls_hip_spectral applies its value to
+ * every problem, amls_hip_project returns the total p, amls_hip_reduced_solve
+ * returns count x want values and amls_hip_get_reduced the block-diagonal
+ * Mtilde.
+ */
+amls_status amls_hip_solve_batch(amls_ctx* ctx, int64_t count, const int64_t* n,
+                                 const int64_t* const* Kcolptr, const int64_t* const* Krowidx,
+                                 const double* const* Kval, const int64_t* const* Mcolptr,
+                                 const int64_t* const* Mrowidx, const double* const* Mval,
+                                 const double* const* coords, const amls_options* opt,
+                                 double* theta, double* Z, int64_t* p_out, double* timing);
 
 /*
  * Eigenvalue sensitivities with respect to per-element stiffness and density

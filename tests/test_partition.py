@@ -93,3 +93,28 @@ def test_plane_separators_on_hex27():
     # root separator = one 21 x 20 node plane, 3 dofs per node
     assert t["size"][-1] == 1260
     assert t["maxLevel"] == 6
+
+
+def test_batch_forest_equals_single_trees():
+    """A batch of independent pencils is partitioned problem by problem: the
+    forest's nodes, sizes, boundaries and permutation are the single trees',
+    shifted by each problem's row and node offsets."""
+    from amls_amd import Pencil
+    rng = np.random.default_rng(2)
+    pbs = [cases.hex_elasticity(1, *(int(v) for v in rng.integers(3, 7, 3))) for _ in range(4)]
+    c = HipContext(-1)
+    c.set_problem_batch([Pencil(p.K, p.M, p.coords) for p in pbs], 3, 40)
+    t = c.tree()
+    rows, nodes = c.batch_info()
+    c.close()
+    assert rows[-1] == sum(p.n for p in pbs) and len(t["size"]) == nodes[-1]
+    for i, pb in enumerate(pbs):
+        s = analyse(pb.K, pb.M, pb.coords, 3, 40)
+        a, b = nodes[i], nodes[i + 1]
+        assert np.array_equal(t["size"][a:b], s["size"])
+        assert np.array_equal(t["offset"][a:b] - rows[i], s["offset"])
+        assert np.array_equal(t["bsize"][a:b], s["bsize"])
+        assert np.array_equal(t["level"][a:b], s["level"])
+        par = t["parent"][a:b]
+        assert np.array_equal(np.where(par >= 0, par - a, -1), s["parent"])
+        assert np.array_equal(t["perm"][rows[i]:rows[i + 1]] - rows[i], s["perm"])
