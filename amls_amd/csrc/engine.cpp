@@ -1494,8 +1494,8 @@ void addDenseLdlt(Plan& plan, const std::vector<double*>& As, int64_t n, int64_t
 
 // ---------------------------------------------------------------------------
 // Subspace iteration with the AMLS block factors (AMLS-SIMb): Q <- K^-1 M Q
-// through L, Khat and L^T; FP64 M-orthonormalisation between steps; one FP64
-// Rayleigh-Ritz step with the original K and M at the end.
+// through L, Khat and L^T; columns scaled to unit M-norm between steps; one
+// FP64 Rayleigh-Ritz step with the original K and M at the end.
 //
 // fromBasis = true starts from the whole AMLS subspace, Q0 = L^-T blockdiag(Z)
 // (q = p, no reduced eigenvectors needed); otherwise from the q recovered
@@ -1540,7 +1540,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     A.plan(to[N] * 8);                     // fronts of the forward solve
     A.plan(zbLevel * 8);                   // gathered x_B of one level
     const int64_t NS = nseg;               // one q x q problem per segment
-    A.plan(NS * q * q * 8 * 5);            // Kc, Mc, C, T, W
+    A.plan(NS * q * q * 8 * 4);            // Kc, Mc, C, T
     A.plan(NS * q * 64 * 8);               // panel scratch
     A.plan(NS * q * 8 * 13);               // eig vectors
     A.plan(NS * q * nev * 8 * 2);          // Yt, V
@@ -1559,7 +1559,6 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     double* Mc = A.take<double>(NS * q * q);
     double* Cm = A.take<double>(NS * q * q);
     double* Tm = A.take<double>(NS * q * q);
-    double* Wq = A.take<double>(NS * q * q);
     double* Wp = A.take<double>(NS * q * 64);
     double* d = A.take<double>(NS * q);
     double* e = A.take<double>(NS * q);
@@ -1645,11 +1644,14 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
 
     for (int it = 0; it < steps; ++it) {
         const bool last = it + 1 == steps;
-        // ---- forward: fronts take their own rows of R, children's updates
+        // ---- forward: fronts take their own rows of M Q (R in the first
+        // step, the scaled M Y of the previous step after that), children's
+        // updates
+        const double* src = it == 0 ? R : MY;
         plan.fn([=](hipStream_t s) { launch_fill(T, to[S.nNodes], 0.0, s); });
         Copy2dBatch cown;
         for (int64_t p = 0; p < N; ++p)
-            cown.add(cdesc(R + S.off[p], n, T + to[p], S.f(p), S.size[p], q, 0));
+            cown.add(cdesc(src + S.off[p], n, T + to[p], S.f(p), S.size[p], q, 0));
         plan.copy(cown);
         for (int64_t l = 0; l <= S.maxLevel; ++l) {
             const auto& nodes = S.byLevel[l];
@@ -1731,69 +1733,16 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
         plan.copy(cw);
         addBacksolve(Yv);
         addSpmm(plan, n, dPMp, dPMi, dPMx, nnzM, Yv, MY, q, Tr1, Tr2);   // MY = M Y
-        if (!last) {
-            // M-orthonormalise each problem's block: Y^T M Y = L D L^T,
-            // W = L^-T D^-1/2, Q = Y W and M Q = (M Y) W
-            GemmBatch gg;
-            for (int64_t sg = 0; sg < NS; ++sg) {
-                const auto [r0, ns] = rowsOf(sg);
-                GemmDesc g = gdesc();
-                g.A = Yv + r0;
-                g.lda = n;
-                g.ta = 1;
-                g.B = MY + r0;
-                g.ldb = n;
-                g.k = static_cast<int32_t>(ns);
-                g.C = McS[sg];
-                g.ldc = q;
-                g.m = g.n = qi;
-                g.lower = 1;
-                gg.add(g);
-            }
-            plan.gemm(gg, false);
-            addDenseLdlt(plan, McS, q, q, WpS, err, -2);
-            std::vector<VecDesc> vd(NS), vdg(NS);
-            Copy2dBatch cd;
-            TrsmBatch tw;
-            GemmBatch gq;
-            for (int64_t sg = 0; sg < NS; ++sg) {
-                const auto [r0, ns] = rowsOf(sg);
-                std::memset(&vd[sg], 0, sizeof(VecDesc));
-                vd[sg].src = McS[sg];
-                vd[sg].dst = dsq + sg * q;
-                vd[sg].ld = q;
-                vd[sg].n = qi;
-                vd[sg].op = 0;
-                std::memset(&vdg[sg], 0, sizeof(VecDesc));
-                vdg[sg].dst = Wq + sg * qq;
-                vdg[sg].ld = q;
-                vdg[sg].n = qi;
-                vdg[sg].op = 2;                         // W = I
-                cd.add(cdesc(Wq + sg * qq, q, Wq + sg * qq, q, q, q, 0, dsq + sg * q));
-                tw.add(McS[sg], q, Wq + sg * qq, q, q, q, 1);   // W = L^-T D^-1/2
-                GemmDesc a = gdesc();
-                a.A = Yv + r0;
-                a.lda = n;
-                a.B = Wq + sg * qq;
-                a.ldb = q;
-                a.k = qi;
-                a.C = Q + r0;
-                a.ldc = n;
-                a.m = static_cast<int32_t>(ns);
-                a.n = qi;
-                gq.add(a);
-                GemmDesc b = a;
-                b.A = MY + r0;
-                b.C = R + r0;
-                gq.add(b);
-            }
-            plan.vec(vd);
-            plan.fn([=](hipStream_t s) { launch_fill(Wq, NS * q * q, 0.0, s); });
-            plan.vec(vdg);
-            plan.copy(cd);                              // W = D^-1/2
-            plan.trsm(tw);
-            plan.gemm(gq, false);
-        }
+        // scale every column of Y and M Y to unit M-norm (per problem): the
+        // next step applies K^-1 to (M Y) S, the Rayleigh-Ritz at the end
+        // projects onto Y S.  A full M-orthonormalisation is not needed: it
+        // only changes the basis of the same subspace, and with unit columns
+        // the projected pencil stays well conditioned for a few steps.
+        (void)last;
+        const int64_t* seg = dSegRow;
+        const int nsg = static_cast<int>(NS);
+        double* nrmS = dsq;
+        plan.fn([=](hipStream_t s) { launch_colnorm_seg(seg, nsg, MY, Yv, qi, nn, nrmS, s, true); });
     }
     // ---- final Rayleigh-Ritz with the original FP64 K and M, per problem:
     // Kc = Y^T K Y, Mc = Y^T M Y; lowest nev pairs of (Kc, Mc) through
@@ -1814,10 +1763,19 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
             g1.ldc = q;
             g1.m = g1.n = qi;
             g1.lower = 1;
-            gr.add(g1);
             GemmDesc g2 = g1;
             g2.B = MY + r0;
             g2.C = McS[sg];
+            if (shift != 0.0) {            // (K + shift M): Kc += shift Y^T M Y
+                g1.A2 = Yv + r0;
+                g1.lda2 = n;
+                g1.ta2 = 1;
+                g1.B2 = MY + r0;
+                g1.ldb2 = n;
+                g1.k2 = static_cast<int32_t>(ns);
+                g1.alpha2 = shift;
+            }
+            gr.add(g1);
             gr.add(g2);
         }
         plan.gemm(gr, false);
@@ -1915,10 +1873,18 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     plan.fn([=](hipStream_t s) { launch_permute_rows(nn, perm, Q, xo_, nv, nn, s); });
     addSpmm(plan, n, mp, mi, mx, static_cast<double>(Mp[n]), xo_, MY, nev, Tr1, Tr2);
     plan.fn([=](hipStream_t s) { launch_colnorm_seg(seg, nsg, xo_, MY, nv, nn, nrm, s); });
+    // theta_j = x_j^T K x_j of the M-normalised vectors: the Ritz value
+    // restated as the Rayleigh quotient of the vector actually returned, so
+    // the pair stays consistent to rounding however the projected pencil
+    // was conditioned
+    addSpmm(plan, n, dKp, dKi, dKx, nnzK, xo_, MY, nev, Tr1, Tr2);
+    plan.fn([=](hipStream_t s) { launch_coldot_seg(seg, nsg, xo_, MY, nv, nn, th, s); });
     plan.run(stream);
     thetaOut.assign(NS * nev, 0.0);
     int ecode = 0;
     AMLS_HIP_CHECK(hipMemcpyAsync(thetaOut.data(), th, NS * nev * 8, hipMemcpyDeviceToHost, stream));
+    sync();
+    for (double& t : thetaOut) t += shift;   // x^T (K + shift M) x with x^T M x = 1
     AMLS_HIP_CHECK(hipMemcpyAsync(&ecode, err, sizeof(int), hipMemcpyDeviceToHost, stream));
     sync();
     if (ecode != INT_MAX)

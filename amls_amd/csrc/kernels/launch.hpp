@@ -43,9 +43,13 @@ void launch_spmm_rm(int64_t n, const int64_t* rowptr, const int64_t* colidx, con
 void launch_coldot(int64_t n, const double* x, const double* y, int ncols, int64_t ld, double* out,
                    hipStream_t s);
 // M-normalisation per problem segment: nrm(s, c) = x_s(:, c)^T y_s(:, c) over
-// rows [seg[s], seg[s+1]), then x_s(:, c) /= sqrt(nrm(s, c)); nrm is nseg x ncols
-void launch_colnorm_seg(const int64_t* seg, int nseg, double* x, const double* y, int ncols,
-                        int64_t ld, double* nrm, hipStream_t s);
+// rows [seg[s], seg[s+1]), then x_s(:, c) /= sqrt(nrm(s, c)) (and y_s(:, c)
+// too if scaleY); nrm is nseg x ncols
+void launch_colnorm_seg(const int64_t* seg, int nseg, double* x, double* y, int ncols,
+                        int64_t ld, double* nrm, hipStream_t s, bool scaleY = false);
+// out[s ncols + c] = x_s(:, c)^T y_s(:, c) over rows [seg[s], seg[s+1])
+void launch_coldot_seg(const int64_t* seg, int nseg, const double* x, const double* y, int ncols,
+                       int64_t ld, double* out, hipStream_t s);
 // x(r, c) *= 1 / sqrt(max(nrm[c], tiny))
 void launch_colscale_rsqrt(int64_t n, double* x, int ncols, int64_t ld, const double* nrm,
                            hipStream_t s);

@@ -40,28 +40,51 @@ __global__ void __launch_bounds__(256) spmm_kernel(int64_t n, const int64_t* __r
     }
 }
 
+constexpr unsigned NXCD = 8;   // MI355X: 8 XCDs, workgroups dealt round-robin
+
 // Row-major operands: Y(r, :) = sum_e val[e] X(col[e], :), one wave per
 // (row, 64-column chunk), lanes over contiguous columns so every nonzero is
 // one coalesced 512-byte read of X.  Chunks are the slow grid dimension, so
-// the whole GPU streams through one 64-column slab of X at a time.
+// the whole GPU streams through one 64-column slab of X at a time.  The grid
+// (a multiple of 8 in x) is remapped so that each XCD takes one contiguous
+// eighth of the rows: the X rows its rows touch stay in that XCD's L2 instead
+// of being fetched by all eight.
 __global__ void __launch_bounds__(256) spmm_rm_kernel(int64_t n, const int64_t* __restrict__ rowptr,
                                                       const int64_t* __restrict__ colidx,
                                                       const double* __restrict__ vals,
                                                       const double* __restrict__ x,
                                                       double* __restrict__ y, int ncols) {
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const unsigned per = gridDim.x / NXCD;
+    const unsigned blk = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+    const int64_t row = (int64_t)blk * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
     const int c = blockIdx.y * 64 + lane;
     const bool on = c < ncols;
+    const int cc = on ? c : 0;   // in-bounds column for the loads of idle lanes
     const int64_t b = rowptr[row], e = rowptr[row + 1];
-    double acc = 0.0;
-    for (int64_t k = b; k < e; ++k) {
-        const double a = vals[k];
-        const int64_t col = colidx[k];
-        if (on) acc += a * x[col * ncols + c];
+    // eight nonzeros per round: their indices, then their X rows, are all in
+    // flight together instead of one dependent round trip per nonzero
+    double acc0 = 0.0, acc1 = 0.0;
+    int64_t k = b;
+    for (; k + 8 <= e; k += 8) {
+        int64_t col[8];
+        double a[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            col[u] = colidx[k + u];
+            a[u] = vals[k + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = x[col[u] * ncols + cc];
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+            acc0 += a[u] * xv[u];
+            acc1 += a[u + 1] * xv[u + 1];
+        }
     }
-    if (on) y[row * ncols + c] = acc;
+    for (; k < e; ++k) acc0 += vals[k] * x[colidx[k] * ncols + cc];
+    if (on) y[row * ncols + c] = acc0 + acc1;
 }
 
 // One block per column.
@@ -80,10 +103,11 @@ __global__ void __launch_bounds__(256) coldot_kernel(int64_t n, const double* __
 
 // Per problem segment: out[s + c nseg] = sum over rows [seg[s], seg[s+1]) of
 // x(r, c) y(r, c).  One block per (segment, column).
+// segMajor = 0: out[s + c nseg]; 1: out[s ncols + c]
 __global__ void __launch_bounds__(256) coldot_seg_kernel(const int64_t* __restrict__ seg, int nseg,
                                                          const double* __restrict__ x,
                                                          const double* __restrict__ y, int64_t ld,
-                                                         double* __restrict__ out) {
+                                                         double* __restrict__ out, int segMajor) {
     __shared__ double red[4];
     const int sg = blockIdx.x, c = blockIdx.y;
     const int64_t r0 = seg[sg], r1 = seg[sg + 1];
@@ -92,7 +116,9 @@ __global__ void __launch_bounds__(256) coldot_seg_kernel(const int64_t* __restri
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) out[sg + (int64_t)c * nseg] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0)
+        out[segMajor ? (int64_t)sg * gridDim.y + c : sg + (int64_t)c * nseg] =
+            red[0] + red[1] + red[2] + red[3];
 }
 
 __global__ void __launch_bounds__(256) colscale_seg_kernel(const int64_t* __restrict__ seg,
@@ -132,8 +158,10 @@ void launch_spmm(int64_t n, const int64_t* rowptr, const int64_t* colidx, const 
 void launch_spmm_rm(int64_t n, const int64_t* rowptr, const int64_t* colidx, const double* vals,
                     const double* x, double* y, int ncols, hipStream_t s) {
     if (n <= 0 || ncols <= 0) return;
-    hipLaunchKernelGGL(dev::spmm_rm_kernel, dim3((unsigned)((n + 3) / 4), (unsigned)((ncols + 63) / 64)),
-                       dim3(256), 0, s, n, rowptr, colidx, vals, x, y, ncols);
+    const unsigned nb = (unsigned)((n + 3) / 4);
+    const unsigned nbx = (nb + dev::NXCD - 1) / dev::NXCD * dev::NXCD;
+    hipLaunchKernelGGL(dev::spmm_rm_kernel, dim3(nbx, (unsigned)((ncols + 63) / 64)), dim3(256), 0, s,
+                       n, rowptr, colidx, vals, x, y, ncols);
 }
 
 void launch_coldot(int64_t n, const double* x, const double* y, int ncols, int64_t ld, double* out,
@@ -142,13 +170,23 @@ void launch_coldot(int64_t n, const double* x, const double* y, int ncols, int64
     hipLaunchKernelGGL(dev::coldot_kernel, dim3(ncols), dim3(256), 0, s, n, x, y, ld, out);
 }
 
-void launch_colnorm_seg(const int64_t* seg, int nseg, double* x, const double* y, int ncols,
-                        int64_t ld, double* nrm, hipStream_t s) {
+void launch_colnorm_seg(const int64_t* seg, int nseg, double* x, double* y, int ncols,
+                        int64_t ld, double* nrm, hipStream_t s, bool scaleY) {
     if (ncols <= 0 || nseg <= 0) return;
     hipLaunchKernelGGL(dev::coldot_seg_kernel, dim3(nseg, ncols), dim3(256), 0, s, seg, nseg, x, y,
-                       ld, nrm);
+                       ld, nrm, 0);
     hipLaunchKernelGGL(dev::colscale_seg_kernel, dim3(nseg, ncols), dim3(256), 0, s, seg, nseg, x,
                        ld, nrm);
+    if (scaleY)
+        hipLaunchKernelGGL(dev::colscale_seg_kernel, dim3(nseg, ncols), dim3(256), 0, s, seg, nseg,
+                           y, ld, nrm);
+}
+
+void launch_coldot_seg(const int64_t* seg, int nseg, const double* x, const double* y, int ncols,
+                       int64_t ld, double* out, hipStream_t s) {
+    if (ncols <= 0 || nseg <= 0) return;
+    hipLaunchKernelGGL(dev::coldot_seg_kernel, dim3(nseg, ncols), dim3(256), 0, s, seg, nseg, x, y,
+                       ld, out, 1);
 }
 
 void launch_colscale_rsqrt(int64_t n, double* x, int ncols, int64_t ld, const double* nrm,

@@ -164,9 +164,9 @@ amls_status amls_hip_backtransform(amls_ctx* ctx, double* Z);
 
 /*
  * Subspace iteration with the AMLS block factors as the inverse
- * (Q <- K^-1 M Q through L, Khat, L^T): `steps` applications, an FP64
- * M-orthonormalisation between them, then one FP64 Rayleigh-Ritz with the
- * original K and M that keeps the lowest nev pairs.  The q start columns are
+ * (Q <- K^-1 M Q through L, Khat, L^T): `steps` applications with the
+ * columns scaled to unit M-norm between them, then one FP64 Rayleigh-Ritz
+ * with the original K and M that keeps the lowest nev pairs.  The q start columns are
  * the lowest q Ritz vectors of the last reduced solve.  theta (nev) and
  * Z (n x nev, M-normalised) to the host if non-null.  No counterpart in the
  * reference's sources (README.md:14 claims it); algorithm AMLS-SIMb of

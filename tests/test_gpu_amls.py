@@ -193,3 +193,27 @@ def test_subspace_iteration_matches_oracle(blas1, prec, qmode):
     # iteration slightly differently (Rayleigh-Ritz stays exact FP64)
     slack = 1e-3 if prec == "fp64" else 0.1
     assert np.all(resid(sol.theta, sol.Z) <= resid(t_or, Z_or) * (1 + slack) + 1e-12)
+
+
+@pytest.mark.parametrize("si", [0, 2])
+def test_shift_matches_oracle(blas1, si):
+    """K <- K + shift M (src/amls/solver.cpp:114-122): the eigenvalues of the
+    shifted pencil are returned, the vectors are unchanged."""
+    pb = small_hex(seed=41, dims=(4, 4, 5))
+    tree, t = tree_of(pb.K, pb.M, pb.coords, 3, 40)
+    nev, shift = 8, 0.37
+    ref = O.amls_solve(pb.K, pb.M, tree, nev=nev, shift=shift)
+    sol = AmlsSolver(0).solve(Pencil(pb.K, pb.M, pb.coords),
+                              AmlsOptions(nev=nev, nLevels=3, leafSize=40, shift=shift,
+                                          siSteps=si, verbosity=0))
+    if si == 0:
+        assert np.abs(sol.theta - ref.theta).max() <= 1e-10 * ref.theta.max()
+    else:
+        exact = O.reference_eigs(pb.K, pb.M, nev) + shift
+        assert np.all(sol.theta >= exact * (1 - 1e-12))
+        assert O.rel_err(sol.theta, exact).max() < O.rel_err(ref.theta, exact).max()
+    if si > 0:
+        Ks = pb.K + shift * pb.M
+        r = Ks @ sol.Z - (pb.M @ sol.Z) * sol.theta
+        rel = np.linalg.norm(r, axis=0) / np.linalg.norm(Ks @ sol.Z, axis=0)
+        assert rel.max() <= 0.05
