@@ -41,6 +41,8 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // vector of column i) from the shared vectors, so one launch per step
 // suffices.  Work item (x = matrix, y = tile row, z = tile col, w = 1 for the
 // matrix's leader, which writes d, e, tau, v_i and the previous reflector).
+constexpr int SY_KR = 8;   // strided rows per thread held in registers (n - i <= 2048)
+
 __global__ void __launch_bounds__(256) sytrd_tile_kernel(const EigDesc* __restrict__ descs,
                                                          const int4* __restrict__ work, int i,
                                                          double* __restrict__ sv_all,
@@ -61,80 +63,128 @@ __global__ void __launch_bounds__(256) sytrd_tile_kernel(const EigDesc* __restri
     double* spw = sp_all + 3 * soff[wk.x] + (int64_t)(i % 3) * n;               // p_i
     double* spz = sp_all + 3 * soff[wk.x] + (int64_t)((i + 1) % 3) * n;         // p_{i+1}
     const bool hasPrev = i > 0;
+    const double* Ai = A + (int64_t)i * lda;   // column i (of A after update i-2)
+
+    // ---- every load that does not depend on this step's scalars is issued
+    // first, so the step costs one round trip to memory before its two
+    // block reductions: the strided rows r = i + t + 256 j of p_{i-1},
+    // v_{i-1} and column i, this workgroup's tile, and the tile's row / column
+    // entries of the same three vectors.
     const double tauP = hasPrev ? d.tau[i - 1] : 0.0;
-    // w_{i-1} = tau p + alpha v with alpha = -1/2 tau^2 (p.v)
+    const double vI = hasPrev ? svr[i] : 0.0, pI = hasPrev ? spr[i] : 0.0, aII = Ai[i];
+    const bool has1 = i + 1 < n;
+    const double v1 = (hasPrev && has1) ? svr[i + 1] : 0.0, p1 = (hasPrev && has1) ? spr[i + 1] : 0.0;
+    const double a1 = has1 ? Ai[i + 1] : 0.0;
+    double pr[SY_KR], vv[SY_KR], ac[SY_KR];
+#pragma unroll
+    for (int j = 0; j < SY_KR; ++j) {
+        const int r = i + t + 256 * j;
+        const bool in = r < n;
+        pr[j] = (hasPrev && in) ? spr[r] : 0.0;
+        vv[j] = (hasPrev && in) ? svr[r] : 0.0;
+        ac[j] = in ? Ai[r] : 0.0;
+    }
+    const int r0 = wk.y * 64, c0 = wk.z * 64;
+    const bool haveTile = wk.y >= 0 && n - i - 1 > 0;
+    double ta[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = t + 256 * k, rr = e & 63, cc = e >> 6;
+        const int r = r0 + rr, c = c0 + cc;
+        ta[k] = (haveTile && r < n && c > i && r >= c) ? A[r + (int64_t)c * lda] : 0.0;
+    }
+    // tile vectors: thread t < 64 takes row r0 + t, 64 <= t < 128 column c0 + t - 64
+    const int vidx = t < 64 ? r0 + t : c0 + t - 64;
+    const bool vok = haveTile && t < 128 && vidx > i && vidx < n;
+    const double vv_t = (vok && hasPrev) ? svr[vidx] : 0.0;
+    const double pv_t = (vok && hasPrev) ? spr[vidx] : 0.0;
+    const double av_t = vok ? Ai[vidx] : 0.0;
+
+    // ---- w_{i-1} = tau p + alpha v with alpha = -1/2 tau^2 (p.v)
     double alphaP = 0.0;
     if (hasPrev) {
         double part = 0.0;
-        for (int r = i + t; r < n; r += 256) part += spr[r] * svr[r];
+#pragma unroll
+        for (int j = 0; j < SY_KR; ++j) part += pr[j] * vv[j];
+        for (int r = i + t + 256 * SY_KR; r < n; r += 256) part += spr[r] * svr[r];
         alphaP = -0.5 * tauP * tauP * block_sum(part, red);
     }
-    auto wprev = [&](int r) { return hasPrev ? tauP * spr[r] + alphaP * svr[r] : 0.0; };
-    auto vprev = [&](int r) { return hasPrev ? svr[r] : 0.0; };
-    const double vi = vprev(i), wi = wprev(i);
+    const double wI = tauP * pI + alphaP * vI;
+    // column i below the diagonal after update i-1: x_r = a_r - (v_r w_i + w_r v_i)
+    auto xof = [&](double a, double v, double p) { return a - (v * wI + (tauP * p + alphaP * v) * vI); };
     const int m = n - i - 1;
     if (m <= 0) {
-        if (wk.w && t == 0) d.d[i] = A[i + (int64_t)i * lda] - 2.0 * vi * wi;
+        if (wk.w && t == 0) d.d[i] = aII - 2.0 * vI * wI;
         return;
     }
-    // column i below the diagonal after update i-1
-    auto xcol = [&](int r) {
-        return A[r + (int64_t)i * lda] - (vprev(r) * wi + wprev(r) * vi);
-    };
     double part = 0.0;
-    for (int r = i + 2 + t; r < n; r += 256) {
-        const double x = xcol(r);
+#pragma unroll
+    for (int j = 0; j < SY_KR; ++j) {
+        const int r = i + t + 256 * j;
+        if (r >= i + 2 && r < n) {
+            const double x = xof(ac[j], vv[j], pr[j]);
+            part += x * x;
+        }
+    }
+    for (int r = i + t + 256 * SY_KR; r < n; r += 256) {
+        const double x = xof(Ai[r], hasPrev ? svr[r] : 0.0, hasPrev ? spr[r] : 0.0);
         part += x * x;
     }
     const double xnorm2 = block_sum(part, red);
-    const double alpha = xcol(i + 1);
+    const double alpha = xof(a1, v1, p1);
     double tau = 0.0, beta = alpha, scal = 0.0;
     if (xnorm2 > 0.0) {
         beta = -copysign(sqrt(alpha * alpha + xnorm2), alpha);
         tau = (beta - alpha) / beta;
         scal = 1.0 / (alpha - beta);
     }
-    auto vcur = [&](int r) { return r == i + 1 ? 1.0 : xcol(r) * scal; };
     if (wk.w) {
+        // leader: d, e, tau, v_i, clear p_{i+1}, store the reflector of step i-1
         if (t == 0) {
-            d.d[i] = A[i + (int64_t)i * lda] - 2.0 * vi * wi;
+            d.d[i] = aII - 2.0 * vI * wI;
             d.e[i] = beta;
             d.tau[i] = tau;
             svw[i] = 0.0;
         }
-        for (int r = i + 1 + t; r < n; r += 256) {
-            svw[r] = vcur(r);
+#pragma unroll
+        for (int j = 0; j < SY_KR; ++j) {
+            const int r = i + t + 256 * j;
+            if (r >= i + 1 && r < n) {
+                svw[r] = r == i + 1 ? 1.0 : xof(ac[j], vv[j], pr[j]) * scal;
+                spz[r] = 0.0;
+                if (hasPrev) A[r + (int64_t)(i - 1) * lda] = vv[j];
+            }
+        }
+        for (int r = i + t + 256 * SY_KR; r < n; r += 256) {
+            const double v = hasPrev ? svr[r] : 0.0;
+            svw[r] = xof(Ai[r], v, hasPrev ? spr[r] : 0.0) * scal;
             spz[r] = 0.0;
-            if (hasPrev) A[r + (int64_t)(i - 1) * lda] = svr[r];   // reflector of step i-1
+            if (hasPrev) A[r + (int64_t)(i - 1) * lda] = v;
         }
     }
     if (wk.y < 0) return;
-    const int r0 = wk.y * 64, c0 = wk.z * 64;
-    if (t < 64) {
-        const int r = r0 + t;
-        const bool ok = r > i && r < n;
-        vr[t] = ok ? vprev(r) : 0.0;
-        wr[t] = ok ? wprev(r) : 0.0;
-        xr[t] = ok ? vcur(r) : 0.0;
-    } else if (t < 128) {
-        const int c = c0 + t - 64;
-        const bool ok = c > i && c < n;
-        vc[t - 64] = ok ? vprev(c) : 0.0;
-        wc[t - 64] = ok ? wprev(c) : 0.0;
-        xc[t - 64] = ok ? vcur(c) : 0.0;
+    if (t < 128) {
+        const double w = tauP * pv_t + alphaP * vv_t;
+        const double x = vok ? (vidx == i + 1 ? 1.0 : xof(av_t, vv_t, pv_t) * scal) : 0.0;
+        if (t < 64) {
+            vr[t] = vv_t;
+            wr[t] = vok ? w : 0.0;
+            xr[t] = x;
+        } else {
+            vc[t - 64] = vv_t;
+            wc[t - 64] = vok ? w : 0.0;
+            xc[t - 64] = x;
+        }
     }
     __syncthreads();
-    for (int e = t; e < 64 * 64; e += 256) {
-        const int rr = e & 63, cc = e >> 6;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = t + 256 * k, rr = e & 63, cc = e >> 6;
         const int r = r0 + rr, c = c0 + cc;
-        double a = 0.0;
-        if (r < n && c > i && r >= c) {
-            double* pa = A + r + (int64_t)c * lda;
-            a = *pa;
-            if (hasPrev) {
-                a -= vr[rr] * wc[cc] + wr[rr] * vc[cc];
-                *pa = a;
-            }
+        double a = ta[k];
+        if (r < n && c > i && r >= c && hasPrev) {
+            a -= vr[rr] * wc[cc] + wr[rr] * vc[cc];
+            A[r + (int64_t)c * lda] = a;
         }
         tile[rr * 65 + cc] = a;
     }
