@@ -95,7 +95,6 @@ struct ExtBatch {
 
 struct TrsmBatch {
     std::vector<TrsmDesc> descs;
-    std::vector<TrsmWork> work;
     void add(const double* L, int64_t ldl, double* B, int64_t ldb, int64_t n, int64_t nrhs,
              int trans) {
         if (n <= 0 || nrhs <= 0) return;
@@ -108,9 +107,7 @@ struct TrsmBatch {
         d.n = static_cast<int32_t>(n);
         d.nrhs = static_cast<int32_t>(nrhs);
         d.trans = trans;
-        const int id = static_cast<int>(descs.size());
         descs.push_back(d);
-        for (int c = 0; c < cdiv(nrhs, 64); ++c) work.push_back({id, c, 0, 0});
     }
 };
 
@@ -187,20 +184,74 @@ struct Plan {
             launch_extadd(bl->at<ExtAddDesc>(od), bl->at<int4>(ow), nw, s);
         });
     }
+    // Blocked right-looking unit-triangular solve of every descriptor: per
+    // 64-row block (in solve order) one launch of diagonal-block solves, then
+    // one GEMM batch applying the solved block to the rows still to come.
     void trsm(const TrsmBatch& b) {
-        if (b.work.empty()) return;
-        const size_t od = blob->add(b.descs), ow = blob->add(b.work);
-        const int nw = static_cast<int>(b.work.size());
+        if (b.descs.empty()) return;
+        const size_t od = blob->add(b.descs);
         Blob* bl = blob;
+        int maxnb = 0;
+        for (const auto& d : b.descs) maxnb = std::max(maxnb, cdiv(d.n, 64));
         double fl = 0, by = 0;
         for (const auto& d : b.descs) {
             const double n = d.n, r = d.nrhs;
             fl += n * (n - 1) * r;
             by += (n * (n + 1) / 2 + 2 * n * r) * 8.0;
         }
-        timed("trsm", fl, by, [=](hipStream_t s) {
-            launch_trsm(bl->at<TrsmDesc>(od), bl->at<TrsmWork>(ow), nw, s);
-        });
+        const int nd = static_cast<int>(b.descs.size());
+        for (int s = 0; s < maxnb; ++s) {
+            std::vector<TrsmWork> work;
+            GemmBatch gb;
+            for (int id = 0; id < nd; ++id) {
+                const TrsmDesc& d = b.descs[id];
+                const int nb = cdiv(d.n, 64);
+                if (s >= nb) continue;
+                const int rb = d.trans ? nb - 1 - s : s;
+                const int r0 = rb * 64, mv = std::min(64, d.n - r0);
+                // a lower-triangular RHS is zero above each column block's
+                // diagonal: only column blocks <= rb are nonzero in row block rb
+                const int ncb = d.rhsLower && !d.trans ? std::min(cdiv(d.nrhs, 64), rb + 1)
+                                                       : cdiv(d.nrhs, 64);
+                for (int c = 0; c < ncb; ++c) work.push_back({id, c, rb, 0});
+                const int ncols = std::min(d.nrhs, ncb * 64);
+                GemmDesc g = gdesc();
+                if (!d.trans && r0 + mv < d.n) {
+                    // B(r0+mv:, :) -= L(r0+mv:, r0:r0+mv) X(r0:r0+mv, :)
+                    g.A = d.L + (r0 + mv) + static_cast<int64_t>(r0) * d.ldl;
+                    g.lda = d.ldl;
+                    g.B = d.B + r0;
+                    g.ldb = d.ldb;
+                    g.C = d.B + (r0 + mv);
+                    g.ldc = d.ldb;
+                    g.m = d.n - (r0 + mv);
+                } else if (d.trans && r0 > 0) {
+                    // B(:r0, :) -= L(r0:r0+mv, :r0)^T X(r0:r0+mv, :)
+                    g.A = d.L + r0;
+                    g.lda = d.ldl;
+                    g.ta = 1;
+                    g.B = d.B + r0;
+                    g.ldb = d.ldb;
+                    g.C = d.B;
+                    g.ldc = d.ldb;
+                    g.m = r0;
+                } else {
+                    continue;
+                }
+                g.n = ncols;
+                g.k = mv;
+                g.alpha1 = -1.0;
+                g.beta = 1.0;
+                gb.add(g);
+            }
+            const size_t ow = blob->add(work);
+            const int nw = static_cast<int>(work.size());
+            const bool first = s == 0;
+            timed("trsm", first ? fl : 0.0, first ? by : 0.0, [=](hipStream_t st) {
+                launch_trsm_diag(bl->at<TrsmDesc>(od), bl->at<TrsmWork>(ow), nw, st);
+            });
+            gemm(gb, false);
+        }
     }
     void gather(const GatherBatch& b) {
         if (b.work.empty()) return;
@@ -327,7 +378,9 @@ void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
         }
         if (descs[m].k > 0) {
             nmax = std::max(nmax, descs[m].n);
-            while (swork.size() % 64) swork.push_back(make_int2(m, -1));   // pad to the block
+            // pad to whole workgroups: every workgroup serves one matrix
+            while (work.size() % 4) work.push_back(make_int2(m, -1));
+            while (swork.size() % 64) swork.push_back(make_int2(m, -1));
         }
         // LDS-resident column blocks of at most 8 columns and 64 KiB
         const int n = descs[m].n;
@@ -339,7 +392,7 @@ void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
         }
     }
     if (work.empty()) return;
-    woff.resize(std::max(woff.size(), swork.size()), 0);
+    woff.resize(std::max({woff.size(), swork.size(), work.size()}), 0);
     const size_t od = plan.blob->add(descs), ow = plan.blob->add(work), oo = plan.blob->add(woff),
                  oq = plan.blob->add(owork), os = plan.blob->add(swork);
     Blob* bl = plan.blob;
@@ -351,7 +404,7 @@ void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
             if (kst) kst->run(name, 0.0, 0.0, s, f);
             else f();
         };
-        run("bisect", [&] { launch_bisect(bl->at<EigDesc>(od), bl->at<int2>(ow), nw, s); });
+        run("bisect", [&] { launch_bisect(bl->at<EigDesc>(od), bl->at<int2>(ow), nw, nmax, s); });
         if (vectors) {
             run("stein", [&] {
                 launch_stein(bl->at<EigDesc>(od), bl->at<int2>(os), ns, bl->at<int64_t>(oo), nmax, s);
@@ -1044,8 +1097,10 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
         std::vector<int32_t> cnt(N, 0);
         const size_t oc = cb.add(cnt);
         cb.upload(stream);
+        int nmaxc = 1;
+        for (int64_t p = 0; p < N; ++p) nmaxc = std::max<int>(nmaxc, static_cast<int>(S.size[p]));
         launch_eig_count(cb.at<EigDesc>(od), static_cast<int>(N), cb.at<double>(ot),
-                         const_cast<int32_t*>(cb.at<int32_t>(oc)), stream);
+                         const_cast<int32_t*>(cb.at<int32_t>(oc)), nmaxc, stream);
         AMLS_HIP_CHECK(hipMemcpyAsync(cnt.data(), cb.at<int32_t>(oc), N * sizeof(int32_t),
                                       hipMemcpyDeviceToHost, stream));
         sync();

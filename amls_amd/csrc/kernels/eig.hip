@@ -240,57 +240,98 @@ __device__ void gersh(const double* dd, const double* ee, int n, double& lo, dou
 }
 
 // counts[mat] = number of eigenvalues >= thr[mat] (capped at cap[mat])
-__global__ void eig_count_kernel(const EigDesc* __restrict__ descs, int nmat,
-                                 const double* __restrict__ thr, int32_t* __restrict__ counts) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nmat) return;
-    const EigDesc d = descs[m];
-    if (d.n <= 0) { counts[m] = 0; return; }
-    double lo, hi, pivmin, tn;
-    gersh(d.d, d.e, d.n, lo, hi, pivmin, tn);
-    const int below = sturm_count(d.d, d.e, d.n, thr[m], pivmin);
-    int k = d.n - below;
-    if (k > d.kcap) k = d.kcap;
-    counts[m] = k;
-}
-
-// lam[j] = (j+1)-th largest eigenvalue, j < k; one wave per work item
-// (x = matrix, y = j).  Multisection: each round the 64 lanes evaluate the
-// Sturm count at 64 interior points of the current bracket, which shrinks the
-// bracket 65-fold per round (instead of 2-fold for plain bisection).
-__global__ void __launch_bounds__(256) bisect_kernel(const EigDesc* __restrict__ descs,
-                                                     const int2* __restrict__ work, int nwork) {
-    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (wid >= nwork) return;
-    const int2 wk = work[wid];
-    const EigDesc d = descs[wk.x];
-    const int n = d.n;
-    const int j = wk.y;
-    // Gershgorin bounds, lane-parallel
-    double lo = 1e308, hi = -1e308, emax = 0.0;
+// Gershgorin bounds and pivmin of a tridiagonal (dd: n, ee: n - 1 entries,
+// any memory space), lanes of one wave striding the rows.
+__device__ void gersh_wave(const double* dd, const double* ee, int n, int lane, double& lo,
+                           double& hi, double& pivmin, double& tn) {
+    lo = 1e308;
+    hi = -1e308;
+    double emax = 0.0;
     for (int r = lane; r < n; r += 64) {
-        const double el = r > 0 ? fabs(d.e[r - 1]) : 0.0;
-        const double er = r + 1 < n ? fabs(d.e[r]) : 0.0;
-        lo = fmin(lo, d.d[r] - el - er);
-        hi = fmax(hi, d.d[r] + el + er);
-        if (r + 1 < n) emax = fmax(emax, d.e[r] * d.e[r]);
+        const double el = r > 0 ? fabs(ee[r - 1]) : 0.0;
+        const double er = r + 1 < n ? fabs(ee[r]) : 0.0;
+        lo = fmin(lo, dd[r] - el - er);
+        hi = fmax(hi, dd[r] + el + er);
+        if (r + 1 < n) emax = fmax(emax, ee[r] * ee[r]);
     }
     for (int o = 32; o > 0; o >>= 1) {
         lo = fmin(lo, __shfl_xor(lo, o, 64));
         hi = fmax(hi, __shfl_xor(hi, o, 64));
         emax = fmax(emax, __shfl_xor(emax, o, 64));
     }
-    const double tn = fmax(fabs(lo), fabs(hi));
+    tn = fmax(fabs(lo), fabs(hi));
     lo -= 2.0 * DBL_EPSILON * tn + 1e-300;
     hi += 2.0 * DBL_EPSILON * tn + 1e-300;
-    const double pivmin = DBL_MIN * fmax(1.0, emax);
+    pivmin = DBL_MIN * fmax(1.0, emax);
+}
+
+// Copies matrix d's diagonal and off-diagonal into LDS (sde: 2 nmax doubles)
+// when useLds, and returns where to read them from.
+__device__ void stage_tridiag(const EigDesc& d, double* sde, int useLds, const double*& dd,
+                              const double*& ee) {
+    dd = d.d;
+    ee = d.e;
+    if (useLds) {
+        for (int r = threadIdx.x; r < d.n; r += blockDim.x) {
+            sde[r] = d.d[r];
+            sde[d.n + r] = r + 1 < d.n ? d.e[r] : 0.0;
+        }
+        __syncthreads();
+        dd = sde;
+        ee = sde + d.n;
+    }
+}
+
+// counts[mat] = number of eigenvalues >= thr[mat] (capped at kcap); one
+// 64-lane workgroup per matrix, its tridiagonal staged in LDS
+__global__ void __launch_bounds__(64) eig_count_kernel(const EigDesc* __restrict__ descs,
+                                                       const double* __restrict__ thr,
+                                                       int32_t* __restrict__ counts, int useLds) {
+    extern __shared__ double sde[];
+    const int m = blockIdx.x;
+    const EigDesc d = descs[m];
+    if (d.n <= 0) {
+        if (threadIdx.x == 0) counts[m] = 0;
+        return;
+    }
+    const double *dd, *ee;
+    stage_tridiag(d, sde, useLds, dd, ee);
+    double lo, hi, pivmin, tn;
+    gersh_wave(dd, ee, d.n, threadIdx.x, lo, hi, pivmin, tn);
+    if (threadIdx.x == 0) {
+        const int below = sturm_count(dd, ee, d.n, thr[m], pivmin);
+        counts[m] = min(d.n - below, d.kcap);
+    }
+}
+
+// lam[j] = (j+1)-th largest eigenvalue, j < k; one wave per work item
+// (x = matrix, y = j; the host pads each matrix's items to whole workgroups
+// of four with y = -1, so a workgroup stages one tridiagonal in LDS).
+// Multisection: each round the 64 lanes evaluate the Sturm count at 64
+// interior points of the current bracket, which shrinks the bracket 65-fold
+// per round (instead of 2-fold for plain bisection).
+__global__ void __launch_bounds__(256) bisect_kernel(const EigDesc* __restrict__ descs,
+                                                     const int2* __restrict__ work, int nwork,
+                                                     int useLds) {
+    extern __shared__ double sde[];
+    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const EigDesc d = descs[work[blockIdx.x * 4].x];
+    const double *dd, *ee;
+    stage_tridiag(d, sde, useLds, dd, ee);
+    if (wid >= nwork) return;
+    const int2 wk = work[wid];
+    if (wk.y < 0) return;
+    const int n = d.n;
+    const int j = wk.y;
+    double lo, hi, pivmin, tn;
+    gersh_wave(dd, ee, n, lane, lo, hi, pivmin, tn);
     const int a = n - 1 - j;    // ascending index
     for (int round = 0; round < 16; ++round) {
         const double tol = 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
         if (hi - lo <= tol) break;
         const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
-        const int c = sturm_count(d.d, d.e, n, x, pivmin);
+        const int c = sturm_count(dd, ee, n, x, pivmin);
         // counts grow with x: lanes with c <= a form a prefix
         const unsigned long long mask = __ballot(c <= a);
         const int np = __popcll(mask);
@@ -514,15 +555,20 @@ void launch_sytrd_step(const dev::EigDesc* d, const int4* work, int nwork, int i
 }
 
 void launch_eig_count(const dev::EigDesc* d, int nmat, const double* thr, int32_t* counts,
-                      hipStream_t s) {
+                      int nmax, hipStream_t s) {
     if (nmat <= 0) return;
-    hipLaunchKernelGGL(dev::eig_count_kernel, dim3((nmat + 63) / 64), dim3(64), 0, s, d, nmat, thr,
-                       counts);
+    const size_t lds = (size_t)2 * nmax * sizeof(double);
+    const int useLds = lds <= 60000 ? 1 : 0;
+    hipLaunchKernelGGL(dev::eig_count_kernel, dim3(nmat), dim3(64), useLds ? lds : 0, s, d, thr,
+                       counts, useLds);
 }
 
-void launch_bisect(const dev::EigDesc* d, const int2* work, int nwork, hipStream_t s) {
+void launch_bisect(const dev::EigDesc* d, const int2* work, int nwork, int nmax, hipStream_t s) {
     if (nwork <= 0) return;
-    hipLaunchKernelGGL(dev::bisect_kernel, dim3((nwork + 3) / 4), dim3(256), 0, s, d, work, nwork);
+    const size_t lds = (size_t)2 * nmax * sizeof(double);
+    const int useLds = lds <= 60000 ? 1 : 0;
+    hipLaunchKernelGGL(dev::bisect_kernel, dim3((nwork + 3) / 4), dim3(256), useLds ? lds : 0, s, d,
+                       work, nwork, useLds);
 }
 
 void launch_stein(const dev::EigDesc* d, const int2* work, int nwork, const int64_t* woff,

@@ -175,14 +175,15 @@ __global__ void __launch_bounds__(256) gemm_batched_kernel(const GemmDesc* __res
 }
 
 // ---------------------------------------------------------------------------
-// Triangular solve, one workgroup per (descriptor, 64-column RHS block).
-// Left-looking over 64-row blocks: the off-diagonal part is an MFMA tile
-// product against already-solved rows, the diagonal block is solved in LDS.
+// Triangular solve, diagonal-block part of the blocked right-looking
+// algorithm: one workgroup per (descriptor, 64-column RHS block, 64-row
+// block rb) solves L_rb,rb X = B (or L_rb,rb^T X = B) in LDS, in place.  The
+// host plans the row blocks in order and applies the off-diagonal updates
+// between them as GEMMs, so every step is one wide launch instead of a
+// sequential walk inside one workgroup per column block.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) trsm_unit_kernel(const TrsmDesc* __restrict__ descs,
+__global__ void __launch_bounds__(256) trsm_diag_kernel(const TrsmDesc* __restrict__ descs,
                                                         const TrsmWork* __restrict__ work) {
-    __shared__ double sA[KC * LDSP];
-    __shared__ double sB[KC * LDSP];
     __shared__ double sT[TM * (TM + 1)];
     __shared__ double sL[TM * (TM + 1)];
     const TrsmWork wk = work[blockIdx.x];
@@ -190,64 +191,36 @@ __global__ void __launch_bounds__(256) trsm_unit_kernel(const TrsmDesc* __restri
     const int n = d.n;
     const int c0 = wk.cb * TM;
     const int nv = min(TM, d.nrhs - c0);
-    const int nb = (n + TM - 1) / TM;
+    const int r0 = wk.pad0 * TM;
+    const int mv = min(TM, n - r0);
     const int t = threadIdx.x;
     double* X = d.B + (int64_t)c0 * d.ldb;
-    // a lower-triangular right-hand side is zero above the block's first
-    // column, so the forward solve can start at that row block
-    const int first = (d.rhsLower && !d.trans) ? wk.cb : 0;
-    for (int it = first; it < nb; ++it) {
-        const int rb = d.trans ? nb - 1 - it : it;
-        const int r0 = rb * TM;
-        const int mv = min(TM, n - r0);
-        TileAcc<double> acc;
-        acc.zero();
-        if (!d.trans) {
-            const int kb = first * TM;
-            if (r0 > kb)
-                tile_mma<double>(acc, d.L + r0, d.ldl, 0, mv, X, d.ldb, 0, nv, kb, r0, 1.0, sA, sB);
-        } else {
-            const int kb = r0 + TM;
-            if (kb < n)
-                tile_mma<double>(acc, d.L + (int64_t)r0 * d.ldl, d.ldl, 1, mv, X, d.ldb, 0, nv,
-                                 kb, n, 1.0, sA, sB);
+    for (int e = t; e < TM * TM; e += 256) {
+        const int r = e & 63, c = e >> 6;
+        sT[r * (TM + 1) + c] = (r < mv && c < nv) ? X[r0 + r + (int64_t)c * d.ldb] : 0.0;
+        sL[r * (TM + 1) + c] = (r < mv && c < mv) ? d.L[r0 + r + (int64_t)(r0 + c) * d.ldl] : 0.0;
+    }
+    __syncthreads();
+    const int c = t & 63;
+    if (!d.trans) {
+        for (int j = 0; j < mv - 1; ++j) {
+            const double xj = sT[j * (TM + 1) + c];
+            for (int i = j + 1 + (t >> 6); i < mv; i += 4)
+                sT[i * (TM + 1) + c] -= sL[i * (TM + 1) + j] * xj;
+            __syncthreads();
         }
-        // T = B_tile - acc
-        for (int e = t; e < TM * TM; e += 256) {
-            const int r = e & 63, c = e >> 6;
-            sT[r * (TM + 1) + c] = (r < mv && c < nv) ? X[r0 + r + (int64_t)c * d.ldb] : 0.0;
-            // diagonal block of L: sL[i][j] = L(r0+i, r0+j)
-            const int i = e & 63, j = e >> 6;
-            sL[i * (TM + 1) + j] =
-                (i < mv && j < mv) ? d.L[r0 + i + (int64_t)(r0 + j) * d.ldl] : 0.0;
+    } else {
+        // L^T upper unit: x_j -= sum_{i > j} L(i, j) x_i, backward
+        for (int j = mv - 1; j > 0; --j) {
+            const double xj = sT[j * (TM + 1) + c];
+            for (int i = (t >> 6); i < j; i += 4)
+                sT[i * (TM + 1) + c] -= sL[j * (TM + 1) + i] * xj;
+            __syncthreads();
         }
-        __syncthreads();
-        tile_foreach<double>(acc, [&](int r, int c, double v) {
-            sT[r * (TM + 1) + c] -= v;
-        });
-        __syncthreads();
-        const int c = t & 63;
-        if (!d.trans) {
-            for (int j = 0; j < mv - 1; ++j) {
-                const double xj = sT[j * (TM + 1) + c];
-                for (int i = j + 1 + (t >> 6); i < mv; i += 4)
-                    sT[i * (TM + 1) + c] -= sL[i * (TM + 1) + j] * xj;
-                __syncthreads();
-            }
-        } else {
-            // L^T upper unit: x_j -= sum_{i > j} L(i, j) x_i, backward
-            for (int j = mv - 1; j > 0; --j) {
-                const double xj = sT[j * (TM + 1) + c];
-                for (int i = (t >> 6); i < j; i += 4)
-                    sT[i * (TM + 1) + c] -= sL[j * (TM + 1) + i] * xj;
-                __syncthreads();
-            }
-        }
-        for (int e = t; e < TM * TM; e += 256) {
-            const int r = e & 63, cc = e >> 6;
-            if (r < mv && cc < nv) X[r0 + r + (int64_t)cc * d.ldb] = sT[r * (TM + 1) + cc];
-        }
-        __syncthreads();
+    }
+    for (int e = t; e < TM * TM; e += 256) {
+        const int r = e & 63, cc = e >> 6;
+        if (r < mv && cc < nv) X[r0 + r + (int64_t)cc * d.ldb] = sT[r * (TM + 1) + cc];
     }
 }
 
@@ -262,9 +235,9 @@ void launch_gemm(const dev::GemmDesc* d, const dev::TileRef* tiles, int ntiles, 
         hipLaunchKernelGGL(dev::gemm_batched_kernel<double>, dim3(ntiles), dim3(256), 0, s, d, tiles);
 }
 
-void launch_trsm(const dev::TrsmDesc* d, const dev::TrsmWork* w, int nwork, hipStream_t s) {
+void launch_trsm_diag(const dev::TrsmDesc* d, const dev::TrsmWork* w, int nwork, hipStream_t s) {
     if (nwork <= 0) return;
-    hipLaunchKernelGGL(dev::trsm_unit_kernel, dim3(nwork), dim3(256), 0, s, d, w);
+    hipLaunchKernelGGL(dev::trsm_diag_kernel, dim3(nwork), dim3(256), 0, s, d, w);
 }
 
 }  // namespace amls

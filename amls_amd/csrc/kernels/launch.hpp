@@ -9,7 +9,10 @@ namespace amls {
 
 void launch_gemm(const dev::GemmDesc* d, const dev::TileRef* tiles, int ntiles, bool fp32,
                  hipStream_t s);
-void launch_trsm(const dev::TrsmDesc* d, const dev::TrsmWork* w, int nwork, hipStream_t s);
+// Diagonal 64 x 64 block solves of a blocked unit-triangular solve: work item
+// (descriptor, RHS column block, -, -) with pad0 = the row block; the caller
+// applies the off-diagonal updates between row blocks.
+void launch_trsm_diag(const dev::TrsmDesc* d, const dev::TrsmWork* w, int nwork, hipStream_t s);
 // Panel s0 of the fronts in `list` (indices into d): diagonal block, then the
 // rows below it in chunks of 256 (rows = (front index, chunk) work items).
 void launch_ldlt_panel(const dev::FrontDesc* d, const int32_t* list, int count, int s0, int* err,
@@ -25,9 +28,12 @@ void launch_gather(const dev::GatherDesc* d, const int4* work, int nwork, hipStr
 // of n per matrix and sp 3 (soff[m] = element offset / stride unit n_m).
 void launch_sytrd_step(const dev::EigDesc* d, const int4* work, int nwork, int i, double* sv,
                        double* sp, const int64_t* soff, hipStream_t s);
+// counts[m] = number of eigenvalues of matrix m >= thr[m], capped at kcap
+// (nmax: the largest n)
 void launch_eig_count(const dev::EigDesc* d, int nmat, const double* thr, int32_t* counts,
-                      hipStream_t s);
-void launch_bisect(const dev::EigDesc* d, const int2* work, int nwork, hipStream_t s);
+                      int nmax, hipStream_t s);
+// work: (matrix, j) padded per matrix to multiples of 4 with j = -1
+void launch_bisect(const dev::EigDesc* d, const int2* work, int nwork, int nmax, hipStream_t s);
 // work: (matrix, j) padded per matrix to multiples of 64 with j = -1
 void launch_stein(const dev::EigDesc* d, const int2* work, int nwork, const int64_t* woff,
                   int nmax, hipStream_t s);
