@@ -370,6 +370,11 @@ void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
     std::vector<int4> owork;
     std::vector<int64_t> woff;
     int lds = 0, nmax = 1;
+    // columns per back-transformation workgroup: aim at ~512 workgroups (each
+    // sweeps all reflectors of its matrix sequentially), at most 8 columns
+    int64_t ktot = 0;
+    for (const auto& d : descs) ktot += d.k;
+    const int want = static_cast<int>(std::min<int64_t>(8, std::max<int64_t>(1, (ktot + 511) / 512)));
     for (int m = 0; m < static_cast<int>(descs.size()); ++m) {
         for (int j = 0; j < descs[m].k; ++j) {
             work.push_back(make_int2(m, j));
@@ -384,7 +389,7 @@ void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
         }
         // LDS-resident column blocks of at most 8 columns and 64 KiB
         const int n = descs[m].n;
-        const int per = std::max(1, std::min(8, 8000 / std::max(n, 1)));
+        const int per = std::max(1, std::min(want, 8000 / std::max(n, 1)));
         for (int c = 0; c < descs[m].k; c += per) {
             const int nc = std::min(per, descs[m].k - c);
             owork.push_back(make_int4(m, c, nc, 0));

@@ -434,17 +434,22 @@ __global__ void __launch_bounds__(64) stein_kernel(const EigDesc* __restrict__ d
         S_(ud, n - 1) = udk;
         S_(u1, n - 1) = 0.0;
         S_(u2, n - 1) = 0.0;
-        // deterministic start vector
+        // inverse iteration: two solves for an isolated eigenvalue, three inside
+        // a cluster; the start vector is generated inside the first forward
+        // pass, and the normalisation of each solve is applied when the next
+        // pass reads the vector
         uint32_t s = 2463534242u ^ (uint32_t)(j * 2654435761u) ^ (uint32_t)(wk.x * 40503u);
-        for (int r = 0; r < n; ++r) {
+        auto start = [&]() {
             s ^= s << 13; s ^= s >> 17; s ^= s << 5;
-            S_(x, r) = 1.0 + (double)(s & 0xffff) * (1.0 / 65536.0);
-        }
-        for (int it = 0; it < 4; ++it) {
+            return 1.0 + (double)(s & 0xffff) * (1.0 / 65536.0);
+        };
+        const int iters = j > j0 ? 3 : 2;
+        double sc = 1.0;
+        for (int it = 0; it < iters; ++it) {
             // forward: row interchanges and multipliers
-            double xk = S_(x, 0);
+            double xk = it == 0 ? start() : S_(x, 0) * sc;
             for (int k = 0; k + 1 < n; ++k) {
-                double xn = S_(x, k + 1);
+                double xn = it == 0 ? start() : S_(x, k + 1) * sc;
                 if (S_(pv, k) != 0.0) {
                     const double tmp = xk;
                     xk = xn;
@@ -453,55 +458,66 @@ __global__ void __launch_bounds__(64) stein_kernel(const EigDesc* __restrict__ d
                 S_(x, k) = xk;
                 xk = xn - S_(ml, k) * xk;
             }
-            S_(x, n - 1) = xk;
-            // back substitution with U (bandwidth 3)
-            double x1 = S_(x, n - 1) / S_(ud, n - 1);
+            // back substitution with U (bandwidth 3), accumulating the norm
+            double nrm = 0.0, amax = 0.0, sgn = 1.0;
+            auto acc = [&](double v) {
+                nrm += v * v;
+                if (fabs(v) > amax) { amax = fabs(v); sgn = v < 0.0 ? -1.0 : 1.0; }
+            };
+            double x1 = xk / S_(ud, n - 1);
             S_(x, n - 1) = x1;
+            acc(x1);
             double x2 = 0.0;
             if (n > 1) {
                 const double v = (S_(x, n - 2) - S_(u1, n - 2) * x1) / S_(ud, n - 2);
                 S_(x, n - 2) = v;
+                acc(v);
                 x2 = x1;
                 x1 = v;
             }
             for (int k = n - 3; k >= 0; --k) {
                 const double v = (S_(x, k) - S_(u1, k) * x1 - S_(u2, k) * x2) / S_(ud, k);
                 S_(x, k) = v;
+                acc(v);
                 x2 = x1;
                 x1 = v;
             }
-            // orthogonalise against earlier members of the cluster
+            // orthogonalise against earlier members of the cluster (modified
+            // Gram-Schmidt; the last subtraction pass recomputes the norm)
             for (int q = j0; q < j; ++q) {
                 const double* y = d.Y + (int64_t)q * n;
                 double dot = 0.0;
                 for (int r = 0; r < n; ++r) dot += S_(x, r) * y[r];
-                for (int r = 0; r < n; ++r) S_(x, r) -= dot * y[r];
+                const bool lastq = q + 1 == j;
+                if (lastq) { nrm = 0.0; amax = 0.0; sgn = 1.0; }
+                for (int r = 0; r < n; ++r) {
+                    const double v = S_(x, r) - dot * y[r];
+                    S_(x, r) = v;
+                    if (lastq) acc(v);
+                }
             }
-            double nrm = 0.0, amax = 0.0, sgn = 1.0;
-            for (int r = 0; r < n; ++r) {
-                const double v = S_(x, r);
-                nrm += v * v;
-                if (fabs(v) > amax) { amax = fabs(v); sgn = v < 0.0 ? -1.0 : 1.0; }
-            }
-            const double sc = sgn / sqrt(nrm);
-            for (int r = 0; r < n; ++r) S_(x, r) *= sc;
+            sc = sgn / sqrt(nrm);
         }
         double* y = d.Y + (int64_t)j * n;
-        for (int r = 0; r < n; ++r) y[r] = S_(x, r);
+        for (int r = 0; r < n; ++r) y[r] = S_(x, r) * sc;
     }
 #undef S_
 }
 
 // Y := Q Y for the reflectors stored by the tridiagonalisation.  One
 // 256-thread workgroup per (matrix, block of nc <= 8 columns); the block of Y
-// lives in LDS for the whole sweep over the reflectors, so each reflector
-// costs one read of its vector from memory and LDS traffic for the rest.
+// lives in LDS for the whole sweep over the reflectors.  Thread t owns the
+// rows r = t + 256 m of Y for every reflector, so the only cross-thread step
+// per reflector is the reduction of the dot products (one barrier, partial
+// sums double-buffered between consecutive reflectors), and each reflector's vector is
+// fetched into registers three reflectors ahead of its use.
 // Work item (x = matrix, y = first column, z = column count).
+constexpr int OR_KR = 8;   // rows per thread in registers (n <= 2048)
+
 __global__ void __launch_bounds__(256) ormtr_kernel(const EigDesc* __restrict__ descs,
                                                     const int4* __restrict__ work) {
     extern __shared__ double sY[];       // n x nc, column-major (ld n)
-    __shared__ double part[4][8];
-    __shared__ double coef[8];
+    __shared__ double part[2][4][8];
     const int4 wk = work[blockIdx.x];
     const EigDesc d = descs[wk.x];
     const int n = d.n;
@@ -512,36 +528,84 @@ __global__ void __launch_bounds__(256) ormtr_kernel(const EigDesc* __restrict__ 
     const int t = threadIdx.x;
     const int lane = t & 63, w = t >> 6;
     for (int e = t; e < n * nc; e += 256) sY[e] = Y[e];
-    __syncthreads();
-    for (int i = n - 2; i >= 0; --i) {
-        const double tau = d.tau[i];
-        if (tau == 0.0) continue;
+    // v_i(r): 0 for r <= i, 1 at r = i + 1, stored below
+    auto load = [&](double (&v)[OR_KR], double& tau, int i) {
+        tau = i >= 0 ? d.tau[i] : 0.0;
+#pragma unroll
+        for (int m = 0; m < OR_KR; ++m) {
+            const int r = t + 256 * m;
+            v[m] = (i < 0 || r <= i || r >= n) ? 0.0 : (r == i + 1 ? 1.0 : A[r + (int64_t)i * lda]);
+        }
+    };
+    int par = 0;   // partial-sum buffer, alternating over the applied reflectors
+    auto step = [&](const double (&v)[OR_KR], double tau, int i) {
+        if (i < 0 || tau == 0.0) return;   // uniform over the workgroup
         double s[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) s[c] = 0.0;
-        for (int r = i + 1 + t; r < n; r += 256) {
-            const double vr = r == i + 1 ? 1.0 : A[r + (int64_t)i * lda];
+#pragma unroll
+        for (int m = 0; m < OR_KR; ++m) {
+            const int r = t + 256 * m;
+            if (r > i && r < n) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (c < nc) s[c] += v[m] * sY[r + c * n];
+            }
+        }
+        for (int r = t + 256 * OR_KR; r < n; r += 256) {
+            const double vr = r <= i ? 0.0 : (r == i + 1 ? 1.0 : A[r + (int64_t)i * lda]);
 #pragma unroll
             for (int c = 0; c < 8; ++c)
                 if (c < nc) s[c] += vr * sY[r + c * n];
         }
+        double (*pb)[8] = part[par];
+        par ^= 1;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            double v = s[c];
-            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-            if (lane == 0) part[w][c] = v;
+            if (c < nc) {
+                double x = s[c];
+                for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+                if (lane == 0) pb[w][c] = x;
+            }
         }
         __syncthreads();
-        if (t < 8) coef[t] = tau * (part[0][t] + part[1][t] + part[2][t] + part[3][t]);
-        __syncthreads();
-        for (int r = i + 1 + t; r < n; r += 256) {
-            const double vr = r == i + 1 ? 1.0 : A[r + (int64_t)i * lda];
+        double coef[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            coef[c] = c < nc ? tau * (pb[0][c] + pb[1][c] + pb[2][c] + pb[3][c]) : 0.0;
+#pragma unroll
+        for (int m = 0; m < OR_KR; ++m) {
+            const int r = t + 256 * m;
+            if (r > i && r < n) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (c < nc) sY[r + c * n] -= coef[c] * v[m];
+            }
+        }
+        for (int r = t + 256 * OR_KR; r < n; r += 256) {
+            const double vr = r <= i ? 0.0 : (r == i + 1 ? 1.0 : A[r + (int64_t)i * lda]);
 #pragma unroll
             for (int c = 0; c < 8; ++c)
                 if (c < nc) sY[r + c * n] -= coef[c] * vr;
         }
-        __syncthreads();
+    };
+    double va[OR_KR], vb[OR_KR], vc[OR_KR], vd[OR_KR];
+    double ta, tb, tc, td;
+    load(va, ta, n - 2);
+    load(vb, tb, n - 3);
+    load(vc, tc, n - 4);
+    __syncthreads();
+    for (int i = n - 2; i >= 0; i -= 4) {
+        load(vd, td, i - 3);
+        step(va, ta, i);
+        load(va, ta, i - 4);
+        step(vb, tb, i - 1);
+        load(vb, tb, i - 5);
+        step(vc, tc, i - 2);
+        load(vc, tc, i - 6);
+        step(vd, td, i - 3);
     }
+    __syncthreads();
     for (int e = t; e < n * nc; e += 256) Y[e] = sY[e];
 }
 
