@@ -350,17 +350,105 @@ void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<i
 }
 
 // Y = A X for column-major X, Y (n x ncols, ld n): transpose X into the
-// row-major scratch t1, CSR product into t2, transpose back into Y.
-void addSpmm(Plan& plan, int64_t n, const int64_t* rp, const int64_t* ci, const double* vx,
-             double nnz, const double* X, double* Y, int64_t ncols, double* t1, double* t2) {
+// row-major scratch t1, sparse product into t2, transpose back into Y.
+// Algorithmic bytes: the matrix once (values and indices) plus X and Y.
+void addSpmm(Plan& plan, const SparseOp& A, const double* X, double* Y, int64_t ncols, double* t1,
+             double* t2) {
+    const int64_t n = A.n;
     Copy2dBatch to, back;
     to.add(cdesc(X, n, t1, ncols, ncols, n, 1));     // t1 (ncols x n col-major) = X^T
     back.add(cdesc(t2, ncols, Y, n, n, ncols, 1));   // Y = t2^T
     plan.copy(to);
     const int nc = static_cast<int>(ncols);
-    plan.timed("spmm", 2.0 * nnz * ncols, nnz * 16.0 + (n + 1) * 8.0 + 2.0 * n * ncols * 8.0,
-               [=](hipStream_t s) { launch_spmm_rm(n, rp, ci, vx, t1, t2, nc, s); });
+    const SparseOp a = A;
+    if (a.nb > 0) {
+        const double nblk = a.nnz / 9.0;
+        plan.timed("spmm", 2.0 * a.nnz * ncols,
+                   nblk * (72.0 + 4.0) + (a.nb + 1) * 8.0 + 2.0 * n * ncols * 8.0,
+                   [=](hipStream_t s) { launch_spmm_bsr3(a.nb, a.bptr, a.bcol, a.bval, t1, t2, nc, s); });
+    } else {
+        plan.timed("spmm", 2.0 * a.nnz * ncols, a.nnz * 16.0 + (n + 1) * 8.0 + 2.0 * n * ncols * 8.0,
+                   [=](hipStream_t s) { launch_spmm_rm(n, a.rp, a.ci, a.vx, t1, t2, nc, s); });
+    }
     plan.copy(back);
+}
+
+// 3 x 3 block rows of a CSR matrix (rows may be unsorted): fails unless n is
+// a multiple of 3 and every aligned row triple has the same pattern made of
+// aligned column triples.
+struct Bsr3Host {
+    std::vector<int64_t> ptr;
+    std::vector<int32_t> col;
+    std::vector<double> val;
+};
+
+bool buildBsr3Sorted(int64_t n, const int64_t* rp, const int64_t* ci, const double* vx,
+                     Bsr3Host& out);
+
+bool buildBsr3(int64_t n, const int64_t* rp, const int64_t* ci, const double* vx, Bsr3Host& out) {
+    if (n <= 0 || n % 3 != 0 || n / 3 > INT32_MAX) return false;
+    // fast path: the three rows of every triple list the same column triples
+    // in the same order (true for sorted rows and for symmetric permutations
+    // that move whole triples); otherwise sort each row and try again
+    const int64_t nb = n / 3;
+    out.ptr.assign(1, 0);
+    out.col.clear();
+    out.val.clear();
+    out.col.reserve(static_cast<size_t>(rp[n] / 9));
+    out.val.reserve(static_cast<size_t>(rp[n]));
+    bool ok = true;
+    for (int64_t b = 0; b < nb && ok; ++b) {
+        const int64_t r0 = 3 * b, len = rp[r0 + 1] - rp[r0];
+        if (len % 3 != 0 || rp[r0 + 2] - rp[r0 + 1] != len || rp[r0 + 3] - rp[r0 + 2] != len) {
+            ok = false;
+            break;
+        }
+        for (int64_t t = 0; t < len && ok; t += 3) {
+            const int64_t c = ci[rp[r0] + t];
+            if (c % 3 != 0) { ok = false; break; }
+            for (int i = 0; i < 3 && ok; ++i)
+                for (int j = 0; j < 3; ++j)
+                    if (ci[rp[r0 + i] + t + j] != c + j) { ok = false; break; }
+            if (!ok) break;
+            out.col.push_back(static_cast<int32_t>(c / 3));
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) out.val.push_back(vx[rp[r0 + i] + t + j]);
+        }
+        out.ptr.push_back(static_cast<int64_t>(out.col.size()));
+    }
+    return ok || buildBsr3Sorted(n, rp, ci, vx, out);
+}
+
+bool buildBsr3Sorted(int64_t n, const int64_t* rp, const int64_t* ci, const double* vx,
+                     Bsr3Host& out) {
+    const int64_t nb = n / 3;
+    out.ptr.assign(1, 0);
+    out.col.clear();
+    out.val.clear();
+    std::vector<std::pair<int64_t, double>> row[3];
+    for (int64_t b = 0; b < nb; ++b) {
+        for (int i = 0; i < 3; ++i) {
+            const int64_t r = 3 * b + i;
+            row[i].clear();
+            for (int64_t e = rp[r]; e < rp[r + 1]; ++e) row[i].push_back({ci[e], vx[e]});
+            std::sort(row[i].begin(), row[i].end(),
+                      [](const auto& x, const auto& y) { return x.first < y.first; });
+        }
+        const size_t len = row[0].size();
+        if (len % 3 != 0 || row[1].size() != len || row[2].size() != len) return false;
+        for (size_t t = 0; t < len; t += 3) {
+            const int64_t c = row[0][t].first;
+            if (c % 3 != 0) return false;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    if (row[i][t + j].first != c + j) return false;
+            out.col.push_back(static_cast<int32_t>(c / 3));
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) out.val.push_back(row[i][t + j].second);
+        }
+        out.ptr.push_back(static_cast<int64_t>(out.col.size()));
+    }
+    return true;
 }
 
 // Bisection, inverse iteration and back-transformation for descriptors with
@@ -719,6 +807,50 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
     up(dPMi, pmi.data(), nm * 8);
     up(dPMp, pmp.data(), (n + 1) * 8);
     up(dSegRow, segRow.data(), (nseg + 1) * 8);
+    // operators: CSR always, 3 x 3 blocks when the pattern allows
+    auto csrOp = [&](const int64_t* rp, const int64_t* ci, const double* vx, double nnz) {
+        SparseOp o;
+        o.n = n;
+        o.rp = rp;
+        o.ci = ci;
+        o.vx = vx;
+        o.nnz = nnz;
+        return o;
+    };
+    opK = csrOp(dKp, dKi, dKx, static_cast<double>(nk));
+    opM = csrOp(dMp, dMi, dMx, static_cast<double>(nm));
+    opPK = csrOp(dPKp, dPKi, dPKx, static_cast<double>(nk));
+    opPM = csrOp(dPMp, dPMi, dPMx, static_cast<double>(nm));
+    Bsr3Host bK, bM, bPK, bPM;
+    const bool blocks = buildBsr3(n, Kp.data(), Ki.data(), Kx.data(), bK) &&
+                        buildBsr3(n, Mp.data(), Mi.data(), Mx.data(), bM) &&
+                        buildBsr3(n, pkp.data(), pki.data(), pkx.data(), bPK) &&
+                        buildBsr3(n, pmp.data(), pmi.data(), pmx.data(), bPM);
+    bsrMem.reset();
+    if (blocks) {
+        bsrMem.reset(new DevArena());
+        auto& B = *bsrMem;
+        for (const Bsr3Host* h : {&bK, &bM, &bPK, &bPM}) {
+            B.plan(h->ptr.size() * 8);
+            B.plan(h->col.size() * 4);
+            B.plan(h->val.size() * 8);
+        }
+        B.commit();
+        SparseOp* ops[4] = {&opK, &opM, &opPK, &opPM};
+        const Bsr3Host* hs[4] = {&bK, &bM, &bPK, &bPM};
+        for (int q = 0; q < 4; ++q) {
+            int64_t* pp = B.take<int64_t>(hs[q]->ptr.size());
+            int32_t* pc = B.take<int32_t>(hs[q]->col.size());
+            double* pv = B.take<double>(hs[q]->val.size());
+            up(pp, hs[q]->ptr.data(), hs[q]->ptr.size() * 8);
+            up(pc, hs[q]->col.data(), hs[q]->col.size() * 4);
+            up(pv, hs[q]->val.data(), hs[q]->val.size() * 8);
+            ops[q]->nb = n / 3;
+            ops[q]->bptr = pp;
+            ops[q]->bcol = pc;
+            ops[q]->bval = pv;
+        }
+    }
     sync();   // host vectors above go out of scope
 }
 
@@ -1477,11 +1609,9 @@ void Ctx::backtransform() {
     }
     const int64_t nn = n;
     int64_t* perm = dperm;
-    int64_t *mp = dMp, *mi = dMi;
-    double* mx = dMx;
     double *zf = dZfull, *xo_ = dXout;
     plan.fn([=](hipStream_t s) { launch_permute_rows(nn, perm, zf, xo_, static_cast<int>(w), nn, s); });
-    addSpmm(plan, n, mp, mi, mx, static_cast<double>(Mp[n]), xo_, Ytmp, w, Tr1, Tr2);
+    addSpmm(plan, opM, xo_, Ytmp, w, Tr1, Tr2);
     const int64_t* seg = dSegRow;
     const int ns = static_cast<int>(nseg);
     plan.fn([=](hipStream_t s) {
@@ -1648,7 +1778,6 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     Plan plan(siBlob.get(), &kstats);
     const int qi = static_cast<int>(q);
     const int64_t nn = n;
-    const double nnzM = static_cast<double>(Mp[n]), nnzK = static_cast<double>(Kp[n]);
 
     // top-down application of L^-T: rows of node p become
     // Q(p) = Q(p) - X_p Q(B_p), ancestors first
@@ -1700,7 +1829,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     } else {
         AMLS_HIP_CHECK(hipMemcpyAsync(Q, dZfull, n * q * 8, hipMemcpyDeviceToDevice, stream));
     }
-    addSpmm(plan, n, dPMp, dPMi, dPMx, nnzM, Q, R, q, Tr1, Tr2);     // R = M Q
+    addSpmm(plan, opPM, Q, R, q, Tr1, Tr2);     // R = M Q
 
     for (int it = 0; it < steps; ++it) {
         const bool last = it + 1 == steps;
@@ -1792,7 +1921,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
             cw.add(cdesc(T + to[p], S.f(p), Yv + S.off[p], n, S.size[p], q, 0));
         plan.copy(cw);
         addBacksolve(Yv);
-        addSpmm(plan, n, dPMp, dPMi, dPMx, nnzM, Yv, MY, q, Tr1, Tr2);   // MY = M Y
+        addSpmm(plan, opPM, Yv, MY, q, Tr1, Tr2);   // MY = M Y
         // scale every column of Y and M Y to unit M-norm (per problem): the
         // next step applies K^-1 to (M Y) S, the Rayleigh-Ritz at the end
         // projects onto Y S.  A full M-orthonormalisation is not needed: it
@@ -1807,7 +1936,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     // ---- final Rayleigh-Ritz with the original FP64 K and M, per problem:
     // Kc = Y^T K Y, Mc = Y^T M Y; lowest nev pairs of (Kc, Mc) through
     // Kc = L D L^T, C = D^-1/2 L^-1 Mc L^-T D^-1/2, nu = 1/theta
-    addSpmm(plan, n, dPKp, dPKi, dPKx, nnzK, Yv, R, q, Tr1, Tr2);
+    addSpmm(plan, opPK, Yv, R, q, Tr1, Tr2);
     {
         GemmBatch gr;
         for (int64_t sg = 0; sg < NS; ++sg) {
@@ -1924,20 +2053,18 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     }
     dXout = outBuf;
     int64_t* perm = dperm;
-    int64_t *mp = dMp, *mi = dMi;
-    double* mx = dMx;
     double* xo_ = dXout;
     double* nrm = dsq;                     // NS x nev <= NS q
     const int64_t* seg = dSegRow;
     const int nsg = static_cast<int>(NS);
     plan.fn([=](hipStream_t s) { launch_permute_rows(nn, perm, Q, xo_, nv, nn, s); });
-    addSpmm(plan, n, mp, mi, mx, static_cast<double>(Mp[n]), xo_, MY, nev, Tr1, Tr2);
+    addSpmm(plan, opM, xo_, MY, nev, Tr1, Tr2);
     plan.fn([=](hipStream_t s) { launch_colnorm_seg(seg, nsg, xo_, MY, nv, nn, nrm, s); });
     // theta_j = x_j^T K x_j of the M-normalised vectors: the Ritz value
     // restated as the Rayleigh quotient of the vector actually returned, so
     // the pair stays consistent to rounding however the projected pencil
     // was conditioned
-    addSpmm(plan, n, dKp, dKi, dKx, nnzK, xo_, MY, nev, Tr1, Tr2);
+    addSpmm(plan, opK, xo_, MY, nev, Tr1, Tr2);
     plan.fn([=](hipStream_t s) { launch_coldot_seg(seg, nsg, xo_, MY, nv, nn, th, s); });
     plan.run(stream);
     thetaOut.assign(NS * nev, 0.0);

@@ -111,6 +111,20 @@ struct KernelStats {
     ~KernelStats();
 };
 
+// A sparse symmetric operator for Y = A X: CSR (rp, ci, vx; CSC of a
+// symmetric matrix is its CSR), plus a 3 x 3 block form (nb > 0) when every
+// aligned row triple shares one pattern of aligned column triples.
+struct SparseOp {
+    int64_t n = 0;
+    const int64_t *rp = nullptr, *ci = nullptr;
+    const double* vx = nullptr;
+    double nnz = 0.0;
+    int64_t nb = 0;
+    const int64_t* bptr = nullptr;
+    const int32_t* bcol = nullptr;
+    const double* bval = nullptr;
+};
+
 struct Ctx {
     int device = -1;
     bool gpu = false;
@@ -139,6 +153,10 @@ struct Ctx {
     int64_t nasmK = 0, nasmM = 0;
     int64_t *dperm = nullptr, *dbidx = nullptr, *drel = nullptr;
     int64_t* dSegRow = nullptr;   // segRow on the device
+    // K and M as operators, original and permuted ordering (3 x 3 blocks when
+    // the pattern has them)
+    SparseOp opK, opM, opPK, opPM;
+    std::unique_ptr<DevArena> bsrMem;
     // permuted CSR of K and M (for the subspace iteration)
     int64_t *dPKp = nullptr, *dPKi = nullptr, *dPMp = nullptr, *dPMi = nullptr;
     double *dPKx = nullptr, *dPMx = nullptr;
