@@ -133,9 +133,11 @@ void gemmWork(const GemmBatch& b, bool fp32, double& flops, double& bytes) {
         const double m = d.m, n = d.n;
         double mn = m * n;
         if (d.lower && m >= n) mn = m * n - n * (n - 1) / 2;
-        const double k = (d.A ? d.k : 0) + (d.A2 ? d.k2 : 0);
-        flops += 2.0 * mn * k;
-        bytes += (m * k + k * n + 2.0 * mn) * w;
+        const double k1 = d.A ? d.k : 0, k2 = d.A2 ? d.k2 : 0;
+        // a triangular op(A) (m = k) meets only half of k on average
+        const double k1eff = d.triA ? 0.5 * (k1 + 1) : k1;
+        flops += 2.0 * mn * (k1eff + k2);
+        bytes += ((d.triA ? 0.5 : 1.0) * m * k1 + m * k2 + (k1 + k2) * n + 2.0 * mn) * w;
     }
 }
 
@@ -1040,6 +1042,7 @@ void Ctx::runGE(int prec) {
                 g.A = Li;
                 g.lda = np;
                 g.ta = 1;
+                g.triA = 2;                    // L^-T is upper triangular
                 g.B = F + np;
                 g.ldb = f;
                 g.tb = 1;
@@ -1153,6 +1156,7 @@ void Ctx::spectralPrepare() {
         GemmDesc a = gdesc();                                  // T = L^-1 Mhat
         a.A = Li;
         a.lda = np;
+        a.triA = 1;
         a.B = C;
         a.ldb = np;
         a.k = static_cast<int32_t>(np);
@@ -1160,16 +1164,20 @@ void Ctx::spectralPrepare() {
         a.ldc = np;
         a.m = a.n = static_cast<int32_t>(np);
         g1.add(a);
-        GemmDesc b = gdesc();                                  // C = T L^-T
-        b.A = T;
+        // C = T L^-T is symmetric, so C = (T L^-T)^T = L^-1 T^T: triangular
+        // A, and only the lower half (the symmetric completion follows)
+        GemmDesc b = gdesc();
+        b.A = Li;
         b.lda = np;
-        b.B = Li;
+        b.triA = 1;
+        b.B = T;
         b.ldb = np;
         b.tb = 1;
         b.k = static_cast<int32_t>(np);
         b.C = C;
         b.ldc = np;
         b.m = b.n = static_cast<int32_t>(np);
+        b.lower = 1;
         g2.add(b);
         c3.add(cdesc(C, np, C, np, np, np, 0, dDsq + vo[p], dDsq + vo[p]));
         c4.add(cdesc(C, np, C, np, np, np, 2));                // exact symmetry
@@ -1311,6 +1319,7 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
         g.A = dLinv + co[p];
         g.lda = np;
         g.ta = 1;
+        g.triA = 2;
         g.B = dY + zo[p];
         g.ldb = np;
         g.k = static_cast<int32_t>(np);
@@ -1888,6 +1897,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                 GemmDesc a = gdesc();
                 a.A = dLinv + co[p];
                 a.lda = np;
+                a.triA = 1;
                 a.B = Tp;
                 a.ldb = f;
                 a.k = static_cast<int32_t>(np);
@@ -1901,6 +1911,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                 b2.A = dLinv + co[p];
                 b2.lda = np;
                 b2.ta = 1;
+                b2.triA = 2;
                 b2.B = Tr1 + S.off[p];
                 b2.ldb = n;
                 b2.k = static_cast<int32_t>(np);

@@ -24,8 +24,9 @@ struct GemmDesc {
     int32_t m, n, k, k2;
     int32_t ta, tb, ta2, tb2;
     double alpha1, alpha2, beta;
-    int32_t lower;
-    int32_t pad;
+    int32_t lower;      // write only the lower triangle of C (gr >= gc)
+    int32_t triA;       // op(A) of the first product is 1: lower, 2: upper
+                        // triangular (square, m = k): its zero half is skipped
 };
 
 // One 64 x 64 output tile of descriptor `desc`.

@@ -2,7 +2,7 @@
 //
 // Tile: 64 x 64 outputs per 256-thread workgroup (four wave64s, each a 32 x 32
 // quarter made of 2 x 2 MFMA 16x16 blocks).  Operands are staged through LDS
-// in 16-deep k-chunks.  The FP64 path uses v_mfma_f64_16x16x4f64; the mixed
+// in 32-deep k-chunks, the next chunk prefetched into registers.  The FP64 path uses v_mfma_f64_16x16x4f64; the mixed
 // path converts the FP64 operands to FP32 while staging and runs
 // v_mfma_f32_16x16x4f32 with FP32 accumulation.
 //
@@ -37,8 +37,9 @@ template <> struct Mfma<float> {
 };
 
 constexpr int TM = 64;      // tile rows / cols
-constexpr int KC = 16;      // k-chunk staged in LDS
-constexpr int LDSP = 80;    // padded LDS row (elements)
+constexpr int KC = 32;      // k-chunk staged in LDS
+constexpr int LDSP = 65;    // padded LDS row (elements): odd, so the k-major stores of
+                            // transposed operands spread over the banks
 
 template <typename T>
 struct TileAcc {
@@ -53,6 +54,8 @@ struct TileAcc {
 // Accumulate op(A)[0:mv, kb:ke] * op(B)[kb:ke, 0:nv] (scaled by `scale`)
 // into the tile accumulator.  A, B point at the tile's first row / column
 // (see the element formulas below); out-of-range elements read as zero.
+// The next k-chunk is fetched into registers while the matrix cores work on
+// the chunk in LDS, so memory latency overlaps the MFMA issue.
 template <typename T>
 __device__ void tile_mma(TileAcc<T>& acc, const double* __restrict__ A, int64_t lda,
                          int ta, int mv, const double* __restrict__ B, int64_t ldb,
@@ -61,52 +64,49 @@ __device__ void tile_mma(TileAcc<T>& acc, const double* __restrict__ A, int64_t 
     const int lane = t & 63;
     const int w = t >> 6;
     const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
+    // element j of this thread's share of a chunk: (row or column, k offset)
+    // coalesced along the contiguous dimension of each operand
+    auto aIdx = [&](int j, int& r, int& kk) {
+        if (ta == 0) { r = t & 63; kk = (t >> 6) + 4 * j; }
+        else { kk = t & 31; r = (t >> 5) + 8 * j; }
+    };
+    auto bIdx = [&](int j, int& c, int& kk) {
+        if (tb == 0) { kk = t & 31; c = (t >> 5) + 8 * j; }
+        else { c = t & 63; kk = (t >> 6) + 4 * j; }
+    };
+    double ra[8], rb[8];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            int r, kk;
+            aIdx(j, r, kk);
+            const int kg = k0 + kk;
+            ra[j] = (r < mv && kg < ke) ? (ta == 0 ? A[r + (int64_t)kg * lda] : A[kg + (int64_t)r * lda])
+                                        : 0.0;
+            int c, kb2;
+            bIdx(j, c, kb2);
+            const int kg2 = k0 + kb2;
+            rb[j] = (c < nv && kg2 < ke) ? (tb == 0 ? B[kg2 + (int64_t)c * ldb] : B[c + (int64_t)kg2 * ldb])
+                                         : 0.0;
+        }
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            int r, kk;
+            aIdx(j, r, kk);
+            sA[kk * LDSP + r] = T(ra[j] * scale);
+            int c, kb2;
+            bIdx(j, c, kb2);
+            sB[kb2 * LDSP + c] = T(rb[j]);
+        }
+    };
+    if (kb >= ke) return;
+    fetch(kb);
     for (int k0 = kb; k0 < ke; k0 += KC) {
-        // ---- stage A chunk: sA[kk][r] = op(A)(r, k0 + kk)
-        if (ta == 0) {
-            const int r = t & 63;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int kk = (t >> 6) + 4 * j;
-                const int kg = k0 + kk;
-                double v = 0.0;
-                if (r < mv && kg < ke) v = A[r + (int64_t)kg * lda] * scale;
-                sA[kk * LDSP + r] = T(v);
-            }
-        } else {
-            const int kk = t & 15;
-            const int kg = k0 + kk;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int r = (t >> 4) + 16 * j;
-                double v = 0.0;
-                if (r < mv && kg < ke) v = A[kg + (int64_t)r * lda] * scale;
-                sA[kk * LDSP + r] = T(v);
-            }
-        }
-        // ---- stage B chunk: sB[kk][c] = op(B)(k0 + kk, c)
-        if (tb == 0) {
-            const int kk = t & 15;
-            const int kg = k0 + kk;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int c = (t >> 4) + 16 * j;
-                double v = 0.0;
-                if (c < nv && kg < ke) v = B[kg + (int64_t)c * ldb];
-                sB[kk * LDSP + c] = T(v);
-            }
-        } else {
-            const int c = t & 63;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int kk = (t >> 6) + 4 * j;
-                const int kg = k0 + kk;
-                double v = 0.0;
-                if (c < nv && kg < ke) v = B[c + (int64_t)kg * ldb];
-                sB[kk * LDSP + c] = T(v);
-            }
-        }
+        stage();
         __syncthreads();
+        if (k0 + KC < ke) fetch(k0 + KC);
 #pragma unroll
         for (int ks = 0; ks < KC / 4; ++ks) {
             const int kr = ks * 4 + (lane >> 4);
@@ -154,7 +154,11 @@ __global__ void __launch_bounds__(256) gemm_batched_kernel(const GemmDesc* __res
     if (d.k > 0 && d.A) {
         const double* A = d.ta ? d.A + (int64_t)r0 * d.lda : d.A + r0;
         const double* B = d.tb ? d.B + c0 : d.B + (int64_t)c0 * d.ldb;
-        tile_mma<T>(acc, A, d.lda, d.ta, mv, B, d.ldb, d.tb, nv, 0, d.k, d.alpha1, sA, sB);
+        // triangular op(A): row block tm only meets k < (tm+1) 64 (lower) or
+        // k >= tm 64 (upper)
+        const int kb = d.triA == 2 ? r0 : 0;
+        const int ke = d.triA == 1 ? min(d.k, r0 + TM) : d.k;
+        tile_mma<T>(acc, A, d.lda, d.ta, mv, B, d.ldb, d.tb, nv, kb, ke, d.alpha1, sA, sB);
     }
     if (d.k2 > 0 && d.A2) {
         const double* A = d.ta2 ? d.A2 + (int64_t)r0 * d.lda2 : d.A2 + r0;
