@@ -34,21 +34,30 @@ __global__ void __launch_bounds__(256) ldlt_diag_kernel(const FrontDesc* __restr
         sD[i * (NB + 1) + j] = v;
     }
     __syncthreads();
+    // right-looking with the columns left unscaled: step j subtracts
+    // a_ij a_kj / d_j from every (i, k) with i >= k > j; thread t owns column
+    // t & 63 and rows (t >> 6) + 4 r, so one barrier per step suffices (column
+    // j is only read in step j, and d_{j+1} is final after it)
+    const int kc = t & 63, i0 = t >> 6;
     for (int j = 0; j < jb; ++j) {
         const double dj = sD[j * (NB + 1) + j];
         if (t == 0 && (!(dj > 0.0) || !(dj < INFINITY))) atomicMin(err, d.node);
-        const int m = jb - j - 1;
-        for (int e = t; e < m * m; e += 256) {
-            const int i = j + 1 + e / m, k = j + 1 + e % m;
-            if (k <= i) sD[i * (NB + 1) + k] -= sD[i * (NB + 1) + j] * sD[k * (NB + 1) + j] / dj;
+        const double rdj = 1.0 / dj;
+        if (kc > j && kc < jb) {
+            const double akj = sD[kc * (NB + 1) + j] * rdj;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = i0 + 4 * r;
+                if (i >= kc && i < jb) sD[i * (NB + 1) + kc] -= sD[i * (NB + 1) + j] * akj;
+            }
         }
         __syncthreads();
-        for (int i = j + 1 + t; i < jb; i += 256) sD[i * (NB + 1) + j] /= dj;
-        __syncthreads();
     }
+    // L_ij = a_ij / d_j
     for (int e = t; e < jb * jb; e += 256) {
         const int i = e % jb, j = e / jb;
-        if (i >= j) F[(s0 + i) + (int64_t)(s0 + j) * f] = sD[i * (NB + 1) + j];
+        if (i > j) F[(s0 + i) + (int64_t)(s0 + j) * f] = sD[i * (NB + 1) + j] / sD[j * (NB + 1) + j];
+        else if (i == j) F[(s0 + i) + (int64_t)(s0 + j) * f] = sD[i * (NB + 1) + j];
     }
 }
 
