@@ -159,6 +159,15 @@ struct Plan {
 
     void gemm(const GemmBatch& b, bool fp32) {
         if (b.empty()) return;
+        // split-k products accumulate into C: clear it first when beta = 0
+        Copy2dBatch clr;
+        for (const auto& d : b.descs)
+            if (d.kchunk > 0 && d.beta == 0.0) {
+                Copy2dDesc c = cdesc(nullptr, 0, d.C, d.ldc, d.m, d.n, 3, nullptr, nullptr, 0.0);
+                c.pad = d.lower;
+                clr.add(c);
+            }
+        copy(clr);
         const size_t od = blob->add(b.descs), ot = blob->add(b.tiles);
         const int nt = static_cast<int>(b.tiles.size());
         Blob* bl = blob;
@@ -598,16 +607,32 @@ void Blob::upload(hipStream_t s) {
     AMLS_HIP_CHECK(hipMemcpyAsync(dev_, host_.data(), host_.size(), hipMemcpyHostToDevice, s));
 }
 
-void GemmBatch::add(const GemmDesc& d) {
-    if (d.m <= 0 || d.n <= 0) return;
+void GemmBatch::add(const GemmDesc& d0) {
+    if (d0.m <= 0 || d0.n <= 0) return;
+    GemmDesc d = d0;
     const int id = static_cast<int>(descs.size());
-    descs.push_back(d);
     const int tm = cdiv(d.m, 64), tn = cdiv(d.n, 64);
-    for (int i = 0; i < tm; ++i)
-        for (int j = 0; j < tn; ++j) {
-            if (d.lower && j > i) continue;
-            tiles.push_back({id, i, j, 0});
+    int ntile = 0;
+    for (int i = 0; i < tm; ++i) ntile += d.lower ? std::min(i + 1, tn) : tn;
+    // long-k products with few tiles (the Gram matrices of the subspace
+    // iteration: k = n) are split over k so the whole device takes part;
+    // the pieces add into C atomically (beta must be 0 or 1)
+    int nsplit = 1;
+    if (d.k >= 8192 && d.triA == 0 && !(d.A2 && d.k2 > 0) && (d.beta == 0.0 || d.beta == 1.0) &&
+        ntile < 2048) {
+        nsplit = std::min(cdiv(d.k, 2048), std::max(1, cdiv(2048, ntile)));
+        if (nsplit > 1) {
+            d.kchunk = cdiv(cdiv(d.k, nsplit), 32) * 32;
+            nsplit = cdiv(d.k, d.kchunk);
         }
+    }
+    descs.push_back(d);
+    for (int c = 0; c < nsplit; ++c)
+        for (int i = 0; i < tm; ++i)
+            for (int j = 0; j < tn; ++j) {
+                if (d.lower && j > i) continue;
+                tiles.push_back({id, i, j, c});
+            }
 }
 
 void Ctx::sync() {

@@ -27,6 +27,10 @@ struct GemmDesc {
     int32_t lower;      // write only the lower triangle of C (gr >= gc)
     int32_t triA;       // op(A) of the first product is 1: lower, 2: upper
                         // triangular (square, m = k): its zero half is skipped
+    int32_t kchunk;     // > 0: split-k, tile work item `pad` takes k in
+                        // [pad kchunk, (pad + 1) kchunk) and adds its part to C
+                        // atomically (C must already hold beta C)
+    int32_t pad0;
 };
 
 // One 64 x 64 output tile of descriptor `desc`.
@@ -96,7 +100,8 @@ struct Copy2dDesc {
     const double* cs;
     int64_t lds, ldd;
     int32_t rows, cols;
-    int32_t mode, pad;
+    int32_t mode, pad;  // mode 0 copy, 1 transpose, 2 symmetric completion
+                        // from the lower triangle, 3 fill with alpha
     double alpha;
 };
 

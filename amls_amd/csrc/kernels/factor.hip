@@ -159,17 +159,43 @@ __global__ void fill_kernel(double* __restrict__ p, int64_t count, double v) {
 // completion of src's lower triangle.
 __global__ void __launch_bounds__(256) copy2d_kernel(const Copy2dDesc* __restrict__ descs,
                                                      const int4* __restrict__ work) {
+    __shared__ double tileT[64 * 65];
     const int4 w = work[blockIdx.x];
     const Copy2dDesc d = descs[w.x];
     const int t = threadIdx.x;
+    if (d.mode == 1) {
+        // transpose through LDS: read along src's contiguous dimension, write
+        // along dst's
+        const int i0 = w.y * 64, j0 = w.z * 64;
+        for (int q = 0; q < 16; ++q) {
+            const int jj = t & 63, ii = (t >> 6) + 4 * q;   // src(j0 + jj, i0 + ii)
+            const int i = i0 + ii, j = j0 + jj;
+            tileT[ii * 65 + jj] = (i < d.rows && j < d.cols) ? d.src[j + (int64_t)i * d.lds] : 0.0;
+        }
+        __syncthreads();
+        const int i = i0 + (t & 63);
+        if (i >= d.rows) return;
+        const double rsv = d.rs ? d.rs[i] : 1.0;
+        for (int q = 0; q < 16; ++q) {
+            const int jj = (t >> 6) + 4 * q, j = j0 + jj;
+            if (j >= d.cols) break;
+            double s = d.alpha * rsv;
+            if (d.cs) s *= d.cs[j];
+            d.dst[i + (int64_t)j * d.ldd] = s * tileT[(t & 63) * 65 + jj];
+        }
+        return;
+    }
     const int i = w.y * 64 + (t & 63);
     if (i >= d.rows) return;
     for (int q = 0; q < 16; ++q) {
         const int j = w.z * 64 + (t >> 6) + 4 * q;
         if (j >= d.cols) break;
+        if (d.mode == 3) {   // fill; pad = 1 keeps to the lower triangle
+            if (!d.pad || i >= j) d.dst[i + (int64_t)j * d.ldd] = d.alpha;
+            continue;
+        }
         double v;
-        if (d.mode == 1) v = d.src[j + (int64_t)i * d.lds];
-        else if (d.mode == 2) v = i >= j ? d.src[i + (int64_t)j * d.lds] : d.src[j + (int64_t)i * d.lds];
+        if (d.mode == 2) v = i >= j ? d.src[i + (int64_t)j * d.lds] : d.src[j + (int64_t)i * d.lds];
         else v = d.src[i + (int64_t)j * d.lds];
         double s = d.alpha;
         if (d.rs) s *= d.rs[i];

@@ -155,9 +155,13 @@ __global__ void __launch_bounds__(256) gemm_batched_kernel(const GemmDesc* __res
         const double* A = d.ta ? d.A + (int64_t)r0 * d.lda : d.A + r0;
         const double* B = d.tb ? d.B + c0 : d.B + (int64_t)c0 * d.ldb;
         // triangular op(A): row block tm only meets k < (tm+1) 64 (lower) or
-        // k >= tm 64 (upper)
-        const int kb = d.triA == 2 ? r0 : 0;
-        const int ke = d.triA == 1 ? min(d.k, r0 + TM) : d.k;
+        // k >= tm 64 (upper); split-k: this work item's chunk of k
+        int kb = d.triA == 2 ? r0 : 0;
+        int ke = d.triA == 1 ? min(d.k, r0 + TM) : d.k;
+        if (d.kchunk > 0) {
+            kb = tr.pad * d.kchunk;
+            ke = min(d.k, kb + d.kchunk);
+        }
         tile_mma<T>(acc, A, d.lda, d.ta, mv, B, d.ldb, d.tb, nv, kb, ke, d.alpha1, sA, sB);
     }
     if (d.k2 > 0 && d.A2) {
@@ -169,11 +173,13 @@ __global__ void __launch_bounds__(256) gemm_batched_kernel(const GemmDesc* __res
     const int lower = d.lower;
     double* C = d.C;
     const int64_t ldc = d.ldc;
+    const bool split = d.kchunk > 0;
     tile_foreach<T>(acc, [&](int r, int c, double v) {
         const int gr = r0 + r, gc = c0 + c;
         if (r < mv && c < nv && (!lower || gr >= gc)) {
             double* p = C + gr + (int64_t)gc * ldc;
-            *p = beta == 0.0 ? v : v + beta * *p;
+            if (split) atomicAdd(p, v);
+            else *p = beta == 0.0 ? v : v + beta * *p;
         }
     });
 }
