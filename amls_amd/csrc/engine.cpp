@@ -324,21 +324,52 @@ struct SytrdPlan {
 // soff[m]: per-matrix offset in units of its n (sv holds 2 n and sp 3 n
 // doubles per matrix from 2 soff / 3 soff on)
 void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<int64_t>& soff,
-              double* sv, double* sp) {
+              double* sv, double* sp, SytrdCache* cache = nullptr) {
     std::vector<int> ns;
     int64_t spLen = 0;
     for (size_t m = 0; m < descs.size(); ++m) {
         ns.push_back(descs[m].n);
         spLen = std::max<int64_t>(spLen, 3 * (soff[m] + descs[m].n));
     }
-    SytrdPlan sp_;
-    sp_.build(ns);
-    if (sp_.steps == 0) return;
-    const size_t od = plan.blob->add(descs), ow = plan.blob->add(sp_.work),
-                 oo = plan.blob->add(soff);
+    const bool one = descs.size() == 1;
+    std::vector<size_t> start;
+    std::vector<int> count;
+    size_t ow = 0;
+    const int4* cached = nullptr;
+    if (one) {
+        // steps computed from the block index (sytrd_tile1_kernel)
+        start.assign(static_cast<size_t>(ns[0]), 0);
+        count.assign(static_cast<size_t>(ns[0]), 0);
+    } else if (cache) {
+        if (cache->ns != ns) {
+            SytrdPlan sp_;
+            sp_.build(ns);
+            const size_t bytes = sp_.work.size() * sizeof(int4);
+            if (bytes > cache->cap) {
+                if (cache->work) AMLS_HIP_CHECK(hipFree(cache->work));
+                cache->work = nullptr;
+                AMLS_HIP_CHECK(hipMalloc(&cache->work, std::max<size_t>(bytes, 16)));
+                cache->cap = bytes;
+            }
+            if (bytes)
+                AMLS_HIP_CHECK(hipMemcpy(cache->work, sp_.work.data(), bytes, hipMemcpyHostToDevice));
+            cache->ns = ns;
+            cache->start = sp_.start;
+            cache->count = sp_.count;
+        }
+        start = cache->start;
+        count = cache->count;
+        cached = cache->work;
+    } else {
+        SytrdPlan sp_;
+        sp_.build(ns);
+        ow = plan.blob->add(sp_.work);
+        start = sp_.start;
+        count = sp_.count;
+    }
+    if (start.empty()) return;
+    const size_t od = plan.blob->add(descs), oo = plan.blob->add(soff);
     Blob* bl = plan.blob;
-    const auto start = sp_.start;
-    const auto count = sp_.count;
     // algorithmic traffic per step: each active matrix reads and writes the
     // lower triangle of its trailing block once
     double bytes = 0.0;
@@ -347,13 +378,20 @@ void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<i
             if (v > i) bytes += 8.0 * double(v - i - 1) * double(v - i);
     KernelStats* kst = plan.st;
     const int64_t nsteps = static_cast<int64_t>(start.size());
+    const EigDesc d1 = descs[0];
+    double* sv1 = sv + 2 * soff[0];
+    double* sp1 = sp + 3 * soff[0];
     plan.fn([=](hipStream_t s) {
         launch_fill(sp, spLen, 0.0, s);
+        const int4* work = cached ? cached : (one ? nullptr : bl->at<int4>(ow));
         // the steps run back to back; one event pair brackets the loop
         auto loop = [&] {
-            for (int i = 0; i < static_cast<int>(start.size()); ++i)
-                launch_sytrd_step(bl->at<EigDesc>(od), bl->at<int4>(ow) + start[i], count[i], i,
-                                  sv, sp, bl->at<int64_t>(oo), s);
+            for (int i = 0; i < static_cast<int>(start.size()); ++i) {
+                if (one) launch_sytrd_step1(d1, i, sv1, sp1, s);
+                else
+                    launch_sytrd_step(bl->at<EigDesc>(od), work + start[i], count[i], i, sv, sp,
+                                      bl->at<int64_t>(oo), s);
+            }
         };
         if (kst) kst->run("sytrd_step", 2.0 * bytes / 8.0, bytes, s, loop, nsteps);
         else loop();
@@ -524,12 +562,25 @@ void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
 DevArena::~DevArena() { release(); }
 
 void DevArena::commit() {
-    release();
     // every take() rounds its request to 256 bytes; leave room for that
     planned_ += 64 * 1024;
+    used_ = 0;
+    if (base_ && planned_ <= cap_) return;   // re-planned phase: reuse the block
+    release();
     AMLS_HIP_CHECK(hipMalloc(&base_, planned_));
     cap_ = planned_;
-    used_ = 0;
+}
+
+Blob& fresh(std::unique_ptr<Blob>& b) {
+    if (!b) b.reset(new Blob());
+    b->clear();
+    return *b;
+}
+
+DevArena& fresh(std::unique_ptr<DevArena>& a) {
+    if (!a) a.reset(new DevArena());
+    a->begin();
+    return *a;
 }
 
 void DevArena::release() {
@@ -780,8 +831,7 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
     permCsr(Kp, Ki, Kx, pkp, pki, pkx);
     permCsr(Mp, Mi, Mx, pmp, pmi, pmx);
 
-    probMem.reset(new DevArena());
-    auto& A = *probMem;
+    auto& A = fresh(probMem);
     const int64_t nk = Kp[n], nm = Mp[n];
     A.plan(nk * 8 * 2 + (n + 1) * 8);        // Kx, Ki, Kp
     A.plan(nm * 8 * 2 + (n + 1) * 8);
@@ -855,8 +905,7 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
                         buildBsr3(n, pmp.data(), pmi.data(), pmx.data(), bPM);
     bsrMem.reset();
     if (blocks) {
-        bsrMem.reset(new DevArena());
-        auto& B = *bsrMem;
+        auto& B = fresh(bsrMem);
         for (const Bsr3Host* h : {&bK, &bM, &bPK, &bPM}) {
             B.plan(h->ptr.size() * 8);
             B.plan(h->col.size() * 4);
@@ -891,8 +940,7 @@ void Ctx::runGE(int prec) {
     geDone = specReady = selected = projected = reduced = recovered = false;
     gePrec = prec;
     const int64_t N = S.nNodes;
-    geMem.reset(new DevArena());
-    auto& A = *geMem;
+    auto& A = fresh(geMem);
     A.plan(ftot * 8);
     A.plan(ftot * 8);
     A.plan(xtot * 8);
@@ -920,7 +968,7 @@ void Ctx::runGE(int prec) {
     launch_scatter(dsrcM, dtgtM, dMx, dG, nasmM, 1.0, false, stream);
     if (shift != 0.0) launch_scatter(dsrcM, dtgtM, dMx, dF, nasmM, shift, true, stream);
 
-    geBlob.reset(new Blob());
+    fresh(geBlob);
     Plan plan(geBlob.get(), &kstats);
     const bool f32 = prec != 0;
     for (int64_t l = 0; l <= S.maxLevel; ++l) {
@@ -1151,8 +1199,7 @@ void Ctx::spectralPrepare() {
     if (!geDone) throw StatusError(AMLS_E_STATE, "spectral_prepare before ge");
     specReady = selected = projected = reduced = recovered = false;
     const int64_t N = S.nNodes;
-    spMem.reset(new DevArena());
-    auto& A = *spMem;
+    auto& A = fresh(spMem);
     A.plan(co[N] * 8 * 2);
     A.plan(vo[N] * 8 * 3);
     A.plan(vo[N] * 8 * 5);
@@ -1164,7 +1211,7 @@ void Ctx::spectralPrepare() {
     dTau = A.take<double>(vo[N]);
     dSv = A.take<double>(2 * vo[N]);
     dSp = A.take<double>(3 * vo[N]);
-    spBlob.reset(new Blob());
+    fresh(spBlob);
     Plan plan(spBlob.get(), &kstats);
     // C_i = D^-1/2 L^-1 Mhat L^-T D^-1/2 with the explicit L^-1 (two GEMMs)
     Copy2dBatch c1, c3, c4;
@@ -1224,7 +1271,7 @@ void Ctx::spectralPrepare() {
     plan.gemm(g2, false);
     plan.copy(c3);
     plan.copy(c4);
-    addSytrd(plan, ed, soff, dSv, dSp);
+    addSytrd(plan, ed, soff, dSv, dSp, &syLocal);
     plan.run(stream);
     sync();
     specReady = true;
@@ -1294,8 +1341,7 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
         segP[sg] = segMoff[sg + 1] - segMoff[sg];
         mto[sg + 1] = mto[sg] + segP[sg] * segP[sg];
     }
-    passMem.reset(new DevArena());
-    auto& A = *passMem;
+    auto& A = fresh(passMem);
     A.plan(ptot * 8 * 3);
     A.plan(zo[N] * 8 * 2);
     A.plan(wk * 8);
@@ -1306,7 +1352,7 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
     dY = A.take<double>(zo[N]);
     dZ = A.take<double>(zo[N]);
     double* dWk = A.take<double>(wk);
-    passBlob.reset(new Blob());
+    fresh(passBlob);
     Plan plan(passBlob.get(), &kstats);
     std::vector<EigDesc> ed;
     std::vector<VecDesc> vd;
@@ -1383,15 +1429,13 @@ void Ctx::project(int prec) {
             throw StatusError(AMLS_E_NOCONV, "reduced problem is empty (every node truncated to 0)");
     // R fronts and Mtilde (one p_s x p_s block per problem) go to a second
     // arena tied to the pass
-    auto* A = new DevArena();
-    std::unique_ptr<DevArena> hold(A);
+    DevArena* A = &fresh(projMem);
     A->plan(ro[N] * 8);
     A->plan(mto[nseg] * 8);
     A->commit();
     dR = A->take<double>(ro[N]);
     dMt = A->take<double>(mto[nseg]);
-    projMem.swap(hold);
-    passBlob2.reset(new Blob());
+    fresh(passBlob2);
     Plan plan(passBlob2.get(), &kstats);
     const bool f32 = prec != 0;
     plan.fn([this](hipStream_t s) {
@@ -1497,8 +1541,7 @@ void Ctx::reducedSolve(int64_t w, bool vectors) {
     want = w;
     // one matrix per problem: S_s = mu^-1/2 Mtilde_s mu^-1/2 (p_s x p_s)
     const int64_t P = ptot, NS = nseg;
-    redMem.reset(new DevArena());
-    auto& A = *redMem;
+    auto& A = fresh(redMem);
     A.plan(mto[NS] * 8);
     A.plan(P * 8 * 9);
     A.plan(NS * w * 8 * 3);
@@ -1518,7 +1561,7 @@ void Ctx::reducedSolve(int64_t w, bool vectors) {
     double* Y = A.take<double>(P * w);
     dQ = A.take<double>(P * w);
     double* wk = vectors ? A.take<double>(w * 6 * P) : nullptr;
-    redBlob.reset(new Blob());
+    fresh(redBlob);
     Plan plan(redBlob.get(), &kstats);
     std::vector<VecDesc> v1(1);
     std::memset(v1.data(), 0, sizeof(VecDesc));
@@ -1561,7 +1604,7 @@ void Ctx::reducedSolve(int64_t w, bool vectors) {
         cq.add(cdesc(Y + m0 * w, ps, dQ + m0, P, ps, w, 0, murs + m0, sth + sg * w));
     }
     plan.copy(cs);
-    addSytrd(plan, eds, soff, sv, sp);
+    addSytrd(plan, eds, soff, sv, sp, &syReduced);
     addSelect(plan, eds, vectors);
     plan.vec(v2);
     if (vectors) plan.copy(cq);
@@ -1583,8 +1626,7 @@ void Ctx::backtransform() {
     const int64_t N = S.nNodes, w = want;
     std::vector<int64_t> zbo(N + 1, 0);
     for (int64_t p = 0; p < N; ++p) zbo[p + 1] = zbo[p] + S.b(p) * w;
-    vecMem.reset(new DevArena());
-    auto& A = *vecMem;
+    auto& A = fresh(vecMem);
     A.plan(n * w * 8 * 5);
     A.plan(zbo[N] * 8);
     A.plan(nseg * w * 8);
@@ -1596,7 +1638,7 @@ void Ctx::backtransform() {
     double* Tr2 = A.take<double>(n * w);
     double* zB = A.take<double>(zbo[N]);
     double* nrm = A.take<double>(nseg * w);
-    vecBlob.reset(new Blob());
+    fresh(vecBlob);
     Plan plan(vecBlob.get(), &kstats);
     for (int64_t l = S.maxLevel; l >= 0; --l) {
         GatherBatch gt;
@@ -1758,8 +1800,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
         }
         zbLevel = std::max(zbLevel, acc);
     }
-    siMem.reset(new DevArena());
-    auto& A = *siMem;
+    auto& A = fresh(siMem);
     A.plan(n * q * 8 * 6);                 // Q, R, Y, MY, two transposes
     A.plan(to[N] * 8);                     // fronts of the forward solve
     A.plan(zbLevel * 8);                   // gathered x_B of one level
@@ -1808,7 +1849,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     int* err = A.take<int>(1);
     const int big = INT_MAX;
     AMLS_HIP_CHECK(hipMemcpyAsync(err, &big, sizeof(int), hipMemcpyHostToDevice, stream));
-    siBlob.reset(new Blob());
+    fresh(siBlob);
     Plan plan(siBlob.get(), &kstats);
     const int qi = static_cast<int>(q);
     const int64_t nn = n;

@@ -40,6 +40,9 @@ public:
     DevArena& operator=(const DevArena&) = delete;
     ~DevArena();
     void plan(size_t bytes) { planned_ += (bytes + 255) & ~size_t(255); }
+    // start planning a new layout; commit() keeps the current block when it
+    // is large enough, so a phase run again does not reallocate
+    void begin() { planned_ = used_ = 0; }
     void commit();   // allocate exactly what was planned
     template <class T> T* take(size_t count) {
         const size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
@@ -57,6 +60,9 @@ private:
 };
 
 // Host staging of descriptor arrays, uploaded in one copy per phase.
+class DevArena;
+DevArena& fresh(std::unique_ptr<DevArena>& a);   // create or begin()
+
 class Blob {
 public:
     template <class T> size_t add(const std::vector<T>& v) {
@@ -66,6 +72,7 @@ public:
         return off;
     }
     void upload(hipStream_t s);
+    void clear() { host_.clear(); }   // new contents; the device buffer is kept
     template <class T> const T* at(size_t off) const {
         return reinterpret_cast<const T*>(dev_ + off);
     }
@@ -76,6 +83,8 @@ private:
     char* dev_ = nullptr;
     size_t devcap_ = 0;
 };
+
+Blob& fresh(std::unique_ptr<Blob>& b);   // create or clear()
 
 struct GemmBatch {
     std::vector<dev::GemmDesc> descs;
@@ -125,8 +134,23 @@ struct SparseOp {
     const double* bval = nullptr;
 };
 
+// Work list of the batched tridiagonalisation (every active tile of every
+// step), kept on the device between solves of the same problem: it depends
+// only on the matrix orders.
+struct SytrdCache {
+    std::vector<int> ns;
+    int4* work = nullptr;
+    size_t cap = 0;
+    std::vector<size_t> start;
+    std::vector<int> count;
+    ~SytrdCache() {
+        if (work) (void)hipFree(work);
+    }
+};
+
 struct Ctx {
     int device = -1;
+    SytrdCache syLocal, syReduced;
     bool gpu = false;
     hipStream_t stream = nullptr;
     std::string err;

@@ -43,25 +43,23 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // matrix's leader, which writes d, e, tau, v_i and the previous reflector).
 constexpr int SY_KR = 8;   // strided rows per thread held in registers (n - i <= 2048)
 
-__global__ void __launch_bounds__(256) sytrd_tile_kernel(const EigDesc* __restrict__ descs,
-                                                         const int4* __restrict__ work, int i,
-                                                         double* __restrict__ sv_all,
-                                                         double* __restrict__ sp_all,
-                                                         const int64_t* __restrict__ soff) {
+// Step i for tile (wk.y, wk.z) of matrix d (wk.y < 0: the leader's last
+// step i = n - 1 only); sv_m / sp_m: the matrix's 2n / 3n scratch.
+__device__ __forceinline__ void sytrd_step_body(const EigDesc& d, int i, int4 wk,
+                                                double* __restrict__ sv_m,
+                                                double* __restrict__ sp_m) {
     __shared__ double tile[64 * 65];
     __shared__ double vr[64], wr[64], xr[64], vc[64], wc[64], xc[64];
     __shared__ double red[8];
-    const int4 wk = work[blockIdx.x];
-    const EigDesc d = descs[wk.x];
     const int n = d.n;
     const int64_t lda = d.lda;
     double* A = d.A;
     const int t = threadIdx.x;
-    const double* svr = sv_all + 2 * soff[wk.x] + (int64_t)((i + 1) & 1) * n;   // v_{i-1}
-    double* svw = sv_all + 2 * soff[wk.x] + (int64_t)(i & 1) * n;               // v_i
-    const double* spr = sp_all + 3 * soff[wk.x] + (int64_t)((i + 2) % 3) * n;   // p_{i-1}
-    double* spw = sp_all + 3 * soff[wk.x] + (int64_t)(i % 3) * n;               // p_i
-    double* spz = sp_all + 3 * soff[wk.x] + (int64_t)((i + 1) % 3) * n;         // p_{i+1}
+    const double* svr = sv_m + (int64_t)((i + 1) & 1) * n;   // v_{i-1}
+    double* svw = sv_m + (int64_t)(i & 1) * n;               // v_i
+    const double* spr = sp_m + (int64_t)((i + 2) % 3) * n;   // p_{i-1}
+    double* spw = sp_m + (int64_t)(i % 3) * n;               // p_i
+    double* spz = sp_m + (int64_t)((i + 1) % 3) * n;         // p_{i+1}
     const bool hasPrev = i > 0;
     const double* Ai = A + (int64_t)i * lda;   // column i (of A after update i-2)
 
@@ -202,6 +200,42 @@ __global__ void __launch_bounds__(256) sytrd_tile_kernel(const EigDesc* __restri
             if (r0 + rr != c) s += tile[rr * 65 + cc] * xr[rr];
         if (c > i && c < n) atomicAdd(spw + c, s);
     }
+}
+
+// Batched step: work item (matrix, tile row, tile col, leader flag).
+__global__ void __launch_bounds__(256) sytrd_tile_kernel(const EigDesc* __restrict__ descs,
+                                                         const int4* __restrict__ work, int i,
+                                                         double* __restrict__ sv_all,
+                                                         double* __restrict__ sp_all,
+                                                         const int64_t* __restrict__ soff) {
+    const int4 wk = work[blockIdx.x];
+    const EigDesc d = descs[wk.x];
+    sytrd_step_body(d, i, wk, sv_all + 2 * soff[wk.x], sp_all + 3 * soff[wk.x]);
+}
+
+// One matrix: the descriptor comes by value and the tile from the block
+// index (tiles R >= C >= t0 in row order, t0 = (i + 1) / 64), so a step
+// starts its data loads without first fetching work items and descriptors.
+// last != 0: the single-workgroup final step i = n - 1.
+__global__ void __launch_bounds__(256) sytrd_tile1_kernel(EigDesc d, int i, int t0, int last,
+                                                          double* __restrict__ sv,
+                                                          double* __restrict__ sp) {
+    int4 wk;
+    wk.x = 0;
+    if (last) {
+        wk.y = wk.z = -1;
+        wk.w = 1;
+    } else {
+        int idx = blockIdx.x, R = t0;
+        while (idx > R - t0) {
+            idx -= R - t0 + 1;
+            ++R;
+        }
+        wk.y = R;
+        wk.z = t0 + idx;
+        wk.w = (R == t0 && wk.z == t0) ? 1 : 0;
+    }
+    sytrd_step_body(d, i, wk, sv, sp);
 }
 
 // Sturm count: number of eigenvalues of T strictly below x (pivmin guarded).
@@ -610,6 +644,18 @@ __global__ void __launch_bounds__(256) ormtr_kernel(const EigDesc* __restrict__ 
 }
 
 }  // namespace dev
+
+void launch_sytrd_step1(const dev::EigDesc& d, int i, double* sv, double* sp, hipStream_t s) {
+    const int n = d.n;
+    if (i >= n) return;
+    if (i == n - 1) {
+        hipLaunchKernelGGL(dev::sytrd_tile1_kernel, dim3(1), dim3(256), 0, s, d, i, 0, 1, sv, sp);
+        return;
+    }
+    const int t0 = (i + 1) / 64, t1 = (n - 1) / 64;
+    const int nt = (t1 - t0 + 1) * (t1 - t0 + 2) / 2;
+    hipLaunchKernelGGL(dev::sytrd_tile1_kernel, dim3(nt), dim3(256), 0, s, d, i, t0, 0, sv, sp);
+}
 
 void launch_sytrd_step(const dev::EigDesc* d, const int4* work, int nwork, int i, double* sv,
                        double* sp, const int64_t* soff, hipStream_t s) {

@@ -28,6 +28,8 @@ void launch_gather(const dev::GatherDesc* d, const int4* work, int nwork, hipStr
 // of n per matrix and sp 3 (soff[m] = element offset / stride unit n_m).
 void launch_sytrd_step(const dev::EigDesc* d, const int4* work, int nwork, int i, double* sv,
                        double* sp, const int64_t* soff, hipStream_t s);
+// Step i of one matrix (descriptor by value; sv 2n, sp 3n doubles).
+void launch_sytrd_step1(const dev::EigDesc& d, int i, double* sv, double* sp, hipStream_t s);
 // counts[m] = number of eigenvalues of matrix m >= thr[m], capped at kcap
 // (nmax: the largest n)
 void launch_eig_count(const dev::EigDesc* d, int nmat, const double* thr, int32_t* counts,
