@@ -512,17 +512,28 @@ void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
     int64_t ktot = 0;
     for (const auto& d : descs) ktot += d.k;
     const int want = static_cast<int>(std::min<int64_t>(8, std::max<int64_t>(1, (ktot + 511) / 512)));
+    // inverse-iteration lanes per 64-wide workgroup: one lane carries one
+    // eigenvector's latency-bound recurrences, so a short list of
+    // eigenvalues is spread over more workgroups (16 lanes: each
+    // workgroup's scratch rows are one 128-byte line per array and its
+    // working set stays within one XCD's L2), a long one packs 64 lanes
+    const int slanes = ktot <= 8192 ? 16 : 64;
     for (int m = 0; m < static_cast<int>(descs.size()); ++m) {
         for (int j = 0; j < descs[m].k; ++j) {
             work.push_back(make_int2(m, j));
-            swork.push_back(make_int2(m, j));
             woff.push_back(0);
         }
+        for (int j = 0; j < descs[m].k; j += slanes)
+            for (int u = 0; u < 64; ++u)
+                swork.push_back(make_int2(m, u < slanes && j + u < descs[m].k ? j + u : -1));
         if (descs[m].k > 0) {
+            // the inverse iteration addresses its scratch (6 n k doubles) and
+            // Y with 32-bit byte offsets
+            if (vectors && (int64_t)48 * descs[m].n * descs[m].k >= ((int64_t)1 << 31))
+                throw StatusError(AMLS_E_ARG, "eigenvector scratch of one matrix exceeds 2 GiB");
             nmax = std::max(nmax, descs[m].n);
             // pad to whole workgroups: every workgroup serves one matrix
             while (work.size() % 4) work.push_back(make_int2(m, -1));
-            while (swork.size() % 64) swork.push_back(make_int2(m, -1));
         }
         // LDS-resident column blocks of at most 8 columns and 64 KiB
         const int n = descs[m].n;

@@ -338,12 +338,44 @@ __global__ void __launch_bounds__(64) eig_count_kernel(const EigDesc* __restrict
     }
 }
 
+// Two Sturm counts at once (x1, x2), interleaved so the two division chains
+// overlap.  e^2 / q uses the hardware reciprocal refined by two Newton steps
+// (relative error a few ulp, like a rounded division: the counts stay
+// those of a tiny relative perturbation of T, the property bisection needs;
+// |q| >= pivmin keeps the reciprocal finite).
+__device__ __forceinline__ double sturm_div(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    r = fma(r, fma(-b, r, 1.0), r);
+    return a * r;
+}
+
+__device__ void sturm_count2(const double* __restrict__ dd, const double* __restrict__ ee, int n,
+                             double x1, double x2, double pivmin, int& c1, int& c2) {
+    double q1 = dd[0] - x1, q2 = dd[0] - x2;
+    if (fabs(q1) < pivmin) q1 = -pivmin;
+    if (fabs(q2) < pivmin) q2 = -pivmin;
+    int n1 = q1 < 0.0, n2 = q2 < 0.0;
+    for (int j = 1; j < n; ++j) {
+        const double e = ee[j - 1], dj = dd[j];
+        const double e2 = e * e;
+        q1 = (dj - x1) - sturm_div(e2, q1);
+        q2 = (dj - x2) - sturm_div(e2, q2);
+        if (fabs(q1) < pivmin) q1 = -pivmin;
+        if (fabs(q2) < pivmin) q2 = -pivmin;
+        n1 += q1 < 0.0;
+        n2 += q2 < 0.0;
+    }
+    c1 = n1;
+    c2 = n2;
+}
+
 // lam[j] = (j+1)-th largest eigenvalue, j < k; one wave per work item
 // (x = matrix, y = j; the host pads each matrix's items to whole workgroups
 // of four with y = -1, so a workgroup stages one tridiagonal in LDS).
-// Multisection: each round the 64 lanes evaluate the Sturm count at 64
-// interior points of the current bracket, which shrinks the bracket 65-fold
-// per round (instead of 2-fold for plain bisection).
+// Multisection: each round the 64 lanes evaluate the Sturm count at 128
+// interior points of the current bracket (two per lane), which shrinks the
+// bracket 129-fold per round (instead of 2-fold for plain bisection).
 __global__ void __launch_bounds__(256) bisect_kernel(const EigDesc* __restrict__ descs,
                                                      const int2* __restrict__ work, int nwork,
                                                      int useLds) {
@@ -361,16 +393,18 @@ __global__ void __launch_bounds__(256) bisect_kernel(const EigDesc* __restrict__
     double lo, hi, pivmin, tn;
     gersh_wave(dd, ee, n, lane, lo, hi, pivmin, tn);
     const int a = n - 1 - j;    // ascending index
+    constexpr double SEC = 129.0;
     for (int round = 0; round < 16; ++round) {
         const double tol = 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
         if (hi - lo <= tol) break;
-        const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
-        const int c = sturm_count(dd, ee, n, x, pivmin);
-        // counts grow with x: lanes with c <= a form a prefix
-        const unsigned long long mask = __ballot(c <= a);
-        const int np = __popcll(mask);
-        const double nlo = np > 0 ? lo + (hi - lo) * (double)np / 65.0 : lo;
-        const double nhi = np < 64 ? lo + (hi - lo) * (double)(np + 1) / 65.0 : hi;
+        const double w = hi - lo;
+        int c1, c2;
+        sturm_count2(dd, ee, n, lo + w * (double)(lane + 1) / SEC, lo + w * (double)(lane + 65) / SEC,
+                     pivmin, c1, c2);
+        // counts grow with x: points with c <= a form a prefix of the 128
+        const int np = __popcll(__ballot(c1 <= a)) + __popcll(__ballot(c2 <= a));
+        const double nlo = np > 0 ? lo + w * (double)np / SEC : lo;
+        const double nhi = np < 128 ? lo + w * (double)(np + 1) / SEC : hi;
         if (!(nhi < hi || nlo > lo)) break;   // no progress at this resolution
         lo = nlo;
         hi = nhi;
@@ -381,33 +415,137 @@ __global__ void __launch_bounds__(256) bisect_kernel(const EigDesc* __restrict__
 // Inverse iteration.  Thread per eigenvalue; a thread whose eigenvalue starts
 // a cluster (gap to the previous one above ctol*|T|) computes the whole
 // cluster in order, orthogonalising each vector against the earlier ones.
-// Scratch of matrix m is interleaved by eigenvalue, element (r, j) of array
-// q at work[(q n + r) k + j], so a wave's accesses are contiguous.
-__global__ void __launch_bounds__(64) stein_kernel(const EigDesc* __restrict__ descs,
-                                                   const int2* __restrict__ work, int nwork,
-                                                   const int64_t* __restrict__ woff,
-                                                   int useLds) {
-    // every 64-item block belongs to one matrix (the host pads per matrix);
-    // its tridiagonal is staged in LDS when it fits (useLds)
-    extern __shared__ double sde[];
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    const int2 wk0 = work[blockIdx.x * blockDim.x];
-    const EigDesc d = descs[wk0.x];
-    const int n = d.n;
-    const double* dd = d.d;
-    const double* ee = d.e;
-    if (useLds) {
-        for (int r = threadIdx.x; r < n; r += blockDim.x) {
-            sde[r] = d.d[r];
-            sde[n + r] = r + 1 < n ? d.e[r] : 0.0;
+//
+// Every recurrence below walks the rows in order with a loop-carried
+// dependence and one lane carries each chain, so a step costs instruction
+// issue plus memory latency:
+//  - scratch of matrix m holds three arrays of 16-byte records, interleaved
+//    by eigenvalue (record (r, j) of array q at byte ((q n + r) k + j) 16):
+//    R0 = (u1, u2) of row r of U, R1 = (1 / pivot, multiplier),
+//    R2 = (interchange flag, x); a step loads whole records, so it issues
+//    few memory instructions and a wave's accesses are contiguous;
+//  - scratch and Y are addressed through buffer instructions: per-lane
+//    column offset plus a row offset shared by the step (the host keeps
+//    48 n k below 2^31);
+//  - each step's inputs are fetched D steps ahead into a register ring
+//    (slot s mod D, static after unrolling), with D chosen so the
+//    instructions in flight stay within the wave's 63-entry memory counter;
+//  - the LU factors store the reciprocal of each pivot (hardware reciprocal
+//    and two Newton steps), so the solves multiply instead of divide.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+struct d2 { double a, b; };
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+__device__ __forceinline__ d2 buf_ld2(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+}
+__device__ __forceinline__ void buf_st(double x, __amdgpu_buffer_rsrc_t r, int vo, int so) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, vo, so, 0);
+}
+__device__ __forceinline__ void buf_st2(double x, double y, __amdgpu_buffer_rsrc_t r, int vo, int so) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, d2{x, y}), r, vo, so, 0);
+}
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int64_t bytes) {
+    // raw buffer (stride 0), dword 3 as for every gfx9 target
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double fast_rcp(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    return fma(r, fma(-b, r, 1.0), r);
+}
+
+// Run step(s, in) for s = 0 .. count-1, where in[0..NA) was filled by
+// load(s, in) D steps earlier (count is uniform over the wave).  For loops
+// that only load.
+template <int D, int NA, typename Load, typename Step>
+__device__ __forceinline__ void prefetched(int count, Load load, Step step) {
+    double buf[D][NA];
+#pragma unroll
+    for (int u = 0; u < D; ++u)
+        if (u < count) load(u, buf[u]);
+    int c = 0;
+    // whole chunks whose every prefetch target exists: no per-step guards
+    for (; c + 2 * D <= count; c += D) {
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            step(c + u, buf[u]);
+            load(c + u + D, buf[u]);
         }
-        __syncthreads();
-        dd = sde;
-        ee = sde + n;
     }
-    if (id >= nwork) return;
-    const int2 wk = work[id];
-    if (wk.y < 0) return;
+    // the last fewer than 2 D steps
+    for (; c < count; c += D) {
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            const int s = c + u;
+            if (s < count) {
+                step(s, buf[u]);
+                if (s + D < count) load(s + D, buf[u]);
+            }
+        }
+    }
+}
+
+// Loops that load and store: step(s, in, out) computes the values that
+// store(s, out) writes, s = 0 .. count-1.  Chunks of D steps alternate
+// between two register buffers, and every memory instruction is issued one
+// chunk of work before anything waits on it: the loads of chunk c + 1 and
+// the stores of chunk c - 1 go out before chunk c is computed.  (The
+// compiler's wait at the loop head drains the memory counter; with this
+// order nothing then in flight is younger than a chunk of work.)  Loads
+// beyond the last step re-read step count - 1 instead of branching.
+template <int D, int NA, int NO, typename Load, typename Step, typename Store>
+__device__ __forceinline__ void pipelined(int count, Load load, Step step, Store store) {
+    double A[D][NA], B[D][NA], oA[D][NO], oB[D][NO];
+    if (count <= 0) return;
+    auto ld = [&](int s, double* in) { load(s < count ? s : count - 1, in); };
+#pragma unroll
+    for (int u = 0; u < D; ++u) ld(u, A[u]);
+    int c = 0;
+    for (; c + 2 * D <= count; c += 2 * D) {
+#pragma unroll
+        for (int u = 0; u < D; ++u) load(c + D + u, B[u]);
+        if (c > 0) {
+#pragma unroll
+            for (int u = 0; u < D; ++u) store(c - D + u, oB[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < D; ++u) step(c + u, A[u], oA[u]);
+#pragma unroll
+        for (int u = 0; u < D; ++u) ld(c + 2 * D + u, A[u]);
+#pragma unroll
+        for (int u = 0; u < D; ++u) store(c + u, oA[u]);
+#pragma unroll
+        for (int u = 0; u < D; ++u) step(c + D + u, B[u], oB[u]);
+    }
+    if (c > 0) {
+#pragma unroll
+        for (int u = 0; u < D; ++u) store(c - D + u, oB[u]);
+    }
+    // fewer than 2 D steps left; A holds steps c .. c + D - 1
+#pragma unroll
+    for (int u = 0; u < D; ++u) ld(c + D + u, B[u]);
+#pragma unroll
+    for (int u = 0; u < D; ++u)
+        if (c + u < count) {
+            step(c + u, A[u], oA[u]);
+            store(c + u, oA[u]);
+        }
+#pragma unroll
+    for (int u = 0; u < D; ++u)
+        if (c + D + u < count) {
+            step(c + D + u, B[u], oB[u]);
+            store(c + D + u, oB[u]);
+        }
+}
+
+// dd / ee: the tridiagonal, in LDS (Lds) or global memory.
+template <bool Lds>
+__device__ __forceinline__ void stein_body(const EigDesc& d, const double* dd, const double* ee,
+                                           int2 wk) {
+    const int n = d.n;
     const int K = d.k;
     const int j0 = wk.y;
     double lo, hi, pivmin, tn;
@@ -416,58 +554,47 @@ __global__ void __launch_bounds__(64) stein_kernel(const EigDesc* __restrict__ d
     if (j0 > 0 && fabs(d.lam[j0 - 1] - d.lam[j0]) <= ctol) return;   // not a leader
     int j1 = j0 + 1;
     while (j1 < K && fabs(d.lam[j1 - 1] - d.lam[j1]) <= ctol) ++j1;
-    (void)woff;
-    double* base = d.work + j0;
-    const int64_t sK = K;
-    double* ud = base;
-    double* u1 = base + (int64_t)n * sK;
-    double* u2 = base + 2 * (int64_t)n * sK;
-    double* ml = base + 3 * (int64_t)n * sK;
-    double* pv = base + 4 * (int64_t)n * sK;
-    double* x = base + 5 * (int64_t)n * sK;
-#define S_(arr, r) arr[(int64_t)(r) * sK]
+    const __amdgpu_buffer_rsrc_t ws = buf_rsrc(d.work, (int64_t)48 * n * K);
+    const __amdgpu_buffer_rsrc_t ys = buf_rsrc(d.Y, (int64_t)8 * n * K);
+    const int vo = 16 * j0;
+    const int rowB = 16 * K;
+    const int a1 = n * rowB, a2 = 2 * n * rowB;   // arrays R1, R2 (R0 at 0)
+    auto vr = [&](int r) { return vo + r * rowB; };
     const double tiny = DBL_EPSILON * fmax(tn, DBL_MIN);
+    auto guard = [&](double p) { return fabs(p) < tiny ? copysign(tiny, p == 0.0 ? 1.0 : p) : p; };
     for (int j = j0; j < j1; ++j) {
         const double lam = d.lam[j] + (j - j0) * 10.0 * DBL_EPSILON * tn;
-        // LU with partial pivoting of T - lam I, kept in registers along the
-        // recurrence and stored for the solves
+        // LU with partial pivoting of T - lam I (the second superdiagonal of
+        // the running row is zero at every step), stored for the solves
         double udk = dd[0] - lam;
         double u1k = n > 1 ? ee[0] : 0.0;
-        double u2k = 0.0;
-        for (int k = 0; k + 1 < n; ++k) {
-            const double sub = ee[k];
-            const double dg = dd[k + 1] - lam;
-            const double sp = k + 2 < n ? ee[k + 1] : 0.0;
-            double nud, nu1;
-            if (fabs(udk) >= fabs(sub)) {
-                double p = udk;
-                if (fabs(p) < tiny) p = copysign(tiny, p == 0.0 ? 1.0 : p);
-                const double l = sub / p;
-                S_(ud, k) = p;
-                S_(u1, k) = u1k;
-                S_(u2, k) = u2k;
-                S_(ml, k) = l;
-                S_(pv, k) = 0.0;
-                nud = dg - l * u1k;
-                nu1 = sp - l * u2k;
-            } else {
-                const double l = udk / sub;
-                S_(ud, k) = sub;
-                S_(u1, k) = dg;
-                S_(u2, k) = sp;
-                S_(ml, k) = l;
-                S_(pv, k) = 1.0;
-                nud = u1k - l * dg;
-                nu1 = u2k - l * sp;
-            }
-            udk = nud;
-            u1k = nu1;
-            u2k = 0.0;
-        }
-        if (fabs(udk) < tiny) udk = copysign(tiny, udk == 0.0 ? 1.0 : udk);
-        S_(ud, n - 1) = udk;
-        S_(u1, n - 1) = 0.0;
-        S_(u2, n - 1) = 0.0;
+        pipelined<8, 3, 5>(
+            n - 1,
+            [&](int k, double* in) {
+                in[0] = ee[k];
+                in[1] = dd[k + 1];
+                in[2] = k + 2 < n ? ee[k + 1] : 0.0;
+            },
+            [&](int, const double* in, double* out) {
+                const double sub = in[0], dg = in[1] - lam, sp = in[2];
+                const bool piv = !(fabs(udk) >= fabs(sub));
+                const double rp = fast_rcp(piv ? sub : guard(udk));
+                const double l = (piv ? udk : sub) * rp;
+                out[0] = piv ? dg : u1k;
+                out[1] = piv ? sp : 0.0;
+                out[2] = rp;
+                out[3] = l;
+                out[4] = piv ? 1.0 : 0.0;
+                const double nud = piv ? u1k - l * dg : dg - l * u1k;
+                u1k = piv ? -l * sp : sp;
+                udk = nud;
+            },
+            [&](int k, const double* out) {
+                buf_st2(out[0], out[1], ws, vr(k), 0);
+                buf_st2(out[2], out[3], ws, vr(k), a1);
+                buf_st(out[4], ws, vr(k), a2);
+            });
+        const double rlast = fast_rcp(guard(udk));
         // inverse iteration: two solves for an isolated eigenvalue, three inside
         // a cluster; the start vector is generated inside the first forward
         // pass, and the normalisation of each solve is applied when the next
@@ -480,62 +607,134 @@ __global__ void __launch_bounds__(64) stein_kernel(const EigDesc* __restrict__ d
         const int iters = j > j0 ? 3 : 2;
         double sc = 1.0;
         for (int it = 0; it < iters; ++it) {
-            // forward: row interchanges and multipliers
-            double xk = it == 0 ? start() : S_(x, 0) * sc;
-            for (int k = 0; k + 1 < n; ++k) {
-                double xn = it == 0 ? start() : S_(x, k + 1) * sc;
-                if (S_(pv, k) != 0.0) {
-                    const double tmp = xk;
-                    xk = xn;
-                    xn = tmp;
-                }
-                S_(x, k) = xk;
-                xk = xn - S_(ml, k) * xk;
+            // forward: row interchanges and multipliers; R2 of row k + 1
+            // brings x(k + 1) and the next step's interchange flag
+            double xk;
+            if (it == 0) {
+                xk = start();
+                pipelined<10, 2, 1>(
+                    n - 1,
+                    [&](int k, double* in) {
+                        in[0] = buf_ld(ws, vr(k), a1 + 8);
+                        in[1] = buf_ld(ws, vr(k), a2);
+                    },
+                    [&](int, const double* in, double* out) {
+                        const double xn = start();
+                        const bool sw = in[1] != 0.0;
+                        const double a = sw ? xn : xk, b = sw ? xk : xn;
+                        out[0] = a;
+                        xk = b - in[0] * a;
+                    },
+                    [&](int k, const double* out) { buf_st(out[0], ws, vr(k), a2 + 8); });
+            } else {
+                const d2 r0 = buf_ld2(ws, vr(0), a2);
+                double pvk = r0.a;
+                xk = r0.b * sc;
+                pipelined<10, 3, 1>(
+                    n - 1,
+                    [&](int k, double* in) {
+                        in[0] = buf_ld(ws, vr(k), a1 + 8);
+                        const d2 r2 = buf_ld2(ws, vr(k + 1), a2);
+                        in[1] = r2.a;
+                        in[2] = r2.b;
+                    },
+                    [&](int, const double* in, double* out) {
+                        const double xn = in[2] * sc;
+                        const bool sw = pvk != 0.0;
+                        const double a = sw ? xn : xk, b = sw ? xk : xn;
+                        out[0] = a;
+                        xk = b - in[0] * a;
+                        pvk = in[1];
+                    },
+                    [&](int k, const double* out) { buf_st(out[0], ws, vr(k), a2 + 8); });
             }
             // back substitution with U (bandwidth 3), accumulating the norm
+            // and the sign of the largest entry
             double nrm = 0.0, amax = 0.0, sgn = 1.0;
             auto acc = [&](double v) {
                 nrm += v * v;
                 if (fabs(v) > amax) { amax = fabs(v); sgn = v < 0.0 ? -1.0 : 1.0; }
             };
-            double x1 = xk / S_(ud, n - 1);
-            S_(x, n - 1) = x1;
+            double x1 = xk * rlast;
+            buf_st(x1, ws, vr(n - 1), a2 + 8);
             acc(x1);
             double x2 = 0.0;
-            if (n > 1) {
-                const double v = (S_(x, n - 2) - S_(u1, n - 2) * x1) / S_(ud, n - 2);
-                S_(x, n - 2) = v;
-                acc(v);
-                x2 = x1;
-                x1 = v;
-            }
-            for (int k = n - 3; k >= 0; --k) {
-                const double v = (S_(x, k) - S_(u1, k) * x1 - S_(u2, k) * x2) / S_(ud, k);
-                S_(x, k) = v;
-                acc(v);
-                x2 = x1;
-                x1 = v;
-            }
+            pipelined<8, 4, 1>(
+                n - 1,
+                [&](int s_, double* in) {
+                    const int k = n - 2 - s_;
+                    const d2 u = buf_ld2(ws, vr(k), 0);
+                    in[0] = u.a;
+                    in[1] = u.b;
+                    in[2] = buf_ld(ws, vr(k), a1);
+                    in[3] = buf_ld(ws, vr(k), a2 + 8);
+                },
+                [&](int, const double* in, double* out) {
+                    const double v = (in[3] - in[0] * x1 - in[1] * x2) * in[2];
+                    out[0] = v;
+                    acc(v);
+                    x2 = x1;
+                    x1 = v;
+                },
+                [&](int s_, const double* out) { buf_st(out[0], ws, vr(n - 2 - s_), a2 + 8); });
             // orthogonalise against earlier members of the cluster (modified
             // Gram-Schmidt; the last subtraction pass recomputes the norm)
             for (int q = j0; q < j; ++q) {
-                const double* y = d.Y + (int64_t)q * n;
+                const int yo = 8 * q * n;
                 double dot = 0.0;
-                for (int r = 0; r < n; ++r) dot += S_(x, r) * y[r];
+                prefetched<16, 2>(
+                    n,
+                    [&](int r, double* in) {
+                        in[0] = buf_ld(ws, vr(r), a2 + 8);
+                        in[1] = buf_ld(ys, yo, 8 * r);
+                    },
+                    [&](int, const double* in) { dot += in[0] * in[1]; });
                 const bool lastq = q + 1 == j;
                 if (lastq) { nrm = 0.0; amax = 0.0; sgn = 1.0; }
-                for (int r = 0; r < n; ++r) {
-                    const double v = S_(x, r) - dot * y[r];
-                    S_(x, r) = v;
-                    if (lastq) acc(v);
-                }
+                pipelined<10, 2, 1>(
+                    n,
+                    [&](int r, double* in) {
+                        in[0] = buf_ld(ws, vr(r), a2 + 8);
+                        in[1] = buf_ld(ys, yo, 8 * r);
+                    },
+                    [&](int, const double* in, double* out) {
+                        const double v = in[0] - dot * in[1];
+                        out[0] = v;
+                        if (lastq) acc(v);
+                    },
+                    [&](int r, const double* out) { buf_st(out[0], ws, vr(r), a2 + 8); });
             }
             sc = sgn / sqrt(nrm);
         }
-        double* y = d.Y + (int64_t)j * n;
-        for (int r = 0; r < n; ++r) y[r] = S_(x, r) * sc;
+        const int yo = 8 * j * n;
+        pipelined<16, 1, 1>(
+            n, [&](int r, double* in) { in[0] = buf_ld(ws, vr(r), a2 + 8); },
+            [&](int, const double* in, double* out) { out[0] = in[0] * sc; },
+            [&](int r, const double* out) { buf_st(out[0], ys, yo, 8 * r); });
     }
-#undef S_
+}
+
+__global__ void __launch_bounds__(64) stein_kernel(const EigDesc* __restrict__ descs,
+                                                   const int2* __restrict__ work, int nwork,
+                                                   int useLds) {
+    // every 64-item block belongs to one matrix (the host pads per matrix);
+    // its tridiagonal is staged in LDS when it fits (useLds)
+    extern __shared__ double sde[];
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    const EigDesc d = descs[work[blockIdx.x * blockDim.x].x];
+    const int n = d.n;
+    if (useLds) {
+        for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            sde[r] = d.d[r];
+            sde[n + r] = r + 1 < n ? d.e[r] : 0.0;
+        }
+        __syncthreads();
+    }
+    if (id >= nwork) return;
+    const int2 wk = work[id];
+    if (wk.y < 0) return;
+    if (useLds) stein_body<true>(d, sde, sde + n, wk);
+    else stein_body<false>(d, d.d, d.e, wk);
 }
 
 // Y := Q Y for the reflectors stored by the tridiagonalisation.  One
@@ -686,8 +885,9 @@ void launch_stein(const dev::EigDesc* d, const int2* work, int nwork, const int6
     if (nwork <= 0) return;
     const size_t lds = (size_t)2 * nmax * sizeof(double);
     const int useLds = lds <= 60000 ? 1 : 0;
+    (void)woff;
     hipLaunchKernelGGL(dev::stein_kernel, dim3((nwork + 63) / 64), dim3(64), useLds ? lds : 0, s, d,
-                       work, nwork, woff, useLds);
+                       work, nwork, useLds);
 }
 
 void launch_ormtr(const dev::EigDesc* d, const int4* work, int nwork, int ldsDoubles, hipStream_t s) {
