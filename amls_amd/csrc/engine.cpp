@@ -504,7 +504,7 @@ bool buildBsr3Sorted(int64_t n, const int64_t* rp, const int64_t* ci, const doub
 // lam/Y/work/k set (work: 6 n k doubles per matrix, see stein_kernel).
 void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
     std::vector<int2> work, swork;
-    std::vector<int4> owork;
+    std::vector<int4> owork, vwork;
     std::vector<int64_t> woff;
     int lds = 0, nmax = 1;
     // columns per back-transformation workgroup: aim at ~512 workgroups (each
@@ -535,7 +535,10 @@ void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
             // pad to whole workgroups: every workgroup serves one matrix
             while (work.size() % 4) work.push_back(make_int2(m, -1));
         }
-        // LDS-resident column blocks of at most 8 columns and 64 KiB
+        // back transformation: one wave per pair of columns, or (order above
+        // 2048) LDS-resident column blocks of at most 8 columns and 64 KiB
+        for (int c = 0; c < descs[m].k; c += 2)
+            vwork.push_back(make_int4(m, c, std::min(2, descs[m].k - c), 0));
         const int n = descs[m].n;
         const int per = std::max(1, std::min(want, 8000 / std::max(n, 1)));
         for (int c = 0; c < descs[m].k; c += per) {
@@ -547,7 +550,10 @@ void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
     if (work.empty()) return;
     woff.resize(std::max({woff.size(), swork.size(), work.size()}), 0);
     const size_t od = plan.blob->add(descs), ow = plan.blob->add(work), oo = plan.blob->add(woff),
-                 oq = plan.blob->add(owork), os = plan.blob->add(swork);
+                 oq = plan.blob->add(owork), os = plan.blob->add(swork),
+                 ov = plan.blob->add(vwork.empty() ? std::vector<int4>(1) : vwork);
+    const bool wave = ormtr_wave_fits(nmax);
+    const int nv = static_cast<int>(vwork.size()), nmat = static_cast<int>(descs.size());
     Blob* bl = plan.blob;
     const int nw = static_cast<int>(work.size()), nq = static_cast<int>(owork.size());
     const int ns = static_cast<int>(swork.size());
@@ -562,7 +568,10 @@ void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
             run("stein", [&] {
                 launch_stein(bl->at<EigDesc>(od), bl->at<int2>(os), ns, bl->at<int64_t>(oo), nmax, s);
             });
-            run("ormtr", [&] { launch_ormtr(bl->at<EigDesc>(od), bl->at<int4>(oq), nq, lds, s); });
+            run("ormtr", [&] {
+                if (wave) launch_ormtr_wave(bl->at<EigDesc>(od), nmat, bl->at<int4>(ov), nv, nmax, s);
+                else launch_ormtr(bl->at<EigDesc>(od), bl->at<int4>(oq), nq, lds, s);
+            });
         }
     });
 }

@@ -842,6 +842,129 @@ __global__ void __launch_bounds__(256) ormtr_kernel(const EigDesc* __restrict__ 
     for (int e = t; e < n * nc; e += 256) Y[e] = sY[e];
 }
 
+// ---------------------------------------------------------------------------
+// Y := Q Y, one wave per (matrix, pair of columns): the wave keeps its two
+// columns in registers (lane l owns rows l + 64 m) and applies the
+// reflectors H_{n-2}, ..., H_0 in turn.  The only cross-lane step per
+// reflector is a wave reduction of two dot products (DPP, no workgroup
+// barrier), and each reflector's column is fetched two reflectors ahead.
+// The reflector columns must hold v_i in full (reflector_prep_kernel): 0 in
+// rows <= i, 1 in row i + 1, so a column is loaded as it stands; rows at or
+// beyond n read as zero (their buffer offset lies past the buffer's end).
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes, returned to every lane: pairs and quads by quad
+// permutes, rows of 16 by row shifts, then the row totals carried across
+// rows by the row broadcasts, so lane 63 ends with the total.
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_mov<0xB1>(v);    // quad_perm 1,0,3,2
+    v += dpp_mov<0x4E>(v);    // quad_perm 2,3,0,1
+    v += dpp_mov<0x114>(v);   // row_shr:4
+    v += dpp_mov<0x118>(v);   // row_shr:8
+    v += dpp_mov<0x142>(v);   // row_bcast:15
+    v += dpp_mov<0x143>(v);   // row_bcast:31
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// Column i of every matrix: rows <= i := 0, row i + 1 := 1 (column n - 1:
+// all zero), so the column holds the reflector vector v_i exactly.  The
+// tridiagonal is kept separately, so nothing else reads these entries.
+// Block (column block of 64, matrix).
+__global__ void __launch_bounds__(256) reflector_prep_kernel(const EigDesc* __restrict__ descs) {
+    const EigDesc d = descs[blockIdx.y];
+    const int n = d.n;
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    if (c >= n) return;
+    double* col = d.A + (int64_t)c * d.lda;
+    const int last = min(c + 1, n - 1);
+    for (int r = threadIdx.x >> 6; r <= last; r += 4) col[r] = (r == c + 1) ? 1.0 : 0.0;
+}
+
+constexpr int ORW_GROUP = 8;   // register rows per skip test (512 matrix rows)
+
+template <int MR>
+__global__ void __launch_bounds__(64) ormtr_wave_kernel(const EigDesc* __restrict__ descs,
+                                                        const int4* __restrict__ work) {
+    const int4 wk = work[blockIdx.x];   // (matrix, first column, columns <= 2)
+    const EigDesc d = descs[wk.x];
+    const int n = d.n;
+    const int l = threadIdx.x;
+    const int ldaB = 8 * (int)d.lda;
+    const __amdgpu_buffer_rsrc_t ar = buf_rsrc(d.A, (int64_t)ldaB * n);
+    int voff[MR];
+    double y0[MR], y1[MR];
+    const double* Y0 = d.Y + (int64_t)wk.y * n;
+    const double* Y1 = Y0 + n;
+    const bool two = wk.z > 1;
+#pragma unroll
+    for (int m = 0; m < MR; ++m) {
+        const int r = l + 64 * m;
+        voff[m] = r < n ? 8 * r : 0x7ffffff0;
+        y0[m] = r < n ? Y0[r] : 0.0;
+        y1[m] = (two && r < n) ? Y1[r] : 0.0;
+    }
+    auto load = [&](int i, double (&v)[MR], double& tau) {
+        if (i < 0) return;
+#pragma unroll
+        for (int m = 0; m < MR; ++m) v[m] = buf_ld(ar, voff[m], i * ldaB);
+        tau = d.tau[i];
+    };
+    auto step = [&](int i, const double (&v)[MR], double tau) {
+        if (i < 0) return;
+        double p0 = 0.0, p1 = 0.0;
+#pragma unroll
+        for (int g = 0; g < MR; g += ORW_GROUP) {
+            if (64 * (g + ORW_GROUP) - 1 > i) {   // some row of the group lies below i
+#pragma unroll
+                for (int m = g; m < g + ORW_GROUP && m < MR; ++m) {
+                    p0 += v[m] * y0[m];
+                    p1 += v[m] * y1[m];
+                }
+            }
+        }
+        const double c0 = tau * wave_sum(p0), c1 = tau * wave_sum(p1);
+#pragma unroll
+        for (int g = 0; g < MR; g += ORW_GROUP) {
+            if (64 * (g + ORW_GROUP) - 1 > i) {
+#pragma unroll
+                for (int m = g; m < g + ORW_GROUP && m < MR; ++m) {
+                    y0[m] -= c0 * v[m];
+                    y1[m] -= c1 * v[m];
+                }
+            }
+        }
+    };
+    double va[MR], vb[MR], ta = 0.0, tb = 0.0;
+    load(n - 2, va, ta);
+    load(n - 3, vb, tb);
+    for (int i = n - 2; i >= 0; i -= 2) {
+        step(i, va, ta);
+        load(i - 2, va, ta);
+        step(i - 1, vb, tb);
+        load(i - 3, vb, tb);
+    }
+    double* W0 = d.Y + (int64_t)wk.y * n;
+    double* W1 = W0 + n;
+#pragma unroll
+    for (int m = 0; m < MR; ++m) {
+        const int r = l + 64 * m;
+        if (r < n) {
+            W0[r] = y0[m];
+            if (two) W1[r] = y1[m];
+        }
+    }
+}
+
 }  // namespace dev
 
 void launch_sytrd_step1(const dev::EigDesc& d, int i, double* sv, double* sp, hipStream_t s) {
@@ -893,6 +1016,22 @@ void launch_stein(const dev::EigDesc* d, const int2* work, int nwork, const int6
 void launch_ormtr(const dev::EigDesc* d, const int4* work, int nwork, int ldsDoubles, hipStream_t s) {
     if (nwork <= 0) return;
     hipLaunchKernelGGL(dev::ormtr_kernel, dim3(nwork), dim3(256), (size_t)ldsDoubles * 8, s, d, work);
+}
+
+bool ormtr_wave_fits(int nmax) { return nmax <= 2048; }
+
+void launch_ormtr_wave(const dev::EigDesc* d, int nmat, const int4* work, int nwork, int nmax,
+                       hipStream_t s) {
+    if (nwork <= 0) return;
+    hipLaunchKernelGGL(dev::reflector_prep_kernel, dim3((nmax + 63) / 64, nmat), dim3(256), 0, s, d);
+    if (nmax <= 512)
+        hipLaunchKernelGGL(dev::ormtr_wave_kernel<8>, dim3(nwork), dim3(64), 0, s, d, work);
+    else if (nmax <= 1024)
+        hipLaunchKernelGGL(dev::ormtr_wave_kernel<16>, dim3(nwork), dim3(64), 0, s, d, work);
+    else if (nmax <= 1536)
+        hipLaunchKernelGGL(dev::ormtr_wave_kernel<24>, dim3(nwork), dim3(64), 0, s, d, work);
+    else
+        hipLaunchKernelGGL(dev::ormtr_wave_kernel<32>, dim3(nwork), dim3(64), 0, s, d, work);
 }
 
 }  // namespace amls

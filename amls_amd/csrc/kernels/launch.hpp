@@ -41,6 +41,12 @@ void launch_stein(const dev::EigDesc* d, const int2* work, int nwork, const int6
                   int nmax, hipStream_t s);
 // work: (matrix, first column, column count <= 8); ldsDoubles >= n * count
 void launch_ormtr(const dev::EigDesc* d, const int4* work, int nwork, int ldsDoubles, hipStream_t s);
+// Wave-per-column-pair variant for matrices of order <= 2048 (ormtr_wave_fits):
+// work items (matrix, first column, columns <= 2); overwrites the entries of
+// A above each reflector with the reflector's own zeros and unit.
+bool ormtr_wave_fits(int nmax);
+void launch_ormtr_wave(const dev::EigDesc* d, int nmat, const int4* work, int nwork, int nmax,
+                       hipStream_t s);
 // y(:, c) = A x(:, c) for CSR A (n x n), column-major x, y with leading dimension n
 void launch_spmm(int64_t n, const int64_t* rowptr, const int64_t* colidx, const double* vals,
                  const double* x, double* y, int ncols, int64_t ld, hipStream_t s);

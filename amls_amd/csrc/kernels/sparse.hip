@@ -89,56 +89,124 @@ __global__ void __launch_bounds__(256) spmm_rm_kernel(int64_t n, const int64_t* 
 
 // 3 x 3 block-sparse rows (elasticity: one block per pair of coupled mesh
 // nodes): Y(3b + i, :) = sum over blocks (b, c) of A_bc(i, :) X(3c .. 3c+2, :).
-// One wave per (block row, 64-column chunk); each block costs three
-// coalesced 512-byte reads of X for nine FMAs per lane, a third of the X
-// traffic of the scalar CSR form, and four blocks are in flight at a time.
-// Same XCD-contiguous row mapping as spmm_rm_kernel.
+// One wave per block row.  The wave stages its row's blocks (values and
+// column indices, at most SPM_MAXB blocks) in LDS once and then sweeps every
+// column of X in chunks of 128 (two columns per lane), so the matrix is read
+// from memory once instead of once per chunk; the block values are LDS
+// broadcasts.  Each block costs six coalesced 512-byte reads of X for 18
+// FMAs per lane, four blocks in flight at a time.  Rows with more blocks
+// than SPM_MAXB read the matrix from global memory per chunk.  Same
+// XCD-contiguous row mapping as spmm_rm_kernel.
+constexpr int SPM_MAXB = 128;
+constexpr int SPM_CPL = 2;   // columns per lane
+
+template <bool Lds>
+__device__ __forceinline__ void bsr3_row(const double* __restrict__ av, const int* __restrict__ ac,
+                                         int nbk, const double* __restrict__ x,
+                                         double* __restrict__ y, int64_t br, int ncols, int lane) {
+    for (int c0 = 0; c0 < ncols; c0 += 64 * SPM_CPL) {
+        int cc[SPM_CPL];
+        bool on[SPM_CPL];
+#pragma unroll
+        for (int q = 0; q < SPM_CPL; ++q) {
+            const int c = c0 + lane + 64 * q;
+            on[q] = c < ncols;
+            cc[q] = on[q] ? c : 0;   // in-bounds column for the loads of idle lanes
+        }
+        double a0[SPM_CPL], a1[SPM_CPL], a2[SPM_CPL];
+#pragma unroll
+        for (int q = 0; q < SPM_CPL; ++q) a0[q] = a1[q] = a2[q] = 0.0;
+        int k = 0;
+        for (; k + 4 <= nbk; k += 4) {
+            double xv[4][3][SPM_CPL];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t xr = ac[k + u];
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int q = 0; q < SPM_CPL; ++q) xv[u][j][q] = x[(xr + j) * ncols + cc[q]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double* a = av + 9 * (k + u);
+#pragma unroll
+                for (int q = 0; q < SPM_CPL; ++q) {
+                    a0[q] += a[0] * xv[u][0][q] + a[1] * xv[u][1][q] + a[2] * xv[u][2][q];
+                    a1[q] += a[3] * xv[u][0][q] + a[4] * xv[u][1][q] + a[5] * xv[u][2][q];
+                    a2[q] += a[6] * xv[u][0][q] + a[7] * xv[u][1][q] + a[8] * xv[u][2][q];
+                }
+            }
+        }
+        for (; k < nbk; ++k) {
+            const int64_t xr = ac[k];
+            const double* a = av + 9 * k;
+#pragma unroll
+            for (int q = 0; q < SPM_CPL; ++q) {
+                const double x0 = x[xr * ncols + cc[q]], x1 = x[(xr + 1) * ncols + cc[q]],
+                             x2 = x[(xr + 2) * ncols + cc[q]];
+                a0[q] += a[0] * x0 + a[1] * x1 + a[2] * x2;
+                a1[q] += a[3] * x0 + a[4] * x1 + a[5] * x2;
+                a2[q] += a[6] * x0 + a[7] * x1 + a[8] * x2;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < SPM_CPL; ++q)
+            if (on[q]) {
+                const int c = c0 + lane + 64 * q;
+                y[(3 * br) * ncols + c] = a0[q];
+                y[(3 * br + 1) * ncols + c] = a1[q];
+                y[(3 * br + 2) * ncols + c] = a2[q];
+            }
+    }
+}
+
 __global__ void __launch_bounds__(256) spmm_bsr3_kernel(int64_t nb, const int64_t* __restrict__ ptr,
                                                         const int32_t* __restrict__ col,
                                                         const double* __restrict__ val,
                                                         const double* __restrict__ x,
                                                         double* __restrict__ y, int ncols) {
+    __shared__ double sv[4][SPM_MAXB * 9];
+    __shared__ int sc[4][SPM_MAXB];
     const unsigned per = gridDim.x / NXCD;
     const unsigned blk = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
-    const int64_t br = (int64_t)blk * 4 + (threadIdx.x >> 6);
+    const int w = threadIdx.x >> 6;
+    const int64_t br = (int64_t)blk * 4 + w;
     const int lane = threadIdx.x & 63;
     if (br >= nb) return;
-    const int c = blockIdx.y * 64 + lane;
-    const bool on = c < ncols;
-    const int cc = on ? c : 0;
     const int64_t b0 = ptr[br], b1 = ptr[br + 1];
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    int64_t k = b0;
-    for (; k + 4 <= b1; k += 4) {
-        double xv[4][3];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int64_t xr = 3 * (int64_t)col[k + u];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) xv[u][j] = x[(xr + j) * ncols + cc];
+    const int nbk = (int)(b1 - b0);
+    if (nbk > SPM_MAXB) {
+        // X row offsets of the blocks are recomputed from col per chunk
+        for (int c0 = 0; c0 < ncols; c0 += 64) {
+            const int c = c0 + lane;
+            const bool on = c < ncols;
+            const int cc = on ? c : 0;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+            for (int64_t k = b0; k < b1; ++k) {
+                const int64_t xr = 3 * (int64_t)col[k];
+                const double x0 = x[xr * ncols + cc], x1 = x[(xr + 1) * ncols + cc],
+                             x2 = x[(xr + 2) * ncols + cc];
+                const double* a = val + 9 * k;
+                a0 += a[0] * x0 + a[1] * x1 + a[2] * x2;
+                a1 += a[3] * x0 + a[4] * x1 + a[5] * x2;
+                a2 += a[6] * x0 + a[7] * x1 + a[8] * x2;
+            }
+            if (on) {
+                y[(3 * br) * ncols + c] = a0;
+                y[(3 * br + 1) * ncols + c] = a1;
+                y[(3 * br + 2) * ncols + c] = a2;
+            }
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const double* a = val + 9 * (k + u);
-            a0 += a[0] * xv[u][0] + a[1] * xv[u][1] + a[2] * xv[u][2];
-            a1 += a[3] * xv[u][0] + a[4] * xv[u][1] + a[5] * xv[u][2];
-            a2 += a[6] * xv[u][0] + a[7] * xv[u][1] + a[8] * xv[u][2];
-        }
+        return;
     }
-    for (; k < b1; ++k) {
-        const int64_t xr = 3 * (int64_t)col[k];
-        const double x0 = x[xr * ncols + cc], x1 = x[(xr + 1) * ncols + cc],
-                     x2 = x[(xr + 2) * ncols + cc];
-        const double* a = val + 9 * k;
-        a0 += a[0] * x0 + a[1] * x1 + a[2] * x2;
-        a1 += a[3] * x0 + a[4] * x1 + a[5] * x2;
-        a2 += a[6] * x0 + a[7] * x1 + a[8] * x2;
-    }
-    if (on) {
-        y[(3 * br) * ncols + c] = a0;
-        y[(3 * br + 1) * ncols + c] = a1;
-        y[(3 * br + 2) * ncols + c] = a2;
-    }
+    // the wave's own LDS slice: written and read by this wave only
+    for (int e = lane; e < nbk * 9; e += 64) sv[w][e] = val[9 * b0 + e];
+    for (int e = lane; e < nbk; e += 64) sc[w][e] = 3 * col[b0 + e];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    bsr3_row<true>(sv[w], sc[w], nbk, x, y, br, ncols, lane);
 }
 
 // One block per column.
@@ -223,8 +291,8 @@ void launch_spmm_bsr3(int64_t nb, const int64_t* ptr, const int32_t* col, const 
     if (nb <= 0 || ncols <= 0) return;
     const unsigned g = (unsigned)((nb + 3) / 4);
     const unsigned gx = (g + dev::NXCD - 1) / dev::NXCD * dev::NXCD;
-    hipLaunchKernelGGL(dev::spmm_bsr3_kernel, dim3(gx, (unsigned)((ncols + 63) / 64)), dim3(256), 0,
-                       s, nb, ptr, col, val, x, y, ncols);
+    hipLaunchKernelGGL(dev::spmm_bsr3_kernel, dim3(gx), dim3(256), 0, s, nb, ptr, col, val, x, y,
+                       ncols);
 }
 
 void launch_coldot(int64_t n, const double* x, const double* y, int ncols, int64_t ld, double* out,
