@@ -34,6 +34,10 @@ HIP_FLAGS = [
     "-munsafe-fp-atomics",
     # no packed FP32 instructions from the compiler
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # matrix-core accumulators in ordinary VGPRs (gfx950 reads and writes
+    # them there): otherwise the GEMM loop copies every accumulator between
+    # the two register files on each k-chunk
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
 ]
 
 

@@ -51,11 +51,25 @@ struct TileAcc {
     }
 };
 
+typedef unsigned int gemm_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t gemm_rsrc(const double* p) {
+    // raw buffer (stride 0, dword 3 as for every gfx9 target); the range
+    // lies past every in-tile offset and below the sentinel GEMM_OOB
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), (short)0, 0x7fffff00, 0x00020000);
+}
+constexpr int GEMM_OOB = 0x7fffff80;   // offset that reads as zero
+
 // Accumulate op(A)[0:mv, kb:ke] * op(B)[kb:ke, 0:nv] (scaled by `scale`)
 // into the tile accumulator.  A, B point at the tile's first row / column
 // (see the element formulas below); out-of-range elements read as zero.
 // The next k-chunk is fetched into registers while the matrix cores work on
-// the chunk in LDS, so memory latency overlaps the MFMA issue.
+// the chunk in LDS, so memory latency overlaps the MFMA issue.  Each
+// chunk's elements are read through a buffer resource based at the chunk,
+// at per-thread byte offsets fixed for the whole loop; an element outside
+// the tile gets an offset past the resource's range, which reads as zero,
+// so the loop body has no branches and the accumulators stay in the
+// matrix-core registers across it (the host keeps 64 lda and 64 ldb below
+// 2^22 elements).
 template <typename T>
 __device__ void tile_mma(TileAcc<T>& acc, const double* __restrict__ A, int64_t lda,
                          int ta, int mv, const double* __restrict__ B, int64_t ldb,
@@ -64,49 +78,50 @@ __device__ void tile_mma(TileAcc<T>& acc, const double* __restrict__ A, int64_t 
     const int lane = t & 63;
     const int w = t >> 6;
     const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
-    // element j of this thread's share of a chunk: (row or column, k offset)
-    // coalesced along the contiguous dimension of each operand
-    auto aIdx = [&](int j, int& r, int& kk) {
-        if (ta == 0) { r = t & 63; kk = (t >> 6) + 4 * j; }
-        else { kk = t & 31; r = (t >> 5) + 8 * j; }
-    };
-    auto bIdx = [&](int j, int& c, int& kk) {
-        if (tb == 0) { kk = t & 31; c = (t >> 5) + 8 * j; }
-        else { c = t & 63; kk = (t >> 6) + 4 * j; }
-    };
+    if (kb >= ke) return;
+    // element j of this thread's share of a chunk: (row or column, k offset),
+    // coalesced along the contiguous dimension of each operand.
+    //   ta = 0: r = t & 63, k = (t >> 6) + 4 j, at r + k lda
+    //   ta = 1: k = t & 31, r = (t >> 5) + 8 j, at k + r lda
+    //   tb = 0: k = t & 31, c = (t >> 5) + 8 j, at k + c ldb
+    //   tb = 1: c = t & 63, k = (t >> 6) + 4 j, at c + k ldb
+    const int la = (int)lda, lb = (int)ldb;
+    const int aR0 = ta ? (t >> 5) : (t & 63), aK0 = ta ? (t & 31) : (t >> 6);
+    const int aRs = ta ? 8 : 0, aKs = ta ? 0 : 4;   // per-j steps of row and k
+    const int bC0 = tb ? (t & 63) : (t >> 5), bK0 = tb ? (t >> 6) : (t & 31);
+    const int bCs = tb ? 0 : 8, bKs = tb ? 4 : 0;
+    const int aOff0 = 8 * (ta ? aK0 + aR0 * la : aR0 + aK0 * la);
+    const int aOffS = 8 * (ta ? 8 * la : 4 * la);
+    const int bOff0 = 8 * (tb ? bC0 + bK0 * lb : bK0 + bC0 * lb);
+    const int bOffS = 8 * (tb ? 4 * lb : 8 * lb);
+    const int64_t aChunk = ta ? 1 : lda, bChunk = tb ? ldb : 1;   // element step per k
     double ra[8], rb[8];
     auto fetch = [&](int k0) {
+        const __amdgpu_buffer_rsrc_t rA = gemm_rsrc(A + (int64_t)k0 * aChunk);
+        const __amdgpu_buffer_rsrc_t rB = gemm_rsrc(B + (int64_t)k0 * bChunk);
+        const int kl = ke - k0;   // valid k offsets in this chunk: < kl
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            int r, kk;
-            aIdx(j, r, kk);
-            const int kg = k0 + kk;
-            ra[j] = (r < mv && kg < ke) ? (ta == 0 ? A[r + (int64_t)kg * lda] : A[kg + (int64_t)r * lda])
-                                        : 0.0;
-            int c, kb2;
-            bIdx(j, c, kb2);
-            const int kg2 = k0 + kb2;
-            rb[j] = (c < nv && kg2 < ke) ? (tb == 0 ? B[kg2 + (int64_t)c * ldb] : B[c + (int64_t)kg2 * ldb])
-                                         : 0.0;
+            const bool okA = aR0 + aRs * j < mv && aK0 + aKs * j < kl;
+            const bool okB = bC0 + bCs * j < nv && bK0 + bKs * j < kl;
+            ra[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                                   rA, okA ? aOff0 + aOffS * j : GEMM_OOB, 0, 0));
+            rb[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                                   rB, okB ? bOff0 + bOffS * j : GEMM_OOB, 0, 0));
         }
     };
     auto stage = [&]() {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            int r, kk;
-            aIdx(j, r, kk);
-            sA[kk * LDSP + r] = T(ra[j] * scale);
-            int c, kb2;
-            bIdx(j, c, kb2);
-            sB[kb2 * LDSP + c] = T(rb[j]);
+            sA[(aK0 + aKs * j) * LDSP + aR0 + aRs * j] = T(ra[j] * scale);
+            sB[(bK0 + bKs * j) * LDSP + bC0 + bCs * j] = T(rb[j]);
         }
     };
-    if (kb >= ke) return;
     fetch(kb);
     for (int k0 = kb; k0 < ke; k0 += KC) {
         stage();
         __syncthreads();
-        if (k0 + KC < ke) fetch(k0 + KC);
+        fetch(k0 + KC < ke ? k0 + KC : k0);   // the last round re-reads (unused)
 #pragma unroll
         for (int ks = 0; ks < KC / 4; ++ks) {
             const int kr = ks * 4 + (lane >> 4);
