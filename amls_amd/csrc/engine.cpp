@@ -680,6 +680,11 @@ void Blob::upload(hipStream_t s) {
 
 void GemmBatch::add(const GemmDesc& d0) {
     if (d0.m <= 0 || d0.n <= 0) return;
+    // the tile kernel addresses a 64-row / 64-column panel of each operand
+    // with 32-bit byte offsets
+    constexpr int64_t kMaxLd = 4000000;
+    if (d0.lda > kMaxLd || d0.ldb > kMaxLd || d0.lda2 > kMaxLd || d0.ldb2 > kMaxLd)
+        throw StatusError(AMLS_E_ARG, "leading dimension above 4e6 elements");
     GemmDesc d = d0;
     const int id = static_cast<int>(descs.size());
     const int tm = cdiv(d.m, 64), tn = cdiv(d.n, 64);
