@@ -50,6 +50,7 @@ __device__ __forceinline__ void sytrd_step_body(const EigDesc& d, int i, int4 wk
                                                 double* __restrict__ sp_m) {
     __shared__ double tile[64 * 65];
     __shared__ double vr[64], wr[64], xr[64], vc[64], wc[64], xc[64];
+    __shared__ double prt[4][64], pct[4][64];
     __shared__ double red[8];
     const int n = d.n;
     const int64_t lda = d.lda;
@@ -187,17 +188,31 @@ __device__ __forceinline__ void sytrd_step_body(const EigDesc& d, int i, int4 wk
         tile[rr * 65 + cc] = a;
     }
     __syncthreads();
+    // products with x_i: thread (quarter q, lane l) sums 16 terms of row l
+    // and 16 of column l (the diagonal element only in the row sum), the
+    // four quarters are added in LDS, and 128 threads add the row and column
+    // totals into p_i
+    {
+        const int q = t >> 6, l = t & 63;
+        double rs = 0.0, cs = 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int k = 16 * q + u;
+            rs += tile[l * 65 + k] * xc[k];
+            if (r0 + k != c0 + l) cs += tile[k * 65 + l] * xr[k];
+        }
+        prt[q][l] = rs;
+        pct[q][l] = cs;
+    }
+    __syncthreads();
     if (t < 64) {
         const int r = r0 + t;
-        double s = 0.0;
-        for (int cc = 0; cc < 64; ++cc) s += tile[t * 65 + cc] * xc[cc];
+        const double s = prt[0][t] + prt[1][t] + prt[2][t] + prt[3][t];
         if (r > i && r < n) atomicAdd(spw + r, s);
     } else if (t < 128) {
         const int cc = t - 64;
         const int c = c0 + cc;
-        double s = 0.0;
-        for (int rr = 0; rr < 64; ++rr)
-            if (r0 + rr != c) s += tile[rr * 65 + cc] * xr[rr];
+        const double s = pct[0][cc] + pct[1][cc] + pct[2][cc] + pct[3][cc];
         if (c > i && c < n) atomicAdd(spw + c, s);
     }
 }
