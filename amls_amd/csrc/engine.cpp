@@ -370,14 +370,18 @@ void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<i
     if (start.empty()) return;
     const size_t od = plan.blob->add(descs), oo = plan.blob->add(soff);
     Blob* bl = plan.blob;
-    // algorithmic traffic per step: each active matrix reads and writes the
-    // lower triangle of its trailing block once
+    // the last sytrd_tail_order() steps of every matrix run in one launch,
+    // a workgroup per matrix with its trailing block in LDS
+    const int i0 = std::max(0, static_cast<int>(start.size()) - sytrd_tail_order());
+    const int nmat = static_cast<int>(descs.size());
+    // algorithmic traffic per tiled step: each active matrix reads and
+    // writes the lower triangle of its trailing block once
     double bytes = 0.0;
-    for (int i = 0; i < static_cast<int>(start.size()); ++i)
+    for (int i = 0; i < i0; ++i)
         for (int v : ns)
             if (v > i) bytes += 8.0 * double(v - i - 1) * double(v - i);
     KernelStats* kst = plan.st;
-    const int64_t nsteps = static_cast<int64_t>(start.size());
+    const int64_t nsteps = i0;
     const EigDesc d1 = descs[0];
     double* sv1 = sv + 2 * soff[0];
     double* sp1 = sp + 3 * soff[0];
@@ -386,15 +390,23 @@ void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<i
         const int4* work = cached ? cached : (one ? nullptr : bl->at<int4>(ow));
         // the steps run back to back; one event pair brackets the loop
         auto loop = [&] {
-            for (int i = 0; i < static_cast<int>(start.size()); ++i) {
+            for (int i = 0; i < i0; ++i) {
                 if (one) launch_sytrd_step1(d1, i, sv1, sp1, s);
                 else
                     launch_sytrd_step(bl->at<EigDesc>(od), work + start[i], count[i], i, sv, sp,
                                       bl->at<int64_t>(oo), s);
             }
         };
-        if (kst) kst->run("sytrd_step", 2.0 * bytes / 8.0, bytes, s, loop, nsteps);
-        else loop();
+        auto tail = [&] {
+            launch_sytrd_tail(bl->at<EigDesc>(od), nmat, i0, sv, sp, bl->at<int64_t>(oo), s);
+        };
+        if (kst) {
+            if (nsteps > 0) kst->run("sytrd_step", 2.0 * bytes / 8.0, bytes, s, loop, nsteps);
+            kst->run("sytrd_tail", 0.0, 0.0, s, tail);
+        } else {
+            loop();
+            tail();
+        }
     });
 }
 

@@ -20,9 +20,33 @@ namespace dev {
 
 constexpr int RS = 64;   // rows per workgroup in a tridiagonalisation step
 
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes, returned to every lane: pairs and quads by quad
+// permutes, rows of 16 by row shifts, then the row totals carried across
+// rows by the row broadcasts, so lane 63 ends with the total.
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_mov<0xB1>(v);    // quad_perm 1,0,3,2
+    v += dpp_mov<0x4E>(v);    // quad_perm 2,3,0,1
+    v += dpp_mov<0x114>(v);   // row_shr:4
+    v += dpp_mov<0x118>(v);   // row_shr:8
+    v += dpp_mov<0x142>(v);   // row_bcast:15
+    v += dpp_mov<0x143>(v);   // row_bcast:31
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ double block_sum(double v, double* red) {
     // 256-thread block reduction; red has >= 4 slots
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    v = wave_sum(v);
     const int t = threadIdx.x;
     __syncthreads();
     if ((t & 63) == 0) red[t >> 6] = v;
@@ -251,6 +275,128 @@ __global__ void __launch_bounds__(256) sytrd_tile1_kernel(EigDesc d, int i, int 
         wk.w = (R == t0 && wk.z == t0) ? 1 : 0;
     }
     sytrd_step_body(d, i, wk, sv, sp);
+}
+
+// ---------------------------------------------------------------------------
+// The last SYT_TAIL steps of the reduction in one workgroup per matrix.
+// Steps i0 .. n-1 (n - i0 <= SYT_TAIL) of matrix blockIdx.x: the trailing
+// block A[i0:n, i0:n] (lower triangle, mirrored) is loaded into LDS with
+// the pending update of step i0 - 1 applied, and each step is then the
+// unblocked Householder reduction on the LDS copy, updates applied at once:
+// no launch per step, no atomics.  Outputs follow the tiled steps exactly:
+// d, e, tau, the reflector v_i (1 in row i + 1) in column i of A below the
+// diagonal, and the reflector of step i0 - 1, which step i0 would store.
+// ---------------------------------------------------------------------------
+constexpr int SYT_TAIL = 128;
+constexpr int SYT_LD = SYT_TAIL + 1;
+
+__global__ void __launch_bounds__(256) sytrd_tail_kernel(const EigDesc* __restrict__ descs, int i0,
+                                                         double* __restrict__ sv_all,
+                                                         double* __restrict__ sp_all,
+                                                         const int64_t* __restrict__ soff) {
+    __shared__ double T[SYT_TAIL * SYT_LD];
+    __shared__ double vl[SYT_TAIL], wl[SYT_TAIL];
+    __shared__ double red[8];
+    const EigDesc d = descs[blockIdx.x];
+    const int n = d.n;
+    const int m = n - i0;
+    if (m <= 0) return;
+    const int t = threadIdx.x;
+    const int64_t lda = d.lda;
+    double* A = d.A;
+    for (int c = t >> 6; c < m; c += 4)
+        for (int r = c + (t & 63); r < m; r += 64) {
+            const double a = A[(i0 + r) + (int64_t)(i0 + c) * lda];
+            T[r * SYT_LD + c] = a;
+            T[c * SYT_LD + r] = a;
+        }
+    if (i0 > 0) {
+        // pending update of step i0 - 1: w = tau p + alpha v, alpha = -tau^2 (p.v) / 2
+        const double* sv_m = sv_all + 2 * soff[blockIdx.x];
+        const double* sp_m = sp_all + 3 * soff[blockIdx.x];
+        const double* vp = sv_m + (int64_t)((i0 + 1) & 1) * n + i0;   // v_{i0-1}
+        const double* pp = sp_m + (int64_t)((i0 + 2) % 3) * n + i0;   // p_{i0-1}
+        const double tauP = d.tau[i0 - 1];
+        double part = 0.0;
+        if (t < m) {
+            const double v = vp[t], pv = pp[t];
+            vl[t] = v;
+            wl[t] = pv;
+            part = pv * v;
+            A[(i0 + t) + (int64_t)(i0 - 1) * lda] = v;   // reflector of step i0 - 1
+        }
+        const double alphaP = -0.5 * tauP * tauP * block_sum(part, red);
+        if (t < m) wl[t] = tauP * wl[t] + alphaP * vl[t];
+        __syncthreads();
+        for (int c = t >> 1; c < m; c += 128) {
+            const double vc = vl[c], wc = wl[c];
+            for (int r = t & 1; r < m; r += 2) T[r * SYT_LD + c] -= vl[r] * wc + wl[r] * vc;
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+        const int i = i0 + k;
+        if (k == m - 1) {
+            if (t == 0) d.d[i] = T[k * SYT_LD + k];
+            break;
+        }
+        // Householder vector of column k below the diagonal
+        const double alpha = T[(k + 1) * SYT_LD + k];
+        double part = 0.0;
+        if (t >= k + 2 && t < m) {
+            const double x = T[t * SYT_LD + k];
+            part = x * x;
+        }
+        const double xnorm2 = block_sum(part, red);
+        double tau = 0.0, beta = alpha, scal = 0.0;
+        if (xnorm2 > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + xnorm2), alpha);
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        if (t == 0) {
+            d.d[i] = T[k * SYT_LD + k];
+            d.e[i] = beta;
+            d.tau[i] = tau;
+        }
+        if (t > k && t < m) {
+            const double v = t == k + 1 ? 1.0 : T[t * SYT_LD + k] * scal;
+            vl[t] = v;
+            A[(i0 + t) + (int64_t)i * lda] = v;
+        }
+        __syncthreads();
+        // p = T v over rows / columns k+1 .. m-1: two threads per row
+        const int r = (t >> 1) + k + 1, h = t & 1;
+        double pr = 0.0;
+        if (r < m) {
+            // four independent partial sums, so the LDS reads pipeline
+            double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+            const double* Tr = T + r * SYT_LD;
+            int c = k + 1 + h;
+            for (; c + 6 < m; c += 8) {
+                q0 += Tr[c] * vl[c];
+                q1 += Tr[c + 2] * vl[c + 2];
+                q2 += Tr[c + 4] * vl[c + 4];
+                q3 += Tr[c + 6] * vl[c + 6];
+            }
+            for (; c < m; c += 2) q0 += Tr[c] * vl[c];
+            pr = (q0 + q1) + (q2 + q3);
+        }
+        pr += __shfl_xor(pr, 1, 64);
+        const double pv = block_sum((r < m && h == 0) ? pr * vl[r] : 0.0, red);
+        const double alpha2 = -0.5 * tau * tau * pv;
+        if (r < m && h == 0) wl[r] = tau * pr + alpha2 * vl[r];
+        __syncthreads();
+        // rank-2 update: thread t takes column k + 1 + t / 2, every other row
+        const int cc = k + 1 + (t >> 1);
+        if (cc < m) {
+            const double vc = vl[cc], wc = wl[cc];
+#pragma unroll 4
+            for (int rr = k + 1 + (t & 1); rr < m; rr += 2)
+                T[rr * SYT_LD + cc] -= vl[rr] * wc + wl[rr] * vc;
+        }
+        __syncthreads();
+    }
 }
 
 // Sturm count: number of eigenvalues of T strictly below x (pivmin guarded).
@@ -857,40 +1003,6 @@ __global__ void __launch_bounds__(256) ormtr_kernel(const EigDesc* __restrict__ 
     for (int e = t; e < n * nc; e += 256) Y[e] = sY[e];
 }
 
-// ---------------------------------------------------------------------------
-// Y := Q Y, one wave per (matrix, pair of columns): the wave keeps its two
-// columns in registers (lane l owns rows l + 64 m) and applies the
-// reflectors H_{n-2}, ..., H_0 in turn.  The only cross-lane step per
-// reflector is a wave reduction of two dot products (DPP, no workgroup
-// barrier), and each reflector's column is fetched two reflectors ahead.
-// The reflector columns must hold v_i in full (reflector_prep_kernel): 0 in
-// rows <= i, 1 in row i + 1, so a column is loaded as it stands; rows at or
-// beyond n read as zero (their buffer offset lies past the buffer's end).
-// ---------------------------------------------------------------------------
-template <int CTRL>
-__device__ __forceinline__ double dpp_mov(double v) {
-    const long long b = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
-    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
-}
-
-// Sum over the 64 lanes, returned to every lane: pairs and quads by quad
-// permutes, rows of 16 by row shifts, then the row totals carried across
-// rows by the row broadcasts, so lane 63 ends with the total.
-__device__ __forceinline__ double wave_sum(double v) {
-    v += dpp_mov<0xB1>(v);    // quad_perm 1,0,3,2
-    v += dpp_mov<0x4E>(v);    // quad_perm 2,3,0,1
-    v += dpp_mov<0x114>(v);   // row_shr:4
-    v += dpp_mov<0x118>(v);   // row_shr:8
-    v += dpp_mov<0x142>(v);   // row_bcast:15
-    v += dpp_mov<0x143>(v);   // row_bcast:31
-    const long long b = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_readlane((int)b, 63);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
-    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
-}
-
 // Column i of every matrix: rows <= i := 0, row i + 1 := 1 (column n - 1:
 // all zero), so the column holds the reflector vector v_i exactly.  The
 // tridiagonal is kept separately, so nothing else reads these entries.
@@ -905,6 +1017,16 @@ __global__ void __launch_bounds__(256) reflector_prep_kernel(const EigDesc* __re
     for (int r = threadIdx.x >> 6; r <= last; r += 4) col[r] = (r == c + 1) ? 1.0 : 0.0;
 }
 
+// ---------------------------------------------------------------------------
+// Y := Q Y, one wave per (matrix, pair of columns): the wave keeps its two
+// columns in registers (lane l owns rows l + 64 m) and applies the
+// reflectors H_{n-2}, ..., H_0 in turn.  The only cross-lane step per
+// reflector is a wave reduction of two dot products (DPP, no workgroup
+// barrier), and each reflector's column is fetched two reflectors ahead.
+// The reflector columns must hold v_i in full (reflector_prep_kernel): 0 in
+// rows <= i, 1 in row i + 1, so a column is loaded as it stands; rows at or
+// beyond n read as zero (their buffer offset lies past the buffer's end).
+// ---------------------------------------------------------------------------
 constexpr int ORW_GROUP = 8;   // register rows per skip test (512 matrix rows)
 
 template <int MR>
@@ -992,6 +1114,14 @@ void launch_sytrd_step1(const dev::EigDesc& d, int i, double* sv, double* sp, hi
     const int t0 = (i + 1) / 64, t1 = (n - 1) / 64;
     const int nt = (t1 - t0 + 1) * (t1 - t0 + 2) / 2;
     hipLaunchKernelGGL(dev::sytrd_tile1_kernel, dim3(nt), dim3(256), 0, s, d, i, t0, 0, sv, sp);
+}
+
+int sytrd_tail_order() { return dev::SYT_TAIL; }
+
+void launch_sytrd_tail(const dev::EigDesc* d, int nmat, int i0, double* sv, double* sp,
+                       const int64_t* soff, hipStream_t s) {
+    if (nmat <= 0) return;
+    hipLaunchKernelGGL(dev::sytrd_tail_kernel, dim3(nmat), dim3(256), 0, s, d, i0, sv, sp, soff);
 }
 
 void launch_sytrd_step(const dev::EigDesc* d, const int4* work, int nwork, int i, double* sv,

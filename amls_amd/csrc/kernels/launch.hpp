@@ -30,6 +30,11 @@ void launch_sytrd_step(const dev::EigDesc* d, const int4* work, int nwork, int i
                        double* sp, const int64_t* soff, hipStream_t s);
 // Step i of one matrix (descriptor by value; sv 2n, sp 3n doubles).
 void launch_sytrd_step1(const dev::EigDesc& d, int i, double* sv, double* sp, hipStream_t s);
+// Steps i0 .. n-1 of every matrix in one workgroup each (n - i0 must not
+// exceed sytrd_tail_order()); matrices with n <= i0 are skipped.
+int sytrd_tail_order();
+void launch_sytrd_tail(const dev::EigDesc* d, int nmat, int i0, double* sv, double* sp,
+                       const int64_t* soff, hipStream_t s);
 // counts[m] = number of eigenvalues of matrix m >= thr[m], capped at kcap
 // (nmax: the largest n)
 void launch_eig_count(const dev::EigDesc* d, int nmat, const double* thr, int32_t* counts,
