@@ -422,11 +422,13 @@ void addSpmm(Plan& plan, const SparseOp& A, const double* X, double* Y, int64_t 
     plan.copy(to);
     const int nc = static_cast<int>(ncols);
     const SparseOp a = A;
-    if (a.nb > 0) {
+    if (a.nq > 0) {
         const double nblk = a.nnz / 9.0;
         plan.timed("spmm", 2.0 * a.nnz * ncols,
-                   nblk * (72.0 + 4.0) + (a.nb + 1) * 8.0 + 2.0 * n * ncols * 8.0,
-                   [=](hipStream_t s) { launch_spmm_bsr3(a.nb, a.bptr, a.bcol, a.bval, t1, t2, nc, s); });
+                   nblk * 72.0 + 2.0 * n * ncols * 8.0,
+                   [=](hipStream_t s) {
+                       launch_spmm_bsr3q(a.nq, a.nb, a.qptr, a.ucol, a.qvptr, a.qval, t1, t2, nc, s);
+                   });
     } else {
         plan.timed("spmm", 2.0 * a.nnz * ncols, a.nnz * 16.0 + (n + 1) * 8.0 + 2.0 * n * ncols * 8.0,
                    [=](hipStream_t s) { launch_spmm_rm(n, a.rp, a.ci, a.vx, t1, t2, nc, s); });
@@ -445,6 +447,52 @@ struct Bsr3Host {
 
 bool buildBsr3Sorted(int64_t n, const int64_t* rp, const int64_t* ci, const double* vx,
                      Bsr3Host& out);
+
+// Block rows grouped by four (the layout of launch_spmm_bsr3q): per quad the
+// merged, sorted column blocks with a mask of the rows holding each, and the
+// blocks themselves in that order.  Fails when a column offset does not fit
+// the 28 bits of the layout.
+struct Bsr3Quads {
+    std::vector<int64_t> qptr, qvptr;
+    std::vector<uint32_t> ucol;
+    std::vector<double> val;
+};
+
+bool buildQuads(const Bsr3Host& b, Bsr3Quads& out) {
+    const int64_t nb = static_cast<int64_t>(b.ptr.size()) - 1;
+    if (nb <= 0 || 3 * nb >= (int64_t(1) << 28)) return false;
+    const int64_t nq = (nb + 3) / 4;
+    out.qptr.assign(1, 0);
+    out.qvptr.assign(1, 0);
+    out.ucol.clear();
+    out.val.clear();
+    out.val.reserve(b.val.size());
+    struct Ent { int32_t col; int32_t row; int64_t blk; };
+    std::vector<Ent> ents;
+    for (int64_t q = 0; q < nq; ++q) {
+        ents.clear();
+        for (int r = 0; r < 4 && 4 * q + r < nb; ++r)
+            for (int64_t k = b.ptr[4 * q + r]; k < b.ptr[4 * q + r + 1]; ++k)
+                ents.push_back({b.col[k], r, k});
+        std::sort(ents.begin(), ents.end(), [](const Ent& x, const Ent& y) {
+            return x.col != y.col ? x.col < y.col : x.row < y.row;
+        });
+        for (size_t e = 0; e < ents.size();) {
+            size_t f = e;
+            uint32_t mask = 0;
+            for (; f < ents.size() && ents[f].col == ents[e].col; ++f) {
+                mask |= 1u << ents[f].row;
+                const double* v = b.val.data() + 9 * ents[f].blk;
+                out.val.insert(out.val.end(), v, v + 9);
+            }
+            out.ucol.push_back(static_cast<uint32_t>(3 * ents[e].col) | (mask << 28));
+            e = f;
+        }
+        out.qptr.push_back(static_cast<int64_t>(out.ucol.size()));
+        out.qvptr.push_back(static_cast<int64_t>(out.val.size() / 9));
+    }
+    return true;
+}
 
 bool buildBsr3(int64_t n, const int64_t* rp, const int64_t* ci, const double* vx, Bsr3Host& out) {
     if (n <= 0 || n % 3 != 0 || n / 3 > INT32_MAX) return false;
@@ -935,33 +983,43 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
     opM = csrOp(dMp, dMi, dMx, static_cast<double>(nm));
     opPK = csrOp(dPKp, dPKi, dPKx, static_cast<double>(nk));
     opPM = csrOp(dPMp, dPMi, dPMx, static_cast<double>(nm));
-    Bsr3Host bK, bM, bPK, bPM;
-    const bool blocks = buildBsr3(n, Kp.data(), Ki.data(), Kx.data(), bK) &&
-                        buildBsr3(n, Mp.data(), Mi.data(), Mx.data(), bM) &&
-                        buildBsr3(n, pkp.data(), pki.data(), pkx.data(), bPK) &&
-                        buildBsr3(n, pmp.data(), pmi.data(), pmx.data(), bPM);
+    Bsr3Quads qK, qM, qPK, qPM;
+    bool blocks = false;
+    {
+        Bsr3Host bK, bM, bPK, bPM;
+        blocks = buildBsr3(n, Kp.data(), Ki.data(), Kx.data(), bK) &&
+                 buildBsr3(n, Mp.data(), Mi.data(), Mx.data(), bM) &&
+                 buildBsr3(n, pkp.data(), pki.data(), pkx.data(), bPK) &&
+                 buildBsr3(n, pmp.data(), pmi.data(), pmx.data(), bPM) && buildQuads(bK, qK) &&
+                 buildQuads(bM, qM) && buildQuads(bPK, qPK) && buildQuads(bPM, qPM);
+    }
     bsrMem.reset();
     if (blocks) {
         auto& B = fresh(bsrMem);
-        for (const Bsr3Host* h : {&bK, &bM, &bPK, &bPM}) {
-            B.plan(h->ptr.size() * 8);
-            B.plan(h->col.size() * 4);
+        for (const Bsr3Quads* h : {&qK, &qM, &qPK, &qPM}) {
+            B.plan(h->qptr.size() * 8);
+            B.plan(h->qvptr.size() * 8);
+            B.plan(h->ucol.size() * 4);
             B.plan(h->val.size() * 8);
         }
         B.commit();
         SparseOp* ops[4] = {&opK, &opM, &opPK, &opPM};
-        const Bsr3Host* hs[4] = {&bK, &bM, &bPK, &bPM};
+        const Bsr3Quads* hs[4] = {&qK, &qM, &qPK, &qPM};
         for (int q = 0; q < 4; ++q) {
-            int64_t* pp = B.take<int64_t>(hs[q]->ptr.size());
-            int32_t* pc = B.take<int32_t>(hs[q]->col.size());
-            double* pv = B.take<double>(hs[q]->val.size());
-            up(pp, hs[q]->ptr.data(), hs[q]->ptr.size() * 8);
-            up(pc, hs[q]->col.data(), hs[q]->col.size() * 4);
-            up(pv, hs[q]->val.data(), hs[q]->val.size() * 8);
+            int64_t* pq = B.take<int64_t>(hs[q]->qptr.size());
+            int64_t* pv = B.take<int64_t>(hs[q]->qvptr.size());
+            uint32_t* pu = B.take<uint32_t>(hs[q]->ucol.size());
+            double* px = B.take<double>(hs[q]->val.size());
+            up(pq, hs[q]->qptr.data(), hs[q]->qptr.size() * 8);
+            up(pv, hs[q]->qvptr.data(), hs[q]->qvptr.size() * 8);
+            up(pu, hs[q]->ucol.data(), hs[q]->ucol.size() * 4);
+            up(px, hs[q]->val.data(), hs[q]->val.size() * 8);
             ops[q]->nb = n / 3;
-            ops[q]->bptr = pp;
-            ops[q]->bcol = pc;
-            ops[q]->bval = pv;
+            ops[q]->nq = static_cast<int64_t>(hs[q]->qptr.size()) - 1;
+            ops[q]->qptr = pq;
+            ops[q]->qvptr = pv;
+            ops[q]->ucol = pu;
+            ops[q]->qval = px;
         }
     }
     sync();   // host vectors above go out of scope

@@ -128,10 +128,11 @@ struct SparseOp {
     const int64_t *rp = nullptr, *ci = nullptr;
     const double* vx = nullptr;
     double nnz = 0.0;
-    int64_t nb = 0;
-    const int64_t* bptr = nullptr;
-    const int32_t* bcol = nullptr;
-    const double* bval = nullptr;
+    // 3 x 3 blocks grouped by four block rows (see launch_spmm_bsr3q)
+    int64_t nb = 0, nq = 0;
+    const int64_t *qptr = nullptr, *qvptr = nullptr;
+    const uint32_t* ucol = nullptr;
+    const double* qval = nullptr;
 };
 
 // Work list of the batched tridiagonalisation (every active tile of every

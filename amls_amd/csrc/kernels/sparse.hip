@@ -88,125 +88,142 @@ __global__ void __launch_bounds__(256) spmm_rm_kernel(int64_t n, const int64_t* 
 }
 
 // 3 x 3 block-sparse rows (elasticity: one block per pair of coupled mesh
-// nodes): Y(3b + i, :) = sum over blocks (b, c) of A_bc(i, :) X(3c .. 3c+2, :).
-// One wave per block row.  The wave stages its row's blocks (values and
-// column indices, at most SPM_MAXB blocks) in LDS once and then sweeps every
-// column of X in chunks of 128 (two columns per lane), so the matrix is read
-// from memory once instead of once per chunk; the block values are LDS
-// broadcasts.  Each block costs six coalesced 512-byte reads of X for 18
-// FMAs per lane, four blocks in flight at a time.  Rows with more blocks
-// than SPM_MAXB read the matrix from global memory per chunk.  Same
-// XCD-contiguous row mapping as spmm_rm_kernel.
-constexpr int SPM_MAXB = 128;
+// nodes): Y(3b + i, :) = sum over blocks (b, c) of A_bc(i, :) X(3c .. 3c+2, :),
+// row-major X and Y.  Every lane holds SPM_CPL columns of a 64 SPM_CPL
+// column chunk, so each X row triple is three coalesced 512-byte reads per
+// column group, and the block values are LDS broadcasts.
 constexpr int SPM_CPL = 2;   // columns per lane
 
+// The block rows are taken four at a time ("quads"):
+// the quad's four column lists are merged, so each distinct column block
+// costs one set of X reads for all four rows (on the elasticity meshes a
+// quad touches ~100 column blocks against ~226 blocks in its rows, so X
+// traffic from L2 drops by ~2.3x).  Per quad: qptr -> merged columns ucol
+// (X row offset 3c in bits 0..27, which of the four rows hold a block in
+// bits 28..31), qvptr -> the blocks that exist, in merged-column order and
+// row order within a column.  One workgroup per quad, its four waves taking
+// every fourth column chunk; the quad's columns and blocks are staged in LDS
+// when they fit (SPQ_MAXU / SPQ_MAXB), otherwise read from global memory.
+constexpr int SPQ_MAXU = 160;
+constexpr int SPQ_MAXB = 320;
+
 template <bool Lds>
-__device__ __forceinline__ void bsr3_row(const double* __restrict__ av, const int* __restrict__ ac,
-                                         int nbk, const double* __restrict__ x,
-                                         double* __restrict__ y, int64_t br, int ncols, int lane) {
-    for (int c0 = 0; c0 < ncols; c0 += 64 * SPM_CPL) {
+__device__ __forceinline__ void bsr3q_quad(const uint32_t* __restrict__ uc, int nu,
+                                           const double* __restrict__ av,
+                                           const double* __restrict__ x, double* __restrict__ y,
+                                           int64_t q, int64_t nb, int ncols, int lane, int c_first,
+                                           int c_step) {
+    for (int c0 = c_first; c0 < ncols; c0 += c_step) {
         int cc[SPM_CPL];
         bool on[SPM_CPL];
 #pragma unroll
-        for (int q = 0; q < SPM_CPL; ++q) {
-            const int c = c0 + lane + 64 * q;
-            on[q] = c < ncols;
-            cc[q] = on[q] ? c : 0;   // in-bounds column for the loads of idle lanes
+        for (int k = 0; k < SPM_CPL; ++k) {
+            const int c = c0 + lane + 64 * k;
+            on[k] = c < ncols;
+            cc[k] = on[k] ? c : 0;
         }
-        double a0[SPM_CPL], a1[SPM_CPL], a2[SPM_CPL];
+        double acc[4][3][SPM_CPL];
 #pragma unroll
-        for (int q = 0; q < SPM_CPL; ++q) a0[q] = a1[q] = a2[q] = 0.0;
-        int k = 0;
-        for (; k + 4 <= nbk; k += 4) {
-            double xv[4][3][SPM_CPL];
+        for (int r = 0; r < 4; ++r)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int64_t xr = ac[k + u];
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int k = 0; k < SPM_CPL; ++k) acc[r][i][k] = 0.0;
+        int bi = 0;
+        // two merged columns per round: their X rows are in flight together
+        int u = 0;
+        for (; u + 2 <= nu; u += 2) {
+            double xv[2][3][SPM_CPL];
+            uint32_t w[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                w[h] = uc[u + h];
+                const int64_t xr = w[h] & 0x0fffffffu;
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
 #pragma unroll
-                    for (int q = 0; q < SPM_CPL; ++q) xv[u][j][q] = x[(xr + j) * ncols + cc[q]];
+                    for (int k = 0; k < SPM_CPL; ++k) xv[h][j][k] = x[(xr + j) * ncols + cc[k]];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const double* a = av + 9 * (k + u);
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t mk = w[h] >> 28;
 #pragma unroll
-                for (int q = 0; q < SPM_CPL; ++q) {
-                    a0[q] += a[0] * xv[u][0][q] + a[1] * xv[u][1][q] + a[2] * xv[u][2][q];
-                    a1[q] += a[3] * xv[u][0][q] + a[4] * xv[u][1][q] + a[5] * xv[u][2][q];
-                    a2[q] += a[6] * xv[u][0][q] + a[7] * xv[u][1][q] + a[8] * xv[u][2][q];
+                for (int r = 0; r < 4; ++r)
+                    if (mk & (1u << r)) {
+                        const double* a = av + 9 * bi;
+                        ++bi;
+#pragma unroll
+                        for (int k = 0; k < SPM_CPL; ++k) {
+                            acc[r][0][k] += a[0] * xv[h][0][k] + a[1] * xv[h][1][k] + a[2] * xv[h][2][k];
+                            acc[r][1][k] += a[3] * xv[h][0][k] + a[4] * xv[h][1][k] + a[5] * xv[h][2][k];
+                            acc[r][2][k] += a[6] * xv[h][0][k] + a[7] * xv[h][1][k] + a[8] * xv[h][2][k];
+                        }
+                    }
+            }
+        }
+        for (; u < nu; ++u) {
+            const uint32_t w = uc[u];
+            const int64_t xr = w & 0x0fffffffu;
+            double xv[3][SPM_CPL];
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < SPM_CPL; ++k) xv[j][k] = x[(xr + j) * ncols + cc[k]];
+            const uint32_t mk = w >> 28;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (mk & (1u << r)) {
+                    const double* a = av + 9 * bi;
+                    ++bi;
+#pragma unroll
+                    for (int k = 0; k < SPM_CPL; ++k) {
+                        acc[r][0][k] += a[0] * xv[0][k] + a[1] * xv[1][k] + a[2] * xv[2][k];
+                        acc[r][1][k] += a[3] * xv[0][k] + a[4] * xv[1][k] + a[5] * xv[2][k];
+                        acc[r][2][k] += a[6] * xv[0][k] + a[7] * xv[1][k] + a[8] * xv[2][k];
+                    }
                 }
-            }
-        }
-        for (; k < nbk; ++k) {
-            const int64_t xr = ac[k];
-            const double* a = av + 9 * k;
-#pragma unroll
-            for (int q = 0; q < SPM_CPL; ++q) {
-                const double x0 = x[xr * ncols + cc[q]], x1 = x[(xr + 1) * ncols + cc[q]],
-                             x2 = x[(xr + 2) * ncols + cc[q]];
-                a0[q] += a[0] * x0 + a[1] * x1 + a[2] * x2;
-                a1[q] += a[3] * x0 + a[4] * x1 + a[5] * x2;
-                a2[q] += a[6] * x0 + a[7] * x1 + a[8] * x2;
-            }
         }
 #pragma unroll
-        for (int q = 0; q < SPM_CPL; ++q)
-            if (on[q]) {
-                const int c = c0 + lane + 64 * q;
-                y[(3 * br) * ncols + c] = a0[q];
-                y[(3 * br + 1) * ncols + c] = a1[q];
-                y[(3 * br + 2) * ncols + c] = a2[q];
+        for (int r = 0; r < 4; ++r) {
+            const int64_t br = 4 * q + r;
+            if (br < nb) {
+#pragma unroll
+                for (int k = 0; k < SPM_CPL; ++k)
+                    if (on[k]) {
+                        const int c = c0 + lane + 64 * k;
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) y[(3 * br + i) * ncols + c] = acc[r][i][k];
+                    }
             }
+        }
     }
 }
 
-__global__ void __launch_bounds__(256) spmm_bsr3_kernel(int64_t nb, const int64_t* __restrict__ ptr,
-                                                        const int32_t* __restrict__ col,
-                                                        const double* __restrict__ val,
-                                                        const double* __restrict__ x,
-                                                        double* __restrict__ y, int ncols) {
-    __shared__ double sv[4][SPM_MAXB * 9];
-    __shared__ int sc[4][SPM_MAXB];
+__global__ void __launch_bounds__(256) spmm_bsr3q_kernel(int64_t nq, int64_t nb,
+                                                         const int64_t* __restrict__ qptr,
+                                                         const uint32_t* __restrict__ ucol,
+                                                         const int64_t* __restrict__ qvptr,
+                                                         const double* __restrict__ val,
+                                                         const double* __restrict__ x,
+                                                         double* __restrict__ y, int ncols) {
+    __shared__ double sv[SPQ_MAXB * 9];
+    __shared__ uint32_t su[SPQ_MAXU];
     const unsigned per = gridDim.x / NXCD;
-    const unsigned blk = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
-    const int w = threadIdx.x >> 6;
-    const int64_t br = (int64_t)blk * 4 + w;
-    const int lane = threadIdx.x & 63;
-    if (br >= nb) return;
-    const int64_t b0 = ptr[br], b1 = ptr[br + 1];
-    const int nbk = (int)(b1 - b0);
-    if (nbk > SPM_MAXB) {
-        // X row offsets of the blocks are recomputed from col per chunk
-        for (int c0 = 0; c0 < ncols; c0 += 64) {
-            const int c = c0 + lane;
-            const bool on = c < ncols;
-            const int cc = on ? c : 0;
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-            for (int64_t k = b0; k < b1; ++k) {
-                const int64_t xr = 3 * (int64_t)col[k];
-                const double x0 = x[xr * ncols + cc], x1 = x[(xr + 1) * ncols + cc],
-                             x2 = x[(xr + 2) * ncols + cc];
-                const double* a = val + 9 * k;
-                a0 += a[0] * x0 + a[1] * x1 + a[2] * x2;
-                a1 += a[3] * x0 + a[4] * x1 + a[5] * x2;
-                a2 += a[6] * x0 + a[7] * x1 + a[8] * x2;
-            }
-            if (on) {
-                y[(3 * br) * ncols + c] = a0;
-                y[(3 * br + 1) * ncols + c] = a1;
-                y[(3 * br + 2) * ncols + c] = a2;
-            }
-        }
+    const int64_t q = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    if (q >= nq) return;
+    const int64_t u0 = qptr[q], b0 = qvptr[q];
+    const int nu = (int)(qptr[q + 1] - u0), nbk = (int)(qvptr[q + 1] - b0);
+    // the four waves take every fourth column chunk of the quad
+    const int cs = 64 * SPM_CPL;
+    if (nu > SPQ_MAXU || nbk > SPQ_MAXB) {
+        bsr3q_quad<false>(ucol + u0, nu, val + 9 * b0, x, y, q, nb, ncols, lane, w * cs, 4 * cs);
         return;
     }
-    // the wave's own LDS slice: written and read by this wave only
-    for (int e = lane; e < nbk * 9; e += 64) sv[w][e] = val[9 * b0 + e];
-    for (int e = lane; e < nbk; e += 64) sc[w][e] = 3 * col[b0 + e];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    bsr3_row<true>(sv[w], sc[w], nbk, x, y, br, ncols, lane);
+    for (int e = t; e < nbk * 9; e += 256) sv[e] = val[9 * b0 + e];
+    for (int e = t; e < nu; e += 256) su[e] = ucol[u0 + e];
+    __syncthreads();
+    bsr3q_quad<true>(su, nu, sv, x, y, q, nb, ncols, lane, w * cs, 4 * cs);
 }
 
 // One block per column.
@@ -286,13 +303,13 @@ void launch_spmm_rm(int64_t n, const int64_t* rowptr, const int64_t* colidx, con
                        n, rowptr, colidx, vals, x, y, ncols);
 }
 
-void launch_spmm_bsr3(int64_t nb, const int64_t* ptr, const int32_t* col, const double* val,
-                      const double* x, double* y, int ncols, hipStream_t s) {
-    if (nb <= 0 || ncols <= 0) return;
-    const unsigned g = (unsigned)((nb + 3) / 4);
-    const unsigned gx = (g + dev::NXCD - 1) / dev::NXCD * dev::NXCD;
-    hipLaunchKernelGGL(dev::spmm_bsr3_kernel, dim3(gx), dim3(256), 0, s, nb, ptr, col, val, x, y,
-                       ncols);
+void launch_spmm_bsr3q(int64_t nq, int64_t nb, const int64_t* qptr, const uint32_t* ucol,
+                       const int64_t* qvptr, const double* val, const double* x, double* y, int ncols,
+                       hipStream_t s) {
+    if (nq <= 0 || ncols <= 0) return;
+    const unsigned gx = (unsigned)((nq + dev::NXCD - 1) / dev::NXCD * dev::NXCD);
+    hipLaunchKernelGGL(dev::spmm_bsr3q_kernel, dim3(gx), dim3(256), 0, s, nq, nb, qptr, ucol, qvptr,
+                       val, x, y, ncols);
 }
 
 void launch_coldot(int64_t n, const double* x, const double* y, int ncols, int64_t ld, double* out,
