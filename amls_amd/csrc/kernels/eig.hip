@@ -119,9 +119,20 @@ __device__ __forceinline__ void sytrd_step_body(const EigDesc& d, int i, int4 wk
     // tile vectors: thread t < 64 takes row r0 + t, 64 <= t < 128 column c0 + t - 64
     const int vidx = t < 64 ? r0 + t : c0 + t - 64;
     const bool vok = haveTile && t < 128 && vidx > i && vidx < n;
-    const double vv_t = (vok && hasPrev) ? svr[vidx] : 0.0;
-    const double pv_t = (vok && hasPrev) ? spr[vidx] : 0.0;
-    const double av_t = vok ? Ai[vidx] : 0.0;
+    // staged in LDS at once (v final; p and the column entry until w and x
+    // replace them), so the loads go out with the others above instead of
+    // being deferred to their use after the reductions
+    if (t < 128) {
+        const double vv_t = (vok && hasPrev) ? svr[vidx] : 0.0;
+        const double pv_t = (vok && hasPrev) ? spr[vidx] : 0.0;
+        const double av_t = vok ? Ai[vidx] : 0.0;
+        double* vd = t < 64 ? vr : vc;
+        double* wd = t < 64 ? wr : wc;
+        double* xd = t < 64 ? xr : xc;
+        vd[t & 63] = vv_t;
+        wd[t & 63] = pv_t;
+        xd[t & 63] = av_t;
+    }
 
     // ---- w_{i-1} = tau p + alpha v with alpha = -1/2 tau^2 (p.v)
     double alphaP = 0.0;
@@ -161,7 +172,10 @@ __device__ __forceinline__ void sytrd_step_body(const EigDesc& d, int i, int4 wk
         tau = (beta - alpha) / beta;
         scal = 1.0 / (alpha - beta);
     }
-    if (wk.w) {
+    // global stores are issued last: a barrier makes every wave wait for the
+    // stores issued before it, so none may precede the barriers below
+    auto leader = [&]() {
+        if (!wk.w) return;
         // leader: d, e, tau, v_i, clear p_{i+1}, store the reflector of step i-1
         if (t == 0) {
             d.d[i] = aII - 2.0 * vI * wI;
@@ -184,22 +198,23 @@ __device__ __forceinline__ void sytrd_step_body(const EigDesc& d, int i, int4 wk
             spz[r] = 0.0;
             if (hasPrev) A[r + (int64_t)(i - 1) * lda] = v;
         }
+    };
+    if (wk.y < 0) {
+        leader();
+        return;
     }
-    if (wk.y < 0) return;
     if (t < 128) {
+        double* wd = t < 64 ? wr : wc;
+        double* xd = t < 64 ? xr : xc;
+        const double vv_t = (t < 64 ? vr : vc)[t & 63];
+        const double pv_t = wd[t & 63], av_t = xd[t & 63];
         const double w = tauP * pv_t + alphaP * vv_t;
         const double x = vok ? (vidx == i + 1 ? 1.0 : xof(av_t, vv_t, pv_t) * scal) : 0.0;
-        if (t < 64) {
-            vr[t] = vv_t;
-            wr[t] = vok ? w : 0.0;
-            xr[t] = x;
-        } else {
-            vc[t - 64] = vv_t;
-            wc[t - 64] = vok ? w : 0.0;
-            xc[t - 64] = x;
-        }
+        wd[t & 63] = vok ? w : 0.0;
+        xd[t & 63] = x;
     }
     __syncthreads();
+    unsigned store = 0;   // tile elements to write back (bit k)
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const int e = t + 256 * k, rr = e & 63, cc = e >> 6;
@@ -207,8 +222,9 @@ __device__ __forceinline__ void sytrd_step_body(const EigDesc& d, int i, int4 wk
         double a = ta[k];
         if (r < n && c > i && r >= c && hasPrev) {
             a -= vr[rr] * wc[cc] + wr[rr] * vc[cc];
-            A[r + (int64_t)c * lda] = a;
+            store |= 1u << k;
         }
+        ta[k] = a;
         tile[rr * 65 + cc] = a;
     }
     __syncthreads();
@@ -239,6 +255,13 @@ __device__ __forceinline__ void sytrd_step_body(const EigDesc& d, int i, int4 wk
         const double s = pct[0][cc] + pct[1][cc] + pct[2][cc] + pct[3][cc];
         if (c > i && c < n) atomicAdd(spw + c, s);
     }
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        if (store & (1u << k)) {
+            const int e = t + 256 * k;
+            A[(r0 + (e & 63)) + (int64_t)(c0 + (e >> 6)) * lda] = ta[k];
+        }
+    leader();
 }
 
 // Batched step: work item (matrix, tile row, tile col, leader flag).
