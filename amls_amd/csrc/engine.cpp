@@ -2107,7 +2107,10 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     }
     // ---- final Rayleigh-Ritz with the original FP64 K and M, per problem:
     // Kc = Y^T K Y, Mc = Y^T M Y; lowest nev pairs of (Kc, Mc) through
-    // Kc = L D L^T, C = D^-1/2 L^-1 Mc L^-T D^-1/2, nu = 1/theta
+    // Kc = L D L^T, C = D^-1/2 L^-1 Mc L^-T D^-1/2, nu = 1/theta.  L^-1 is
+    // formed once (one blocked triangular solve on the identity, into the
+    // storage of Mc after Mc is copied out), and the two congruence products
+    // and the back-transformation of the eigenvectors are GEMMs with it.
     addSpmm(plan, opPK, Yv, R, q, Tr1, Tr2);
     {
         GemmBatch gr;
@@ -2143,28 +2146,55 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     }
     addDenseLdlt(plan, KcS, q, q, WpS, err, -1);
     const int nv = static_cast<int>(nev);
-    Copy2dBatch c1, c2, c3, c4, cv;
-    TrsmBatch u1, u2, tv;
-    std::vector<VecDesc> vd(NS), v2(NS);
+    Copy2dBatch c1, cz, c3, cv;
+    TrsmBatch tinv;
+    std::vector<VecDesc> vd(NS), v2(NS), vid(NS);
     std::vector<EigDesc> eds;
     std::vector<int64_t> soff;
-    GemmBatch gx;
+    GemmBatch gx, gl1, gl2, gv;
     for (int64_t sg = 0; sg < NS; ++sg) {
         const auto [r0, ns] = rowsOf(sg);
         double *kc = KcS[sg], *mc = McS[sg], *cm = Cm + sg * qq, *tm = Tm + sg * qq;
+        double* li = mc;   // L^-1 replaces Mc once Mc is copied to cm
         double* ds = dsq + sg * q;
         c1.add(cdesc(mc, q, cm, q, q, q, 2));
-        u1.add(kc, q, cm, q, q, q, 0);
-        c2.add(cdesc(cm, q, tm, q, q, q, 1));
-        u2.add(kc, q, tm, q, q, q, 0);
+        cz.add(cdesc(nullptr, 0, li, q, q, q, 3, nullptr, nullptr, 0.0));
+        std::memset(&vid[sg], 0, sizeof(VecDesc));
+        vid[sg].dst = li;
+        vid[sg].ld = q;
+        vid[sg].n = qi;
+        vid[sg].op = 2;
+        tinv.add(kc, q, li, q, q, q, 0);
+        tinv.descs.back().rhsLower = 1;
+        GemmDesc l1 = gdesc();           // tm = L^-1 Mc (L^-1 lower triangular)
+        l1.A = li;
+        l1.lda = q;
+        l1.triA = 1;
+        l1.B = cm;
+        l1.ldb = q;
+        l1.k = qi;
+        l1.C = tm;
+        l1.ldc = q;
+        l1.m = l1.n = qi;
+        gl1.add(l1);
+        GemmDesc l2 = gdesc();           // cm = tm L^-T
+        l2.A = tm;
+        l2.lda = q;
+        l2.B = li;
+        l2.ldb = q;
+        l2.tb = 1;
+        l2.k = qi;
+        l2.C = cm;
+        l2.ldc = q;
+        l2.m = l2.n = qi;
+        gl2.add(l2);
         std::memset(&vd[sg], 0, sizeof(VecDesc));
         vd[sg].src = kc;
         vd[sg].dst = ds;
         vd[sg].ld = q;
         vd[sg].n = qi;
         vd[sg].op = 0;
-        c3.add(cdesc(tm, q, tm, q, q, q, 0, ds, ds));
-        c4.add(cdesc(tm, q, cm, q, q, q, 2));
+        c3.add(cdesc(cm, q, cm, q, q, q, 0, ds, ds));
         EigDesc ed;
         std::memset(&ed, 0, sizeof(ed));
         ed.A = cm;
@@ -2186,8 +2216,20 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
         v2[sg].dst2 = sth + sg * nev;
         v2[sg].n = nv;
         v2[sg].op = 1;
-        cv.add(cdesc(Yt + sg * q * nev, q, V + sg * q * nev, q, q, nev, 0, ds, sth + sg * nev));
-        tv.add(kc, q, V + sg * q * nev, q, q, nev, 1);
+        cv.add(cdesc(Yt + sg * q * nev, q, Yt + sg * q * nev, q, q, nev, 0, ds, sth + sg * nev));
+        GemmDesc lv = gdesc();           // V = L^-T (scaled eigenvectors), L^-T upper
+        lv.A = li;
+        lv.lda = q;
+        lv.ta = 1;
+        lv.triA = 2;
+        lv.B = Yt + sg * q * nev;
+        lv.ldb = q;
+        lv.k = qi;
+        lv.C = V + sg * q * nev;
+        lv.ldc = q;
+        lv.m = qi;
+        lv.n = nv;
+        gv.add(lv);
         GemmDesc g3 = gdesc();
         g3.A = Yv + r0;
         g3.lda = n;
@@ -2201,17 +2243,18 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
         gx.add(g3);
     }
     plan.copy(c1);
-    plan.trsm(u1);
-    plan.copy(c2);
-    plan.trsm(u2);
+    plan.copy(cz);
+    plan.vec(vid);
+    plan.trsm(tinv);
+    plan.gemm(gl1, false);
+    plan.gemm(gl2, false);
     plan.vec(vd);
     plan.copy(c3);
-    plan.copy(c4);
     addSytrd(plan, eds, soff, sv, sp);
     addSelect(plan, eds, true);
     plan.vec(v2);
     plan.copy(cv);
-    plan.trsm(tv);
+    plan.gemm(gv, false);
     plan.gemm(gx, false);
     // un-permute and M-normalise into the output block
     // the subspace-iteration result has its own output block (dXout may
