@@ -101,9 +101,12 @@ constexpr int SPM_CPL = 2;   // columns per lane
 // traffic from L2 drops by ~2.3x).  Per quad: qptr -> merged columns ucol
 // (X row offset 3c in bits 0..27, which of the four rows hold a block in
 // bits 28..31), qvptr -> the blocks that exist, in merged-column order and
-// row order within a column.  One workgroup per quad, its four waves taking
-// every fourth column chunk; the quad's columns and blocks are staged in LDS
-// when they fit (SPQ_MAXU / SPQ_MAXB), otherwise read from global memory.
+// row order within a column.  One workgroup per (quad, group of four column
+// chunks), one chunk per wave; the column groups are the slow grid
+// dimension, so the whole GPU streams through one 512-column slab of X at a
+// time (the quad's blocks are read once per slab).  A quad's columns and
+// blocks are staged in LDS when they fit (SPQ_MAXU / SPQ_MAXB), otherwise
+// read from global memory.
 constexpr int SPQ_MAXU = 160;
 constexpr int SPQ_MAXB = 320;
 
@@ -214,16 +217,16 @@ __global__ void __launch_bounds__(256) spmm_bsr3q_kernel(int64_t nq, int64_t nb,
     if (q >= nq) return;
     const int64_t u0 = qptr[q], b0 = qvptr[q];
     const int nu = (int)(qptr[q + 1] - u0), nbk = (int)(qvptr[q + 1] - b0);
-    // the four waves take every fourth column chunk of the quad
+    // column group blockIdx.y (four chunks), one chunk per wave
     const int cs = 64 * SPM_CPL;
     if (nu > SPQ_MAXU || nbk > SPQ_MAXB) {
-        bsr3q_quad<false>(ucol + u0, nu, val + 9 * b0, x, y, q, nb, ncols, lane, w * cs, 4 * cs);
+        bsr3q_quad<false>(ucol + u0, nu, val + 9 * b0, x, y, q, nb, ncols, lane, (blockIdx.y * 4 + w) * cs, 1 << 30);
         return;
     }
     for (int e = t; e < nbk * 9; e += 256) sv[e] = val[9 * b0 + e];
     for (int e = t; e < nu; e += 256) su[e] = ucol[u0 + e];
     __syncthreads();
-    bsr3q_quad<true>(su, nu, sv, x, y, q, nb, ncols, lane, w * cs, 4 * cs);
+    bsr3q_quad<true>(su, nu, sv, x, y, q, nb, ncols, lane, (blockIdx.y * 4 + w) * cs, 1 << 30);
 }
 
 // One block per column.
@@ -308,7 +311,8 @@ void launch_spmm_bsr3q(int64_t nq, int64_t nb, const int64_t* qptr, const uint32
                        hipStream_t s) {
     if (nq <= 0 || ncols <= 0) return;
     const unsigned gx = (unsigned)((nq + dev::NXCD - 1) / dev::NXCD * dev::NXCD);
-    hipLaunchKernelGGL(dev::spmm_bsr3q_kernel, dim3(gx), dim3(256), 0, s, nq, nb, qptr, ucol, qvptr,
+    const unsigned gy = (unsigned)((ncols + 4 * 64 * dev::SPM_CPL - 1) / (4 * 64 * dev::SPM_CPL));
+    hipLaunchKernelGGL(dev::spmm_bsr3q_kernel, dim3(gx, gy), dim3(256), 0, s, nq, nb, qptr, ucol, qvptr,
                        val, x, y, ncols);
 }
 
