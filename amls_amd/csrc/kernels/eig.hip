@@ -214,18 +214,31 @@ __device__ __forceinline__ void sytrd_step_body(const EigDesc& d, int i, int4 wk
         xd[t & 63] = x;
     }
     __syncthreads();
+    // the vector entries of the thread's row and its 16 columns, read into
+    // registers before the loop: the LDS writes of the loop would otherwise
+    // order every read behind the previous element's write
     unsigned store = 0;   // tile elements to write back (bit k)
+    {
+        const int rr = t & 63, cq = t >> 6;
+        const double vrr = vr[rr], wrr = wr[rr];
+        double vcc[16], wcc[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const int e = t + 256 * k, rr = e & 63, cc = e >> 6;
-        const int r = r0 + rr, c = c0 + cc;
-        double a = ta[k];
-        if (r < n && c > i && r >= c && hasPrev) {
-            a -= vr[rr] * wc[cc] + wr[rr] * vc[cc];
-            store |= 1u << k;
+        for (int k = 0; k < 16; ++k) {
+            vcc[k] = vc[cq + 4 * k];
+            wcc[k] = wc[cq + 4 * k];
         }
-        ta[k] = a;
-        tile[rr * 65 + cc] = a;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int cc = cq + 4 * k;
+            const int r = r0 + rr, c = c0 + cc;
+            double a = ta[k];
+            if (r < n && c > i && r >= c && hasPrev) {
+                a -= vrr * wcc[k] + wrr * vcc[k];
+                store |= 1u << k;
+            }
+            ta[k] = a;
+            tile[rr * 65 + cc] = a;
+        }
     }
     __syncthreads();
     // products with x_i: thread (quarter q, lane l) sums 16 terms of row l
@@ -235,11 +248,24 @@ __device__ __forceinline__ void sytrd_step_body(const EigDesc& d, int i, int4 wk
     {
         const int q = t >> 6, l = t & 63;
         double rs = 0.0, cs = 0.0;
+        // two halves of eight terms, each half's LDS reads issued together
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int k = 16 * q + u;
-            rs += tile[l * 65 + k] * xc[k];
-            if (r0 + k != c0 + l) cs += tile[k * 65 + l] * xr[k];
+        for (int h = 0; h < 2; ++h) {
+            double tr_[8], tc_[8], xc_[8], xr_[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = 16 * q + 8 * h + u;
+                tr_[u] = tile[l * 65 + k];
+                tc_[u] = tile[k * 65 + l];
+                xc_[u] = xc[k];
+                xr_[u] = xr[k];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = 16 * q + 8 * h + u;
+                rs += tr_[u] * xc_[u];
+                if (r0 + k != c0 + l) cs += tc_[u] * xr_[u];
+            }
         }
         prt[q][l] = rs;
         pct[q][l] = cs;
