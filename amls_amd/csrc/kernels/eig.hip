@@ -549,15 +549,14 @@ __global__ void __launch_bounds__(64) eig_count_kernel(const EigDesc* __restrict
 }
 
 // Two Sturm counts at once (x1, x2), interleaved so the two division chains
-// overlap.  e^2 / q uses the hardware reciprocal refined by two Newton steps
-// (relative error a few ulp, like a rounded division: the counts stay
-// those of a tiny relative perturbation of T, the property bisection needs;
-// |q| >= pivmin keeps the reciprocal finite).
-__device__ __forceinline__ double sturm_div(double a, double b) {
-    double r = __builtin_amdgcn_rcp(b);
-    r = fma(r, fma(-b, r, 1.0), r);
-    r = fma(r, fma(-b, r, 1.0), r);
-    return a * r;
+// overlap.  e^2 / q uses the hardware reciprocal (relative error 4.6e-8
+// measured on gfx950) refined by one Newton step (2.1e-15, about ten ulp):
+// the counts are those of a relative perturbation of e^2 of that size, so
+// the eigenvalues keep an absolute accuracy of about 1e-15 |T|; |q| >=
+// pivmin keeps the reciprocal finite.
+__device__ __forceinline__ double sturm_rcp(double b) {
+    const double r = __builtin_amdgcn_rcp(b);
+    return fma(r, fma(-b, r, 1.0), r);
 }
 
 __device__ void sturm_count2(const double* __restrict__ dd, const double* __restrict__ ee, int n,
@@ -569,8 +568,8 @@ __device__ void sturm_count2(const double* __restrict__ dd, const double* __rest
     for (int j = 1; j < n; ++j) {
         const double e = ee[j - 1], dj = dd[j];
         const double e2 = e * e;
-        q1 = (dj - x1) - sturm_div(e2, q1);
-        q2 = (dj - x2) - sturm_div(e2, q2);
+        q1 = fma(-e2, sturm_rcp(q1), dj - x1);
+        q2 = fma(-e2, sturm_rcp(q2), dj - x2);
         if (fabs(q1) < pivmin) q1 = -pivmin;
         if (fabs(q2) < pivmin) q2 = -pivmin;
         n1 += q1 < 0.0;
