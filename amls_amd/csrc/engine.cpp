@@ -401,7 +401,10 @@ void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<i
             launch_sytrd_tail(bl->at<EigDesc>(od), nmat, i0, sv, sp, bl->at<int64_t>(oo), s);
         };
         if (kst) {
-            if (nsteps > 0) kst->run("sytrd_step", 2.0 * bytes / 8.0, bytes, s, loop, nsteps);
+            // one matrix (sytrd_tile1_kernel) and batches (sytrd_tile_kernel)
+            // are separate kernels and separate statistics
+            if (nsteps > 0)
+                kst->run(one ? "sytrd_step" : "sytrd_batch", 2.0 * bytes / 8.0, bytes, s, loop, nsteps);
             kst->run("sytrd_tail", 0.0, 0.0, s, tail);
         } else {
             loop();
@@ -2746,6 +2749,16 @@ amls_status amls_hip_profile(amls_ctx* ctx, int32_t enable) {
     return guarded(ctx, [&](amls::Ctx& c) {
         c.kstats.reset();
         c.kstats.enabled = enable != 0;
+        if (c.kstats.enabled && c.gpu) {
+            // "launch_overhead": 512 back-to-back empty kernels under one
+            // event pair, the dispatch gap that a class timed as one bracket
+            // around many launches (the tridiagonalisation steps) includes
+            constexpr int kN = 512;
+            c.kstats.run("launch_overhead", 0.0, 0.0, c.stream, [&] {
+                for (int i = 0; i < kN; ++i) amls::launch_noop(c.stream);
+            }, kN);
+            c.sync();
+        }
     });
 }
 

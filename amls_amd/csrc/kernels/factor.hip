@@ -239,6 +239,10 @@ __global__ void vecop_kernel(const VecDesc* __restrict__ descs) {
     }
 }
 
+// Empty kernel: back-to-back launches of it measure the dispatch gap
+// between kernels on a stream (profiling only).
+__global__ void __launch_bounds__(64) noop_kernel() {}
+
 }  // namespace dev
 
 void launch_vecop(const dev::VecDesc* d, int count, hipStream_t s) {
@@ -266,6 +270,8 @@ void launch_scatter(const int64_t* src, const int64_t* tgt, const double* vals, 
     hipLaunchKernelGGL(dev::scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src, tgt, vals,
                        dst, count, scale, accumulate ? 1 : 0);
 }
+
+void launch_noop(hipStream_t s) { hipLaunchKernelGGL(dev::noop_kernel, dim3(1), dim3(64), 0, s); }
 
 void launch_fill(double* p, int64_t count, double v, hipStream_t s) {
     if (count <= 0) return;

@@ -250,7 +250,10 @@ amls_status amls_hip_adjoint(amls_ctx* ctx, int64_t nelem, int32_t npe, const in
  * flops / bytes of each class (from the launch descriptors: 2mnk and
  * (mk + kn + 2mn) words per GEMM, n(n-1) r for a unit triangular solve, two
  * passes over the trailing block per tridiagonalisation step, 2 nnz r for a
- * CSR product) are always counted.  No counterpart in the reference. */
+ * CSR product) are always counted.  Enabling also times 512 back-to-back
+ * empty kernels as the class "launch_overhead" (the dispatch gap included by
+ * classes timed as one bracket around many launches: sytrd_step and
+ * sytrd_batch).  No counterpart in the reference. */
 amls_status amls_hip_profile(amls_ctx* ctx, int32_t enable);
 /* Class `index` (0-based, sorted by name); AMLS_E_ARG past the last one. */
 amls_status amls_hip_kernel_stats(amls_ctx* ctx, int32_t index, char* name, int32_t name_len,
