@@ -38,17 +38,25 @@ __global__ void __launch_bounds__(256) ldlt_diag_kernel(const FrontDesc* __restr
     // a_ij a_kj / d_j from every (i, k) with i >= k > j; thread t owns column
     // t & 63 and rows (t >> 6) + 4 r, so one barrier per step suffices (column
     // j is only read in step j, and d_{j+1} is final after it)
+    // (all of a step's LDS reads are issued before its writes: otherwise the
+    // compiler orders each read behind the previous element's write)
     const int kc = t & 63, i0 = t >> 6;
     for (int j = 0; j < jb; ++j) {
         const double dj = sD[j * (NB + 1) + j];
         if (t == 0 && (!(dj > 0.0) || !(dj < INFINITY))) atomicMin(err, d.node);
-        const double rdj = 1.0 / dj;
         if (kc > j && kc < jb) {
-            const double akj = sD[kc * (NB + 1) + j] * rdj;
+            const double akj = sD[kc * (NB + 1) + j] / dj;
+            double aj[16], ak[16];
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int i = i0 + 4 * r;
-                if (i >= kc && i < jb) sD[i * (NB + 1) + kc] -= sD[i * (NB + 1) + j] * akj;
+                aj[r] = sD[i * (NB + 1) + j];
+                ak[r] = sD[i * (NB + 1) + kc];
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = i0 + 4 * r;
+                if (i >= kc && i < jb) sD[i * (NB + 1) + kc] = ak[r] - aj[r] * akj;
             }
         }
         __syncthreads();
