@@ -98,6 +98,9 @@ struct TrsmBatch {
     void add(const double* L, int64_t ldl, double* B, int64_t ldb, int64_t n, int64_t nrhs,
              int trans) {
         if (n <= 0 || nrhs <= 0) return;
+        // the diagonal-block kernel addresses 64 columns of B with 32-bit
+        // byte offsets
+        if (ldb > 4000000) throw StatusError(AMLS_E_ARG, "leading dimension above 4e6 elements");
         TrsmDesc d;
         std::memset(&d, 0, sizeof(d));
         d.L = L;
@@ -212,7 +215,7 @@ struct Plan {
         }
         const int nd = static_cast<int>(b.descs.size());
         for (int s = 0; s < maxnb; ++s) {
-            std::vector<TrsmWork> work;
+            std::vector<TrsmWork> work[2];   // by direction: forward, transposed
             GemmBatch gb;
             for (int id = 0; id < nd; ++id) {
                 const TrsmDesc& d = b.descs[id];
@@ -224,7 +227,7 @@ struct Plan {
                 // diagonal: only column blocks <= rb are nonzero in row block rb
                 const int ncb = d.rhsLower && !d.trans ? std::min(cdiv(d.nrhs, 64), rb + 1)
                                                        : cdiv(d.nrhs, 64);
-                for (int c = 0; c < ncb; ++c) work.push_back({id, c, rb, 0});
+                for (int c = 0; c < cdiv(ncb, 4); ++c) work[d.trans ? 1 : 0].push_back({id, c, rb, ncb});
                 const int ncols = std::min(d.nrhs, ncb * 64);
                 GemmDesc g = gdesc();
                 if (!d.trans && r0 + mv < d.n) {
@@ -255,11 +258,12 @@ struct Plan {
                 g.beta = 1.0;
                 gb.add(g);
             }
-            const size_t ow = blob->add(work);
-            const int nw = static_cast<int>(work.size());
+            const size_t ow0 = blob->add(work[0]), ow1 = blob->add(work[1]);
+            const int nw0 = static_cast<int>(work[0].size()), nw1 = static_cast<int>(work[1].size());
             const bool first = s == 0;
             timed("trsm", first ? fl : 0.0, first ? by : 0.0, [=](hipStream_t st) {
-                launch_trsm_diag(bl->at<TrsmDesc>(od), bl->at<TrsmWork>(ow), nw, st);
+                launch_trsm_diag(bl->at<TrsmDesc>(od), bl->at<TrsmWork>(ow0), nw0, false, st);
+                launch_trsm_diag(bl->at<TrsmDesc>(od), bl->at<TrsmWork>(ow1), nw1, true, st);
             });
             gemm(gb, false);
         }

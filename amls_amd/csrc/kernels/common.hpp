@@ -52,8 +52,9 @@ struct TrsmDesc {
 };
 
 struct TrsmWork {
-    int32_t desc, cb, pad0, pad1;   // cb = 64-column block of the right-hand side,
-                                    // pad0 = 64-row block solved by trsm_diag_kernel
+    int32_t desc, cb, pad0, pad1;   // cb = group of four 64-column blocks of the
+                                    // right-hand side, pad0 = 64-row block solved by
+                                    // trsm_diag_kernel, pad1 = column blocks in play
 };
 
 // Partial LDL^T of a front: columns [0, n) of the f x f lower-stored front F

@@ -201,51 +201,110 @@ __global__ void __launch_bounds__(256) gemm_batched_kernel(const GemmDesc* __res
 
 // ---------------------------------------------------------------------------
 // Triangular solve, diagonal-block part of the blocked right-looking
-// algorithm: one workgroup per (descriptor, 64-column RHS block, 64-row
-// block rb) solves L_rb,rb X = B (or L_rb,rb^T X = B) in LDS, in place.  The
-// host plans the row blocks in order and applies the off-diagonal updates
-// between them as GEMMs, so every step is one wide launch instead of a
-// sequential walk inside one workgroup per column block.
+// algorithm: one workgroup per (descriptor, four 64-column RHS blocks, 64-row
+// block rb) solves L_rb,rb X = B (or L_rb,rb^T X = B) in place.  Each wave
+// takes one column block, each lane one column with its 64 rows in
+// registers; the diagonal block of L sits in LDS and every lane reads the
+// same element at the same time (a broadcast), so after the one barrier that
+// publishes L the solve needs no synchronisation.  The host plans the row
+// blocks in order and applies the off-diagonal updates between them as
+// GEMMs, so every step is one wide launch.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) trsm_diag_kernel(const TrsmDesc* __restrict__ descs,
-                                                        const TrsmWork* __restrict__ work) {
-    __shared__ double sT[TM * (TM + 1)];
-    __shared__ double sL[TM * (TM + 1)];
+template <bool TR>
+__global__ void __launch_bounds__(256, TR ? 2 : 1) trsm_diag_kernel(const TrsmDesc* __restrict__ descs,
+                                                           const TrsmWork* __restrict__ work) {
+    // the strictly lower part of the block, packed in the order the solve
+    // consumes it: forward, pair (j, i > j) uses L(i, j) in step j; backward,
+    // pair (j, i < j) uses L(j, i) in step j, j descending
+    constexpr int NP = TM * (TM - 1) / 2, D = 16;
+    __shared__ __align__(16) double sL[NP];
     const TrsmWork wk = work[blockIdx.x];
     const TrsmDesc d = descs[wk.desc];
-    const int n = d.n;
-    const int c0 = wk.cb * TM;
-    const int nv = min(TM, d.nrhs - c0);
     const int r0 = wk.pad0 * TM;
-    const int mv = min(TM, n - r0);
+    const int mv = min(TM, d.n - r0);
     const int t = threadIdx.x;
-    double* X = d.B + (int64_t)c0 * d.ldb;
     for (int e = t; e < TM * TM; e += 256) {
         const int r = e & 63, c = e >> 6;
-        sT[r * (TM + 1) + c] = (r < mv && c < nv) ? X[r0 + r + (int64_t)c * d.ldb] : 0.0;
-        sL[r * (TM + 1) + c] = (r < mv && c < mv) ? d.L[r0 + r + (int64_t)(r0 + c) * d.ldl] : 0.0;
+        if (r <= c) continue;
+        const double v = (r < mv) ? d.L[r0 + r + (int64_t)(r0 + c) * d.ldl] : 0.0;
+        const int p = TR ? NP - r * (r + 1) / 2 + c : c * (TM - 1) - c * (c - 1) / 2 + (r - c - 1);
+        sL[p] = v;
     }
+    // the wave's 64 columns through one buffer resource whose range ends with
+    // the matrix: a lane's column at byte offset lane * ldb * 8 (idle lanes
+    // at an offset that reads as zero), row r at the instruction's immediate
+    // offset r * 8.  Rows past mv read whatever lies there (or zero past the
+    // matrix); they are never stored, and the solves below never let them
+    // reach a live row.
+    const int lane = t & 63, blk = __builtin_amdgcn_readfirstlane(wk.cb * 4 + (t >> 6));
+    const int col = blk * TM + lane;
+    const bool act = blk < wk.pad1 && col < d.nrhs;
+    const int64_t first = (int64_t)blk * TM * d.ldb + r0;
+    const int64_t span = ((int64_t)(d.nrhs - 1) * d.ldb + d.n) - first;
+    // (range capped below the idle offset, which stays below 2^31 with the
+    // row added; the host keeps 64 ldb * 8 bytes below the cap)
+    constexpr int kCap = 0x7ffff000, kIdle = 0x7ffff800;
+    const int nrec = (int)min((int64_t)kCap, max((int64_t)0, span) * 8);
+    const __amdgpu_buffer_rsrc_t rX =
+        __builtin_amdgcn_make_buffer_rsrc(d.B + first, (short)0, nrec, 0x00020000);
+    const int voff = act ? lane * (int)d.ldb * 8 : kIdle;
+    double x[TM];
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+        x[r] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rX, voff + r * 8, 0, 0));
     __syncthreads();
-    const int c = t & 63;
-    if (!d.trans) {
-        for (int j = 0; j < mv - 1; ++j) {
-            const double xj = sT[j * (TM + 1) + c];
-            for (int i = j + 1 + (t >> 6); i < mv; i += 4)
-                sT[i * (TM + 1) + c] -= sL[i * (TM + 1) + j] * xj;
-            __syncthreads();
+    // Forward, a live row i < mv only ever reads x_j with j < i; backward,
+    // steps j >= mv use x_j = 0.  So the full 64-step recurrences leave the
+    // live rows exact (the L entries past mv are zeros).  The L values stream
+    // through a ring of D registers, each read from LDS D pairs before its
+    // use; the scheduling barriers every 8 pairs stop the compiler from
+    // hoisting all reads ahead, which would need more registers than a wave
+    // has.  (The forward solve still needs about 300 registers, so it runs
+    // one wave per SIMD; the backward one fits two.)
+    double buf[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) buf[q] = sL[q];
+    if (!TR) {
+#pragma unroll
+        for (int j = 0; j < TM - 1; ++j) {
+            const double xj = x[j];
+#pragma unroll
+            for (int i = j + 1; i < TM; ++i) {
+                const int p = j * (TM - 1) - j * (j - 1) / 2 + (i - j - 1);
+                const double l = buf[p % D];
+                if (p + D < NP) buf[p % D] = sL[p + D];
+                x[i] -= l * xj;
+                if ((p & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+            }
         }
     } else {
-        // L^T upper unit: x_j -= sum_{i > j} L(i, j) x_i, backward
-        for (int j = mv - 1; j > 0; --j) {
-            const double xj = sT[j * (TM + 1) + c];
-            for (int i = (t >> 6); i < j; i += 4)
-                sT[i * (TM + 1) + c] -= sL[j * (TM + 1) + i] * xj;
-            __syncthreads();
+        // L^T upper unit: once x_j is final, x_i -= L(j, i) x_j for i < j
+#pragma unroll
+        for (int j = TM - 1; j > 0; --j) {
+            const double xj = j < mv ? x[j] : 0.0;
+#pragma unroll
+            for (int i = 0; i < j; ++i) {
+                const int p = NP - j * (j + 1) / 2 + i;
+                const double l = buf[p % D];
+                if (p + D < NP) buf[p % D] = sL[p + D];
+                x[i] -= l * xj;
+                if ((p & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+            }
         }
     }
-    for (int e = t; e < TM * TM; e += 256) {
-        const int r = e & 63, cc = e >> 6;
-        if (r < mv && cc < nv) X[r0 + r + (int64_t)cc * d.ldb] = sT[r * (TM + 1) + cc];
+    // an opaque copy of the offset, so the 64 store addresses are formed
+    // here (as immediates) rather than hoisted and held across the solve
+    int vs = voff;
+    asm volatile("" : "+v"(vs));
+    if (mv == TM) {
+#pragma unroll
+        for (int r = 0; r < TM; ++r)
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(gemm_u32x2, x[r]), rX, vs + r * 8, 0, 0);
+    } else {
+#pragma unroll
+        for (int r = 0; r < TM; ++r)
+            if (r < mv)
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(gemm_u32x2, x[r]), rX, vs + r * 8, 0, 0);
     }
 }
 
@@ -260,9 +319,13 @@ void launch_gemm(const dev::GemmDesc* d, const dev::TileRef* tiles, int ntiles, 
         hipLaunchKernelGGL(dev::gemm_batched_kernel<double>, dim3(ntiles), dim3(256), 0, s, d, tiles);
 }
 
-void launch_trsm_diag(const dev::TrsmDesc* d, const dev::TrsmWork* w, int nwork, hipStream_t s) {
+void launch_trsm_diag(const dev::TrsmDesc* d, const dev::TrsmWork* w, int nwork, bool trans,
+                      hipStream_t s) {
     if (nwork <= 0) return;
-    hipLaunchKernelGGL(dev::trsm_diag_kernel, dim3(nwork), dim3(256), 0, s, d, w);
+    if (trans)
+        hipLaunchKernelGGL(dev::trsm_diag_kernel<true>, dim3(nwork), dim3(256), 0, s, d, w);
+    else
+        hipLaunchKernelGGL(dev::trsm_diag_kernel<false>, dim3(nwork), dim3(256), 0, s, d, w);
 }
 
 }  // namespace amls

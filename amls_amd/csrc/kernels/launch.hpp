@@ -12,7 +12,10 @@ void launch_gemm(const dev::GemmDesc* d, const dev::TileRef* tiles, int ntiles, 
 // Diagonal 64 x 64 block solves of a blocked unit-triangular solve: work item
 // (descriptor, RHS column block, -, -) with pad0 = the row block; the caller
 // applies the off-diagonal updates between row blocks.
-void launch_trsm_diag(const dev::TrsmDesc* d, const dev::TrsmWork* w, int nwork, hipStream_t s);
+// one launch per direction: the work items of a launch all solve with
+// trans == `trans`
+void launch_trsm_diag(const dev::TrsmDesc* d, const dev::TrsmWork* w, int nwork, bool trans,
+                      hipStream_t s);
 // Panel s0 of the fronts in `list` (indices into d): diagonal block, then the
 // rows below it in chunks of 256 (rows = (front index, chunk) work items).
 void launch_ldlt_panel(const dev::FrontDesc* d, const int32_t* list, int count, int s0, int* err,
