@@ -16,56 +16,52 @@ namespace dev {
 
 constexpr int NB = 64;
 
-// One workgroup per front with n > s0: unblocked LDL^T of the diagonal block
-// of panel s0 in LDS; L11 (strict lower) and D (diagonal) are written back.
-__global__ void __launch_bounds__(256) ldlt_diag_kernel(const FrontDesc* __restrict__ descs,
-                                                        const int32_t* __restrict__ list,
-                                                        int s0, int* __restrict__ err) {
-    __shared__ double sD[NB * (NB + 1)];
+// lane `src`'s value of v, as a wave-uniform value
+__device__ __forceinline__ double lane_value(double v, int src) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// One wave per front with n > s0: unblocked LDL^T of the diagonal block of
+// panel s0, lane i holding row i (its lower part) in registers; L11 (strict
+// lower) and D (diagonal) are written back.
+//
+// Right-looking: step j computes l_i = a_ij / d_j on every lane i > j and
+// keeps it in register j, then subtracts a_ij l_k from a_ik for every k > j,
+// l_k read from lane k as a uniform value.  Registers k > i of lane i (the
+// upper part) collect harmless junk and are never read or stored.  No LDS
+// and no barrier: the whole step chain is straight-line code.
+__global__ void __launch_bounds__(64) ldlt_diag_kernel(const FrontDesc* __restrict__ descs,
+                                                       const int32_t* __restrict__ list,
+                                                       int s0, int* __restrict__ err) {
     const FrontDesc d = descs[list[blockIdx.x]];
     const int64_t f = d.f;
     const int jb = min(NB, d.n - s0);
-    double* F = d.F;
-    const int t = threadIdx.x;
-    for (int e = t; e < NB * NB; e += 256) {
-        const int i = e & 63, j = e >> 6;
-        double v = 0.0;
-        if (i < jb && j < jb && i >= j) v = F[(s0 + i) + (int64_t)(s0 + j) * f];
-        sD[i * (NB + 1) + j] = v;
-    }
-    __syncthreads();
-    // right-looking with the columns left unscaled: step j subtracts
-    // a_ij a_kj / d_j from every (i, k) with i >= k > j; thread t owns column
-    // t & 63 and rows (t >> 6) + 4 r, so one barrier per step suffices (column
-    // j is only read in step j, and d_{j+1} is final after it)
-    // (all of a step's LDS reads are issued before its writes: otherwise the
-    // compiler orders each read behind the previous element's write)
-    const int kc = t & 63, i0 = t >> 6;
-    for (int j = 0; j < jb; ++j) {
-        const double dj = sD[j * (NB + 1) + j];
-        if (t == 0 && (!(dj > 0.0) || !(dj < INFINITY))) atomicMin(err, d.node);
-        if (kc > j && kc < jb) {
-            const double akj = sD[kc * (NB + 1) + j] / dj;
-            double aj[16], ak[16];
+    double* F = d.F + s0 + (int64_t)s0 * f;
+    const int i = threadIdx.x;
+    double a[NB];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int i = i0 + 4 * r;
-                aj[r] = sD[i * (NB + 1) + j];
-                ak[r] = sD[i * (NB + 1) + kc];
-            }
+    for (int k = 0; k < NB; ++k) a[k] = (k <= i && i < jb) ? F[i + (int64_t)k * f] : 0.0;
+    bool bad = false;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int i = i0 + 4 * r;
-                if (i >= kc && i < jb) sD[i * (NB + 1) + kc] = ak[r] - aj[r] * akj;
-            }
+    for (int j = 0; j < NB; ++j) {
+        if (j < jb) {
+            const double dj = lane_value(a[j], j);
+            bad = bad || !(dj > 0.0) || !(dj < INFINITY);
+            const double aij = a[j];
+            const double l = aij / dj;
+            if (i > j) a[j] = l;
+#pragma unroll
+            for (int k = j + 1; k < NB; ++k) a[k] -= aij * lane_value(a[j], k);
         }
-        __syncthreads();
     }
-    // L_ij = a_ij / d_j
-    for (int e = t; e < jb * jb; e += 256) {
-        const int i = e % jb, j = e / jb;
-        if (i > j) F[(s0 + i) + (int64_t)(s0 + j) * f] = sD[i * (NB + 1) + j] / sD[j * (NB + 1) + j];
-        else if (i == j) F[(s0 + i) + (int64_t)(s0 + j) * f] = sD[i * (NB + 1) + j];
+    if (bad && i == 0) atomicMin(err, d.node);
+    if (i < jb) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+            if (k <= i) F[i + (int64_t)k * f] = a[k];
     }
 }
 
@@ -261,7 +257,7 @@ void launch_vecop(const dev::VecDesc* d, int count, hipStream_t s) {
 void launch_ldlt_panel(const dev::FrontDesc* d, const int32_t* list, int count, int s0, int* err,
                        const int2* rows, int nrows, hipStream_t s) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(dev::ldlt_diag_kernel, dim3(count), dim3(256), 0, s, d, list, s0, err);
+    hipLaunchKernelGGL(dev::ldlt_diag_kernel, dim3(count), dim3(64), 0, s, d, list, s0, err);
     if (nrows > 0)
         hipLaunchKernelGGL(dev::ldlt_rows_kernel, dim3(nrows), dim3(256), 0, s, d, rows, s0);
 }
