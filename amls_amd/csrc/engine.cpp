@@ -2405,6 +2405,8 @@ amls_status amls_hip_set_problem(amls_ctx* ctx, int64_t n, const int64_t* Kp, co
     });
 }
 
+}  // extern "C"
+
 namespace {
 std::vector<amls::Ctx::ProblemIn> batchInputs(int64_t count, const int64_t* n,
                                               const int64_t* const* Kp, const int64_t* const* Ki,
@@ -2419,6 +2421,8 @@ std::vector<amls::Ctx::ProblemIn> batchInputs(int64_t count, const int64_t* n,
     return v;
 }
 }  // namespace
+
+extern "C" {
 
 amls_status amls_hip_set_problem_batch(amls_ctx* ctx, int64_t count, const int64_t* n,
                                        const int64_t* const* Kp, const int64_t* const* Ki,

@@ -548,43 +548,113 @@ __global__ void __launch_bounds__(64) eig_count_kernel(const EigDesc* __restrict
     }
 }
 
-// Two Sturm counts at once (x1, x2), interleaved so the two division chains
-// overlap.  e^2 / q uses the hardware reciprocal (relative error 4.6e-8
-// measured on gfx950) refined by one Newton step (2.1e-15, about ten ulp):
-// the counts are those of a relative perturbation of e^2 of that size, so
-// the eigenvalues keep an absolute accuracy of about 1e-15 |T|; |q| >=
-// pivmin keeps the reciprocal finite.
-__device__ __forceinline__ double sturm_rcp(double b) {
-    const double r = __builtin_amdgcn_rcp(b);
-    return fma(r, fma(-b, r, 1.0), r);
-}
-
-__device__ void sturm_count2(const double* __restrict__ dd, const double* __restrict__ ee, int n,
-                             double x1, double x2, double pivmin, int& c1, int& c2) {
-    double q1 = dd[0] - x1, q2 = dd[0] - x2;
-    if (fabs(q1) < pivmin) q1 = -pivmin;
-    if (fabs(q2) < pivmin) q2 = -pivmin;
-    int n1 = q1 < 0.0, n2 = q2 < 0.0;
-    for (int j = 1; j < n; ++j) {
-        const double e = ee[j - 1], dj = dd[j];
-        const double e2 = e * e;
-        q1 = fma(-e2, sturm_rcp(q1), dj - x1);
-        q2 = fma(-e2, sturm_rcp(q2), dj - x2);
-        if (fabs(q1) < pivmin) q1 = -pivmin;
-        if (fabs(q2) < pivmin) q2 = -pivmin;
-        n1 += q1 < 0.0;
-        n2 += q2 < 0.0;
+// NC Sturm counts at once (shifts x[0..NC-1]) for the multisection, by the
+// three-term recurrence of the leading principal minors of T - x I:
+//   p_j = (d_j - x) p_{j-1} - e_{j-1}^2 p_{j-2},  p_{-1} = 1, p_{-2} = 0;
+// the number of eigenvalues below x is the number of sign changes along
+// 1, p_0, ..., p_{n-1} (counted on the sign bits, so a zero, +0 or -0,
+// still yields one change across j - 1 .. j + 1 since p_{j+1} = -e^2 p_{j-1}).
+// Only one FMA per row is on the dependent chain (e^2 p_{j-2} is formed a
+// step earlier), against reciprocal, Newton step and update in the ratio
+// form q_j = p_j / p_{j-1}.  The rows are scaled by 2^-k, 2^k > |T| (exact),
+// so |d - x| < 2 and e^2 < 1, and after every chunk of eight rows the pair
+// (p_{j-1}, p_j) is rescaled by a power of two (exact): within a chunk
+// |p| grows at most (1 + sqrt 2)^8 ~ 1.2e3.  Backward stability is that of
+// the ratio form: each row's operations perturb d_j - x and e^2 by a few ulp.
+// The rows are read a chunk ahead (two chunk buffers used in turn), so the
+// loads wait behind eight steps.
+// Pre: dd and ee already hold d 2^-k and (e 2^-k)^2 (the staged LDS copy).
+template <bool Pre, int NC>
+__device__ __forceinline__ void sturm_count_n(const double* __restrict__ dd,
+                                              const double* __restrict__ ee, int n,
+                                              const double* x, int k, int* cnt) {
+    double y[NC], a[NC], b[NC];
+    unsigned s[NC];
+    int c[NC];
+    auto sgn = [](double v) { return (unsigned)(__builtin_bit_cast(unsigned long long, v) >> 63); };
+    const double d0 = Pre ? dd[0] : __builtin_amdgcn_ldexp(dd[0], -k);
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        y[q] = __builtin_amdgcn_ldexp(x[q], -k);
+        a[q] = 1.0;
+        b[q] = d0 - y[q];
+        s[q] = sgn(b[q]);
+        c[q] = (int)s[q];
     }
-    c1 = n1;
-    c2 = n2;
+    constexpr int CH = 8;
+    auto fetch = [&](double* dv, double* ev, int j0) {
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            const int r = min(j0 + u, n - 1);
+            if (Pre) {
+                dv[u] = dd[r];
+                ev[u] = ee[r - 1];
+            } else {
+                dv[u] = __builtin_amdgcn_ldexp(dd[r], -k);
+                const double e = __builtin_amdgcn_ldexp(ee[r - 1], -k);
+                ev[u] = e * e;
+            }
+        }
+    };
+    auto step = [&](double dj, double e2) {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const double p = fma(dj - y[q], b[q], -e2 * a[q]);
+            a[q] = b[q];
+            b[q] = p;
+            const unsigned t = sgn(p);
+            c[q] += (int)(t ^ s[q]);
+            s[q] = t;
+        }
+    };
+    auto chunk = [&](const double* dv, const double* ev) {
+#pragma unroll
+        for (int u = 0; u < CH; ++u) step(dv[u], ev[u]);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const int m = max(__builtin_amdgcn_frexp_exp(a[q]), __builtin_amdgcn_frexp_exp(b[q]));
+            a[q] = __builtin_amdgcn_ldexp(a[q], -m);
+            b[q] = __builtin_amdgcn_ldexp(b[q], -m);
+        }
+    };
+    if (n > 1) {
+        double d1[CH], e1[CH], d2[CH], e2[CH];
+        fetch(d1, e1, 1);
+        int j0 = 1;
+        // two chunks per pass, buffers used in turn: rows j0.. in d1,
+        // j0 + CH.. fetched into d2 while d1's rows are stepped
+        for (; j0 + 2 * CH <= n; j0 += 2 * CH) {
+            fetch(d2, e2, j0 + CH);
+            chunk(d1, e1);
+            fetch(d1, e1, j0 + 2 * CH);
+            chunk(d2, e2);
+        }
+        if (j0 + CH <= n) {
+            fetch(d2, e2, j0 + CH);
+            chunk(d1, e1);
+            j0 += CH;
+#pragma unroll
+            for (int u = 0; u < CH; ++u)
+                if (j0 + u < n) step(d2[u], e2[u]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < CH; ++u)
+                if (j0 + u < n) step(d1[u], e1[u]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q) cnt[q] = c[q];
 }
 
 // lam[j] = (j+1)-th largest eigenvalue, j < k; one wave per work item
 // (x = matrix, y = j; the host pads each matrix's items to whole workgroups
 // of four with y = -1, so a workgroup stages one tridiagonal in LDS).
-// Multisection: each round the 64 lanes evaluate the Sturm count at 128
-// interior points of the current bracket (two per lane), which shrinks the
-// bracket 129-fold per round (instead of 2-fold for plain bisection).
+// Multisection: each round the 64 lanes evaluate the Sturm count at 64
+// interior points of the current bracket, which shrinks the bracket 65-fold
+// per round (instead of 2-fold for plain bisection).  One shift per lane is
+// the fastest measured: the recurrence is bound by instruction issue, so
+// two or four shifts per lane (129 or 257 sections) cost more per round
+// than the rounds they save (0.33 ms against 0.37 and 0.53 ms per launch).
 __global__ void __launch_bounds__(256) bisect_kernel(const EigDesc* __restrict__ descs,
                                                      const int2* __restrict__ work, int nwork,
                                                      int useLds) {
@@ -594,26 +664,47 @@ __global__ void __launch_bounds__(256) bisect_kernel(const EigDesc* __restrict__
     const EigDesc d = descs[work[blockIdx.x * 4].x];
     const double *dd, *ee;
     stage_tridiag(d, sde, useLds, dd, ee);
+    const int n = d.n;
+    double lo, hi, pivmin, tn;
+    gersh_wave(dd, ee, n, lane, lo, hi, pivmin, tn);
+    const int k = __builtin_amdgcn_frexp_exp(tn);   // 2^(k-1) <= |T| < 2^k
+    if (useLds) {
+        // the staged copy becomes what the Sturm counts read: d 2^-k and
+        // (e 2^-k)^2 (every wave of the block has its bounds by now)
+        __syncthreads();
+        for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            sde[r] = __builtin_amdgcn_ldexp(sde[r], -k);
+            const double e = __builtin_amdgcn_ldexp(sde[n + r], -k);
+            sde[n + r] = e * e;
+        }
+        __syncthreads();
+    }
     if (wid >= nwork) return;
     const int2 wk = work[wid];
     if (wk.y < 0) return;
-    const int n = d.n;
     const int j = wk.y;
-    double lo, hi, pivmin, tn;
-    gersh_wave(dd, ee, n, lane, lo, hi, pivmin, tn);
     const int a = n - 1 - j;    // ascending index
-    constexpr double SEC = 129.0;
-    for (int round = 0; round < 16; ++round) {
+    constexpr int NC = 1, NP = 64 * NC;   // shifts per lane, points per round
+    constexpr double SEC = NP + 1.0;
+    for (int round = 0; round < 24; ++round) {
         const double tol = 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
         if (hi - lo <= tol) break;
         const double w = hi - lo;
-        int c1, c2;
-        sturm_count2(dd, ee, n, lo + w * (double)(lane + 1) / SEC, lo + w * (double)(lane + 65) / SEC,
-                     pivmin, c1, c2);
-        // counts grow with x: points with c <= a form a prefix of the 128
-        const int np = __popcll(__ballot(c1 <= a)) + __popcll(__ballot(c2 <= a));
+        double x[NC];
+        int c[NC];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) x[q] = lo + w * (double)(lane + 1 + 64 * q) / SEC;
+        // separate calls, so the staged copy is read with LDS instructions
+        if (useLds)
+            sturm_count_n<true, NC>(sde, sde + n, n, x, k, c);
+        else
+            sturm_count_n<false, NC>(d.d, d.e, n, x, k, c);
+        // counts grow with x: points with c <= a form a prefix of the NP
+        int np = 0;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) np += __popcll(__ballot(c[q] <= a));
         const double nlo = np > 0 ? lo + w * (double)np / SEC : lo;
-        const double nhi = np < 128 ? lo + w * (double)(np + 1) / SEC : hi;
+        const double nhi = np < NP ? lo + w * (double)(np + 1) / SEC : hi;
         if (!(nhi < hi || nlo > lo)) break;   // no progress at this resolution
         lo = nlo;
         hi = nhi;
