@@ -81,3 +81,37 @@ def test_syevx_degenerate(gpu_ctx):
     assert np.abs(lam - np.repeat([40.0, 39, 38, 37], 3)).max() <= 1e-11 * 40
     assert np.abs(V.T @ V - np.eye(12)).max() <= 1e-9
     assert np.linalg.norm(A @ V - V * lam, axis=0).max() <= 1e-10 * 40
+
+
+@pytest.mark.parametrize("scale", [1e-100, 1.0, 1e100])
+def test_syevx_graded_spectrum(gpu_ctx, scale):
+    # eigenvalues over 14 decades, the whole matrix scaled far from 1: the
+    # Sturm counts run on a power-of-two rescaled copy, so the absolute
+    # accuracy stays a few ulp of |A| at any scale
+    n = 200
+    rng = np.random.default_rng(17)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    w = np.logspace(0, -14, n) * np.where(np.arange(n) % 2 == 0, 1.0, -1.0) * scale
+    A = (Q * w) @ Q.T
+    A = (A + A.T) / 2
+    lam, V = gpu_ctx.syevx_top(A, n)
+    ref = np.sort(np.linalg.eigvalsh(A))[::-1]
+    assert np.abs(lam - ref).max() <= 1e-13 * n * scale
+    assert np.abs(V.T @ V - np.eye(n)).max() <= 1e-10
+    assert np.linalg.norm(A @ V - V * lam, axis=0).max() <= 1e-11 * n * scale
+
+
+def test_syevx_decoupled_blocks(gpu_ctx):
+    # two blocks with no coupling (zero off-diagonal in the tridiagonal form)
+    # and norms 1e6 apart
+    rng = np.random.default_rng(23)
+    G1, G2 = rng.standard_normal((60, 60)), rng.standard_normal((40, 40))
+    A = np.zeros((100, 100))
+    A[:60, :60] = (G1 + G1.T) * 1e6
+    A[60:, 60:] = G2 + G2.T
+    lam, V = gpu_ctx.syevx_top(A, 100)
+    ref = np.sort(np.linalg.eigvalsh(A))[::-1]
+    nrm = np.abs(ref).max()
+    assert np.abs(lam - ref).max() <= 1e-13 * 100 * nrm
+    assert np.abs(V.T @ V - np.eye(100)).max() <= 1e-10
+    assert np.linalg.norm(A @ V - V * lam, axis=0).max() <= 1e-11 * 100 * nrm
