@@ -14,6 +14,7 @@
 #include "launch.hpp"
 
 #include <cfloat>
+#include <type_traits>
 
 namespace amls {
 namespace dev {
@@ -329,122 +330,206 @@ __global__ void __launch_bounds__(256) sytrd_tile1_kernel(EigDesc d, int i, int 
 // ---------------------------------------------------------------------------
 // The last SYT_TAIL steps of the reduction in one workgroup per matrix.
 // Steps i0 .. n-1 (n - i0 <= SYT_TAIL) of matrix blockIdx.x: the trailing
-// block A[i0:n, i0:n] (lower triangle, mirrored) is loaded into LDS with
-// the pending update of step i0 - 1 applied, and each step is then the
-// unblocked Householder reduction on the LDS copy, updates applied at once:
-// no launch per step, no atomics.  Outputs follow the tiled steps exactly:
-// d, e, tau, the reflector v_i (1 in row i + 1) in column i of A below the
-// diagonal, and the reflector of step i0 - 1, which step i0 would store.
+// block S = A[i0:n, i0:n] (lower triangle, mirrored) is loaded with the
+// pending update of step i0 - 1 applied and stays in registers: 512
+// threads as a 32 x 16 grid, thread (tr, tc) holding the 6 x 12 block
+// S[6 tr + a][12 tc + b].  Each step is the unblocked Householder reduction
+// with the update applied at once: no launch per step, no atomics, two
+// barriers.
+//   1. the owners of column k write it to LDS;
+//   2. every wave forms the reflector (norm, beta, tau) from it, wave 0
+//      stores the outputs; p = S v: each thread its six row partials,
+//      summed over the 16 column threads of its DPP row, lane tc = 0
+//      writes them;
+//   3. every wave forms p.v itself, each thread w for its rows and columns
+//      and applies S -= v w^T + w v^T to its block.
+// Outputs follow the tiled steps exactly: d, e, tau, the reflector v_i (1
+// in row i + 1) in column i of A below the diagonal, and the reflector of
+// step i0 - 1, which step i0 would store.  Entries of S in rows or columns
+// at or before the current step are never read again; v and w are zero
+// there, so the updates leave them as they are.
 // ---------------------------------------------------------------------------
-constexpr int SYT_TAIL = 128;
-constexpr int SYT_LD = SYT_TAIL + 1;
+constexpr int SYT_TAIL = 192;
+constexpr int SYT_BR = 6, SYT_BC = 12;   // rows, columns of S per thread
 
-__global__ void __launch_bounds__(256) sytrd_tail_kernel(const EigDesc* __restrict__ descs, int i0,
+// sum over the 16 lanes of each DPP row, returned to all of them
+__device__ __forceinline__ double row16_sum(double v) {
+    v += dpp_mov<0xB1>(v);    // quad_perm 1,0,3,2
+    v += dpp_mov<0x4E>(v);    // quad_perm 2,3,0,1
+    v += dpp_mov<0x141>(v);   // row_half_mirror: the other quad of each 8
+    v += dpp_mov<0x140>(v);   // row_mirror: the other 8 of the row
+    return v;
+}
+
+__global__ void __launch_bounds__(512) sytrd_tail_kernel(const EigDesc* __restrict__ descs, int i0,
                                                          double* __restrict__ sv_all,
                                                          double* __restrict__ sp_all,
                                                          const int64_t* __restrict__ soff) {
-    __shared__ double T[SYT_TAIL * SYT_LD];
-    __shared__ double vl[SYT_TAIL], wl[SYT_TAIL];
+    __shared__ double xs[2 * SYT_TAIL], vl[SYT_TAIL], pl[SYT_TAIL], wl[SYT_TAIL];
     __shared__ double red[8];
     const EigDesc d = descs[blockIdx.x];
     const int n = d.n;
     const int m = n - i0;
     if (m <= 0) return;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int tr = t >> 4, tc = t & 15;
+    const int R0 = SYT_BR * tr, C0 = SYT_BC * tc;
     const int64_t lda = d.lda;
     double* A = d.A;
-    for (int c = t >> 6; c < m; c += 4)
-        for (int r = c + (t & 63); r < m; r += 64) {
-            const double a = A[(i0 + r) + (int64_t)(i0 + c) * lda];
-            T[r * SYT_LD + c] = a;
-            T[c * SYT_LD + r] = a;
-        }
+    for (int r = t; r < SYT_TAIL; r += 512) {
+        pl[r] = 0.0;
+        wl[r] = 0.0;
+        vl[r] = 0.0;
+    }
+    // pending update of step i0 - 1: w = tau p + alpha v, alpha = -tau^2 (p.v) / 2
     if (i0 > 0) {
-        // pending update of step i0 - 1: w = tau p + alpha v, alpha = -tau^2 (p.v) / 2
-        const double* sv_m = sv_all + 2 * soff[blockIdx.x];
-        const double* sp_m = sp_all + 3 * soff[blockIdx.x];
-        const double* vp = sv_m + (int64_t)((i0 + 1) & 1) * n + i0;   // v_{i0-1}
-        const double* pp = sp_m + (int64_t)((i0 + 2) % 3) * n + i0;   // p_{i0-1}
+        const double* vp = sv_all + 2 * soff[blockIdx.x] + (int64_t)((i0 + 1) & 1) * n + i0;
+        const double* pp = sp_all + 3 * soff[blockIdx.x] + (int64_t)((i0 + 2) % 3) * n + i0;
         const double tauP = d.tau[i0 - 1];
         double part = 0.0;
         if (t < m) {
             const double v = vp[t], pv = pp[t];
             vl[t] = v;
-            wl[t] = pv;
+            pl[t] = pv;
             part = pv * v;
             A[(i0 + t) + (int64_t)(i0 - 1) * lda] = v;   // reflector of step i0 - 1
         }
-        const double alphaP = -0.5 * tauP * tauP * block_sum(part, red);
-        if (t < m) wl[t] = tauP * wl[t] + alphaP * vl[t];
+        part = wave_sum(part);
+        if (lane == 0) red[wv] = part;
         __syncthreads();
-        for (int c = t >> 1; c < m; c += 128) {
-            const double vc = vl[c], wc = wl[c];
-            for (int r = t & 1; r < m; r += 2) T[r * SYT_LD + c] -= vl[r] * wc + wl[r] * vc;
-        }
+        double pv = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) pv += red[q];
+        const double alphaP = -0.5 * tauP * tauP * pv;
+        if (t < m) wl[t] = tauP * pl[t] + alphaP * vl[t];
     }
     __syncthreads();
-    for (int k = 0; k < m; ++k) {
+    // the thread's block of S, with the pending update
+    double S[SYT_BR][SYT_BC];
+    {
+        double vc_[SYT_BC], wc_[SYT_BC];
+#pragma unroll
+        for (int b = 0; b < SYT_BC; ++b) {
+            vc_[b] = vl[C0 + b];
+            wc_[b] = wl[C0 + b];
+        }
+#pragma unroll
+        for (int a = 0; a < SYT_BR; ++a) {
+            const int r = R0 + a;
+            const double vr = vl[r], wr = wl[r];
+#pragma unroll
+            for (int b = 0; b < SYT_BC; ++b) {
+                const int c = C0 + b;
+                double x = 0.0;
+                if (r < m && c < m) {
+                    const int hi = r > c ? r : c, lo = r > c ? c : r;
+                    x = A[(i0 + hi) + (int64_t)(i0 + lo) * lda] - (vr * wc_[b] + wr * vc_[b]);
+                }
+                S[a][b] = x;
+            }
+        }
+    }
+    // one step; b0 = k % SYT_BC is a compile-time constant of the unrolled
+    // loop below, so the column owners index their registers statically.
+    // Column k goes to LDS (xs, double-buffered by step parity); every wave
+    // then forms the reflector from it on its own, so v needs no second
+    // publication and a step has two barriers.
+    auto step = [&](int k, auto b0c) {
+        constexpr int b0 = decltype(b0c)::value;
         const int i = i0 + k;
-        if (k == m - 1) {
-            if (t == 0) d.d[i] = T[k * SYT_LD + k];
-            break;
+        double* xk = xs + (k & 1) * SYT_TAIL;
+        if (tc == k / SYT_BC) {
+#pragma unroll
+            for (int a = 0; a < SYT_BR; ++a) xk[R0 + a] = S[a][b0];
         }
-        // Householder vector of column k below the diagonal
-        const double alpha = T[(k + 1) * SYT_LD + k];
+        __syncthreads();
         double part = 0.0;
-        if (t >= k + 2 && t < m) {
-            const double x = T[t * SYT_LD + k];
-            part = x * x;
+#pragma unroll
+        for (int q = 0; q < SYT_TAIL / 64; ++q) {
+            const int r = lane + 64 * q;
+            const double x = xk[r];
+            if (r >= k + 2 && r < m) part += x * x;
         }
-        const double xnorm2 = block_sum(part, red);
+        const double xnorm2 = wave_sum(part);
+        const double alpha = k + 1 < m ? xk[k + 1] : 0.0;
         double tau = 0.0, beta = alpha, scal = 0.0;
         if (xnorm2 > 0.0) {
             beta = -copysign(sqrt(alpha * alpha + xnorm2), alpha);
             tau = (beta - alpha) / beta;
             scal = 1.0 / (alpha - beta);
         }
-        if (t == 0) {
-            d.d[i] = T[k * SYT_LD + k];
-            d.e[i] = beta;
-            d.tau[i] = tau;
-        }
-        if (t > k && t < m) {
-            const double v = t == k + 1 ? 1.0 : T[t * SYT_LD + k] * scal;
-            vl[t] = v;
-            A[(i0 + t) + (int64_t)i * lda] = v;
-        }
-        __syncthreads();
-        // p = T v over rows / columns k+1 .. m-1: two threads per row
-        const int r = (t >> 1) + k + 1, h = t & 1;
-        double pr = 0.0;
-        if (r < m) {
-            // four independent partial sums, so the LDS reads pipeline
-            double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
-            const double* Tr = T + r * SYT_LD;
-            int c = k + 1 + h;
-            for (; c + 6 < m; c += 8) {
-                q0 += Tr[c] * vl[c];
-                q1 += Tr[c + 2] * vl[c + 2];
-                q2 += Tr[c + 4] * vl[c + 4];
-                q3 += Tr[c + 6] * vl[c + 6];
+        auto vof = [&](int r) { return r == k + 1 ? 1.0 : (r > k + 1 && r < m ? xk[r] * scal : 0.0); };
+        if (wv == 0) {
+            if (lane == 0) {
+                d.d[i] = xk[k];
+                if (k + 1 < m) {
+                    d.e[i] = beta;
+                    d.tau[i] = tau;
+                }
             }
-            for (; c < m; c += 2) q0 += Tr[c] * vl[c];
-            pr = (q0 + q1) + (q2 + q3);
+#pragma unroll
+            for (int q = 0; q < SYT_TAIL / 64; ++q) {
+                const int r = lane + 64 * q;
+                if (r > k && r < m) A[(i0 + r) + (int64_t)i * lda] = vof(r);
+            }
         }
-        pr += __shfl_xor(pr, 1, 64);
-        const double pv = block_sum((r < m && h == 0) ? pr * vl[r] : 0.0, red);
-        const double alpha2 = -0.5 * tau * tau * pv;
-        if (r < m && h == 0) wl[r] = tau * pr + alpha2 * vl[r];
-        __syncthreads();
-        // rank-2 update: thread t takes column k + 1 + t / 2, every other row
-        const int cc = k + 1 + (t >> 1);
-        if (cc < m) {
-            const double vc = vl[cc], wc = wl[cc];
-#pragma unroll 4
-            for (int rr = k + 1 + (t & 1); rr < m; rr += 2)
-                T[rr * SYT_LD + cc] -= vl[rr] * wc + wl[rr] * vc;
+        if (k + 1 >= m) return;
+        // rows wholly at or before k: p and the update are not needed there
+        // (the wave's 24 rows are skipped together)
+        const bool live = R0 + SYT_BR - 1 > k;
+        // p = S v for this thread's rows: six partials, summed over the 16
+        // column threads of the DPP row
+        double vc_[SYT_BC];
+#pragma unroll
+        for (int b = 0; b < SYT_BC; ++b) vc_[b] = vof(C0 + b);
+        if (live) {
+            double pr[SYT_BR];
+#pragma unroll
+            for (int a = 0; a < SYT_BR; ++a) {
+                double acc = 0.0;
+#pragma unroll
+                for (int b = 0; b < SYT_BC; ++b) acc += S[a][b] * vc_[b];
+                pr[a] = row16_sum(acc);
+            }
+            if (tc == 0) {
+#pragma unroll
+                for (int a = 0; a < SYT_BR; ++a) pl[R0 + a] = pr[a];
+            }
         }
         __syncthreads();
+        // p.v (every wave on its own), w for the rows and columns, update
+        part = 0.0;
+#pragma unroll
+        for (int q = 0; q < SYT_TAIL / 64; ++q) {
+            const int r = lane + 64 * q;
+            if (r > k && r < m) part += pl[r] * vof(r);
+        }
+        const double alpha2 = -0.5 * tau * tau * wave_sum(part);
+        if (!live) return;
+        double wc_[SYT_BC];
+#pragma unroll
+        for (int b = 0; b < SYT_BC; ++b) {
+            const int c = C0 + b;
+            wc_[b] = c > k ? tau * pl[c] + alpha2 * vc_[b] : 0.0;
+        }
+        // row by row, so only one row's v and w are live beside the block
+#pragma unroll
+        for (int a = 0; a < SYT_BR; ++a) {
+            const int r = R0 + a;
+            const double vr = vof(r);
+            const double wr = r > k ? tau * pl[r] + alpha2 * vr : 0.0;
+#pragma unroll
+            for (int b = 0; b < SYT_BC; ++b) S[a][b] -= vr * wc_[b] + wr * vc_[b];
+        }
+    };
+    for (int kb = 0; kb < m; kb += SYT_BC) {
+        step(kb, std::integral_constant<int, 0>());
+#define AMLS_TAIL_STEP(B) \
+        if (kb + B < m) step(kb + B, std::integral_constant<int, B>());
+        AMLS_TAIL_STEP(1) AMLS_TAIL_STEP(2) AMLS_TAIL_STEP(3) AMLS_TAIL_STEP(4)
+        AMLS_TAIL_STEP(5) AMLS_TAIL_STEP(6) AMLS_TAIL_STEP(7) AMLS_TAIL_STEP(8)
+        AMLS_TAIL_STEP(9) AMLS_TAIL_STEP(10) AMLS_TAIL_STEP(11)
+#undef AMLS_TAIL_STEP
     }
 }
 
@@ -1260,7 +1345,7 @@ int sytrd_tail_order() { return dev::SYT_TAIL; }
 void launch_sytrd_tail(const dev::EigDesc* d, int nmat, int i0, double* sv, double* sp,
                        const int64_t* soff, hipStream_t s) {
     if (nmat <= 0) return;
-    hipLaunchKernelGGL(dev::sytrd_tail_kernel, dim3(nmat), dim3(256), 0, s, d, i0, sv, sp, soff);
+    hipLaunchKernelGGL(dev::sytrd_tail_kernel, dim3(nmat), dim3(512), 0, s, d, i0, sv, sp, soff);
 }
 
 void launch_sytrd_step(const dev::EigDesc* d, const int4* work, int nwork, int i, double* sv,
