@@ -285,6 +285,21 @@ struct Plan {
         steps.push_back([=](hipStream_t s) { launch_vecop(bl->at<VecDesc>(od), nv, s); });
     }
     void fn(std::function<void(hipStream_t)> f) { steps.push_back(std::move(f)); }
+    // side's steps run on s2 from this point of the plan on, concurrently
+    // with the steps added here until join(); side shares this plan's blob
+    void fork(Plan& side, hipStream_t s2, hipEvent_t ef, hipEvent_t ej) {
+        auto st = std::make_shared<std::vector<std::function<void(hipStream_t)>>>(std::move(side.steps));
+        side.steps.clear();
+        steps.push_back([=](hipStream_t s) {
+            AMLS_HIP_CHECK(hipEventRecord(ef, s));
+            AMLS_HIP_CHECK(hipStreamWaitEvent(s2, ef, 0));
+            for (auto& f : *st) f(s2);
+            AMLS_HIP_CHECK(hipEventRecord(ej, s2));
+        });
+    }
+    void join(hipEvent_t ej) {
+        steps.push_back([=](hipStream_t s) { AMLS_HIP_CHECK(hipStreamWaitEvent(s, ej, 0)); });
+    }
     void run(hipStream_t s) {
         blob->upload(s);
         for (auto& f : steps) f(s);
@@ -1373,7 +1388,32 @@ void Ctx::spectralPrepare() {
     plan.gemm(g2, false);
     plan.copy(c3);
     plan.copy(c4);
-    addSytrd(plan, ed, soff, dSv, dSp, &syLocal);
+    // The largest node's reduction is the longest chain of latency-bound
+    // steps; it runs alone on the main stream while the other nodes' batch
+    // runs on the second stream.  The split needs the largest node's
+    // scratch (sp is cleared per batch) to lie after every other node's.
+    int big = 0;
+    for (int q = 1; q < static_cast<int>(ed.size()); ++q)
+        if (ed[q].n > ed[big].n) big = q;
+    bool split = ed.size() > 1 && stream2;
+    for (int q = 0; q < static_cast<int>(ed.size()) && split; ++q)
+        if (q != big && soff[q] + ed[q].n > soff[big]) split = false;
+    if (split) {
+        std::vector<EigDesc> er;
+        std::vector<int64_t> sr;
+        for (int q = 0; q < static_cast<int>(ed.size()); ++q)
+            if (q != big) {
+                er.push_back(ed[q]);
+                sr.push_back(soff[q]);
+            }
+        Plan side(plan.blob, plan.st);
+        addSytrd(side, er, sr, dSv, dSp, &syLocal);
+        plan.fork(side, stream2, evFork, evJoin);
+        addSytrd(plan, {ed[big]}, {0}, dSv + 2 * soff[big], dSp + 3 * soff[big]);
+        plan.join(evJoin);
+    } else {
+        addSytrd(plan, ed, soff, dSv, dSp, &syLocal);
+    }
     plan.run(stream);
     sync();
     specReady = true;
@@ -2367,7 +2407,13 @@ amls_status amls_hip_create(int32_t device, uint32_t flags, amls_ctx** out) {
             delete ctx;
             return AMLS_E_HIP;
         }
-        if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->c.stream) != hipSuccess) {
+        if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->c.stream) != hipSuccess ||
+            hipStreamCreate(&ctx->c.stream2) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->c.evFork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->c.evJoin, hipEventDisableTiming) != hipSuccess) {
+            if (ctx->c.evFork) (void)hipEventDestroy(ctx->c.evFork);
+            if (ctx->c.stream2) (void)hipStreamDestroy(ctx->c.stream2);
+            if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
             delete ctx;
             return AMLS_E_HIP;
         }
@@ -2383,11 +2429,18 @@ amls_status amls_hip_destroy(amls_ctx* ctx) {
     if (ctx->c.gpu) {
         (void)hipSetDevice(ctx->c.device);
         (void)hipStreamSynchronize(ctx->c.stream);
+        (void)hipStreamSynchronize(ctx->c.stream2);
     }
-    hipStream_t s = ctx->c.stream;
+    hipStream_t s = ctx->c.stream, s2 = ctx->c.stream2;
+    hipEvent_t e1 = ctx->c.evFork, e2 = ctx->c.evJoin;
     const bool gpu = ctx->c.gpu;
     delete ctx;
-    if (gpu && s) (void)hipStreamDestroy(s);
+    if (gpu) {
+        if (e1) (void)hipEventDestroy(e1);
+        if (e2) (void)hipEventDestroy(e2);
+        if (s2) (void)hipStreamDestroy(s2);
+        if (s) (void)hipStreamDestroy(s);
+    }
     return AMLS_OK;
 }
 

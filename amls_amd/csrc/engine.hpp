@@ -154,6 +154,10 @@ struct Ctx {
     SytrdCache syLocal, syReduced;
     bool gpu = false;
     hipStream_t stream = nullptr;
+    // second stream (default priority) for independent work forked from
+    // `stream` within one plan, and the fork / join events
+    hipStream_t stream2 = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
     std::string err;
 
     // ---- problem (host)
