@@ -533,22 +533,6 @@ __global__ void __launch_bounds__(512) sytrd_tail_kernel(const EigDesc* __restri
     }
 }
 
-// Sturm count: number of eigenvalues of T strictly below x (pivmin guarded).
-__device__ int sturm_count(const double* __restrict__ dd, const double* __restrict__ ee, int n,
-                           double x, double pivmin) {
-    int cnt = 0;
-    double q = dd[0] - x;
-    if (fabs(q) < pivmin) q = -pivmin;
-    if (q < 0.0) ++cnt;
-    for (int j = 1; j < n; ++j) {
-        const double e = ee[j - 1];
-        q = dd[j] - x - e * e / q;
-        if (fabs(q) < pivmin) q = -pivmin;
-        if (q < 0.0) ++cnt;
-    }
-    return cnt;
-}
-
 __device__ void gersh(const double* dd, const double* ee, int n, double& lo, double& hi,
                       double& pivmin, double& tnorm) {
     lo = 1e308;
@@ -608,28 +592,6 @@ __device__ void stage_tridiag(const EigDesc& d, double* sde, int useLds, const d
         __syncthreads();
         dd = sde;
         ee = sde + d.n;
-    }
-}
-
-// counts[mat] = number of eigenvalues >= thr[mat] (capped at kcap); one
-// 64-lane workgroup per matrix, its tridiagonal staged in LDS
-__global__ void __launch_bounds__(64) eig_count_kernel(const EigDesc* __restrict__ descs,
-                                                       const double* __restrict__ thr,
-                                                       int32_t* __restrict__ counts, int useLds) {
-    extern __shared__ double sde[];
-    const int m = blockIdx.x;
-    const EigDesc d = descs[m];
-    if (d.n <= 0) {
-        if (threadIdx.x == 0) counts[m] = 0;
-        return;
-    }
-    const double *dd, *ee;
-    stage_tridiag(d, sde, useLds, dd, ee);
-    double lo, hi, pivmin, tn;
-    gersh_wave(dd, ee, d.n, threadIdx.x, lo, hi, pivmin, tn);
-    if (threadIdx.x == 0) {
-        const int below = sturm_count(dd, ee, d.n, thr[m], pivmin);
-        counts[m] = min(d.n - below, d.kcap);
     }
 }
 
@@ -729,6 +691,30 @@ __device__ __forceinline__ void sturm_count_n(const double* __restrict__ dd,
     }
 #pragma unroll
     for (int q = 0; q < NC; ++q) cnt[q] = c[q];
+}
+
+// counts[mat] = number of eigenvalues >= thr[mat] (capped at kcap); one
+// 64-lane workgroup per matrix, its tridiagonal staged in LDS
+__global__ void __launch_bounds__(64) eig_count_kernel(const EigDesc* __restrict__ descs,
+                                                       const double* __restrict__ thr,
+                                                       int32_t* __restrict__ counts, int useLds) {
+    extern __shared__ double sde[];
+    const int m = blockIdx.x;
+    const EigDesc d = descs[m];
+    if (d.n <= 0) {
+        if (threadIdx.x == 0) counts[m] = 0;
+        return;
+    }
+    const double *dd, *ee;
+    stage_tridiag(d, sde, useLds, dd, ee);
+    double lo, hi, pivmin, tn;
+    gersh_wave(dd, ee, d.n, threadIdx.x, lo, hi, pivmin, tn);
+    if (threadIdx.x == 0) {
+        const double x = thr[m];
+        int below;
+        sturm_count_n<false, 1>(dd, ee, d.n, &x, __builtin_amdgcn_frexp_exp(tn), &below);
+        counts[m] = min(d.n - below, d.kcap);
+    }
 }
 
 // lam[j] = (j+1)-th largest eigenvalue, j < k; one wave per work item

@@ -8,6 +8,7 @@
 // multifrontal extend-add so no two workgroups write one element.
 #include "common.hpp"
 #include "launch.hpp"
+#include "packed_solve.hpp"
 
 #include <climits>
 
@@ -65,59 +66,55 @@ __global__ void __launch_bounds__(64) ldlt_diag_kernel(const FrontDesc* __restri
     }
 }
 
-// Rows below the diagonal block of panel s0, one row per thread:
-// w L11^T = a (forward substitution in registers), L = w D^-1, W = w.
+// Rows below the diagonal block of panel s0, one row per thread: the row's
+// 64 entries in registers, w L11^T = a solved as L11 w^T = a^T against L11
+// packed in LDS in solve order (packed_solve), then L = w D^-1 and W = w.
 // Work item (x = front index, y = chunk of 256 rows).
 __global__ void __launch_bounds__(256) ldlt_rows_kernel(const FrontDesc* __restrict__ descs,
                                                         const int2* __restrict__ work, int s0) {
-    __shared__ double sL[NB * NB];      // sL[j * NB + k] = L11(k, j), k > j, else 0
+    static_assert(NB == PS_N, "the panel solve is 64 wide");
+    __shared__ __align__(16) double sL[PS_NP];
     __shared__ double sdinv[NB];
     const int2 wk = work[blockIdx.x];
     const FrontDesc d = descs[wk.x];
     const int64_t f = d.f;
     const int jb = min(NB, d.n - s0);
-    const double* F = d.F;
     const int t = threadIdx.x;
     for (int e = t; e < NB * NB; e += 256) {
-        const int k = e & 63, j = e >> 6;
-        sL[j * NB + k] = (k > j && k < jb && j < jb) ? F[(s0 + k) + (int64_t)(s0 + j) * f] : 0.0;
+        const int r = e & 63, c = e >> 6;
+        if (r <= c) continue;
+        sL[packed_pos<false>(r, c)] = r < jb ? d.F[(s0 + r) + (int64_t)(s0 + c) * f] : 0.0;
     }
-    if (t < NB) sdinv[t] = t < jb ? 1.0 / F[(s0 + t) + (int64_t)(s0 + t) * f] : 0.0;
+    if (t < NB) sdinv[t] = t < jb ? 1.0 / d.F[(s0 + t) + (int64_t)(s0 + t) * f] : 0.0;
     __syncthreads();
     const int64_t rb = s0 + jb;
-    const int64_t i = rb + (int64_t)wk.y * 256 + t;
-    if (i >= f) return;
-    // left-looking in blocks of 16 columns: earlier w values are re-read from
-    // W (written by this thread), keeping only 16 row values in registers
-    double* W = d.W + (i - rb);
-    double* Fi = d.F + i;
-    for (int J = 0; J * 16 < jb; ++J) {
-        double a[16];
+    const int64_t row0 = rb + (int64_t)wk.y * 256;   // the chunk's first row
+    // The chunk's rows of F's panel columns and of W through buffer
+    // resources: lane t at byte offset 8 t, column k at the scalar offset
+    // 8 k f.  Rows past f and columns past jb get offsets beyond the range
+    // (loads read zero, stores are dropped), so no lane branches.
+    // (a front of order f holds f^2 doubles, so f stays far below the
+    // 2 million at which the range would reach the cap)
+    constexpr int kOut = 0x40000000;   // past every range below; two of them stay below 2^32
+    const int64_t span = (63 * f + 256) * 8;
+    const int nrec = (int)min(span, (int64_t)kOut - 8);
+    const __amdgpu_buffer_rsrc_t rF =
+        __builtin_amdgcn_make_buffer_rsrc(d.F + row0 + (int64_t)s0 * f, (short)0, nrec, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rW =
+        __builtin_amdgcn_make_buffer_rsrc(d.W + (row0 - rb), (short)0, nrec, 0x00020000);
+    const int vo = row0 + t < f ? 8 * t : kOut;
+    const int cstep = 8 * (int)f;
+    double a[NB];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int col = J * 16 + q;
-            a[q] = col < jb ? Fi[(int64_t)(s0 + col) * f] : 0.0;
-        }
-        for (int j = 0; j < J * 16; ++j) {
-            const double wj = W[(int64_t)j * f];
-            const double* lj = sL + j * NB + J * 16;
+    for (int k = 0; k < NB; ++k)
+        a[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                              rF, vo, k < jb ? k * cstep : kOut, 0));
+    packed_solve<false>(a, sL, jb);
 #pragma unroll
-            for (int q = 0; q < 16; ++q) a[q] -= wj * lj[q];
-        }
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const double* lq = sL + (J * 16 + q) * NB + J * 16;
-#pragma unroll
-            for (int r = q + 1; r < 16; ++r) a[r] -= a[q] * lq[r];
-        }
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int col = J * 16 + q;
-            if (col < jb) {
-                W[(int64_t)col * f] = a[q];
-                Fi[(int64_t)(s0 + col) * f] = a[q] * sdinv[col];
-            }
-        }
+    for (int k = 0; k < NB; ++k) {
+        const int so = k < jb ? k * cstep : kOut;
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(ps_u32x2, a[k]), rW, vo, so, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(ps_u32x2, a[k] * sdinv[k]), rF, vo, so, 0);
     }
 }
 

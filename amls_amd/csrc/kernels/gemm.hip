@@ -11,6 +11,7 @@
 // Eigen triangular solves (src/linalg/sparse_direct.cpp:26-31, :56-61).
 #include "common.hpp"
 #include "launch.hpp"
+#include "packed_solve.hpp"
 
 namespace amls {
 namespace dev {
@@ -210,14 +211,13 @@ __global__ void __launch_bounds__(256) gemm_batched_kernel(const GemmDesc* __res
 // blocks in order and applies the off-diagonal updates between them as
 // GEMMs, so every step is one wide launch.
 // ---------------------------------------------------------------------------
+static_assert(TM == PS_N, "the diagonal-block solve is 64 wide");
+
 template <bool TR>
 __global__ void __launch_bounds__(256, TR ? 2 : 1) trsm_diag_kernel(const TrsmDesc* __restrict__ descs,
                                                            const TrsmWork* __restrict__ work) {
-    // the strictly lower part of the block, packed in the order the solve
-    // consumes it: forward, pair (j, i > j) uses L(i, j) in step j; backward,
-    // pair (j, i < j) uses L(j, i) in step j, j descending
-    constexpr int NP = TM * (TM - 1) / 2, D = 16;
-    __shared__ __align__(16) double sL[NP];
+    // the strictly lower part of the block, packed in solve order
+    __shared__ __align__(16) double sL[PS_NP];
     const TrsmWork wk = work[blockIdx.x];
     const TrsmDesc d = descs[wk.desc];
     const int r0 = wk.pad0 * TM;
@@ -227,8 +227,7 @@ __global__ void __launch_bounds__(256, TR ? 2 : 1) trsm_diag_kernel(const TrsmDe
         const int r = e & 63, c = e >> 6;
         if (r <= c) continue;
         const double v = (r < mv) ? d.L[r0 + r + (int64_t)(r0 + c) * d.ldl] : 0.0;
-        const int p = TR ? NP - r * (r + 1) / 2 + c : c * (TM - 1) - c * (c - 1) / 2 + (r - c - 1);
-        sL[p] = v;
+        sL[packed_pos<TR>(r, c)] = v;
     }
     // the wave's 64 columns through one buffer resource whose range ends with
     // the matrix: a lane's column at byte offset lane * ldb * 8 (idle lanes
@@ -253,45 +252,9 @@ __global__ void __launch_bounds__(256, TR ? 2 : 1) trsm_diag_kernel(const TrsmDe
     for (int r = 0; r < TM; ++r)
         x[r] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rX, voff + r * 8, 0, 0));
     __syncthreads();
-    // Forward, a live row i < mv only ever reads x_j with j < i; backward,
-    // steps j >= mv use x_j = 0.  So the full 64-step recurrences leave the
-    // live rows exact (the L entries past mv are zeros).  The L values stream
-    // through a ring of D registers, each read from LDS D pairs before its
-    // use; the scheduling barriers every 8 pairs stop the compiler from
-    // hoisting all reads ahead, which would need more registers than a wave
-    // has.  (The forward solve still needs about 300 registers, so it runs
-    // one wave per SIMD; the backward one fits two.)
-    double buf[D];
-#pragma unroll
-    for (int q = 0; q < D; ++q) buf[q] = sL[q];
-    if (!TR) {
-#pragma unroll
-        for (int j = 0; j < TM - 1; ++j) {
-            const double xj = x[j];
-#pragma unroll
-            for (int i = j + 1; i < TM; ++i) {
-                const int p = j * (TM - 1) - j * (j - 1) / 2 + (i - j - 1);
-                const double l = buf[p % D];
-                if (p + D < NP) buf[p % D] = sL[p + D];
-                x[i] -= l * xj;
-                if ((p & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-            }
-        }
-    } else {
-        // L^T upper unit: once x_j is final, x_i -= L(j, i) x_j for i < j
-#pragma unroll
-        for (int j = TM - 1; j > 0; --j) {
-            const double xj = j < mv ? x[j] : 0.0;
-#pragma unroll
-            for (int i = 0; i < j; ++i) {
-                const int p = NP - j * (j + 1) / 2 + i;
-                const double l = buf[p % D];
-                if (p + D < NP) buf[p % D] = sL[p + D];
-                x[i] -= l * xj;
-                if ((p & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-            }
-        }
-    }
+    // (the forward solve needs about 300 registers, so it runs one wave
+    // per SIMD; the backward one fits two)
+    packed_solve<TR>(x, sL, mv);
     // an opaque copy of the offset, so the 64 store addresses are formed
     // here (as immediates) rather than hoisted and held across the solve
     int vs = voff;
