@@ -166,23 +166,28 @@ __global__ void __launch_bounds__(256) copy2d_kernel(const Copy2dDesc* __restric
     const int t = threadIdx.x;
     if (d.mode == 1) {
         // transpose through LDS: read along src's contiguous dimension, write
-        // along dst's
+        // along dst's; all sixteen loads of a thread are issued before the
+        // first LDS store, and all LDS reads before the first global store
         const int i0 = w.y * 64, j0 = w.z * 64;
+        const int jj = t & 63, j = j0 + jj;
+        double v[16];
+#pragma unroll
         for (int q = 0; q < 16; ++q) {
-            const int jj = t & 63, ii = (t >> 6) + 4 * q;   // src(j0 + jj, i0 + ii)
-            const int i = i0 + ii, j = j0 + jj;
-            tileT[ii * 65 + jj] = (i < d.rows && j < d.cols) ? d.src[j + (int64_t)i * d.lds] : 0.0;
+            const int i = i0 + (t >> 6) + 4 * q;   // src(j, i)
+            v[q] = (i < d.rows && j < d.cols) ? d.src[j + (int64_t)i * d.lds] : 0.0;
         }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) tileT[((t >> 6) + 4 * q) * 65 + jj] = v[q];
         __syncthreads();
         const int i = i0 + (t & 63);
         if (i >= d.rows) return;
-        const double rsv = d.rs ? d.rs[i] : 1.0;
+        const double sr = d.alpha * (d.rs ? d.rs[i] : 1.0);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = tileT[(t & 63) * 65 + (t >> 6) + 4 * q];
+#pragma unroll
         for (int q = 0; q < 16; ++q) {
-            const int jj = (t >> 6) + 4 * q, j = j0 + jj;
-            if (j >= d.cols) break;
-            double s = d.alpha * rsv;
-            if (d.cs) s *= d.cs[j];
-            d.dst[i + (int64_t)j * d.ldd] = s * tileT[(t & 63) * 65 + jj];
+            const int jc = j0 + (t >> 6) + 4 * q;
+            if (jc < d.cols) d.dst[i + (int64_t)jc * d.ldd] = sr * (d.cs ? d.cs[jc] : 1.0) * v[q];
         }
         return;
     }
