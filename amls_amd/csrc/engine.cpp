@@ -390,7 +390,7 @@ void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<i
     const size_t od = plan.blob->add(descs), oo = plan.blob->add(soff);
     Blob* bl = plan.blob;
     // the last sytrd_tail_order() steps of every matrix run in one launch,
-    // a workgroup per matrix with its trailing block in LDS
+    // a workgroup per matrix with its trailing block in registers
     const int i0 = std::max(0, static_cast<int>(start.size()) - sytrd_tail_order());
     const int nmat = static_cast<int>(descs.size());
     // algorithmic traffic per tiled step: each active matrix reads and
