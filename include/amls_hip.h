@@ -244,8 +244,8 @@ amls_status amls_hip_adjoint(amls_ctx* ctx, int64_t nelem, int32_t npe, const in
                              double* grad_r);
 
 /* ---- measurement -------------------------------------------------------
- * enable != 0 brackets every launch on the context's stream with HIP events
- * (grouped by kernel class) and clears the counters; enable = 0 stops the
+ * enable != 0 brackets every launch with HIP events on the stream it runs
+ * on (grouped by kernel class) and clears the counters; enable = 0 stops the
  * event timing and clears the counters.  Launch counts and the algorithmic
  * flops / bytes of each class (from the launch descriptors: 2mnk and
  * (mk + kn + 2mn) words per GEMM, n(n-1) r for a unit triangular solve, two
