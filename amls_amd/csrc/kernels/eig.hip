@@ -46,10 +46,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
-    // 256-thread block reduction; red has >= 4 slots
+    // 256-thread block reduction into 4 slots of red that no thread of the
+    // block reads or writes elsewhere in the kernel (so one barrier suffices)
     v = wave_sum(v);
     const int t = threadIdx.x;
-    __syncthreads();
     if ((t & 63) == 0) red[t >> 6] = v;
     __syncthreads();
     return red[0] + red[1] + red[2] + red[3];
@@ -165,7 +165,7 @@ __device__ __forceinline__ void sytrd_step_body(const EigDesc& d, int i, int4 wk
         const double x = xof(Ai[r], hasPrev ? svr[r] : 0.0, hasPrev ? spr[r] : 0.0);
         part += x * x;
     }
-    const double xnorm2 = block_sum(part, red);
+    const double xnorm2 = block_sum(part, red + 4);
     const double alpha = xof(a1, v1, p1);
     double tau = 0.0, beta = alpha, scal = 0.0;
     if (xnorm2 > 0.0) {
