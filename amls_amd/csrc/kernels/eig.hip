@@ -120,9 +120,9 @@ __device__ __forceinline__ void sytrd_step_body(const EigDesc& d, int i, int4 wk
     // tile vectors: thread t < 64 takes row r0 + t, 64 <= t < 128 column c0 + t - 64
     const int vidx = t < 64 ? r0 + t : c0 + t - 64;
     const bool vok = haveTile && t < 128 && vidx > i && vidx < n;
-    // staged in LDS at once (v final; p and the column entry until w and x
-    // replace them), so the loads go out with the others above instead of
-    // being deferred to their use after the reductions
+    // staged in LDS at once (v and p as they are; the column entry until the
+    // new reflector's entry replaces it), so the loads go out with the others
+    // above instead of being deferred to their use after the reductions
     if (t < 128) {
         const double vv_t = (vok && hasPrev) ? svr[vidx] : 0.0;
         const double pv_t = (vok && hasPrev) ? spr[vidx] : 0.0;
@@ -204,29 +204,29 @@ __device__ __forceinline__ void sytrd_step_body(const EigDesc& d, int i, int4 wk
         leader();
         return;
     }
+    // the tile vectors keep v and p as staged (zero outside the trailing
+    // range): w = tau p + alpha v is formed where the update needs it, and
+    // the new reflector's entries replace the column entries in xr / xc,
+    // which the update does not read, so no barrier separates the two
     if (t < 128) {
-        double* wd = t < 64 ? wr : wc;
         double* xd = t < 64 ? xr : xc;
         const double vv_t = (t < 64 ? vr : vc)[t & 63];
-        const double pv_t = wd[t & 63], av_t = xd[t & 63];
-        const double w = tauP * pv_t + alphaP * vv_t;
-        const double x = vok ? (vidx == i + 1 ? 1.0 : xof(av_t, vv_t, pv_t) * scal) : 0.0;
-        wd[t & 63] = vok ? w : 0.0;
-        xd[t & 63] = x;
+        const double pv_t = (t < 64 ? wr : wc)[t & 63];
+        const double av_t = xd[t & 63];
+        xd[t & 63] = vok ? (vidx == i + 1 ? 1.0 : xof(av_t, vv_t, pv_t) * scal) : 0.0;
     }
-    __syncthreads();
     // the vector entries of the thread's row and its 16 columns, read into
     // registers before the loop: the LDS writes of the loop would otherwise
     // order every read behind the previous element's write
     unsigned store = 0;   // tile elements to write back (bit k)
     {
         const int rr = t & 63, cq = t >> 6;
-        const double vrr = vr[rr], wrr = wr[rr];
+        const double vrr = vr[rr], wrr = tauP * wr[rr] + alphaP * vrr;
         double vcc[16], wcc[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             vcc[k] = vc[cq + 4 * k];
-            wcc[k] = wc[cq + 4 * k];
+            wcc[k] = tauP * wc[cq + 4 * k] + alphaP * vcc[k];
         }
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
