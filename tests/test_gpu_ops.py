@@ -19,6 +19,21 @@ def test_gemm_fp64(gpu_ctx, m, n, k, ta, tb):
     assert np.abs(out - ref).max() <= 1e-13 * k * max(1.0, np.abs(ref).max())
 
 
+@pytest.mark.parametrize("beta", [0.0, 1.0])
+@pytest.mark.parametrize("ta", [False, True])
+def test_gemm_fp64_split_k(gpu_ctx, beta, ta):
+    # few tiles and k >= 8192: the product is split over k and the pieces
+    # are added into C atomically
+    m, n, k = 300, 200, 9000
+    rng = np.random.default_rng(7 + int(beta) + 2 * ta)
+    A = rng.standard_normal((k, m) if ta else (m, k))
+    B = rng.standard_normal((k, n))
+    C0 = rng.standard_normal((m, n))
+    ref = -1.5 * (A.T if ta else A) @ B + beta * C0
+    out = gpu_ctx.gemm(A, B, C0, alpha=-1.5, beta=beta, transA=ta)
+    assert np.abs(out - ref).max() <= 1e-13 * k * max(1.0, np.abs(ref).max())
+
+
 def test_gemm_fp32(gpu_ctx):
     rng = np.random.default_rng(3)
     A = rng.standard_normal((200, 150))
