@@ -104,6 +104,52 @@ def roofline(stats):
             "algorithmic_per_launch": (st["flops"] if name in PEAK else st["bytes"]) / st["launches"]}
 
 
+def roofline_overall(stats, steps, t_solve):
+    """Whole-solve roofline: T_roof = sum over kernel classes of
+    max(flops / peak, bytes / HBM peak), from the library's algorithmic counts
+    per class (FP32 GEMMs at the FP32 MFMA peak, all other flops at the FP64
+    peak), against the measured time of one solve.  Per class: the ideal and
+    the HIP-event milliseconds per solve."""
+    t_roof = 0.0
+    stages = {}
+    for name, st in stats.items():
+        if name == "launch_overhead" or st["launches"] <= 0:
+            continue
+        peak = PEAK["gemm_f32"][0] if name == "gemm_f32" else PEAK["gemm_f64"][0]
+        t = max(st["flops"] / (peak * 1e12), st["bytes"] / (HBM_PEAK_GBS * 1e9)) / steps
+        t_roof += t
+        stages[name] = [round(1e3 * t, 4), round(st["ms"] / steps, 3)]
+    top = dict(sorted(stages.items(), key=lambda kv: -kv[1][1])[:8])
+    return {"t_roof_ms": round(1e3 * t_roof, 3), "t_measured_ms": round(1e3 * t_solve, 3),
+            "frac": round(t_roof / t_solve, 5), "stages_roof_vs_event_ms": top}
+
+
+def cpu_full(pb, tree, ref, nev, eta, si_steps, O):
+    """Two more host timings, single-threaded BLAS: the GPU value's algorithm
+    (two-pass AMLS at eta, then si_steps subspace-iteration steps started from
+    the whole AMLS basis) run by the oracle, and the FP64 ground truth
+    (shift-invert Lanczos at sigma = 0, tol 1e-12)."""
+    from threadpoolctl import threadpool_limits
+    out = {}
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        r = O.amls_solve(pb.K, pb.M, tree, nev=nev, thresholdFactor=eta, recoverVectors=False)
+        _, Q = r.cond.solve_projected(r.p)
+        z = r.cond.backtransform(Q)
+        X = np.zeros_like(z)
+        X[tree.perm] = z
+        th, _ = O.subspace_iteration(pb.K, pb.M, r.ge, X, si_steps, nev, tree.perm)
+        t1 = time.perf_counter()
+        out["amls_si_fp64"] = {"value": round(t1 - t0, 3), "unit": "s/solve", "cores": 1,
+                               "p": int(r.p), "max_rel_err": float(O.rel_err(th, ref).max())}
+        t0 = time.perf_counter()
+        w = O.reference_eigs(pb.K, pb.M, nev)
+        t1 = time.perf_counter()
+        out["lanczos_fp64"] = {"value": round(t1 - t0, 3), "unit": "s/solve", "cores": 1,
+                               "max_rel_err": float(O.rel_err(w, ref).max())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +161,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-table", action="store_true")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also time, on the host, the oracle's AMLS + 2 subspace-iteration "
+                         "steps over the whole AMLS basis (the GPU value's algorithm) and the "
+                         "FP64 shift-invert Lanczos ground truth (minutes of CPU time)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 10 times (the reference kernel "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -158,6 +208,7 @@ def main():
     t_prof, _, stats = timed_solves(ctx, om, args.steps, 0, True)
     err_m, res_m, p = quality(ctx, pb, opts("mixed", args.si_steps), ref)
     rf = roofline(stats)
+    rf_all = roofline_overall(stats, args.steps, t_mixed)
 
     out_fp64 = None
     if not args.no_fp64:
@@ -185,6 +236,8 @@ def main():
                "max_rel_err": float(O.rel_err(r.theta, ref).max()),
                "host_cpus": os.cpu_count(),
                "gpu_speedup_vs_cpu": round(tc / t_mixed, 2)}
+        if args.cpu_full:
+            cpu["more"] = cpu_full(pb, tree, ref, nev, args.eta, args.si_steps, O)
 
     line = {
         "metric": METRIC,
@@ -209,6 +262,7 @@ def main():
         "profiled_s_per_solve": round(t_prof, 6),
         "fp64_gpu": out_fp64,
         "roofline": rf,
+        "roofline_overall": rf_all,
         "cpu_baseline": cpu,
     }
     if args.kernel_table:
