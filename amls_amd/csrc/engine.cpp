@@ -804,6 +804,8 @@ void Ctx::sync() {
 void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t leafSize,
                       double sh) {
     if (probs.empty()) throw StatusError(AMLS_E_ARG, "no problem given");
+    if (nLevels < 0 || nLevels > 40) throw StatusError(AMLS_E_ARG, "nLevels must be in 0..40");
+    if (leafSize < 1) throw StatusError(AMLS_E_ARG, "leafSize must be positive");
     haveProblem = false;
     geDone = specReady = selected = projected = reduced = recovered = false;
     lastVectors = false;
@@ -816,6 +818,12 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
         if (P.n <= 0) throw StatusError(AMLS_E_ARG, "empty matrix");
         if (!P.Kp || !P.Ki || !P.Kx || !P.Mp || !P.Mi || !P.Mx)
             throw StatusError(AMLS_E_ARG, "null matrix array");
+        // column pointers start at 0 and never decrease
+        for (const int64_t* cp : {P.Kp, P.Mp}) {
+            if (cp[0] != 0) throw StatusError(AMLS_E_ARG, "column pointers must start at 0");
+            for (int64_t c = 0; c < P.n; ++c)
+                if (cp[c + 1] < cp[c]) throw StatusError(AMLS_E_ARG, "column pointers decrease");
+        }
         segRow[s + 1] = segRow[s] + P.n;
     }
     n = segRow[nseg];
@@ -1050,7 +1058,12 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
 // ---------------------------------------------------------------------------
 // Block Gaussian elimination (multifrontal form)
 // ---------------------------------------------------------------------------
+static void checkPrecision(int prec) {
+    if (prec != 0 && prec != 1) throw StatusError(AMLS_E_ARG, "precision must be 0 (FP64) or 1 (mixed)");
+}
+
 void Ctx::runGE(int prec) {
+    checkPrecision(prec);
     requireGpu();
     sync();   // earlier phases may still use buffers released below
     if (!haveProblem) throw StatusError(AMLS_E_STATE, "no problem set");
@@ -1423,6 +1436,7 @@ void Ctx::spectralPrepare() {
 // Local eigenpairs + truncation (both passes)
 // ---------------------------------------------------------------------------
 void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
+    if (kind != 0 && kind != 1) throw StatusError(AMLS_E_ARG, "truncation kind must be 0 or 1");
     requireGpu();
     sync();   // earlier phases may still use buffers released below
     if (!specReady) throw StatusError(AMLS_E_STATE, "spectral before spectral_prepare");
@@ -1554,6 +1568,7 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
 // Condensation: projection of M onto the retained modes (multifrontal)
 // ---------------------------------------------------------------------------
 void Ctx::project(int prec) {
+    checkPrecision(prec);
     requireGpu();
     sync();   // earlier phases may still use buffers released below
     if (!selected) throw StatusError(AMLS_E_STATE, "project before spectral");
@@ -2598,6 +2613,11 @@ namespace amls {
 static void solveNumeric(Ctx& c, amls_options o, double* theta, double* Z, int64_t* p_out,
                          double* timing, Clock::time_point t0, Clock::time_point t1) {
     if (o.nev <= 0) throw StatusError(AMLS_E_ARG, "nev must be positive");
+    checkPrecision(o.precision);
+    if (o.siSteps < 0 || o.siVectors < 0 || o.fixedKeep < 0)
+        throw StatusError(AMLS_E_ARG, "siSteps, siVectors and fixedKeep must not be negative");
+    if (o.fixedKeep == 0 && !(o.thresholdFactor > 0.0))
+        throw StatusError(AMLS_E_ARG, "thresholdFactor must be positive");
     c.requireGpu();
     if (!c.haveProblem) throw StatusError(AMLS_E_STATE, "no problem set");
     AMLS_HIP_CHECK(hipSetDevice(c.device));
@@ -2897,6 +2917,8 @@ amls_status amls_hip_gemm(amls_ctx* ctx, int32_t precision, int32_t transA, int3
                           int64_t ldc) {
     return guarded(ctx, [&](amls::Ctx& c) {
         using amls::StatusError;
+        if (precision != 0 && precision != 1)
+            throw StatusError(AMLS_E_ARG, "precision must be 0 (FP64) or 1 (FP32 MFMA)");
         c.requireGpu();
         if (m < 0 || n < 0 || k < 0) throw StatusError(AMLS_E_ARG, "negative size");
         const int64_t ar = transA ? k : m, ac = transA ? m : k;

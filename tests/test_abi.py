@@ -54,3 +54,54 @@ def test_bad_arguments_are_status_codes():
         ctx.set_problem(bad, M, None, 2, 5)
     assert e.value.status == 1
     assert "out of range" in str(e.value)
+
+
+def test_malformed_inputs_are_status_codes():
+    """Malformed column pointers, tree parameters, precision, truncation kind
+    and subspace options come back as AMLS_E_ARG (1), checked before any
+    device work (so a host-only context sees them)."""
+    from amls_amd._lib import dptr, iptr
+    L = _lib.lib()
+    ctx = HipContext(-1)
+    K, M = cases.path_graph(10)
+    Kc, Mc = K.tocsc(), M.tocsc()
+    ki = np.ascontiguousarray(Kc.indices, dtype=np.int64)
+    kx = np.ascontiguousarray(Kc.data, dtype=np.float64)
+    mp = np.ascontiguousarray(Mc.indptr, dtype=np.int64)
+    mi = np.ascontiguousarray(Mc.indices, dtype=np.int64)
+    mx = np.ascontiguousarray(Mc.data, dtype=np.float64)
+    kp = np.ascontiguousarray(Kc.indptr, dtype=np.int64)
+
+    def setp(colptr, nLevels=2, leafSize=5):
+        return L.amls_hip_set_problem(ctx._h, 10, iptr(colptr), iptr(ki), dptr(kx), iptr(mp),
+                                      iptr(mi), dptr(mx), dptr(None), nLevels, leafSize, 0.0)
+
+    def err():
+        return L.amls_hip_last_error(ctx._h).decode()
+
+    swapped = kp.copy()
+    swapped[4], swapped[5] = swapped[5], swapped[4]
+    assert setp(swapped) == 1 and "decrease" in err()
+    shifted = kp.copy()
+    shifted[0] = 1
+    assert setp(shifted) == 1 and "start at 0" in err()
+    assert setp(kp, nLevels=-1) == 1 and "nLevels" in err()
+    assert setp(kp, leafSize=0) == 1 and "leafSize" in err()
+    assert setp(kp) == 0
+    assert L.amls_hip_ge(ctx._h, 7) == 1 and "precision" in err()
+    assert L.amls_hip_project(ctx._h, 2, None) == 1 and "precision" in err()
+    assert L.amls_hip_spectral(ctx._h, 5, 0.0, None) == 1 and "kind" in err()
+    o = _lib.AmlsOptionsC()
+    L.amls_options_default(ctypes.byref(o))
+    o.precision = 3
+    assert L.amls_hip_solve_numeric(ctx._h, ctypes.byref(o), None, None, None, None) == 1
+    o.precision = 1
+    o.siSteps = -1
+    assert L.amls_hip_solve_numeric(ctx._h, ctypes.byref(o), None, None, None, None) == 1
+    o.siSteps = 0
+    o.thresholdFactor = 0.0
+    assert L.amls_hip_solve_numeric(ctx._h, ctypes.byref(o), None, None, None, None) == 1
+    # valid options on a host-only context: the device is what is missing
+    o.thresholdFactor = 2.0
+    assert L.amls_hip_solve_numeric(ctx._h, ctypes.byref(o), None, None, None, None) == 6
+    ctx.close()
