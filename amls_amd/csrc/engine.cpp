@@ -2921,6 +2921,8 @@ amls_status amls_hip_gemm(amls_ctx* ctx, int32_t precision, int32_t transA, int3
             throw StatusError(AMLS_E_ARG, "precision must be 0 (FP64) or 1 (FP32 MFMA)");
         c.requireGpu();
         if (m < 0 || n < 0 || k < 0) throw StatusError(AMLS_E_ARG, "negative size");
+        if ((m > 0 && n > 0 && !C) || (m > 0 && n > 0 && k > 0 && (!A || !B)))
+            throw StatusError(AMLS_E_ARG, "null matrix argument");
         const int64_t ar = transA ? k : m, ac = transA ? m : k;
         const int64_t br = transB ? n : k, bc = transB ? k : n;
         if (lda < std::max<int64_t>(1, ar) || ldb < std::max<int64_t>(1, br) ||
@@ -2968,6 +2970,7 @@ amls_status amls_hip_ldlt_solve(amls_ctx* ctx, int64_t n, const double* A, int64
         using amls::StatusError;
         c.requireGpu();
         if (n <= 0 || nrhs < 0) throw StatusError(AMLS_E_ARG, "bad size");
+        if (!A || (nrhs > 0 && !B)) throw StatusError(AMLS_E_ARG, "null matrix argument");
         amls::DevArena mem;
         mem.plan(n * n * 8);
         mem.plan(n * 64 * 8);
@@ -3038,6 +3041,7 @@ amls_status amls_hip_ldlt_solve(amls_ctx* ctx, int64_t n, const double* A, int64
             vd[0].ld = n;
             vd[0].n = static_cast<int32_t>(n);
             vd[0].op = 0;
+            // dq = D^-1/2 (vecop 0); the row scaling applied twice is D^-1
             amls::Copy2dBatch sc;
             sc.add(amls::cdesc(dX, n, dX, n, n, nrhs, 0, dq));
             plan.trsm(t1);
@@ -3063,6 +3067,7 @@ amls_status amls_hip_syevx_top(amls_ctx* ctx, int64_t n, const double* A, int64_
         using amls::StatusError;
         c.requireGpu();
         if (n <= 0 || k < 0 || k > n) throw StatusError(AMLS_E_ARG, "bad size");
+        if (!A) throw StatusError(AMLS_E_ARG, "null matrix argument");
         amls::DevArena mem;
         mem.plan(n * n * 8 * 2);
         mem.plan(n * 8 * 8);
