@@ -806,6 +806,8 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
     if (probs.empty()) throw StatusError(AMLS_E_ARG, "no problem given");
     if (nLevels < 0 || nLevels > 40) throw StatusError(AMLS_E_ARG, "nLevels must be in 0..40");
     if (leafSize < 1) throw StatusError(AMLS_E_ARG, "leafSize must be positive");
+    ++problemGen;
+    if (gpu) dropGeGraph();
     haveProblem = false;
     geDone = specReady = selected = projected = reduced = recovered = false;
     lastVectors = false;
@@ -1089,17 +1091,35 @@ void Ctx::runGE(int prec) {
     dDsq = A.take<double>(vo[N]);
     dDinv = A.take<double>(vo[N]);
     dErr = A.take<int>(1);
-    const int big = INT_MAX;
-    AMLS_HIP_CHECK(hipMemcpyAsync(dErr, &big, sizeof(int), hipMemcpyHostToDevice, stream));
-    launch_fill(dF, ftot, 0.0, stream);
-    launch_fill(dG, ftot, 0.0, stream);
-    launch_fill(dLinv, co[N], 0.0, stream);
-    launch_scatter(dsrcK, dtgtK, dKx, dF, nasmK, 1.0, false, stream);
-    launch_scatter(dsrcM, dtgtM, dMx, dG, nasmM, 1.0, false, stream);
-    if (shift != 0.0) launch_scatter(dsrcM, dtgtM, dMx, dF, nasmM, shift, true, stream);
-
+    // the launch sequence depends only on the tree, the precision and the
+    // arena's address (the values come from the device): replay the graph
+    // captured for them if there is one
+    if (geGraph && geGraphPrec == prec && geGraphGen == problemGen && geGraphBase == A.base() &&
+        !kstats.enabled) {
+        AMLS_HIP_CHECK(hipGraphLaunch(geGraph, stream));
+        finishGE();
+        return;
+    }
+    dropGeGraph();
     fresh(geBlob);
     Plan plan(geBlob.get(), &kstats);
+    {
+        int* err = dErr;
+        double *F = dF, *G = dG, *Li = dLinv;
+        const int64_t nf = ftot, nl = co[N], nk = nasmK, nm = nasmM;
+        const int64_t *sK = dsrcK, *tK = dtgtK, *sM = dsrcM, *tM = dtgtM;
+        const double *Kx_ = dKx, *Mx_ = dMx;
+        const double sh = shift;
+        plan.fn([=](hipStream_t s) {
+            launch_set_i32(err, INT_MAX, s);
+            launch_fill(F, nf, 0.0, s);
+            launch_fill(G, nf, 0.0, s);
+            launch_fill(Li, nl, 0.0, s);
+            launch_scatter(sK, tK, Kx_, F, nk, 1.0, false, s);
+            launch_scatter(sM, tM, Mx_, G, nm, 1.0, false, s);
+            if (sh != 0.0) launch_scatter(sM, tM, Mx_, F, nm, sh, true, s);
+        });
+    }
     const bool f32 = prec != 0;
     for (int64_t l = 0; l <= S.maxLevel; ++l) {
         const auto& nodes = S.byLevel[l];
@@ -1308,7 +1328,42 @@ void Ctx::runGE(int prec) {
         plan.gemm(gFm, f32);
         plan.gemm(gG22, f32);
     }
-    plan.run(stream);
+    if (kstats.enabled) {
+        plan.run(stream);
+    } else {
+        // descriptors go up first, outside the graph; the graph holds the
+        // kernel launches only (no memset or host-copy nodes)
+        plan.blob->upload(stream);
+        AMLS_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+        hipGraph_t g = nullptr;
+        try {
+            for (auto& f : plan.steps) f(stream);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        AMLS_HIP_CHECK(hipStreamEndCapture(stream, &g));
+        const hipError_t ie = hipGraphInstantiate(&geGraph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        AMLS_HIP_CHECK(ie);
+        geGraphPrec = prec;
+        geGraphGen = problemGen;
+        geGraphBase = A.base();
+        AMLS_HIP_CHECK(hipGraphLaunch(geGraph, stream));
+    }
+    finishGE();
+}
+
+void Ctx::dropGeGraph() {
+    if (geGraph) (void)hipGraphExecDestroy(geGraph);
+    geGraph = nullptr;
+    geGraphPrec = -1;
+    geGraphBase = nullptr;
+}
+
+// reads the pivot flag of the elimination just launched
+void Ctx::finishGE() {
     int e = 0;
     AMLS_HIP_CHECK(hipMemcpyAsync(&e, dErr, sizeof(int), hipMemcpyDeviceToHost, stream));
     sync();
@@ -2445,6 +2500,7 @@ amls_status amls_hip_destroy(amls_ctx* ctx) {
         (void)hipSetDevice(ctx->c.device);
         (void)hipStreamSynchronize(ctx->c.stream);
         (void)hipStreamSynchronize(ctx->c.stream2);
+        ctx->c.dropGeGraph();
     }
     hipStream_t s = ctx->c.stream, s2 = ctx->c.stream2;
     hipEvent_t e1 = ctx->c.evFork, e2 = ctx->c.evJoin;

@@ -53,6 +53,7 @@ public:
     }
     void release();
     size_t capacity() const { return cap_; }
+    const void* base() const { return base_; }
 
 private:
     char* base_ = nullptr;
@@ -200,6 +201,15 @@ struct Ctx {
     int* dErr = nullptr;
     bool geDone = false;
     int gePrec = 0;
+    // GE's launch sequence (every level) as a HIP graph: captured at the
+    // first run for a problem and precision, replayed while the elimination
+    // arena keeps its address; runs with kernel statistics on stay eager
+    hipGraphExec_t geGraph = nullptr;
+    int geGraphPrec = -1;
+    uint64_t problemGen = 0, geGraphGen = 0;
+    const void* geGraphBase = nullptr;
+    void dropGeGraph();
+    void finishGE();
 
     // ---- local spectra
     std::unique_ptr<DevArena> spMem;

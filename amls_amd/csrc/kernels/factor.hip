@@ -150,6 +150,10 @@ __global__ void scatter_kernel(const int64_t* __restrict__ src, const int64_t* _
     else dst[tgt[e]] = v;
 }
 
+__global__ void set_i32_kernel(int* __restrict__ p, int v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
 __global__ void fill_kernel(double* __restrict__ p, int64_t count, double v) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count;
          e += (int64_t)gridDim.x * blockDim.x)
@@ -278,6 +282,10 @@ void launch_scatter(const int64_t* src, const int64_t* tgt, const double* vals, 
 }
 
 void launch_noop(hipStream_t s) { hipLaunchKernelGGL(dev::noop_kernel, dim3(1), dim3(64), 0, s); }
+
+void launch_set_i32(int* p, int v, hipStream_t s) {
+    hipLaunchKernelGGL(dev::set_i32_kernel, dim3(1), dim3(64), 0, s, p, v);
+}
 
 void launch_fill(double* p, int64_t count, double v, hipStream_t s) {
     if (count <= 0) return;

@@ -24,6 +24,7 @@ void launch_extadd(const dev::ExtAddDesc* d, const int4* work, int nwork, hipStr
 void launch_scatter(const int64_t* src, const int64_t* tgt, const double* vals, double* dst,
                     int64_t count, double scale, bool accumulate, hipStream_t s);
 void launch_fill(double* p, int64_t count, double v, hipStream_t s);
+void launch_set_i32(int* p, int v, hipStream_t s);   // *p = v, as a kernel (graph-safe)
 void launch_noop(hipStream_t s);   // empty kernel (dispatch-gap calibration)
 void launch_vecop(const dev::VecDesc* d, int count, hipStream_t s);
 void launch_copy2d(const dev::Copy2dDesc* d, const int4* work, int nwork, hipStream_t s);

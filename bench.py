@@ -181,7 +181,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+              "(the multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
 
     from amls_amd import AmlsOptions, HipContext, cases

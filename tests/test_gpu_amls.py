@@ -217,3 +217,46 @@ def test_shift_matches_oracle(blas1, si):
         r = Ks @ sol.Z - (pb.M @ sol.Z) * sol.theta
         rel = np.linalg.norm(r, axis=0) / np.linalg.norm(Ks @ sol.Z, axis=0)
         assert rel.max() <= 0.05
+
+
+def test_ge_graph_replay_matches_eager():
+    """GE's launch sequence is captured as a HIP graph at its first run for a
+    problem and precision, and replayed on later runs.  Every replay, with
+    other device work and synchronisations between replays, gives the node
+    factors of the eager run (kernel statistics on, no graph) bit for bit; a
+    new precision captures again, and so does a new problem."""
+    pb = small_hex(seed=5, dims=(4, 5, 5))
+    ctx = HipContext(0)
+    ctx.set_problem(pb.K, pb.M, pb.coords, 3, 60)
+    tr = ctx.tree()
+    nodes = sorted({0, len(tr["size"]) // 2, len(tr["size"]) - 1})
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((70, 50))
+    B = rng.standard_normal((50, 40))
+
+    def factors():
+        return [ctx.node_factor(p, int(tr["size"][p])) for p in nodes]
+
+    for prec in ("mixed", "fp64"):
+        ctx.profile(True)
+        ctx.ge(prec)
+        eager = factors()
+        ctx.profile(False)
+        for rep in range(4):          # rep 0 captures, 1..3 replay
+            ctx.ge(prec)
+            for (ld, mh), (ld0, mh0) in zip(factors(), eager):
+                assert np.array_equal(ld, ld0) and np.array_equal(mh, mh0), (prec, rep)
+            C = ctx.gemm(A, B)        # eager work between replays
+            assert np.abs(C - A @ B).max() <= 1e-12 * np.abs(A @ B).max()
+    # a new problem (other values, same sizes) must not replay the old graph
+    pb2 = small_hex(seed=6, dims=(4, 5, 5))
+    ctx.set_problem(pb2.K, pb2.M, pb2.coords, 3, 60)
+    ctx.profile(True)
+    ctx.ge("fp64")
+    eager2 = factors()
+    ctx.profile(False)
+    ctx.ge("fp64")
+    for (ld, mh), (ld0, mh0) in zip(factors(), eager2):
+        assert np.array_equal(ld, ld0) and np.array_equal(mh, mh0)
+    assert not np.array_equal(eager2[0][0], eager[0][0])
+    ctx.close()
