@@ -220,6 +220,20 @@ class HipContext:
         return th, Z
 
     # ---- measurement -----------------------------------------------------
+    def autotune(self) -> np.ndarray:
+        """Measures the GEMM's split-k choices on this device and installs the
+        fastest per shape; returns the table, rows (output tiles, k, split)."""
+        tab = np.zeros(3 * 64, dtype=np.int64)
+        rows = C.c_int64()
+        self._chk(self._L.amls_hip_autotune(self._h, iptr(tab), tab.size, C.byref(rows)))
+        return tab[:3 * rows.value].reshape(-1, 3).copy()
+
+    def set_tuning(self, table):
+        """Installs a split table (rows of output tiles, k, split); an empty
+        table restores the built-in rule."""
+        t = np.ascontiguousarray(np.asarray(table, dtype=np.int64).reshape(-1, 3))
+        self._chk(self._L.amls_hip_set_tuning(self._h, iptr(t) if t.size else None, t.shape[0]))
+
     def profile(self, enable: bool = True):
         self._chk(self._L.amls_hip_profile(self._h, int(enable)))
 

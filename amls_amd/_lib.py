@@ -27,6 +27,7 @@ EXPORTS = [
     "amls_hip_get_reduced", "amls_hip_gemm", "amls_hip_ldlt_solve", "amls_hip_syevx_top",
     "amls_hip_profile", "amls_hip_kernel_stats", "amls_hip_adjoint",
     "amls_hip_set_problem_batch", "amls_hip_batch_info", "amls_hip_solve_batch",
+    "amls_hip_autotune", "amls_hip_set_tuning",
 ]
 
 
@@ -92,6 +93,8 @@ def lib() -> C.CDLL:
                             C.POINTER(AmlsOptionsC), PD, PD, PI64, PD], C.c_int),
         "amls_hip_solve_numeric": ([P, C.POINTER(AmlsOptionsC), PD, PD, PI64, PD], C.c_int),
         "amls_hip_profile": ([P, I32], C.c_int),
+        "amls_hip_autotune": ([P, PI64, I64, PI64], C.c_int),
+        "amls_hip_set_tuning": ([P, PI64, I64], C.c_int),
         "amls_hip_adjoint": ([P, I64, I32, PI64, PD, PD, I64, PD, PD, PD, PD], C.c_int),
         "amls_hip_set_problem_batch": ([P, I64, PI64, PP, PP, PP, PP, PP, PP, PP, I32, I64, D],
                                        C.c_int),

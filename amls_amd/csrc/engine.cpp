@@ -776,13 +776,27 @@ void GemmBatch::add(const GemmDesc& d0) {
     // iteration: k = n) are split over k so the whole device takes part;
     // the pieces add into C atomically (beta must be 0 or 1)
     int nsplit = 1;
-    if (d.k >= 8192 && d.triA == 0 && !(d.A2 && d.k2 > 0) && (d.beta == 0.0 || d.beta == 1.0) &&
-        ntile < 2048) {
-        nsplit = std::min(cdiv(d.k, 2048), std::max(1, cdiv(2048, ntile)));
-        if (nsplit > 1) {
-            d.kchunk = cdiv(cdiv(d.k, nsplit), 32) * 32;
-            nsplit = cdiv(d.k, d.kchunk);
+    const bool splittable = d.triA == 0 && !(d.A2 && d.k2 > 0) && (d.beta == 0.0 || d.beta == 1.0);
+    if (splittable && forceSplit > 0) {
+        nsplit = forceSplit;
+    } else if (splittable && d.k >= 8192 && t_splitTable && !t_splitTable->empty()) {
+        // the measured split of the nearest shape (log distance in tiles and k)
+        double best = 1e300;
+        for (const auto& r : *t_splitTable) {
+            const double dist = std::fabs(std::log2(double(ntile) / r[0])) +
+                                std::fabs(std::log2(double(d.k) / r[1]));
+            if (dist < best) {
+                best = dist;
+                nsplit = static_cast<int>(r[2]);
+            }
         }
+    } else if (splittable && d.k >= 8192 && ntile < 2048) {
+        nsplit = std::min(cdiv(d.k, 2048), std::max(1, cdiv(2048, ntile)));
+    }
+    nsplit = std::max(1, std::min(nsplit, cdiv(d.k, 256)));
+    if (nsplit > 1) {
+        d.kchunk = cdiv(cdiv(d.k, nsplit), 32) * 32;
+        nsplit = cdiv(d.k, d.kchunk);
     }
     descs.push_back(d);
     for (int c = 0; c < nsplit; ++c)
@@ -791,6 +805,86 @@ void GemmBatch::add(const GemmDesc& d0) {
                 if (d.lower && j > i) continue;
                 tiles.push_back({id, i, j, c});
             }
+}
+
+thread_local const SplitTable* t_splitTable = nullptr;
+
+// Times the batched FP64 GEMM's split-k choices on this device, C = A^T B
+// with k-long operands (the form of the Gram products), for output orders
+// 256 .. 2048, full and lower-triangular, k = 8192 and 32768, and keeps the
+// fastest split of each shape as the context's split table.
+void Ctx::autotune() {
+    requireGpu();
+    sync();
+    constexpr int64_t kMax = 32768, mMax = 2048;
+    DevArena mem;
+    mem.plan(kMax * mMax * 8 * 2);
+    mem.plan(mMax * mMax * 8);
+    mem.commit();
+    double* A = mem.take<double>(kMax * mMax);
+    double* B = mem.take<double>(kMax * mMax);
+    double* Cm = mem.take<double>(mMax * mMax);
+    launch_fill(A, kMax * mMax, 1.0, stream);
+    launch_fill(B, kMax * mMax, 0.5, stream);
+    hipEvent_t e0, e1;
+    AMLS_HIP_CHECK(hipEventCreate(&e0));
+    AMLS_HIP_CHECK(hipEventCreate(&e1));
+    SplitTable tab;
+    try {
+        for (const int64_t k : {int64_t(8192), kMax})
+            for (const int64_t m : {256, 512, 1024, 1536, 2048})
+                for (const int lower : {0, 1}) {
+                    float bestMs = 1e30f;
+                    int bestS = 1;
+                    const int tn = cdiv(static_cast<int>(m), 64);
+                    int64_t tiles = 0;
+                    for (int i = 0; i < tn; ++i) tiles += lower ? std::min(i + 1, tn) : tn;
+                    for (const int sp : {1, 2, 3, 4, 6, 8, 12, 16, 24, 32}) {
+                        if (sp > 1 && k / sp < 256) break;
+                        GemmBatch gb;
+                        gb.forceSplit = sp;
+                        GemmDesc g = gdesc();
+                        g.A = A;
+                        g.lda = k;
+                        g.ta = 1;
+                        g.B = B;
+                        g.ldb = k;
+                        g.k = static_cast<int32_t>(k);
+                        g.C = Cm;
+                        g.ldc = m;
+                        g.m = g.n = static_cast<int32_t>(m);
+                        g.lower = lower;
+                        gb.add(g);
+                        Blob blob;
+                        Plan plan(&blob, nullptr);
+                        plan.gemm(gb, false);
+                        plan.run(stream);   // warm-up (and descriptor upload)
+                        float ms = 1e30f;
+                        for (int rep = 0; rep < 3; ++rep) {
+                            AMLS_HIP_CHECK(hipEventRecord(e0, stream));
+                            for (auto& f : plan.steps) f(stream);
+                            AMLS_HIP_CHECK(hipEventRecord(e1, stream));
+                            AMLS_HIP_CHECK(hipEventSynchronize(e1));
+                            float t = 0.f;
+                            AMLS_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+                            ms = std::min(ms, t);
+                        }
+                        if (ms < bestMs) {
+                            bestMs = ms;
+                            bestS = sp;
+                        }
+                    }
+                    tab.push_back({tiles, k, bestS});
+                }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    sync();
+    splitTable = tab;
 }
 
 void Ctx::sync() {
@@ -2426,6 +2520,11 @@ namespace {
 template <class F>
 amls_status guarded(amls_ctx* ctx, F&& f) {
     if (!ctx) return AMLS_E_ARG;
+    // GEMM batches built during the call use this context's split table
+    struct TableScope {
+        explicit TableScope(const amls::SplitTable* t) { amls::t_splitTable = t; }
+        ~TableScope() { amls::t_splitTable = nullptr; }
+    } scope(&ctx->c.splitTable);
     try {
         ctx->c.err.clear();
         f(ctx->c);
@@ -2879,6 +2978,33 @@ amls_status amls_hip_adjoint(amls_ctx* ctx, int64_t nelem, int32_t npe, const in
             AMLS_HIP_CHECK(hipMemcpyAsync(grad_r, dgr, nelem * nev * 8, hipMemcpyDeviceToHost,
                                           c.stream));
         c.sync();
+    });
+}
+
+amls_status amls_hip_autotune(amls_ctx* ctx, int64_t* table, int64_t capacity, int64_t* rows) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (capacity < 0 || (capacity > 0 && !table))
+            throw amls::StatusError(AMLS_E_ARG, "table capacity must be >= 0 with a table");
+        c.autotune();
+        const int64_t nr = static_cast<int64_t>(c.splitTable.size());
+        if (rows) *rows = nr;
+        for (int64_t r = 0; r < std::min(nr, capacity / 3); ++r)
+            for (int j = 0; j < 3; ++j) table[3 * r + j] = c.splitTable[r][j];
+    });
+}
+
+amls_status amls_hip_set_tuning(amls_ctx* ctx, const int64_t* table, int64_t rows) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (rows < 0 || (rows > 0 && !table))
+            throw amls::StatusError(AMLS_E_ARG, "rows must be >= 0 with a table");
+        amls::SplitTable t;
+        for (int64_t r = 0; r < rows; ++r) {
+            const int64_t tiles = table[3 * r], k = table[3 * r + 1], sp = table[3 * r + 2];
+            if (tiles < 1 || k < 1 || sp < 1 || sp > 64)
+                throw amls::StatusError(AMLS_E_ARG, "tuning row needs tiles >= 1, k >= 1, 1 <= split <= 64");
+            t.push_back({tiles, k, sp});
+        }
+        c.splitTable = std::move(t);
     });
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -87,9 +88,17 @@ private:
 
 Blob& fresh(std::unique_ptr<Blob>& b);   // create or clear()
 
+// Split-k table of the batched GEMM: rows (output tiles, k, split), measured
+// on the device by Ctx::autotune or installed by the caller; GemmBatch::add
+// takes the split of the nearest row for a long-k product, or its built-in
+// rule while the calling context's table is empty.
+using SplitTable = std::vector<std::array<int64_t, 3>>;
+extern thread_local const SplitTable* t_splitTable;   // table of the context in the current call
+
 struct GemmBatch {
     std::vector<dev::GemmDesc> descs;
     std::vector<dev::TileRef> tiles;
+    int forceSplit = 0;   // > 0: split every eligible product this many ways (measurement)
     void add(const dev::GemmDesc& d);
     bool empty() const { return tiles.empty(); }
 };
@@ -252,6 +261,9 @@ struct Ctx {
     int64_t lastOutCols = 0;
     double* outBuf = nullptr;
     std::unique_ptr<Blob> siBlob;
+
+    SplitTable splitTable;   // empty: the built-in split-k rule
+    void autotune();
 
     std::map<std::string, double> timing;
     int64_t lastWant = 0;

@@ -161,6 +161,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-table", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="measure the GEMM split-k table on this device first (set-up, untimed)")
     ap.add_argument("--cpu-full", action="store_true",
                     help="also time, on the host, the oracle's AMLS + 2 subspace-iteration "
                          "steps over the whole AMLS basis (the GPU value's algorithm) and the "
@@ -192,6 +194,7 @@ def main():
     nev = 100
     t_gen = time.perf_counter() - t_gen
     ctx = HipContext(0)
+    tuning = ctx.autotune().tolist() if args.autotune else None
     ta = time.perf_counter()
     ctx.set_problem(pb.K, pb.M, pb.coords, 6, 500)
     t_analysis = time.perf_counter() - ta
@@ -259,6 +262,7 @@ def main():
         "max_rel_err": err_m,
         "max_residual": res_m,
         "analysis_s": round(t_analysis, 4),
+        "split_table": tuning,
         "profiled_s_per_solve": round(t_prof, 6),
         "fp64_gpu": out_fp64,
         "roofline": rf,

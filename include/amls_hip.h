@@ -246,6 +246,20 @@ amls_status amls_hip_adjoint(amls_ctx* ctx, int64_t nelem, int32_t npe, const in
                              const double* theta, const double* Z, double* grad_s,
                              double* grad_r);
 
+/*
+ * Split-k tuning of the batched GEMM (no counterpart in the reference, whose
+ * GEMMs are OpenBLAS calls).  amls_hip_autotune times the split choices of
+ * long-k products C = A^T B on this device (output orders 256 .. 2048, full
+ * and lower-triangular, k = 8192 and 32768) and installs the fastest split
+ * per shape as the context's table; table (capacity int64 entries) receives
+ * rows of (output tiles, k, split) and *rows their number.  Later products
+ * with k >= 8192 take the split of the nearest row.  amls_hip_set_tuning
+ * installs a saved table (rows = 0 restores the built-in rule); it works on
+ * host-only contexts too.
+ */
+amls_status amls_hip_autotune(amls_ctx* ctx, int64_t* table, int64_t capacity, int64_t* rows);
+amls_status amls_hip_set_tuning(amls_ctx* ctx, const int64_t* table, int64_t rows);
+
 /* ---- measurement -------------------------------------------------------
  * enable != 0 brackets every launch with HIP events on the stream it runs
  * on (grouped by kernel class) and clears the counters; enable = 0 stops the

@@ -105,3 +105,17 @@ def test_malformed_inputs_are_status_codes():
     o.thresholdFactor = 2.0
     assert L.amls_hip_solve_numeric(ctx._h, ctypes.byref(o), None, None, None, None) == 6
     ctx.close()
+
+
+def test_set_tuning_validates_rows():
+    """A split table (rows of output tiles, k, split) installs on a host-only
+    context; an empty table restores the built-in rule; malformed rows are
+    AMLS_E_ARG."""
+    ctx = HipContext(-1)
+    ctx.set_tuning([[300, 26460, 4], [1024, 8192, 2]])
+    ctx.set_tuning([])
+    for bad in ([[0, 8192, 2]], [[10, 0, 2]], [[10, 8192, 0]], [[10, 8192, 65]]):
+        with pytest.raises(AmlsError) as e:
+            ctx.set_tuning(bad)
+        assert e.value.status == 1
+    ctx.close()

@@ -130,3 +130,26 @@ def test_syevx_decoupled_blocks(gpu_ctx):
     assert np.abs(lam - ref).max() <= 1e-13 * 100 * nrm
     assert np.abs(V.T @ V - np.eye(100)).max() <= 1e-10
     assert np.linalg.norm(A @ V - V * lam, axis=0).max() <= 1e-11 * 100 * nrm
+
+
+def test_autotune_split_table():
+    """The autotuner times the GEMM's split-k choices per shape and installs
+    the fastest; long-k products then take the split of the nearest row.
+    Products under the tuned table, and under single-row tables forcing 1
+    and 32 pieces, match numpy at 1e-12."""
+    from amls_amd import HipContext
+    ctx = HipContext(0)
+    tab = ctx.autotune()
+    assert tab.shape == (20, 3)
+    assert (tab[:, 0] >= 1).all() and set(tab[:, 1]) == {8192, 32768}
+    assert ((tab[:, 2] >= 1) & (tab[:, 2] <= 32)).all()
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal((9000, 300))
+    B = rng.standard_normal((9000, 200))
+    ref = A.T @ B
+    for table in (tab, [[15, 9000, 1]], [[15, 9000, 32]]):
+        ctx.set_tuning(table)
+        C = ctx.gemm(A, B, transA=True)
+        assert np.abs(C - ref).max() <= 1e-12 * np.abs(ref).max()
+    ctx.set_tuning([])
+    ctx.close()
