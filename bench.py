@@ -108,20 +108,33 @@ def roofline_overall(stats, steps, t_solve):
     """Whole-solve roofline: T_roof = sum over kernel classes of
     max(flops / peak, bytes / HBM peak), from the library's algorithmic counts
     per class (FP32 GEMMs at the FP32 MFMA peak, all other flops at the FP64
-    peak), against the measured time of one solve.  Per class: the ideal and
-    the HIP-event milliseconds per solve."""
+    peak), against the measured time of one solve.  Per class (the ten with
+    the most event time): the ideal and the HIP-event milliseconds per solve,
+    and the achieved rate against the bound that sets the ideal (TF/s against
+    the MFMA peak, or GB/s against HBM)."""
     t_roof = 0.0
     stages = {}
     for name, st in stats.items():
-        if name == "launch_overhead" or st["launches"] <= 0:
+        if name == "launch_overhead" or st["launches"] <= 0 or st["ms"] <= 0:
             continue
         peak = PEAK["gemm_f32"][0] if name == "gemm_f32" else PEAK["gemm_f64"][0]
-        t = max(st["flops"] / (peak * 1e12), st["bytes"] / (HBM_PEAK_GBS * 1e9)) / steps
+        t_fl = st["flops"] / (peak * 1e12)
+        t_by = st["bytes"] / (HBM_PEAK_GBS * 1e9)
+        t = max(t_fl, t_by) / steps
         t_roof += t
-        stages[name] = [round(1e3 * t, 4), round(st["ms"] / steps, 3)]
-    top = dict(sorted(stages.items(), key=lambda kv: -kv[1][1])[:8])
+        sec = st["ms"] * 1e-3
+        if st["flops"] <= 0 and st["bytes"] <= 0:  # no algorithmic count for this class
+            ach, unit, frac = None, None, None
+        elif t_fl >= t_by:                        # compute-bound class: TF/s against its peak
+            ach, unit, frac = st["flops"] / sec / 1e12, "TFLOP/s", st["flops"] / sec / 1e12 / peak
+        else:                                     # memory-bound class: GB/s against HBM
+            ach, unit, frac = st["bytes"] / sec / 1e9, "GB/s", st["bytes"] / sec / 1e9 / HBM_PEAK_GBS
+        stages[name] = {"roof_ms": round(1e3 * t, 4), "event_ms": round(st["ms"] / steps, 3),
+                        "achieved": None if ach is None else round(ach, 2), "unit": unit,
+                        "frac": None if frac is None else round(frac, 4)}
+    top = dict(sorted(stages.items(), key=lambda kv: -kv[1]["event_ms"])[:10])
     return {"t_roof_ms": round(1e3 * t_roof, 3), "t_measured_ms": round(1e3 * t_solve, 3),
-            "frac": round(t_roof / t_solve, 5), "stages_roof_vs_event_ms": top}
+            "frac": round(t_roof / t_solve, 5), "stages": top}
 
 
 def cpu_full(pb, tree, ref, nev, eta, si_steps, O):
