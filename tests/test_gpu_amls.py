@@ -181,7 +181,7 @@ def test_subspace_iteration_matches_oracle(blas1, prec, qmode):
     assert sol.theta.shape == (nev,)
     assert np.all(sol.theta >= exact * (1 - 1e-12))
     rel = np.abs(sol.theta - t_ref) / t_ref
-    assert rel.max() <= (1e-9 if prec == "fp64" else 1e-6)
+    assert rel.max() <= (1e-9 if prec == "fp64" else 1e-7)
     assert O.rel_err(sol.theta, exact).max() < O.rel_err(ref.theta, exact).max()
     assert np.abs(np.einsum("ij,ij->j", sol.Z, pb.M @ sol.Z) - 1).max() <= 1e-10
 
@@ -260,3 +260,31 @@ def test_ge_graph_replay_matches_eager():
         assert np.array_equal(ld, ld0) and np.array_equal(mh, mh0)
     assert not np.array_equal(eager2[0][0], eager[0][0])
     ctx.close()
+
+
+def test_c2_subspace_iteration_matches_oracle():
+    """The benchmark workload at full size (27-node hexahedra, 10^3 elements,
+    n = 26,460, lowest 100, eta = 4, two subspace-iteration steps over the
+    whole AMLS basis): the FP64 and mixed-precision GPU eigenvalues agree with
+    the oracle's FP64 AMLS + subspace iteration on the same tree within 1e-7
+    relative (about 1e-10 measured), and both reach 1e-6 against the
+    committed FP64 ground-truth fixture."""
+    import os
+    pb = cases.case_hex27_10()
+    tree, _ = tree_of(pb.K, pb.M, pb.coords, 6, 500)
+    nev = 100
+    ref = O.amls_solve(pb.K, pb.M, tree, nev=nev, thresholdFactor=4.0, recoverVectors=False)
+    _, Q = ref.cond.solve_projected(ref.p)
+    z = ref.cond.backtransform(Q)
+    X = np.zeros_like(z)
+    X[tree.perm] = z
+    t_or, _ = O.subspace_iteration(pb.K, pb.M, ref.ge, X, 2, nev, tree.perm)
+    exact = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "hex27_10_lowest100.txt"))
+    solver = AmlsSolver(0)
+    for prec in ("fp64", "mixed"):
+        sol = solver.solve(Pencil(pb.K, pb.M, pb.coords),
+                           AmlsOptions(nev=nev, nLevels=6, leafSize=500, thresholdFactor=4.0,
+                                       precision=prec, siSteps=2, siVectors=10**6, verbosity=0))
+        assert sol.p == ref.p
+        assert (np.abs(sol.theta - t_or) / t_or).max() <= 1e-7, prec
+        assert O.rel_err(sol.theta, exact).max() <= 1e-6, prec
