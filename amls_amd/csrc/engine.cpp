@@ -1186,15 +1186,16 @@ void Ctx::runGE(int prec) {
     dDinv = A.take<double>(vo[N]);
     dErr = A.take<int>(1);
     // the launch sequence depends only on the tree, the precision and the
-    // arena's address (the values come from the device): replay the graph
-    // captured for them if there is one
-    if (geGraph && geGraphPrec == prec && geGraphGen == problemGen && geGraphBase == A.base() &&
-        !kstats.enabled) {
+    // arena's address (the values come from the device): the first run for
+    // them is eager, the second captures a graph, later runs replay it
+    const bool sameKey = geGraphPrec == prec && geGraphGen == problemGen && geGraphBase == A.base();
+    if (geGraph && sameKey && !kstats.enabled) {
         AMLS_HIP_CHECK(hipGraphLaunch(geGraph, stream));
         finishGE();
         return;
     }
-    dropGeGraph();
+    if (geGraph) (void)hipGraphExecDestroy(geGraph);
+    geGraph = nullptr;
     fresh(geBlob);
     Plan plan(geBlob.get(), &kstats);
     {
@@ -1422,7 +1423,7 @@ void Ctx::runGE(int prec) {
         plan.gemm(gFm, f32);
         plan.gemm(gG22, f32);
     }
-    if (kstats.enabled) {
+    if (kstats.enabled || !sameKey) {
         plan.run(stream);
     } else {
         // descriptors go up first, outside the graph; the graph holds the
@@ -1441,11 +1442,11 @@ void Ctx::runGE(int prec) {
         const hipError_t ie = hipGraphInstantiate(&geGraph, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         AMLS_HIP_CHECK(ie);
-        geGraphPrec = prec;
-        geGraphGen = problemGen;
-        geGraphBase = A.base();
         AMLS_HIP_CHECK(hipGraphLaunch(geGraph, stream));
     }
+    geGraphPrec = prec;
+    geGraphGen = problemGen;
+    geGraphBase = A.base();
     finishGE();
 }
 

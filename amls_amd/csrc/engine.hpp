@@ -210,9 +210,10 @@ struct Ctx {
     int* dErr = nullptr;
     bool geDone = false;
     int gePrec = 0;
-    // GE's launch sequence (every level) as a HIP graph: captured at the
-    // first run for a problem and precision, replayed while the elimination
-    // arena keeps its address; runs with kernel statistics on stay eager
+    // GE's launch sequence (every level) as a HIP graph: the first run for a
+    // problem, precision and arena address is eager, the second captures the
+    // graph, later ones replay it; runs with kernel statistics on stay eager.
+    // geGraphPrec / Gen / Base: the key of the last run.
     hipGraphExec_t geGraph = nullptr;
     int geGraphPrec = -1;
     uint64_t problemGen = 0, geGraphGen = 0;

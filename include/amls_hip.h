@@ -128,9 +128,10 @@ amls_status amls_hip_tree_get(const amls_ctx* ctx, int64_t* treeId, int64_t* off
  * the M congruence update, passed to the parent by extend-add.  Replaces
  * GaussianElimination::run/runLevel/runDag (src/amls/gaussian_elimination.cpp
  * :458-866).  precision: 0 = FP64, 1 = mixed (other values: AMLS_E_ARG).
- * The launch sequence is captured as a HIP graph at the first call for a
- * problem and precision and replayed by later calls (K and M values are read
- * from device memory at every call); with profiling on, launches are eager.
+ * The first call for a problem and precision launches eagerly, the second
+ * captures the launch sequence as a HIP graph and later calls replay it (K
+ * and M values are read from device memory at every call); with profiling
+ * on, launches are eager.
  */
 amls_status amls_hip_ge(amls_ctx* ctx, int32_t precision);
 

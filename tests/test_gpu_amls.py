@@ -220,11 +220,12 @@ def test_shift_matches_oracle(blas1, si):
 
 
 def test_ge_graph_replay_matches_eager():
-    """GE's launch sequence is captured as a HIP graph at its first run for a
-    problem and precision, and replayed on later runs.  Every replay, with
-    other device work and synchronisations between replays, gives the node
-    factors of the eager run (kernel statistics on, no graph) bit for bit; a
-    new precision captures again, and so does a new problem."""
+    """GE runs eagerly the first time for a problem and precision, captures
+    its launch sequence as a HIP graph the second time and replays it after
+    that.  The captured run and every replay, with other device work and
+    synchronisations between replays, give the node factors of the eager run
+    (kernel statistics on) bit for bit; a new precision captures again, and
+    so does a new problem."""
     pb = small_hex(seed=5, dims=(4, 5, 5))
     ctx = HipContext(0)
     ctx.set_problem(pb.K, pb.M, pb.coords, 3, 60)
@@ -242,7 +243,7 @@ def test_ge_graph_replay_matches_eager():
         ctx.ge(prec)
         eager = factors()
         ctx.profile(False)
-        for rep in range(4):          # rep 0 captures, 1..3 replay
+        for rep in range(4):          # rep 0 captures (the eager run set the key), 1..3 replay
             ctx.ge(prec)
             for (ld, mh), (ld0, mh0) in zip(factors(), eager):
                 assert np.array_equal(ld, ld0) and np.array_equal(mh, mh0), (prec, rep)
