@@ -197,20 +197,35 @@ __global__ void __launch_bounds__(256) copy2d_kernel(const Copy2dDesc* __restric
     }
     const int i = w.y * 64 + (t & 63);
     if (i >= d.rows) return;
-    for (int q = 0; q < 16; ++q) {
-        const int j = w.z * 64 + (t >> 6) + 4 * q;
-        if (j >= d.cols) break;
-        if (d.mode == 3) {   // fill; pad = 1 keeps to the lower triangle
-            if (!d.pad || i >= j) d.dst[i + (int64_t)j * d.ldd] = d.alpha;
-            continue;
+    const int jb = w.z * 64 + (t >> 6);
+    if (d.mode == 3) {   // fill; pad = 1 keeps to the lower triangle
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int j = jb + 4 * q;
+            if (j < d.cols && (!d.pad || i >= j)) d.dst[i + (int64_t)j * d.ldd] = d.alpha;
         }
-        double v;
-        if (d.mode == 2) v = i >= j ? d.src[i + (int64_t)j * d.lds] : d.src[j + (int64_t)i * d.lds];
-        else v = d.src[i + (int64_t)j * d.lds];
-        double s = d.alpha;
-        if (d.rs) s *= d.rs[i];
-        if (d.cs) s *= d.cs[j];
-        d.dst[i + (int64_t)j * d.ldd] = s * v;
+        return;
+    }
+    // all sixteen loads of the thread go out before its first store (src and
+    // dst may be one block: an element is then read and written by the same
+    // thread, or, for the symmetric completion, rewritten with its own value)
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int j = jb + 4 * q;
+        v[q] = 0.0;
+        if (j < d.cols)
+            v[q] = (d.mode == 2 && i < j) ? d.src[j + (int64_t)i * d.lds] : d.src[i + (int64_t)j * d.lds];
+    }
+    const double sr = d.rs ? d.alpha * d.rs[i] : d.alpha;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int j = jb + 4 * q;
+        if (j < d.cols) {
+            double sc = sr;
+            if (d.cs) sc *= d.cs[j];
+            d.dst[i + (int64_t)j * d.ldd] = sc * v[q];
+        }
     }
 }
 
@@ -222,11 +237,18 @@ __global__ void gather_rows_kernel(const GatherDesc* __restrict__ descs, const i
     const int r = w.y * 64 + (t & 63);
     if (r >= d.rows) return;
     const int64_t sr = d.rows_idx[r];
+    const int cb = w.z * 64 + (t >> 6);
+    const int64_t rs = d.scatter ? r : sr, rd = d.scatter ? sr : r;   // source / target row
+    double v[16];   // loads first, then stores
+#pragma unroll
     for (int q = 0; q < 16; ++q) {
-        const int c = w.z * 64 + (t >> 6) + 4 * q;
-        if (c >= d.cols) break;
-        if (d.scatter) d.dst[sr + (int64_t)c * d.ldd] = d.src[r + (int64_t)c * d.lds];
-        else d.dst[r + (int64_t)c * d.ldd] = d.src[sr + (int64_t)c * d.lds];
+        const int c = cb + 4 * q;
+        v[q] = c < d.cols ? d.src[rs + (int64_t)c * d.lds] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int c = cb + 4 * q;
+        if (c < d.cols) d.dst[rd + (int64_t)c * d.ldd] = v[q];
     }
 }
 
