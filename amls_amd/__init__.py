@@ -84,9 +84,10 @@ class Pencil:
 
 
 def _csc(A):
-    A = sp.csc_matrix(A)
-    if not A.has_sorted_indices:
-        A = A.sorted_indices()
+    A = sp.csc_matrix(A, copy=True)
+    # one entry per (row, col), rows ascending (the library sums duplicates
+    # too; doing it here keeps the arrays it receives small)
+    A.sum_duplicates()
     return (np.ascontiguousarray(A.indptr, dtype=np.int64),
             np.ascontiguousarray(A.indices, dtype=np.int64),
             np.ascontiguousarray(A.data, dtype=np.float64))
@@ -267,8 +268,10 @@ class HipContext:
     def adjoint(self, conn, Ke0, Me0, theta, Z=None):
         """Element sensitivities (grad_s, grad_r), each nelem x nev:
         d theta_j / d s_e = x_je^T Ke0 x_je and d theta_j / d r_e =
-        -theta_j x_je^T Me0 x_je.  Z None uses the eigenvectors of the last
-        solve kept in device memory."""
+        -(theta_j - shift) x_je^T Me0 x_je, theta as returned by the solve
+        of the problem set on this context (shift: its K <- K + shift M).
+        Z None uses the eigenvectors of the last solve kept in device
+        memory."""
         conn = np.ascontiguousarray(conn, dtype=np.int64)
         nelem, npe = conn.shape
         K0 = np.asfortranarray(Ke0, dtype=np.float64)

@@ -582,6 +582,41 @@ bool buildBsr3Sorted(int64_t n, const int64_t* rp, const int64_t* ci, const doub
     return true;
 }
 
+// Appends the nn columns of one CSC matrix (rows shifted by `base`) to the
+// block-diagonal CSC (cp, ri, vx), each column with ascending rows and its
+// duplicate entries summed.  Columns that are already strictly ascending are
+// copied as they are.
+void appendCanonical(int64_t nn, int64_t base, const int64_t* pp, const int64_t* pi,
+                     const double* px, std::vector<int64_t>& cp, std::vector<int64_t>& ri,
+                     std::vector<double>& vx) {
+    std::vector<std::pair<int64_t, double>> col;
+    for (int64_t c = 0; c < nn; ++c) {
+        const int64_t e0 = pp[c], e1 = pp[c + 1];
+        bool sorted = true;
+        for (int64_t e = e0 + 1; e < e1 && sorted; ++e) sorted = pi[e] > pi[e - 1];
+        if (sorted) {
+            for (int64_t e = e0; e < e1; ++e) {
+                ri.push_back(pi[e] + base);
+                vx.push_back(px[e]);
+            }
+        } else {
+            col.clear();
+            for (int64_t e = e0; e < e1; ++e) col.push_back({pi[e], px[e]});
+            std::stable_sort(col.begin(), col.end(),
+                             [](const auto& a, const auto& b) { return a.first < b.first; });
+            for (size_t t = 0; t < col.size();) {
+                double v = 0.0;
+                size_t u = t;
+                for (; u < col.size() && col[u].first == col[t].first; ++u) v += col[u].second;
+                ri.push_back(col[t].first + base);
+                vx.push_back(v);
+                t = u;
+            }
+        }
+        cp.push_back(static_cast<int64_t>(ri.size()));
+    }
+}
+
 // Bisection, inverse iteration and back-transformation for descriptors with
 // lam/Y/work/k set (work: 6 n k doubles per matrix, see stein_kernel).
 void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
@@ -938,19 +973,11 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
             if (P.Ki[e] < 0 || P.Ki[e] >= nn) throw StatusError(AMLS_E_ARG, "K row index out of range");
         for (int64_t e = 0; e < P.Mp[nn]; ++e)
             if (P.Mi[e] < 0 || P.Mi[e] >= nn) throw StatusError(AMLS_E_ARG, "M row index out of range");
-        const int64_t k0 = static_cast<int64_t>(Ki.size()), m0 = static_cast<int64_t>(Mi.size());
-        for (int64_t c = 0; c < nn; ++c) {
-            Kp.push_back(k0 + P.Kp[c + 1]);
-            Mp.push_back(m0 + P.Mp[c + 1]);
-        }
-        for (int64_t e = 0; e < P.Kp[nn]; ++e) {
-            Ki.push_back(P.Ki[e] + base);
-            Kx.push_back(P.Kx[e]);
-        }
-        for (int64_t e = 0; e < P.Mp[nn]; ++e) {
-            Mi.push_back(P.Mi[e] + base);
-            Mx.push_back(P.Mx[e]);
-        }
+        // canonical columns: rows ascending, duplicate (row, col) entries
+        // summed (the reference's Eigen matrices hold each entry once; the
+        // front assembly maps one slot per entry)
+        appendCanonical(nn, base, P.Kp, P.Ki, P.Kx, Kp, Ki, Kx);
+        appendCanonical(nn, base, P.Mp, P.Mi, P.Mx, Mp, Mi, Mx);
         const Graph gs = graphFromCsc(nn, P.Kp, P.Ki, P.Mp, P.Mi);
         const PartitionResult ps = recursiveBisection(gs, P.coords, nLevels, leafSize);
         parts.push_back(analyze(ps, gs));
@@ -2471,12 +2498,14 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     // un-permute and M-normalise into the output block
     // the subspace-iteration result has its own output block (dXout may
     // point into the recovery arena of an earlier, narrower solve)
-    if (!outMem || lastOutCols < nev) {
+    // sized by elements: a later problem of the same nev but larger n needs
+    // a larger block
+    if (!outMem || outElems < n * nev) {
         outMem.reset(new DevArena());
         outMem->plan(n * nev * 8);
         outMem->commit();
         outBuf = outMem->take<double>(n * nev);
-        lastOutCols = nev;
+        outElems = n * nev;
     }
     dXout = outBuf;
     int64_t* perm = dperm;
@@ -2748,6 +2777,8 @@ amls_status amls_hip_backtransform(amls_ctx* ctx, double* Z) {
 amls_status amls_hip_subspace_iter(amls_ctx* ctx, int32_t steps, int64_t q, int64_t nev,
                                    double* theta, double* Z) {
     return guarded(ctx, [&](amls::Ctx& c) {
+        if (steps <= 0) throw amls::StatusError(AMLS_E_ARG, "steps must be positive");
+        if (nev <= 0) throw amls::StatusError(AMLS_E_ARG, "nev must be positive");
         std::vector<double> th;
         c.subspaceIter(steps, q, nev, th, false);
         c.lastWant = nev;
@@ -2963,7 +2994,12 @@ amls_status amls_hip_adjoint(amls_ctx* ctx, int64_t nelem, int32_t npe, const in
         up(dconn, conn, nelem * npe * 8);
         up(dK0, Ke0, npe * npe * 8);
         up(dM0, Me0, npe * npe * 8);
-        up(dth, theta, nev * 8);
+        // a shifted solve returns theta = lambda + shift for the pencil
+        // (K + shift M, M); d theta / d r_e = x^T (shift Me0 - theta Me0) x
+        // = -lambda x^T Me0 x, so the kernel takes lambda = theta - shift
+        std::vector<double> lam(theta, theta + nev);
+        for (double& v : lam) v -= c.shift;
+        up(dth, lam.data(), nev * 8);
         if (Z) up(dZ, Z, nz * 8);
         c.kstats.run("adjoint", 4.0 * (npe * npe + npe) * nelem * nev,
                      8.0 * (nelem * npe * (1 + nev) + 2 * nelem * nev), c.stream, [&] {
@@ -3102,8 +3138,10 @@ amls_status amls_hip_gemm(amls_ctx* ctx, int32_t precision, int32_t transA, int3
         using amls::StatusError;
         if (precision != 0 && precision != 1)
             throw StatusError(AMLS_E_ARG, "precision must be 0 (FP64) or 1 (FP32 MFMA)");
-        c.requireGpu();
         if (m < 0 || n < 0 || k < 0) throw StatusError(AMLS_E_ARG, "negative size");
+        if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX)
+            throw StatusError(AMLS_E_ARG, "gemm: m, n and k must fit in 32 bits");
+        c.requireGpu();
         if ((m > 0 && n > 0 && !C) || (m > 0 && n > 0 && k > 0 && (!A || !B)))
             throw StatusError(AMLS_E_ARG, "null matrix argument");
         const int64_t ar = transA ? k : m, ac = transA ? m : k;
@@ -3151,9 +3189,11 @@ amls_status amls_hip_ldlt_solve(amls_ctx* ctx, int64_t n, const double* A, int64
                                 const double* B, double* LD, double* X) {
     return guarded(ctx, [&](amls::Ctx& c) {
         using amls::StatusError;
-        c.requireGpu();
         if (n <= 0 || nrhs < 0) throw StatusError(AMLS_E_ARG, "bad size");
+        if (n > INT32_MAX || nrhs > INT32_MAX)
+            throw StatusError(AMLS_E_ARG, "ldlt_solve: n and nrhs must fit in 32 bits");
         if (!A || (nrhs > 0 && !B)) throw StatusError(AMLS_E_ARG, "null matrix argument");
+        c.requireGpu();
         amls::DevArena mem;
         mem.plan(n * n * 8);
         mem.plan(n * 64 * 8);
@@ -3248,9 +3288,10 @@ amls_status amls_hip_syevx_top(amls_ctx* ctx, int64_t n, const double* A, int64_
                                double* V) {
     return guarded(ctx, [&](amls::Ctx& c) {
         using amls::StatusError;
-        c.requireGpu();
         if (n <= 0 || k < 0 || k > n) throw StatusError(AMLS_E_ARG, "bad size");
+        if (n > INT32_MAX) throw StatusError(AMLS_E_ARG, "syevx_top: n must fit in 32 bits");
         if (!A) throw StatusError(AMLS_E_ARG, "null matrix argument");
+        c.requireGpu();
         amls::DevArena mem;
         mem.plan(n * n * 8 * 2);
         mem.plan(n * 8 * 8);

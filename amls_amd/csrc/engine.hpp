@@ -259,7 +259,7 @@ struct Ctx {
 
     // ---- subspace iteration
     std::unique_ptr<DevArena> siMem, outMem;
-    int64_t lastOutCols = 0;
+    int64_t outElems = 0;        // doubles outBuf holds (n x nev of the solve that sized it)
     double* outBuf = nullptr;
     std::unique_ptr<Blob> siBlob;
 

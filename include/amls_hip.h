@@ -231,9 +231,12 @@ amls_status amls_hip_solve_batch(amls_ctx* ctx, int64_t count, const int64_t* n,
  * Eigenvalue sensitivities with respect to per-element stiffness and density
  * scales, for pencils assembled as K = sum_e s_e P_e^T Ke0 P_e and
  * M = sum_e r_e P_e^T Me0 P_e from one reference element pair:
- *   grad_s(e, j) = d theta_j / d s_e =          x_je^T Ke0 x_je
- *   grad_r(e, j) = d theta_j / d r_e = -theta_j x_je^T Me0 x_je
- * for M-normalised eigenvectors x_j gathered on element e's dofs.
+ *   grad_s(e, j) = d theta_j / d s_e =                    x_je^T Ke0 x_je
+ *   grad_r(e, j) = d theta_j / d r_e = -(theta_j - shift) x_je^T Me0 x_je
+ * for M-normalised eigenvectors x_j gathered on element e's dofs.  theta is
+ * what the solver returned for the problem set on the context: with a shift
+ * (K <- K + shift M) it is lambda + shift, and the shift term's own
+ * dependence on r_e is included.
  * conn: nelem x npe row-major dof indices in the original ordering (-1 for a
  * constrained dof); Ke0, Me0: npe x npe column-major, symmetric; theta: nev.
  * Z: n x nev column-major, or null to use the eigenvectors of the last solve

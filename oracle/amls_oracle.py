@@ -27,12 +27,27 @@ libamls_hip's own nested dissection):
 * back-transformation, un-permutation and M-normalisation --
   src/amls/subspace_condensation.cpp:263-319, src/amls/solver.cpp:296-311.
 
+* the local dense / shift-invert-Lanczos dispatch (n >= 200 and fewer than
+  0.15 n^2 non-zeros in K-hat; index path only for il = 1 and k < 4n/5;
+  dense fallback when SIL returns fewer than k pairs or none) --
+  src/amls/eigenpair_solver.hpp:32-82, and SIL itself (sigma rule, ncv
+  schedule, M-inner-product Lanczos with two full modified-Gram-Schmidt
+  passes, residual estimate, pick rule, Ritz vectors) --
+  src/amls/shift_invert_lanczos.hpp:32-203.
+
 Deviations, all value-preserving: the sparse Eigen LDL^T of leaves and the
 dense Eigen LDLT of separators are both done as a dense Cholesky (same
-factorisation of an SPD matrix, different rounding); the SIL local solver
-(src/amls/shift_invert_lanczos.hpp) is replaced by the dense solver it
-approximates (it is only selected for nodes with density < 0.15, which the
-hexahedral workloads do not have).
+factorisation of an SPD matrix, different rounding).  The SIL start vector
+is std::srand(12345) + Eigen's Vec::Random as glibc and Eigen <= 3.4 produce
+it (glibc_rand restates glibc's generator; the reference's own platform
+library may draw other numbers, so its unconverged Ritz values are pinned
+only for that generator).
+
+SIL's residual estimate beta_{m-1} s_{m-1,j} reads beta_{m-1}, which
+iterate() never stores (it leaves the loop first), so it is always zero:
+solveIndex stops at its first ncv, solveInterval stops unless no Ritz value
+or every Ritz value lies below hi, and the pick keeps every Ritz value in
+(lo, hi].  This module does the same, literally.
 
 Parity pins for this module live in tests/test_oracle.py: the closed-form
 spectra and structural properties of the reference's unit and regression
@@ -243,6 +258,213 @@ def local_eig(Kh: np.ndarray, Mh: np.ndarray, kind: str, value: float):
     return w, V
 
 
+# --------------------------------------------------------------------------
+# Shift-invert Lanczos local solver (src/amls/shift_invert_lanczos.hpp) and
+# the dense / SIL dispatch of EigenpairSolver (src/amls/eigenpair_solver.hpp)
+# --------------------------------------------------------------------------
+
+RAND_MAX = 2147483647
+
+
+def glibc_rand(seed: int, count: int) -> np.ndarray:
+    """The first `count` values of glibc's rand() after srand(seed): the
+    TYPE_3 additive feedback generator (r_i = r_{i-31} + r_{i-3} mod 2^32
+    after a Park-Miller seeding of 31 words and 310 discarded outputs,
+    output r_i >> 1).  This is what std::srand / std::rand give the
+    reference's Lanczos start vector on Linux (shift_invert_lanczos.hpp
+    :49-50)."""
+    r = np.zeros(344 + count, dtype=np.int64)
+    r[0] = seed if seed != 0 else 1
+    for i in range(1, 31):
+        hi, lo = divmod(int(r[i - 1]), 127773)
+        w = 16807 * lo - 2836 * hi
+        r[i] = w + 2147483647 if w < 0 else w
+    r[31:34] = r[0:3]
+    for i in range(34, 344 + count):
+        r[i] = (r[i - 31] + r[i - 3]) & 0xFFFFFFFF
+    return r[344:] >> 1
+
+
+def sil_start_vector(n: int, seed: int = 12345) -> np.ndarray:
+    """Vec::Random(n) after std::srand(seed) (shift_invert_lanczos.hpp:49-50):
+    Eigen (<= 3.4) draws x + (y - x) rand() / RAND_MAX with (x, y) = (-1, 1),
+    coefficient 0 first."""
+    r = glibc_rand(seed, n).astype(np.float64)
+    return -1.0 + (2.0 * r) / float(RAND_MAX)
+
+
+class ShiftInvertLanczos:
+    """sil::ShiftInvertLanczos (shift_invert_lanczos.hpp:32-122): M-inner-
+    product Lanczos on C = (K - sigma M)^-1 M with two passes of full
+    (modified Gram-Schmidt) reorthogonalisation per step."""
+
+    def __init__(self, Kd: np.ndarray, Md: np.ndarray, sigma: float):
+        self.n = Kd.shape[0]
+        self.shift = sigma
+        # sparseView(1e-14) (:37-38) keeps every non-zero entry
+        self.M = sp.csr_matrix(Md)
+        A = Kd - sigma * Md
+        try:                              # EigenSimplicialLDLTSolver: D > 0
+            self.fac = sla.cho_factor(A, lower=True, check_finite=False)
+        except np.linalg.LinAlgError as e:
+            raise np.linalg.LinAlgError("SIL: K - sigma M is not SPD") from e
+        self.m = 0
+
+    def _applyC(self, x: np.ndarray) -> np.ndarray:
+        return sla.cho_solve(self.fac, self.M @ x, check_finite=False)
+
+    def iterate(self, ncv: int, seed: int = 12345) -> int:
+        """iterate (:44-85): at most ncv steps; T's eigenpairs ascending."""
+        n, M = self.n, self.M
+        ncv = min(ncv, n)
+        alpha = np.zeros(ncv)
+        beta = np.zeros(ncv)
+        q = sil_start_vector(n, seed)
+        q = q / math.sqrt(max(float(q @ (M @ q)), 1e-300))
+        Q = [q]
+        m = 0
+        for j in range(ncv):
+            w = self._applyC(Q[j])
+            alpha[j] = float(Q[j] @ (M @ w))
+            for _ in range(2):            # :61-64
+                for l in range(j + 1):
+                    w = w - float(Q[l] @ (M @ w)) * Q[l]
+            bj = math.sqrt(max(float(w @ (M @ w)), 0.0))
+            m = j + 1
+            if bj < 1e-13 or m == ncv:    # :67-68, before beta[j] is stored
+                break
+            beta[j] = bj
+            Q.append(w / bj)
+        self.m = m
+        self.alpha, self.beta = alpha, beta
+        self.Q = np.stack(Q[:m], axis=1)
+        T = np.diag(alpha[:m]) + np.diag(beta[:m - 1], 1) + np.diag(beta[:m - 1], -1)
+        self.nu, self.S = np.linalg.eigh(T)   # SelfAdjointEigenSolver: ascending
+        return m
+
+    def lam(self, j: int) -> float:
+        """lambda (:90): 1 / nu + sigma."""
+        return 1.0 / float(self.nu[j]) + self.shift
+
+    def res(self, j: int) -> float:
+        """res (:92-95): |beta_{m-1} s_{m-1, j}|.  iterate() leaves the
+        loop before storing beta_{m-1}, so this reads the zero it was
+        initialised with: every convergence test below passes."""
+        bm = self.beta[self.m - 1] if self.m > 1 else 0.0
+        return abs(bm * float(self.S[self.m - 1, j]))
+
+    def ritz_vector(self, j: int) -> np.ndarray:
+        """ritzVector (:97-102): x = Q s_j, M-normalised with Q."""
+        return self.Q @ self.S[:, j]
+
+
+def _sil_sigma(Kd: np.ndarray, lo: Optional[float] = None) -> float:
+    """sigma of solveInterval (:128-135) / solveIndex (:175-178): just left of
+    min(0, lo) by max(1e-10, 1e-9 max_j |K_jj|)."""
+    scale = float(np.abs(np.diag(Kd)).max()) if Kd.size else 0.0
+    eps = max(1e-10, 1e-9 * (scale if scale > 0.0 else 1.0))
+    if lo is None:
+        return -eps
+    lo_sane = lo if lo > -1e100 else 0.0
+    return min(0.0, lo_sane) - eps
+
+
+def sil_solve_interval(Kd, Md, lo: float, hi: float, tol: float = 1e-9):
+    """sil::solveInterval (:126-168): Ritz pairs with lambda in (lo, hi],
+    ncv 96 doubling to min(n, 1536).  Returned in the reference's order:
+    ascending nu, i.e. descending lambda."""
+    n = Kd.shape[0]
+    lz = ShiftInvertLanczos(Kd, Md, _sil_sigma(Kd, lo))
+    ncv = min(n, 96)
+    cap = min(n, 1536)
+    while True:
+        m = lz.iterate(ncv)
+        cnt = sum(1 for j in range(m) if lz.lam(j) <= hi)
+        if cnt < m and cnt > 0 and lz.res(cnt - 1) <= tol * max(1.0, float(lz.nu[m - 1])):
+            break
+        if ncv >= cap or m < ncv:
+            break
+        ncv = min(cap, ncv * 2)
+    pick = []
+    for j in range(lz.m):
+        lam = lz.lam(j)
+        if lam > hi or lam <= lo:
+            continue
+        if lz.res(j) > 100 * tol * max(1.0, float(lz.nu[lz.m - 1])):
+            continue
+        pick.append(j)
+    w = np.array([lz.lam(j) for j in pick])
+    V = np.stack([lz.ritz_vector(j) for j in pick], axis=1) if pick else np.zeros((n, 0))
+    return w, V, lz
+
+
+def sil_solve_index(Kd, Md, il: int, iu: int, tol: float = 1e-10):
+    """sil::solveIndex (:171-203): the smallest k = iu - il + 1 pairs
+    (il = 1), ncv = min(n, max(4 k + 16, 48)) doubling to min(n, 1536)
+    while the top k Ritz values are unconverged; ascending lambda."""
+    n = Kd.shape[0]
+    k = iu - il + 1
+    lz = ShiftInvertLanczos(Kd, Md, _sil_sigma(Kd))
+    cap = min(n, 1536)
+    ncv = min(n, max(4 * k + 16, 48))
+    while True:
+        m = lz.iterate(ncv)
+        want = min(k, m)
+        ok = all(lz.res(m - 1 - t) <= tol * max(1.0, float(lz.nu[m - 1])) for t in range(want))
+        if ok or ncv >= cap or m < ncv:
+            break
+        ncv = min(cap, ncv * 2)
+    m = lz.m
+    kk = min(k, m)
+    js = [m - 1 - t for t in range(kk)]
+    w = np.array([lz.lam(j) for j in js])
+    V = np.stack([lz.ritz_vector(j) for j in js], axis=1) if js else np.zeros((n, 0))
+    return w, V, lz
+
+
+def use_sil(Kd: np.ndarray, sil_mode: int = 0) -> bool:
+    """EigenpairSolver::useSil_ (eigenpair_solver.hpp:68-82): n >= 200 and
+    fewer than 0.15 n^2 non-zero entries in the dense K block."""
+    if sil_mode > 0:
+        return True
+    if sil_mode < 0:
+        return False
+    n = Kd.shape[0]
+    if n < 200:
+        return False
+    return float(np.count_nonzero(Kd)) < 0.15 * float(n) * float(n)
+
+
+def local_eig_dispatch(Kh: np.ndarray, Mh: np.ndarray, kind: str, value: float,
+                       sil_mode: int = 0):
+    """EigenpairSolver::solveLocalIndex / solveLocalInterval / solveLocalAll
+    (eigenpair_solver.hpp:32-63) as called by spectralLevel with the
+    truncation's hint (subspace_condensation.cpp:67-80,
+    spectral_truncation.hpp:50-82).  Returns (w, V, path) with path 'sil' or
+    'dense'."""
+    n = Kh.shape[0]
+    if kind == "fixed" and int(value) > 0:
+        il, iu = 1, int(value)                      # FixedCount indexHint
+        if il == 1 and use_sil(Kh, sil_mode):
+            k = iu - il + 1
+            if k < (n * 4) // 5:
+                w, V, _ = sil_solve_index(Kh, Mh, il, iu)
+                if w.size == k:
+                    return w, V, "sil"
+        w, V = local_eig(Kh, Mh, "fixed", iu)
+        return w, V, "dense"
+    if kind == "threshold":
+        lo, hi = -1e300, float(value)               # Threshold intervalHint
+        if use_sil(Kh, sil_mode):
+            w, V, _ = sil_solve_interval(Kh, Mh, lo, hi)
+            if w.size:
+                return w, V, "sil"
+        w, V = local_eig(Kh, Mh, "threshold", hi)
+        return w, V, "dense"
+    w, V = local_eig(Kh, Mh, "fixed", 0)            # no hint: solveLocalAll
+    return w, V, "dense"
+
+
 class Condensation:
     """SubspaceCondensation restated (src/amls/subspace_condensation.cpp)."""
 
@@ -250,21 +472,25 @@ class Condensation:
         self.ge = ge
 
     def run(self, kind: str, value: float, pinned_k: Optional[np.ndarray] = None,
-            plant: Optional[str] = None):
+            plant: Optional[str] = None, sil_mode: int = 0):
         """plant: "drop_mid" skips the mid update M_a,kk -= K_a,i Khat^-1 M_i,kk
-        (a deliberate mistake for the tests of the parity checks)."""
+        (a deliberate mistake for the tests of the parity checks).
+        sil_mode: EigenpairSolver::silMode (eigenpair_solver.hpp:28-29), 0 =
+        the density switch, 1 = always SIL, -1 = always dense."""
         ge, tr = self.ge, self.ge.tree
         N = tr.N
         self.k = np.zeros(N, dtype=np.int64)
         self.mu: List[np.ndarray] = [np.zeros(0)] * N
         self.Z: List[np.ndarray] = [np.zeros((int(tr.size[p]), 0)) for p in range(N)]
+        self.path = ["dense"] * N
         # spectralLevel (:45-100): local spectra + select
         for p in range(N):
             if pinned_k is not None:
                 w, V = local_eig(ge.Khat[p], ge.Mhat[p], "fixed", max(int(pinned_k[p]), 0)) \
                     if pinned_k[p] > 0 else (np.zeros(0), np.zeros((int(tr.size[p]), 0)))
             else:
-                w, V = local_eig(ge.Khat[p], ge.Mhat[p], kind, value)
+                w, V, self.path[p] = local_eig_dispatch(ge.Khat[p], ge.Mhat[p], kind, value,
+                                                        sil_mode)
             if kind == "threshold" and pinned_k is None:
                 kk = 0
                 while kk < w.size and w[kk] <= value:
@@ -366,8 +592,9 @@ class Condensation:
 def amls_solve(K: sp.spmatrix, M: sp.spmatrix, tree: Tree, *, nev: int = 100,
                thresholdFactor: float = 2.0, fixedKeep: int = 0, shift: float = 0.0,
                twoPass: bool = True, recoverVectors: bool = True,
-               pinned_k: Optional[np.ndarray] = None) -> OracleResult:
-    """AmlsSolver::solve (src/amls/solver.cpp:100-319) on the given tree."""
+               pinned_k: Optional[np.ndarray] = None, sil_mode: int = 0) -> OracleResult:
+    """AmlsSolver::solve (src/amls/solver.cpp:100-319) on the given tree.
+    sil_mode as in Condensation.run (0: the reference's density switch)."""
     t0 = time.perf_counter()
     if nev <= 0:
         raise ValueError("nev must be positive")
@@ -388,19 +615,20 @@ def amls_solve(K: sp.spmatrix, M: sp.spmatrix, tree: Tree, *, nev: int = 100,
     if pinned_k is not None:
         cond.run("fixed", 0, pinned_k=pinned_k)
     elif fixedKeep > 0:
-        cond.run("fixed", fixedKeep)
+        cond.run("fixed", fixedKeep, sil_mode=sil_mode)
     else:
         if not twoPass:
             raise ValueError("twoPass=False requires fixedKeep > 0")
         N = tree.N
         k0 = max(4, (3 * nev + N - 1) // N)
-        cond.run("fixed", k0)
+        cond.run("fixed", k0, sil_mode=sil_mode)
         p0 = cond.p
+        cond.pathA = list(cond.path)
         wA = cond.solve_projected(min(nev, cond.p), vectors=False)
         if wA.size < nev:
             raise RuntimeError("pass A reduced problem smaller than nev")
         lam_cut = thresholdFactor * float(wA[nev - 1])
-        cond.run("threshold", lam_cut)
+        cond.run("threshold", lam_cut, sil_mode=sil_mode)
     t3 = time.perf_counter()
     want = min(nev, cond.p)
     res = cond.solve_projected(want, vectors=recoverVectors)
@@ -514,16 +742,18 @@ def residual(K, M, theta: float, z: np.ndarray) -> float:
 # --------------------------------------------------------------------------
 
 def eig_adjoint(conn: np.ndarray, Ke0: np.ndarray, Me0: np.ndarray, theta: np.ndarray,
-                Z: np.ndarray):
+                Z: np.ndarray, shift: float = 0.0):
     """d theta_j / d s_e = x_je^T Ke0 x_je, d theta_j / d r_e =
-    -theta_j x_je^T Me0 x_je for K = sum_e s_e P_e^T Ke0 P_e,
+    -(theta_j - shift) x_je^T Me0 x_je for K = sum_e s_e P_e^T Ke0 P_e,
     M = sum_e r_e P_e^T Me0 P_e and M-normalised x_j (the first-order
-    perturbation of a simple eigenvalue, d theta = x^T (dK - theta dM) x).
-    Returns (grad_s, grad_r), nelem x nev."""
+    perturbation of a simple eigenvalue of (K + shift M, M),
+    d theta = x^T (dK + shift dM - theta dM) x).  Returns (grad_s, grad_r),
+    nelem x nev."""
     conn = np.asarray(conn)
     nev = len(theta)
     Zp = np.vstack([Z[:, :nev], np.zeros((1, nev))])      # row -1 -> zeros
     X = Zp[np.where(conn >= 0, conn, Z.shape[0])]           # nelem x npe x nev
     gs = np.einsum("eij,ik,ekj->ej", X, Ke0, X)
-    gr = -np.einsum("eij,ik,ekj->ej", X, Me0, X) * np.asarray(theta)[None, :]
+    lam = np.asarray(theta) - shift
+    gr = -np.einsum("eij,ik,ekj->ej", X, Me0, X) * lam[None, :]
     return gs, gr

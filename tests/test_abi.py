@@ -119,3 +119,68 @@ def test_set_tuning_validates_rows():
             ctx.set_tuning(bad)
         assert e.value.status == 1
     ctx.close()
+
+
+def test_operator_sizes_beyond_int32_are_rejected():
+    """Sizes the kernels address with 32-bit counts come back as AMLS_E_ARG
+    instead of wrapping (checked before the device is needed, so a host-only
+    context sees them), as do non-positive subspace-iteration counts."""
+    L = _lib.lib()
+    ctx = HipContext(-1)
+    big = 2 ** 31
+    assert L.amls_hip_gemm(ctx._h, 0, 0, 0, big, 4, 4, 1.0, None, big, None, 4, 0.0,
+                           None, big) == 1
+    assert "32 bits" in L.amls_hip_last_error(ctx._h).decode()
+    assert L.amls_hip_gemm(ctx._h, 0, 0, 0, 4, 4, big, 1.0, None, 4, None, big, 0.0,
+                           None, 4) == 1
+    assert L.amls_hip_ldlt_solve(ctx._h, big, None, 1, None, None, None) == 1
+    assert "32 bits" in L.amls_hip_last_error(ctx._h).decode()
+    assert L.amls_hip_syevx_top(ctx._h, big, None, 1, None, None) == 1
+    assert L.amls_hip_subspace_iter(ctx._h, 2, 10, 0, None, None) == 1
+    assert "nev" in L.amls_hip_last_error(ctx._h).decode()
+    assert L.amls_hip_subspace_iter(ctx._h, 2, 10, -5, None, None) == 1
+    assert L.amls_hip_subspace_iter(ctx._h, 0, 10, 4, None, None) == 1
+    ctx.close()
+
+
+def test_duplicate_entries_are_summed():
+    """A CSC with a split (row, col) entry and unsorted rows describes the
+    same matrix as its canonical form: the library's analysis sees the same
+    pattern, and the wrapper hands over the summed entries."""
+    import scipy.sparse as sp
+    from amls_amd import _csc
+    K, M = cases.grid_graph_2d(6, 5)
+    Kc = sp.csc_matrix(K)
+    rows, cols = Kc.nonzero()
+    vals = np.asarray(Kc[rows, cols]).ravel()
+    # split every diagonal entry in two halves, reverse each column's rows
+    d = rows == cols
+    R = np.concatenate([rows, rows[d]])
+    Cc = np.concatenate([cols, cols[d]])
+    V = np.concatenate([np.where(d, 0.5 * vals, vals), 0.5 * vals[d]])
+    order = np.lexsort((-R, Cc))
+    colptr = np.zeros(K.shape[0] + 1, dtype=np.int64)
+    np.add.at(colptr, Cc[order] + 1, 1)
+    colptr = np.cumsum(colptr)
+    ri = np.ascontiguousarray(R[order], dtype=np.int64)
+    rv = np.ascontiguousarray(V[order], dtype=np.float64)
+    # (data, indices, indptr) keeps the duplicates as given
+    Ks = sp.csc_matrix((rv.copy(), ri.copy(), colptr.copy()), shape=K.shape)
+    assert Ks.nnz == Kc.nnz + K.shape[0]
+    p, i, x = _csc(Ks)
+    assert np.array_equal(p, Kc.indptr) and np.array_equal(i, Kc.indices)
+    assert np.allclose(x, Kc.data, rtol=0, atol=1e-15)
+    # raw arrays with duplicates straight through the C-ABI
+    from amls_amd._lib import dptr, iptr
+    mp, mi, mx = _csc(M)
+    L = _lib.lib()
+    trees = []
+    for (cp, rr, vv) in ((colptr, ri, rv), _csc(K)):
+        ctx = HipContext(-1)
+        st = L.amls_hip_set_problem(ctx._h, K.shape[0], iptr(cp), iptr(rr), dptr(vv), iptr(mp),
+                                    iptr(mi), dptr(mx), dptr(None), 2, 8, 0.0)
+        assert st == 0
+        trees.append(ctx.tree())
+        ctx.close()
+    for key in ("treeId", "offset", "size", "parent", "bsize", "perm"):
+        assert np.array_equal(trees[0][key], trees[1][key]), key

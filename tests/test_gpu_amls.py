@@ -289,3 +289,63 @@ def test_c2_subspace_iteration_matches_oracle():
         assert sol.p == ref.p
         assert (np.abs(sol.theta - t_or) / t_or).max() <= 1e-7, prec
         assert O.rel_err(sol.theta, exact).max() <= 1e-6, prec
+
+
+def test_context_reuse_smaller_then_larger_problem(blas1):
+    """One AmlsSolver (one context) solves a small problem with subspace
+    iteration and then a larger one with the same nev: the output block is
+    sized by n x nev, so the second solve matches a fresh context's result."""
+    nev = 6
+    opt = AmlsOptions(nev=nev, nLevels=3, leafSize=40, siSteps=2, siVectors=16, verbosity=0)
+    small = small_hex(seed=51, dims=(3, 3, 4))
+    large = small_hex(seed=52, dims=(5, 5, 6))
+    s = AmlsSolver(0)
+    s.solve(Pencil(small.K, small.M, small.coords), opt)
+    got = s.solve(Pencil(large.K, large.M, large.coords), opt)
+    ref = AmlsSolver(0).solve(Pencil(large.K, large.M, large.coords), opt)
+    assert got.Z.shape == (large.n, nev)
+    assert np.abs(got.theta - ref.theta).max() <= 1e-12 * ref.theta.max()
+    sg = np.sign(np.einsum("ij,ij->j", got.Z, large.M @ ref.Z))
+    assert np.abs(got.Z * sg - ref.Z).max() <= 1e-9 * np.abs(ref.Z).max()
+
+
+def test_duplicate_entries_summed_on_device_path(blas1):
+    """Raw CSC arrays with every diagonal entry split in two and rows in
+    descending order, straight through amls_hip_solve: the same eigenvalues
+    as the canonical matrices (the library sums duplicates before building
+    the front assembly maps and the sparse operators)."""
+    import ctypes as C
+    import scipy.sparse as sp
+    from amls_amd import _csc, _lib
+    from amls_amd._lib import dptr, iptr
+    pb = small_hex(seed=53, dims=(4, 4, 4))
+    nev = 6
+
+    def split(A):
+        A = sp.csc_matrix(A)
+        r, c = A.nonzero()
+        v = np.asarray(A[r, c]).ravel()
+        d = r == c
+        R = np.concatenate([r, r[d]])
+        Cc = np.concatenate([c, c[d]])
+        V = np.concatenate([np.where(d, 0.5 * v, v), 0.5 * v[d]])
+        o = np.lexsort((-R, Cc))
+        cp = np.zeros(A.shape[0] + 1, dtype=np.int64)
+        np.add.at(cp, Cc[o] + 1, 1)
+        return (np.cumsum(cp), np.ascontiguousarray(R[o], dtype=np.int64),
+                np.ascontiguousarray(V[o], dtype=np.float64))
+
+    opt = AmlsOptions(nev=nev, nLevels=3, leafSize=40, siSteps=1, siVectors=12, verbosity=0)
+    ref = AmlsSolver(0).solve(Pencil(pb.K, pb.M, pb.coords), opt)
+    s = AmlsSolver(0)
+    (kp, ki, kx), (mp, mi, mx) = split(pb.K), split(pb.M)
+    th = np.zeros(nev)
+    p = C.c_int64()
+    oc = opt.to_c()
+    cc = np.ascontiguousarray(pb.coords, dtype=np.float64)
+    st = _lib.lib().amls_hip_solve(s.ctx._h, pb.n, iptr(kp), iptr(ki), dptr(kx), iptr(mp),
+                                   iptr(mi), dptr(mx), dptr(cc), C.byref(oc), dptr(th), None,
+                                   C.byref(p), None)
+    s.ctx._chk(st)
+    assert p.value == ref.p
+    assert np.abs(th - ref.theta).max() <= 1e-12 * ref.theta.max()
