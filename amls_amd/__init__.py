@@ -30,7 +30,9 @@ PRECISIONS = {"fp64": 0, "mixed": 1}
 @dataclass
 class AmlsOptions:
     """AmlsOptions of src/amls/solver.hpp:18-30 (same names and defaults),
-    plus the GPU forms: precision 'fp64' | 'mixed', siSteps, siVectors."""
+    plus the GPU forms: precision 'fp64' | 'mixed', siSteps, siVectors, and
+    silMode (EigenpairSolver::silMode, src/amls/eigenpair_solver.hpp:28-29:
+    0 density switch, 1 always shift-invert Lanczos, -1 always dense)."""
     nLevels: int = 6
     leafSize: int = 500
     thresholdFactor: float = 2.0
@@ -45,6 +47,7 @@ class AmlsOptions:
     precision: str = "fp64"
     siSteps: int = 0
     siVectors: int = 0
+    silMode: int = 0
 
     def to_c(self) -> AmlsOptionsC:
         o = AmlsOptionsC()
@@ -62,6 +65,7 @@ class AmlsOptions:
         o.precision = PRECISIONS[self.precision]
         o.siSteps = self.siSteps
         o.siVectors = self.siVectors
+        o.silMode = self.silMode
         return o
 
 
@@ -191,6 +195,22 @@ class HipContext:
 
     def spectral_prepare(self):
         self._chk(self._L.amls_hip_spectral_prepare(self._h))
+
+    def set_sil_mode(self, mode: int):
+        """Local solver switch of the following spectral_prepare / spectral
+        calls: 0 density switch, 1 always shift-invert Lanczos, -1 never."""
+        self._chk(self._L.amls_hip_set_sil_mode(self._h, int(mode)))
+
+    def sil_info(self):
+        """(path, steps) per post-order node after spectral(): path 0 dense,
+        1 shift-invert Lanczos, 2 Lanczos fell back to dense; steps = the
+        Lanczos steps whose Ritz pairs were kept."""
+        N = self.tree()["size"].shape[0]
+        path = np.zeros(N, dtype=np.int32)
+        steps = np.zeros(N, dtype=np.int64)
+        self._chk(self._L.amls_hip_get_sil(self._h, path.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           iptr(steps)))
+        return path, steps
 
     def spectral(self, kind: str, value: float) -> np.ndarray:
         N = self.tree()["size"].shape[0]

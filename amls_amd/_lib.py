@@ -27,7 +27,7 @@ EXPORTS = [
     "amls_hip_get_reduced", "amls_hip_gemm", "amls_hip_ldlt_solve", "amls_hip_syevx_top",
     "amls_hip_profile", "amls_hip_kernel_stats", "amls_hip_adjoint",
     "amls_hip_set_problem_batch", "amls_hip_batch_info", "amls_hip_solve_batch",
-    "amls_hip_autotune", "amls_hip_set_tuning",
+    "amls_hip_autotune", "amls_hip_set_tuning", "amls_hip_set_sil_mode", "amls_hip_get_sil",
 ]
 
 
@@ -47,6 +47,7 @@ class AmlsOptionsC(C.Structure):
         ("precision", C.c_int32),
         ("siSteps", C.c_int32),
         ("siVectors", C.c_int64),
+        ("silMode", C.c_int32),
     ]
 
 
@@ -108,6 +109,8 @@ def lib() -> C.CDLL:
         "amls_hip_gemm": ([P, I32, I32, I32, I64, I64, I64, D, PD, I64, PD, I64, D, PD, I64], C.c_int),
         "amls_hip_ldlt_solve": ([P, I64, PD, I64, PD, PD, PD], C.c_int),
         "amls_hip_syevx_top": ([P, I64, PD, I64, PD, PD], C.c_int),
+        "amls_hip_set_sil_mode": ([P, I32], C.c_int),
+        "amls_hip_get_sil": ([P, C.POINTER(C.c_int32), PI64], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

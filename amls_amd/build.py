@@ -21,7 +21,9 @@ SOURCES = [
     CSRC / "kernels" / "eig.hip",
     CSRC / "kernels" / "sparse.hip",
     CSRC / "kernels" / "adjoint.hip",
+    CSRC / "kernels" / "lanczos.hip",
     CSRC / "host" / "tree.cpp",
+    CSRC / "sil.cpp",
     CSRC / "engine.cpp",
 ]
 

@@ -8,6 +8,7 @@
 // eigenvalue lambda_hat, the per-node counts k_i of pass B, the reduced
 // eigenvalues, and the error word of the factorisations).
 #include "engine.hpp"
+#include "plan.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -20,292 +21,7 @@
 
 namespace amls {
 
-using dev::Copy2dDesc;
-using dev::EigDesc;
-using dev::ExtAddDesc;
-using dev::FrontDesc;
-using dev::GatherDesc;
-using dev::GemmDesc;
-using dev::TileRef;
-using dev::TrsmDesc;
-using dev::TrsmWork;
-using dev::VecDesc;
-
 namespace {
-
-inline int cdiv(int64_t a, int64_t b) { return static_cast<int>((a + b - 1) / b); }
-
-using Clock = std::chrono::steady_clock;
-inline double secs(Clock::time_point a, Clock::time_point b) {
-    return std::chrono::duration<double>(b - a).count();
-}
-
-GemmDesc gdesc() {
-    GemmDesc d;
-    std::memset(&d, 0, sizeof(d));
-    d.alpha1 = 1.0;
-    d.alpha2 = 1.0;
-    return d;
-}
-
-Copy2dDesc cdesc(const double* src, int64_t lds, double* dst, int64_t ldd, int64_t rows,
-                 int64_t cols, int mode, const double* rs = nullptr, const double* cs = nullptr,
-                 double alpha = 1.0) {
-    Copy2dDesc d;
-    std::memset(&d, 0, sizeof(d));
-    d.src = src;
-    d.dst = dst;
-    d.lds = lds;
-    d.ldd = ldd;
-    d.rows = static_cast<int32_t>(rows);
-    d.cols = static_cast<int32_t>(cols);
-    d.mode = mode;
-    d.rs = rs;
-    d.cs = cs;
-    d.alpha = alpha;
-    return d;
-}
-
-struct Copy2dBatch {
-    std::vector<Copy2dDesc> descs;
-    std::vector<int4> work;
-    void add(const Copy2dDesc& d) {
-        if (d.rows <= 0 || d.cols <= 0) return;
-        const int id = static_cast<int>(descs.size());
-        descs.push_back(d);
-        for (int i = 0; i < cdiv(d.rows, 64); ++i)
-            for (int j = 0; j < cdiv(d.cols, 64); ++j) work.push_back(make_int4(id, i, j, 0));
-    }
-};
-
-struct ExtBatch {
-    std::vector<ExtAddDesc> descs;
-    std::vector<int4> work;
-    void add(const ExtAddDesc& d) {
-        if (d.rows <= 0 || d.cols <= 0) return;
-        const int id = static_cast<int>(descs.size());
-        descs.push_back(d);
-        for (int i = 0; i < cdiv(d.rows, 64); ++i)
-            for (int j = 0; j < cdiv(d.cols, 64); ++j) {
-                if (d.symmetric && j > i) continue;
-                work.push_back(make_int4(id, i, j, 0));
-            }
-    }
-};
-
-struct TrsmBatch {
-    std::vector<TrsmDesc> descs;
-    void add(const double* L, int64_t ldl, double* B, int64_t ldb, int64_t n, int64_t nrhs,
-             int trans) {
-        if (n <= 0 || nrhs <= 0) return;
-        // the diagonal-block kernel addresses 64 columns of B with 32-bit
-        // byte offsets
-        if (ldb > 4000000) throw StatusError(AMLS_E_ARG, "leading dimension above 4e6 elements");
-        TrsmDesc d;
-        std::memset(&d, 0, sizeof(d));
-        d.L = L;
-        d.B = B;
-        d.ldl = ldl;
-        d.ldb = ldb;
-        d.n = static_cast<int32_t>(n);
-        d.nrhs = static_cast<int32_t>(nrhs);
-        d.trans = trans;
-        descs.push_back(d);
-    }
-};
-
-struct GatherBatch {
-    std::vector<GatherDesc> descs;
-    std::vector<int4> work;
-    void add(const GatherDesc& d) {
-        if (d.rows <= 0 || d.cols <= 0) return;
-        const int id = static_cast<int>(descs.size());
-        descs.push_back(d);
-        for (int i = 0; i < cdiv(d.rows, 64); ++i)
-            for (int j = 0; j < cdiv(d.cols, 64); ++j) work.push_back(make_int4(id, i, j, 0));
-    }
-};
-
-// A sequence of launches whose descriptor arrays live in one blob.
-// algorithmic work of a GEMM batch: 2 m n k flops per product and
-// (m k + k n + 2 m n) words, counting only the written triangle for `lower`
-void gemmWork(const GemmBatch& b, bool fp32, double& flops, double& bytes) {
-    flops = bytes = 0.0;
-    const double w = fp32 ? 4.0 : 8.0;
-    for (const auto& d : b.descs) {
-        const double m = d.m, n = d.n;
-        double mn = m * n;
-        if (d.lower && m >= n) mn = m * n - n * (n - 1) / 2;
-        const double k1 = d.A ? d.k : 0, k2 = d.A2 ? d.k2 : 0;
-        // a triangular op(A) (m = k) meets only half of k on average
-        const double k1eff = d.triA ? 0.5 * (k1 + 1) : k1;
-        flops += 2.0 * mn * (k1eff + k2);
-        bytes += ((d.triA ? 0.5 : 1.0) * m * k1 + m * k2 + (k1 + k2) * n + 2.0 * mn) * w;
-    }
-}
-
-struct Plan {
-    Blob* blob;
-    KernelStats* st;
-    std::vector<std::function<void(hipStream_t)>> steps;
-    Plan(Blob* b, KernelStats* k) : blob(b), st(k) {}
-
-    void timed(const std::string& name, double flops, double bytes,
-               std::function<void(hipStream_t)> f) {
-        KernelStats* k = st;
-        if (!k) {
-            steps.push_back(std::move(f));
-            return;
-        }
-        steps.push_back([=](hipStream_t s) { k->run(name, flops, bytes, s, [&] { f(s); }); });
-    }
-
-    void gemm(const GemmBatch& b, bool fp32) {
-        if (b.empty()) return;
-        // split-k products accumulate into C: clear it first when beta = 0
-        Copy2dBatch clr;
-        for (const auto& d : b.descs)
-            if (d.kchunk > 0 && d.beta == 0.0) {
-                Copy2dDesc c = cdesc(nullptr, 0, d.C, d.ldc, d.m, d.n, 3, nullptr, nullptr, 0.0);
-                c.pad = d.lower;
-                clr.add(c);
-            }
-        copy(clr);
-        const size_t od = blob->add(b.descs), ot = blob->add(b.tiles);
-        const int nt = static_cast<int>(b.tiles.size());
-        Blob* bl = blob;
-        double fl, by;
-        gemmWork(b, fp32, fl, by);
-        timed(fp32 ? "gemm_f32" : "gemm_f64", fl, by, [=](hipStream_t s) {
-            launch_gemm(bl->at<GemmDesc>(od), bl->at<TileRef>(ot), nt, fp32, s);
-        });
-    }
-    void copy(const Copy2dBatch& b) {
-        if (b.work.empty()) return;
-        const size_t od = blob->add(b.descs), ow = blob->add(b.work);
-        const int nw = static_cast<int>(b.work.size());
-        Blob* bl = blob;
-        steps.push_back([=](hipStream_t s) {
-            launch_copy2d(bl->at<Copy2dDesc>(od), bl->at<int4>(ow), nw, s);
-        });
-    }
-    void ext(const ExtBatch& b) {
-        if (b.work.empty()) return;
-        const size_t od = blob->add(b.descs), ow = blob->add(b.work);
-        const int nw = static_cast<int>(b.work.size());
-        Blob* bl = blob;
-        steps.push_back([=](hipStream_t s) {
-            launch_extadd(bl->at<ExtAddDesc>(od), bl->at<int4>(ow), nw, s);
-        });
-    }
-    // Blocked right-looking unit-triangular solve of every descriptor: per
-    // 64-row block (in solve order) one launch of diagonal-block solves, then
-    // one GEMM batch applying the solved block to the rows still to come.
-    void trsm(const TrsmBatch& b) {
-        if (b.descs.empty()) return;
-        const size_t od = blob->add(b.descs);
-        Blob* bl = blob;
-        int maxnb = 0;
-        for (const auto& d : b.descs) maxnb = std::max(maxnb, cdiv(d.n, 64));
-        double fl = 0, by = 0;
-        for (const auto& d : b.descs) {
-            const double n = d.n, r = d.nrhs;
-            fl += n * (n - 1) * r;
-            by += (n * (n + 1) / 2 + 2 * n * r) * 8.0;
-        }
-        const int nd = static_cast<int>(b.descs.size());
-        for (int s = 0; s < maxnb; ++s) {
-            std::vector<TrsmWork> work[2];   // by direction: forward, transposed
-            GemmBatch gb;
-            for (int id = 0; id < nd; ++id) {
-                const TrsmDesc& d = b.descs[id];
-                const int nb = cdiv(d.n, 64);
-                if (s >= nb) continue;
-                const int rb = d.trans ? nb - 1 - s : s;
-                const int r0 = rb * 64, mv = std::min(64, d.n - r0);
-                // a lower-triangular RHS is zero above each column block's
-                // diagonal: only column blocks <= rb are nonzero in row block rb
-                const int ncb = d.rhsLower && !d.trans ? std::min(cdiv(d.nrhs, 64), rb + 1)
-                                                       : cdiv(d.nrhs, 64);
-                for (int c = 0; c < cdiv(ncb, 4); ++c) work[d.trans ? 1 : 0].push_back({id, c, rb, ncb});
-                const int ncols = std::min(d.nrhs, ncb * 64);
-                GemmDesc g = gdesc();
-                if (!d.trans && r0 + mv < d.n) {
-                    // B(r0+mv:, :) -= L(r0+mv:, r0:r0+mv) X(r0:r0+mv, :)
-                    g.A = d.L + (r0 + mv) + static_cast<int64_t>(r0) * d.ldl;
-                    g.lda = d.ldl;
-                    g.B = d.B + r0;
-                    g.ldb = d.ldb;
-                    g.C = d.B + (r0 + mv);
-                    g.ldc = d.ldb;
-                    g.m = d.n - (r0 + mv);
-                } else if (d.trans && r0 > 0) {
-                    // B(:r0, :) -= L(r0:r0+mv, :r0)^T X(r0:r0+mv, :)
-                    g.A = d.L + r0;
-                    g.lda = d.ldl;
-                    g.ta = 1;
-                    g.B = d.B + r0;
-                    g.ldb = d.ldb;
-                    g.C = d.B;
-                    g.ldc = d.ldb;
-                    g.m = r0;
-                } else {
-                    continue;
-                }
-                g.n = ncols;
-                g.k = mv;
-                g.alpha1 = -1.0;
-                g.beta = 1.0;
-                gb.add(g);
-            }
-            const size_t ow0 = blob->add(work[0]), ow1 = blob->add(work[1]);
-            const int nw0 = static_cast<int>(work[0].size()), nw1 = static_cast<int>(work[1].size());
-            const bool first = s == 0;
-            timed("trsm", first ? fl : 0.0, first ? by : 0.0, [=](hipStream_t st) {
-                launch_trsm_diag(bl->at<TrsmDesc>(od), bl->at<TrsmWork>(ow0), nw0, false, st);
-                launch_trsm_diag(bl->at<TrsmDesc>(od), bl->at<TrsmWork>(ow1), nw1, true, st);
-            });
-            gemm(gb, false);
-        }
-    }
-    void gather(const GatherBatch& b) {
-        if (b.work.empty()) return;
-        const size_t od = blob->add(b.descs), ow = blob->add(b.work);
-        const int nw = static_cast<int>(b.work.size());
-        Blob* bl = blob;
-        steps.push_back([=](hipStream_t s) {
-            launch_gather(bl->at<GatherDesc>(od), bl->at<int4>(ow), nw, s);
-        });
-    }
-    void vec(const std::vector<VecDesc>& v) {
-        if (v.empty()) return;
-        const size_t od = blob->add(v);
-        const int nv = static_cast<int>(v.size());
-        Blob* bl = blob;
-        steps.push_back([=](hipStream_t s) { launch_vecop(bl->at<VecDesc>(od), nv, s); });
-    }
-    void fn(std::function<void(hipStream_t)> f) { steps.push_back(std::move(f)); }
-    // side's steps run on s2 from this point of the plan on, concurrently
-    // with the steps added here until join(); side shares this plan's blob
-    void fork(Plan& side, hipStream_t s2, hipEvent_t ef, hipEvent_t ej) {
-        auto st = std::make_shared<std::vector<std::function<void(hipStream_t)>>>(std::move(side.steps));
-        side.steps.clear();
-        steps.push_back([=](hipStream_t s) {
-            AMLS_HIP_CHECK(hipEventRecord(ef, s));
-            AMLS_HIP_CHECK(hipStreamWaitEvent(s2, ef, 0));
-            for (auto& f : *st) f(s2);
-            AMLS_HIP_CHECK(hipEventRecord(ej, s2));
-        });
-    }
-    void join(hipEvent_t ej) {
-        steps.push_back([=](hipStream_t s) { AMLS_HIP_CHECK(hipStreamWaitEvent(s, ej, 0)); });
-    }
-    void run(hipStream_t s) {
-        blob->upload(s);
-        for (auto& f : steps) f(s);
-        AMLS_HIP_CHECK(hipGetLastError());
-    }
-};
 
 // Tridiagonalisation schedule for a batch of matrices: launch i processes
 // every matrix with n_m > i, one work item per 64 x 64 tile of the trailing
@@ -340,10 +56,12 @@ struct SytrdPlan {
     }
 };
 
+}  // namespace
+
 // soff[m]: per-matrix offset in units of its n (sv holds 2 n and sp 3 n
 // doubles per matrix from 2 soff / 3 soff on)
 void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<int64_t>& soff,
-              double* sv, double* sp, SytrdCache* cache = nullptr) {
+              double* sv, double* sp, SytrdCache* cache) {
     std::vector<int> ns;
     int64_t spLen = 0;
     for (size_t m = 0; m < descs.size(); ++m) {
@@ -457,6 +175,8 @@ void addSpmm(Plan& plan, const SparseOp& A, const double* X, double* Y, int64_t 
     }
     plan.copy(back);
 }
+
+namespace {
 
 // 3 x 3 block rows of a CSR matrix (rows may be unsorted): fails unless n is
 // a multiple of 3 and every aligned row triple has the same pattern made of
@@ -617,9 +337,12 @@ void appendCanonical(int64_t nn, int64_t base, const int64_t* pp, const int64_t*
     }
 }
 
+}  // namespace
+
 // Bisection, inverse iteration and back-transformation for descriptors with
 // lam/Y/work/k set (work: 6 n k doubles per matrix, see stein_kernel).
-void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
+// backTransform = false leaves Y as eigenvectors of the tridiagonal itself.
+void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors, bool backTransform) {
     std::vector<int2> work, swork;
     std::vector<int4> owork, vwork;
     std::vector<int64_t> woff;
@@ -685,15 +408,14 @@ void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors) {
             run("stein", [&] {
                 launch_stein(bl->at<EigDesc>(od), bl->at<int2>(os), ns, bl->at<int64_t>(oo), nmax, s);
             });
-            run("ormtr", [&] {
-                if (wave) launch_ormtr_wave(bl->at<EigDesc>(od), nmat, bl->at<int4>(ov), nv, nmax, s);
-                else launch_ormtr(bl->at<EigDesc>(od), bl->at<int4>(oq), nq, lds, s);
-            });
+            if (backTransform)
+                run("ormtr", [&] {
+                    if (wave) launch_ormtr_wave(bl->at<EigDesc>(od), nmat, bl->at<int4>(ov), nv, nmax, s);
+                    else launch_ormtr(bl->at<EigDesc>(od), bl->at<int4>(oq), nq, lds, s);
+                });
         }
     });
 }
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 DevArena::~DevArena() { release(); }
@@ -1201,6 +923,7 @@ void Ctx::runGE(int prec) {
     A.plan(wtot * 8);
     A.plan(co[N] * 8);
     A.plan(vo[N] * 8 * 2);
+    A.plan(N * 2 * 8);
     A.plan(256);
     A.commit();
     dF = A.take<double>(ftot);
@@ -1211,6 +934,7 @@ void Ctx::runGE(int prec) {
     dLinv = A.take<double>(co[N]);
     dDsq = A.take<double>(vo[N]);
     dDinv = A.take<double>(vo[N]);
+    dStat = A.take<double>(N * 2);
     dErr = A.take<int>(1);
     // the launch sequence depends only on the tree, the precision and the
     // arena's address (the values come from the device): the first run for
@@ -1227,13 +951,14 @@ void Ctx::runGE(int prec) {
     Plan plan(geBlob.get(), &kstats);
     {
         int* err = dErr;
-        double *F = dF, *G = dG, *Li = dLinv;
+        double *F = dF, *G = dG, *Li = dLinv, *st = dStat;
         const int64_t nf = ftot, nl = co[N], nk = nasmK, nm = nasmM;
         const int64_t *sK = dsrcK, *tK = dtgtK, *sM = dsrcM, *tM = dtgtM;
         const double *Kx_ = dKx, *Mx_ = dMx;
         const double sh = shift;
         plan.fn([=](hipStream_t s) {
             launch_set_i32(err, INT_MAX, s);
+            launch_fill(st, 2 * N, 0.0, s);
             launch_fill(F, nf, 0.0, s);
             launch_fill(G, nf, 0.0, s);
             launch_fill(Li, nl, 0.0, s);
@@ -1281,6 +1006,30 @@ void Ctx::runGE(int prec) {
             nmaxl = std::max(nmaxl, S.size[p]);
         }
         const size_t ofd = geBlob->add(fds);
+        {
+            // non-zeros and largest |diagonal| of each K-hat (the fronts'
+            // leading blocks, complete now): the density switch of the local
+            // eigensolver reads them after GE
+            std::vector<dev::FrontStatDesc> sd;
+            std::vector<int2> sw;
+            for (int64_t p : nodes) {
+                dev::FrontStatDesc d;
+                d.F = dF + fo[p];
+                d.f = S.f(p);
+                d.n = static_cast<int32_t>(S.size[p]);
+                d.node = static_cast<int32_t>(p);
+                for (int c = 0; c < cdiv(S.size[p], 64); ++c)
+                    sw.push_back(make_int2(static_cast<int>(sd.size()), c));
+                sd.push_back(d);
+            }
+            const size_t osd = geBlob->add(sd), osw = geBlob->add(sw);
+            const int nsw = static_cast<int>(sw.size());
+            Blob* bl = geBlob.get();
+            double* st = dStat;
+            plan.fn([=](hipStream_t s) {
+                launch_front_stats(bl->at<dev::FrontStatDesc>(osd), bl->at<int2>(osw), nsw, st, s);
+            });
+        }
         for (int64_t s0 = 0; s0 < nmaxl; s0 += 64) {
             std::vector<int32_t> list;
             GemmBatch g64, gp;
@@ -1487,7 +1236,10 @@ void Ctx::dropGeGraph() {
 // reads the pivot flag of the elimination just launched
 void Ctx::finishGE() {
     int e = 0;
+    statHost.assign(2 * S.nNodes, 0.0);
     AMLS_HIP_CHECK(hipMemcpyAsync(&e, dErr, sizeof(int), hipMemcpyDeviceToHost, stream));
+    AMLS_HIP_CHECK(hipMemcpyAsync(statHost.data(), dStat, 2 * S.nNodes * sizeof(double),
+                                  hipMemcpyDeviceToHost, stream));
     sync();
     if (e != INT_MAX)
         throw StatusError(AMLS_E_NOT_SPD,
@@ -1606,6 +1358,9 @@ void Ctx::spectralPrepare() {
     }
     plan.run(stream);
     sync();
+    // operators of the nodes the density switch sends to shift-invert
+    // Lanczos (the dense reduction above stays their fallback)
+    silPrepare();
     specReady = true;
 }
 
@@ -1656,6 +1411,9 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
         sync();
         for (int64_t p = 0; p < N; ++p) k[p] = cnt[p];
     }
+    // nodes on the shift-invert Lanczos path: their k and Lanczos steps
+    std::vector<int> mUse;
+    silSelect(kind, valueSeg, mUse);
     moff.assign(N + 1, 0);
     zo.assign(N + 1, 0);
     int64_t wk = 0;
@@ -1691,10 +1449,56 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
     std::vector<VecDesc> vd;
     Copy2dBatch cz;
     GemmBatch gz;
+    // shift-invert Lanczos nodes: the k largest eigenpairs (nu, s) of the
+    // tridiagonal T_m, mu = 1 / nu + sigma, Z = R^-T U_m s = L_M^-T D_M^-1/2 U_m s
+    std::vector<EigDesc> sed;
+    std::vector<VecDesc> svd;
+    GemmBatch sgz;
+    Copy2dBatch ssc;
+    TrsmBatch str;
     int64_t wo2 = 0;
     for (int64_t p = 0; p < N; ++p) {
         const int64_t np = S.size[p], kp = k[p];
         if (kp == 0) continue;
+        if (mUse[p] > 0) {
+            const SilNode& s = sil[silOf[p]];
+            const int m = mUse[p];
+            EigDesc e;
+            std::memset(&e, 0, sizeof(e));
+            e.lda = m;
+            e.n = m;
+            e.d = s.alpha;
+            e.e = s.beta;
+            e.lam = dLam + moff[p];
+            e.Y = dY + zo[p];             // m x kp eigenvectors of T (ld m <= np)
+            e.work = dWk + wo2;
+            e.k = e.kcap = static_cast<int32_t>(kp);
+            e.node = static_cast<int32_t>(p);
+            sed.push_back(e);
+            wo2 += 6 * np * kp;
+            VecDesc v;
+            std::memset(&v, 0, sizeof(v));
+            v.src = dLam + moff[p];
+            v.dst = dMu + moff[p];
+            v.n = static_cast<int32_t>(kp);
+            v.op = 7;
+            v.a = s.sigma;
+            svd.push_back(v);
+            GemmDesc g = gdesc();
+            g.A = s.U;
+            g.lda = np;
+            g.B = dY + zo[p];
+            g.ldb = m;
+            g.k = m;
+            g.C = dZ + zo[p];
+            g.ldc = np;
+            g.m = static_cast<int32_t>(np);
+            g.n = static_cast<int32_t>(kp);
+            sgz.add(g);
+            ssc.add(cdesc(dZ + zo[p], np, dZ + zo[p], np, np, kp, 0, s.dmhi));
+            str.add(s.Mf, np, dZ + zo[p], np, np, kp, 1);
+            continue;
+        }
         EigDesc e;
         std::memset(&e, 0, sizeof(e));
         e.A = dC + co[p];
@@ -1737,6 +1541,11 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
     plan.vec(vd);
     plan.copy(cz);
     plan.gemm(gz, false);
+    addSelect(plan, sed, true, false);
+    plan.vec(svd);
+    plan.gemm(sgz, false);
+    plan.copy(ssc);
+    plan.trsm(str);
     plan.run(stream);
     selected = true;
 }
@@ -2032,52 +1841,48 @@ void Ctx::backtransform() {
 }
 
 // ---------------------------------------------------------------------------
-// Dense helpers on one matrix (appended to a plan)
+// Dense helpers (appended to a plan)
 // ---------------------------------------------------------------------------
-namespace {
-
-// In-place LDL^T of the n x n lower triangles of the matrices As[m] (all of
-// order n, leading dimension lda), 64-column panels; Ws[m]: lda x 64 scratch.
-void addDenseLdlt(Plan& plan, const std::vector<double*>& As, int64_t n, int64_t lda,
-                  const std::vector<double*>& Ws, int* err, int node) {
+// In-place LDL^T of the lower triangles of the square matrices As[m] (order
+// and leading dimension ns[m]), 64-column panels over all matrices at once;
+// Ws[m]: ns[m] x 64 scratch.  A bad pivot of matrix m sets *err to at most
+// nodes[m].
+void addLdltBatch(Plan& plan, const std::vector<double*>& As, const std::vector<int64_t>& ns,
+                  const std::vector<double*>& Ws, int* err, const std::vector<int>& nodes) {
     const int nm = static_cast<int>(As.size());
+    if (nm == 0) return;
     std::vector<FrontDesc> fd(nm);
-    std::vector<int32_t> list(nm);
+    int64_t nmax = 0;
     for (int m = 0; m < nm; ++m) {
         std::memset(&fd[m], 0, sizeof(FrontDesc));
         fd[m].F = As[m];
         fd[m].W = Ws[m];
-        fd[m].f = lda;
-        fd[m].n = static_cast<int32_t>(n);
-        fd[m].node = node;
-        list[m] = m;
+        fd[m].f = ns[m];
+        fd[m].n = static_cast<int32_t>(ns[m]);
+        fd[m].node = nodes[m];
+        nmax = std::max(nmax, ns[m]);
     }
     const size_t ofd = plan.blob->add(fd);
-    const size_t ol = plan.blob->add(list);
     Blob* bl = plan.blob;
-    for (int64_t s0 = 0; s0 < n; s0 += 64) {
-        const int64_t jb = std::min<int64_t>(64, n - s0), r1 = s0 + jb;
+    for (int64_t s0 = 0; s0 < nmax; s0 += 64) {
+        std::vector<int32_t> list;
         std::vector<int2> rows;
-        for (int m = 0; m < nm; ++m)
-            for (int c = 0; c < cdiv(lda - r1, 256); ++c) rows.push_back(make_int2(m, c));
-        const size_t orw = plan.blob->add(rows);
-        const int nrw = static_cast<int>(rows.size());
-        const int s0i = static_cast<int>(s0);
-        plan.fn([=](hipStream_t s) {
-            launch_ldlt_panel(bl->at<FrontDesc>(ofd), bl->at<int32_t>(ol), nm, s0i, err,
-                              bl->at<int2>(orw), nrw, s);
-        });
-        if (n > r1) {
-            GemmBatch gb;
-            for (int m = 0; m < nm; ++m) {
+        GemmBatch gb;
+        for (int m = 0; m < nm; ++m) {
+            const int64_t n = ns[m];
+            if (n <= s0) continue;
+            const int64_t jb = std::min<int64_t>(64, n - s0), r1 = s0 + jb;
+            list.push_back(m);
+            for (int c = 0; c < cdiv(n - r1, 256); ++c) rows.push_back(make_int2(m, c));
+            if (n > r1) {
                 GemmDesc d = gdesc();
-                d.A = As[m] + r1 + s0 * lda;
-                d.lda = lda;
+                d.A = As[m] + r1 + s0 * n;
+                d.lda = n;
                 d.B = Ws[m];
-                d.ldb = lda;
+                d.ldb = n;
                 d.tb = 1;
-                d.C = As[m] + r1 + r1 * lda;
-                d.ldc = lda;
+                d.C = As[m] + r1 + r1 * n;
+                d.ldc = n;
                 d.m = d.n = static_cast<int32_t>(n - r1);
                 d.k = static_cast<int32_t>(jb);
                 d.alpha1 = -1.0;
@@ -2085,12 +1890,17 @@ void addDenseLdlt(Plan& plan, const std::vector<double*>& As, int64_t n, int64_t
                 d.lower = 1;
                 gb.add(d);
             }
-            plan.gemm(gb, false);
         }
+        const size_t ol = plan.blob->add(list), orw = plan.blob->add(rows);
+        const int cnt = static_cast<int>(list.size()), nrw = static_cast<int>(rows.size());
+        const int s0i = static_cast<int>(s0);
+        plan.fn([=](hipStream_t s) {
+            launch_ldlt_panel(bl->at<FrontDesc>(ofd), bl->at<int32_t>(ol), cnt, s0i, err,
+                              bl->at<int2>(orw), nrw, s);
+        });
+        plan.gemm(gb, false);
     }
 }
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // Subspace iteration with the AMLS block factors (AMLS-SIMb): Q <- K^-1 M Q
@@ -2383,7 +2193,8 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
         }
         plan.gemm(gr, false);
     }
-    addDenseLdlt(plan, KcS, q, q, WpS, err, -1);
+    addLdltBatch(plan, KcS, std::vector<int64_t>(KcS.size(), q), WpS, err,
+                 std::vector<int>(KcS.size(), -1));
     const int nv = static_cast<int>(nev);
     Copy2dBatch c1, cz, c3, cv;
     TrsmBatch tinv;
@@ -2591,6 +2402,7 @@ amls_status amls_options_default(amls_options* o) {
     o->precision = 0;
     o->siSteps = 0;
     o->siVectors = 0;
+    o->silMode = 0;
     return AMLS_OK;
 }
 
@@ -2739,6 +2551,24 @@ amls_status amls_hip_spectral_prepare(amls_ctx* ctx) {
     return guarded(ctx, [&](amls::Ctx& c) { c.spectralPrepare(); });
 }
 
+amls_status amls_hip_set_sil_mode(amls_ctx* ctx, int32_t mode) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (mode < -1 || mode > 1) throw amls::StatusError(AMLS_E_ARG, "silMode must be -1, 0 or 1");
+        c.silMode = mode;
+    });
+}
+
+amls_status amls_hip_get_sil(amls_ctx* ctx, int32_t* path, int64_t* steps) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (!c.selected) throw amls::StatusError(AMLS_E_STATE, "spectral not run");
+        const int64_t N = c.S.nNodes;
+        for (int64_t p = 0; p < N; ++p) {
+            if (path) path[p] = c.lastPath.empty() ? 0 : c.lastPath[p];
+            if (steps) steps[p] = c.lastSteps.empty() ? 0 : c.lastSteps[p];
+        }
+    });
+}
+
 amls_status amls_hip_spectral(amls_ctx* ctx, int32_t kind, double value, int64_t* k_out) {
     return guarded(ctx, [&](amls::Ctx& c) {
         c.spectralSelect(kind, value);
@@ -2805,7 +2635,9 @@ static void solveNumeric(Ctx& c, amls_options o, double* theta, double* Z, int64
         throw StatusError(AMLS_E_ARG, "siSteps, siVectors and fixedKeep must not be negative");
     if (o.fixedKeep == 0 && !(o.thresholdFactor > 0.0))
         throw StatusError(AMLS_E_ARG, "thresholdFactor must be positive");
+    if (o.silMode < -1 || o.silMode > 1) throw StatusError(AMLS_E_ARG, "silMode must be -1, 0 or 1");
     c.requireGpu();
+    c.silMode = o.silMode;
     if (!c.haveProblem) throw StatusError(AMLS_E_STATE, "no problem set");
     AMLS_HIP_CHECK(hipSetDevice(c.device));
     const int64_t NS = c.nseg;

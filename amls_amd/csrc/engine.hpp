@@ -221,6 +221,30 @@ struct Ctx {
     void dropGeGraph();
     void finishGE();
 
+    // ---- local shift-invert Lanczos (nodes that EigenpairSolver's density
+    // switch sends to sil::*, src/amls/eigenpair_solver.hpp:38-82)
+    int silMode = 0;                 // silMode of EigenpairSolver: 0 switch, 1 always, -1 never
+    double* dStat = nullptr;         // per node (nnz, max |diag|) of K-hat, written by GE
+    std::vector<double> statHost;    // its host copy after GE
+    struct SilNode {
+        int64_t p;                   // post-order node
+        int n, cap;                  // order, Lanczos capacity min(n, 1536)
+        double sigma;                // -max(1e-10, 1e-9 max |K-hat_jj|)
+        double *S, *Mf, *X, *U, *alpha, *beta, *u0, *dmhi, *dai, *dhalf, *Wp;
+        int32_t* state;              // (steps done, breakdown flag) on the device
+        int mdone = 0;
+        bool broke = false;
+    };
+    std::vector<SilNode> sil;        // SIL-eligible nodes of the current factorisation
+    std::vector<int> silOf;          // node -> index into sil, or -1
+    std::vector<int32_t> lastPath;   // per node, last spectral call: 0 dense, 1 SIL, 2 SIL fell back
+    std::vector<int64_t> lastSteps;  // per node, Lanczos steps the last call used
+    std::unique_ptr<DevArena> silMem;
+    std::unique_ptr<Blob> silBlob;
+    void silPrepare();               // S = R^T (K-hat - sigma M-hat)^-1 R per SIL node
+    void silAdvance(const std::vector<int>& idx, const std::vector<int>& ncv);
+    void silSelect(int kind, const std::vector<double>& valueSeg, std::vector<int>& mUse);
+
     // ---- local spectra
     std::unique_ptr<DevArena> spMem;
     std::unique_ptr<Blob> spBlob;

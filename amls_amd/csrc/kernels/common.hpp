@@ -102,7 +102,9 @@ struct Copy2dDesc {
     int64_t lds, ldd;
     int32_t rows, cols;
     int32_t mode, pad;  // mode 0 copy, 1 transpose, 2 symmetric completion
-                        // from the lower triangle, 3 fill with alpha
+                        // from the lower triangle, 3 fill with alpha, 4 unit
+                        // lower triangle of src (strict lower part, 1 on the
+                        // diagonal, 0 above: L of an LDL^T factor)
     double alpha;
 };
 
@@ -123,12 +125,39 @@ struct GatherDesc {
 //   op 3: dst[i] = 1 / sqrt(src[i])
 //   op 4: dst[i] = sqrt(1 / src[i])
 //   op 5: dst[i] = 1 / src[i + i*ld]             (D^{-1} from an LDL^T diagonal)
+//   op 6: dst[i] = sqrt(src[i + i*ld])           (D^{1/2} from an LDL^T diagonal)
+//   op 7: dst[i] = 1 / src[i] + a                (shift-invert Ritz values)
 struct VecDesc {
     const double* src;
     double* dst;
     double* dst2;
     int64_t ld;
     int32_t n, op;
+    double a;
+};
+
+// Shift-invert Lanczos on one dense symmetric S (n x n, full, ld n): runs
+// steps state[0] .. ncv-1 (stopping early at a breakdown, beta < 1e-13,
+// which sets state[1]); U holds the Lanczos vectors (n x cap, ld n, column
+// j = u_j), alpha / beta the tridiagonal (beta[j] couples u_j and u_{j+1}).
+// u0 (unnormalised) starts the run when state[0] = 0.
+struct LanczosDesc {
+    const double* S;
+    double* U;
+    double* alpha;
+    double* beta;
+    const double* u0;
+    int32_t* state;   // [0] steps done, [1] breakdown flag
+    int32_t n, ncv;
+};
+
+// Statistics of a front's leading n x n block before its factorisation:
+// stat[2 node] = non-zero entries of the symmetric block (both triangles,
+// counted from the stored lower one), stat[2 node + 1] = max |diagonal|.
+struct FrontStatDesc {
+    const double* F;
+    int64_t f;
+    int32_t n, node;
 };
 
 }  // namespace dev

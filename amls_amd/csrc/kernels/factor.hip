@@ -214,8 +214,13 @@ __global__ void __launch_bounds__(256) copy2d_kernel(const Copy2dDesc* __restric
     for (int q = 0; q < 16; ++q) {
         const int j = jb + 4 * q;
         v[q] = 0.0;
-        if (j < d.cols)
-            v[q] = (d.mode == 2 && i < j) ? d.src[j + (int64_t)i * d.lds] : d.src[i + (int64_t)j * d.lds];
+        if (j < d.cols) {
+            if (d.mode == 4)
+                v[q] = i > j ? d.src[i + (int64_t)j * d.lds] : (i == j ? 1.0 : 0.0);
+            else
+                v[q] = (d.mode == 2 && i < j) ? d.src[j + (int64_t)i * d.lds]
+                                              : d.src[i + (int64_t)j * d.lds];
+        }
     }
     const double sr = d.rs ? d.alpha * d.rs[i] : d.alpha;
 #pragma unroll
@@ -266,8 +271,51 @@ __global__ void vecop_kernel(const VecDesc* __restrict__ descs) {
             case 2: d.dst[i + (int64_t)i * d.ld] = 1.0; break;
             case 3: d.dst[i] = 1.0 / sqrt(d.src[i]); break;
             case 5: d.dst[i] = 1.0 / d.src[i + (int64_t)i * d.ld]; break;
+            case 6: d.dst[i] = sqrt(d.src[i + (int64_t)i * d.ld]); break;
+            case 7: d.dst[i] = 1.0 / d.src[i] + d.a; break;
             default: d.dst[i] = sqrt(1.0 / d.src[i]); break;
         }
+    }
+}
+
+// Non-zeros of a front's leading n x n block (lower triangle stored; both
+// triangles counted) and its largest |diagonal|.  Work item (x = front, y =
+// block of 64 columns); a workgroup adds its count into stat[2 node] and
+// raises stat[2 node + 1] to its maximum (both cleared beforehand): counts
+// are integers far below 2^53, so the sum is exact in any order, and the
+// maximum of non-negative doubles is the maximum of their bit patterns.
+__global__ void __launch_bounds__(256) front_stats_kernel(const FrontStatDesc* __restrict__ descs,
+                                                          const int2* __restrict__ work,
+                                                          double* __restrict__ stat) {
+    const int2 wk = work[blockIdx.x];
+    const FrontStatDesc d = descs[wk.x];
+    __shared__ double scnt[256];
+    __shared__ double smax[256];
+    double cnt = 0.0, mx = 0.0;
+    const int t = threadIdx.x;
+    const int c0 = wk.y * 64, c1 = min(d.n, c0 + 64);
+    for (int c = c0; c < c1; ++c) {
+        const double* col = d.F + (int64_t)c * d.f;
+        for (int r = c + t; r < d.n; r += 256) {
+            const double v = col[r];
+            if (v != 0.0) cnt += r == c ? 1.0 : 2.0;
+            if (r == c) mx = fmax(mx, fabs(v));
+        }
+    }
+    scnt[t] = cnt;
+    smax[t] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            scnt[t] += scnt[t + s];
+            smax[t] = fmax(smax[t], smax[t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        atomicAdd(stat + 2 * d.node, scnt[0]);
+        atomicMax(reinterpret_cast<unsigned long long*>(stat + 2 * d.node + 1),
+                  __builtin_bit_cast(unsigned long long, smax[0]));
     }
 }
 
@@ -288,6 +336,12 @@ void launch_ldlt_panel(const dev::FrontDesc* d, const int32_t* list, int count, 
     hipLaunchKernelGGL(dev::ldlt_diag_kernel, dim3(count), dim3(64), 0, s, d, list, s0, err);
     if (nrows > 0)
         hipLaunchKernelGGL(dev::ldlt_rows_kernel, dim3(nrows), dim3(256), 0, s, d, rows, s0);
+}
+
+void launch_front_stats(const dev::FrontStatDesc* d, const int2* work, int nwork, double* stat,
+                        hipStream_t s) {
+    if (nwork <= 0) return;
+    hipLaunchKernelGGL(dev::front_stats_kernel, dim3(nwork), dim3(256), 0, s, d, work, stat);
 }
 
 void launch_extadd(const dev::ExtAddDesc* d, const int4* work, int nwork, hipStream_t s) {

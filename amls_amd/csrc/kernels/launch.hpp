@@ -21,6 +21,14 @@ void launch_trsm_diag(const dev::TrsmDesc* d, const dev::TrsmWork* w, int nwork,
 void launch_ldlt_panel(const dev::FrontDesc* d, const int32_t* list, int count, int s0, int* err,
                        const int2* rows, int nrows, hipStream_t s);
 void launch_extadd(const dev::ExtAddDesc* d, const int4* work, int nwork, hipStream_t s);
+// Non-zero count and largest |diagonal| of the fronts' leading blocks: work
+// items (front index, 64-column block); stat (2 per node, cleared first).
+void launch_front_stats(const dev::FrontStatDesc* d, const int2* work, int nwork, double* stat,
+                        hipStream_t s);
+// Shift-invert Lanczos steps, one workgroup per descriptor (n <= 8192 and
+// ncv <= 1536: LDS holds two n-vectors and the step's coefficients).
+int lanczos_max_order();
+void launch_lanczos(const dev::LanczosDesc* d, int count, int nmax, int cmax, hipStream_t s);
 void launch_scatter(const int64_t* src, const int64_t* tgt, const double* vals, double* dst,
                     int64_t count, double scale, bool accumulate, hipStream_t s);
 void launch_fill(double* p, int64_t count, double v, hipStream_t s);

@@ -61,6 +61,10 @@ typedef struct amls_options {
                                    Schur, M-congruence and projection GEMMs */
     int32_t siSteps;            /* subspace-iteration Rayleigh-Ritz steps (0 = plain AMLS) */
     int64_t siVectors;          /* subspace width q (0: max(2*nev, nev+8), capped at p) */
+    int32_t silMode;            /* local solver switch, EigenpairSolver::silMode
+                                   (src/amls/eigenpair_solver.hpp:28-29): 0 = shift-
+                                   invert Lanczos for nodes of order >= 200 whose K-hat
+                                   has < 0.15 n^2 non-zeros, 1 = always, -1 = never */
 } amls_options;
 
 /* Fills *opt with the defaults above. */
@@ -136,8 +140,22 @@ amls_status amls_hip_tree_get(const amls_ctx* ctx, int64_t* treeId, int64_t* off
 amls_status amls_hip_ge(amls_ctx* ctx, int32_t precision);
 
 /* Standard-form local pencils C_i = (L sqrt(D))^-1 Mhat (L sqrt(D))^-T and
- * their tridiagonal reduction, for all nodes (after amls_hip_ge). */
+ * their tridiagonal reduction, for all nodes (after amls_hip_ge); for the
+ * nodes the local solver switch sends to shift-invert Lanczos, also
+ * S_i = R^T (Khat - sigma Mhat)^-1 R with Mhat = R R^T and
+ * sigma = -max(1e-10, 1e-9 max |Khat_jj|) (sil::ShiftInvertLanczos,
+ * src/amls/shift_invert_lanczos.hpp:32-41, 128-135, 175-178). */
 amls_status amls_hip_spectral_prepare(amls_ctx* ctx);
+
+/* The local solver switch for the following amls_hip_spectral_prepare /
+ * amls_hip_spectral calls (mode as amls_options.silMode; amls_hip_solve and
+ * amls_hip_solve_numeric set it from their options). */
+amls_status amls_hip_set_sil_mode(amls_ctx* ctx, int32_t mode);
+/* Per node (post-order) after amls_hip_spectral: path[p] = 0 dense solver,
+ * 1 shift-invert Lanczos, 2 Lanczos tried and the dense solver used instead
+ * (EigenpairSolver's fallbacks); steps[p] = Lanczos steps m of the Ritz
+ * pairs (0 on the dense path).  Either pointer may be null. */
+amls_status amls_hip_get_sil(amls_ctx* ctx, int32_t* path, int64_t* steps);
 
 /*
  * Local eigenpairs and truncation for every node: kind 0 = FixedCount(value)
@@ -146,6 +164,11 @@ amls_status amls_hip_spectral_prepare(amls_ctx* ctx);
  * SubspaceCondensation::spectralLevel + EigenpairSolver + the truncation
  * policies (src/amls/subspace_condensation.cpp:45-100,
  * src/amls/eigenpair_solver.hpp:32-63, src/amls/spectral_truncation.hpp:35-88).
+ * Nodes on the shift-invert Lanczos path follow sil::solveIndex /
+ * solveInterval (src/amls/shift_invert_lanczos.hpp:126-203): the same
+ * ncv rules, start vector (glibc's rand() after srand(12345), as Eigen's
+ * Vec::Random draws it), kept Ritz values and dense fallbacks; within such a
+ * node the modes are in ascending order.
  */
 amls_status amls_hip_spectral(amls_ctx* ctx, int32_t kind, double value, int64_t* k_out);
 

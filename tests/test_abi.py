@@ -184,3 +184,23 @@ def test_duplicate_entries_are_summed():
         ctx.close()
     for key in ("treeId", "offset", "size", "parent", "bsize", "perm"):
         assert np.array_equal(trees[0][key], trees[1][key]), key
+
+
+def test_sil_mode_entry_points():
+    """The local solver switch takes -1, 0 or 1; the per-node paths exist
+    only after a spectral call (host-only context: state error)."""
+    L = _lib.lib()
+    ctx = HipContext(-1)
+    for m in (-1, 0, 1):
+        ctx.set_sil_mode(m)
+    for bad in (2, -2):
+        with pytest.raises(AmlsError) as e:
+            ctx.set_sil_mode(bad)
+        assert e.value.status == 1
+    assert L.amls_hip_get_sil(ctx._h, None, None) == 6
+    o = _lib.AmlsOptionsC()
+    L.amls_options_default(ctypes.byref(o))
+    assert o.silMode == 0
+    o.silMode = 3
+    assert L.amls_hip_solve_numeric(ctx._h, ctypes.byref(o), None, None, None, None) == 1
+    ctx.close()
