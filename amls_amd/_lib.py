@@ -28,6 +28,7 @@ EXPORTS = [
     "amls_hip_profile", "amls_hip_kernel_stats", "amls_hip_adjoint",
     "amls_hip_set_problem_batch", "amls_hip_batch_info", "amls_hip_solve_batch",
     "amls_hip_autotune", "amls_hip_set_tuning", "amls_hip_set_sil_mode", "amls_hip_get_sil",
+    "amls_hip_ldlt_solve_noalloc",
 ]
 
 
@@ -110,6 +111,7 @@ def lib() -> C.CDLL:
         "amls_hip_ldlt_solve": ([P, I64, PD, I64, PD, PD, PD], C.c_int),
         "amls_hip_syevx_top": ([P, I64, PD, I64, PD, PD], C.c_int),
         "amls_hip_set_sil_mode": ([P, I32], C.c_int),
+        "amls_hip_ldlt_solve_noalloc": ([P, I64, P, I64, P], C.c_int),
         "amls_hip_get_sil": ([P, C.POINTER(C.c_int32), PI64], C.c_int),
     }
     for name, (args, res) in sig.items():

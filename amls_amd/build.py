@@ -24,6 +24,7 @@ SOURCES = [
     CSRC / "kernels" / "lanczos.hip",
     CSRC / "host" / "tree.cpp",
     CSRC / "sil.cpp",
+    CSRC / "noalloc.cpp",
     CSRC / "engine.cpp",
 ]
 

@@ -501,11 +501,15 @@ KernelStats::~KernelStats() {
 }
 
 Blob::~Blob() {
-    if (dev_) (void)hipFree(dev_);
+    if (dev_ && !ext_) (void)hipFree(dev_);
 }
 
 void Blob::upload(hipStream_t s) {
     if (host_.empty()) return;
+    if (ext_ && devcap_ < host_.size())
+        throw StatusError(AMLS_E_ARG, "launch descriptors need " + std::to_string(host_.size()) +
+                                          " bytes, the caller's buffer holds " +
+                                          std::to_string(devcap_));
     if (devcap_ < host_.size()) {
         if (dev_) {
             AMLS_HIP_CHECK(hipStreamSynchronize(s));
@@ -1902,6 +1906,76 @@ void addLdltBatch(Plan& plan, const std::vector<double*>& As, const std::vector<
     }
 }
 
+// LDL^T of the n x n lower triangle of A in place (64-column panels) and,
+// for nrhs > 0, X := A^-1 X (forward solve, D^-1, backward solve).  W:
+// n x 64 panel scratch, dq: n doubles, *err: set to INT_MAX first, lowered
+// to 0 by a non-positive or non-finite pivot.
+void addLdltSolve(Plan& plan, double* dA, int64_t n, double* dX, int64_t nrhs, double* dW,
+                  double* dq, int* derr) {
+    Blob& blob = *plan.blob;
+    plan.fn([=](hipStream_t s) { launch_set_i32(derr, INT_MAX, s); });
+    std::vector<dev::FrontDesc> fd(1);
+    fd[0].F = dA;
+    fd[0].W = dW;
+    fd[0].f = n;
+    fd[0].n = static_cast<int32_t>(n);
+    fd[0].node = 0;
+    const size_t ofd = blob.add(fd);
+    std::vector<int32_t> list{0};
+    const size_t ol = blob.add(list);
+    Blob* bl = &blob;
+    for (int64_t s0 = 0; s0 < n; s0 += 64) {
+        const int64_t jb = std::min<int64_t>(64, n - s0), r1 = s0 + jb;
+        const int s0i = static_cast<int>(s0);
+        std::vector<int2> rows;
+        for (int c = 0; c < cdiv(n - r1, 256); ++c) rows.push_back(make_int2(0, c));
+        const size_t orw = blob.add(rows);
+        const int nrw = static_cast<int>(rows.size());
+        plan.fn([=](hipStream_t s) {
+            launch_ldlt_panel(bl->at<dev::FrontDesc>(ofd), bl->at<int32_t>(ol), 1, s0i, derr,
+                              bl->at<int2>(orw), nrw, s);
+        });
+        if (n > r1) {
+            GemmBatch gb;
+            dev::GemmDesc d = gdesc();
+            d.A = dA + r1 + s0 * n;
+            d.lda = n;
+            d.B = dW;
+            d.ldb = n;
+            d.tb = 1;
+            d.C = dA + r1 + r1 * n;
+            d.ldc = n;
+            d.m = d.n = static_cast<int32_t>(n - r1);
+            d.k = static_cast<int32_t>(jb);
+            d.alpha1 = -1.0;
+            d.beta = 1.0;
+            d.lower = 1;
+            gb.add(d);
+            plan.gemm(gb, false);
+        }
+    }
+    if (nrhs > 0) {
+        TrsmBatch t1, t2;
+        t1.add(dA, n, dX, n, n, nrhs, 0);
+        t2.add(dA, n, dX, n, n, nrhs, 1);
+        std::vector<dev::VecDesc> vd(1);
+        std::memset(vd.data(), 0, sizeof(dev::VecDesc));
+        vd[0].src = dA;
+        vd[0].dst = dq;
+        vd[0].ld = n;
+        vd[0].n = static_cast<int32_t>(n);
+        vd[0].op = 0;
+        // dq = D^-1/2 (vecop 0); the row scaling applied twice is D^-1
+        Copy2dBatch sc;
+        sc.add(cdesc(dX, n, dX, n, n, nrhs, 0, dq));
+        plan.trsm(t1);
+        plan.vec(vd);
+        plan.copy(sc);
+        plan.copy(sc);
+        plan.trsm(t2);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Subspace iteration with the AMLS block factors (AMLS-SIMb): Q <- K^-1 M Q
 // through L, Khat and L^T; columns scaled to unit M-norm between steps; one
@@ -2418,7 +2492,8 @@ amls_status amls_hip_create(int32_t device, uint32_t flags, amls_ctx** out) {
             delete ctx;
             return AMLS_E_HIP;
         }
-        if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->c.stream) != hipSuccess ||
+        if (hipSetDevice(device) != hipSuccess || amls::scratchInit(device) != hipSuccess ||
+            hipStreamCreate(&ctx->c.stream) != hipSuccess ||
             hipStreamCreate(&ctx->c.stream2) != hipSuccess ||
             hipEventCreateWithFlags(&ctx->c.evFork, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&ctx->c.evJoin, hipEventDisableTiming) != hipSuccess) {
@@ -3038,73 +3113,12 @@ amls_status amls_hip_ldlt_solve(amls_ctx* ctx, int64_t n, const double* A, int64
         double* dX = mem.take<double>(n * std::max<int64_t>(nrhs, 1));
         double* dq = mem.take<double>(n);
         int* derr = mem.take<int>(1);
-        const int big = INT_MAX;
-        AMLS_HIP_CHECK(hipMemcpyAsync(derr, &big, sizeof(int), hipMemcpyHostToDevice, c.stream));
         AMLS_HIP_CHECK(hipMemcpyAsync(dA, A, n * n * 8, hipMemcpyHostToDevice, c.stream));
         if (nrhs > 0)
             AMLS_HIP_CHECK(hipMemcpyAsync(dX, B, n * nrhs * 8, hipMemcpyHostToDevice, c.stream));
         amls::Blob blob;
         amls::Plan plan(&blob, &c.kstats);
-        std::vector<amls::dev::FrontDesc> fd(1);
-        fd[0].F = dA;
-        fd[0].W = dW;
-        fd[0].f = n;
-        fd[0].n = static_cast<int32_t>(n);
-        fd[0].node = 0;
-        const size_t ofd = blob.add(fd);
-        std::vector<int32_t> list{0};
-        const size_t ol = blob.add(list);
-        amls::Blob* bl = &blob;
-        for (int64_t s0 = 0; s0 < n; s0 += 64) {
-            const int64_t jb = std::min<int64_t>(64, n - s0), r1 = s0 + jb;
-            const int s0i = static_cast<int>(s0);
-            std::vector<int2> rows;
-            for (int c = 0; c < amls::cdiv(n - r1, 256); ++c) rows.push_back(make_int2(0, c));
-            const size_t orw = blob.add(rows);
-            const int nrw = static_cast<int>(rows.size());
-            plan.fn([=](hipStream_t s) {
-                amls::launch_ldlt_panel(bl->at<amls::dev::FrontDesc>(ofd), bl->at<int32_t>(ol), 1, s0i,
-                                        derr, bl->at<int2>(orw), nrw, s);
-            });
-            if (n > r1) {
-                amls::GemmBatch gb;
-                amls::dev::GemmDesc d = amls::gdesc();
-                d.A = dA + r1 + s0 * n;
-                d.lda = n;
-                d.B = dW;
-                d.ldb = n;
-                d.tb = 1;
-                d.C = dA + r1 + r1 * n;
-                d.ldc = n;
-                d.m = d.n = static_cast<int32_t>(n - r1);
-                d.k = static_cast<int32_t>(jb);
-                d.alpha1 = -1.0;
-                d.beta = 1.0;
-                d.lower = 1;
-                gb.add(d);
-                plan.gemm(gb, false);
-            }
-        }
-        if (nrhs > 0) {
-            amls::TrsmBatch t1, t2;
-            t1.add(dA, n, dX, n, n, nrhs, 0);
-            t2.add(dA, n, dX, n, n, nrhs, 1);
-            std::vector<amls::dev::VecDesc> vd(1);
-            std::memset(vd.data(), 0, sizeof(amls::dev::VecDesc));
-            vd[0].src = dA;
-            vd[0].dst = dq;
-            vd[0].ld = n;
-            vd[0].n = static_cast<int32_t>(n);
-            vd[0].op = 0;
-            // dq = D^-1/2 (vecop 0); the row scaling applied twice is D^-1
-            amls::Copy2dBatch sc;
-            sc.add(amls::cdesc(dX, n, dX, n, n, nrhs, 0, dq));
-            plan.trsm(t1);
-            plan.vec(vd);
-            plan.copy(sc);
-            plan.copy(sc);
-            plan.trsm(t2);
-        }
+        amls::addLdltSolve(plan, dA, n, dX, nrhs, dW, dq, derr);
         plan.run(c.stream);
         int e = 0;
         AMLS_HIP_CHECK(hipMemcpyAsync(&e, derr, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -3113,6 +3127,14 @@ amls_status amls_hip_ldlt_solve(amls_ctx* ctx, int64_t n, const double* A, int64
             AMLS_HIP_CHECK(hipMemcpyAsync(X, dX, n * nrhs * 8, hipMemcpyDeviceToHost, c.stream));
         c.sync();
         if (e != INT_MAX) throw StatusError(AMLS_E_NOT_SPD, "LDL^T: non-positive or non-finite pivot");
+    });
+}
+
+amls_status amls_hip_ldlt_solve_noalloc(amls_ctx* ctx, int64_t n, double* A, int64_t nrhs,
+                                        double* B) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (c.gpu) AMLS_HIP_CHECK(hipSetDevice(c.device));
+        amls::ldltSolveNoalloc(c, n, A, nrhs, B);
     });
 }
 

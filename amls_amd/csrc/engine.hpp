@@ -75,6 +75,15 @@ public:
     }
     void upload(hipStream_t s);
     void clear() { host_.clear(); }   // new contents; the device buffer is kept
+    // upload into caller-owned device memory of `cap` bytes instead of an
+    // allocation of the blob's own (upload then fails with AMLS_E_ARG when
+    // the descriptors do not fit)
+    void useExternal(void* dev, size_t cap) {
+        ext_ = true;
+        dev_ = static_cast<char*>(dev);
+        devcap_ = cap;
+    }
+    size_t size() const { return host_.size(); }
     template <class T> const T* at(size_t off) const {
         return reinterpret_cast<const T*>(dev_ + off);
     }
@@ -84,6 +93,7 @@ private:
     std::vector<char> host_;
     char* dev_ = nullptr;
     size_t devcap_ = 0;
+    bool ext_ = false;
 };
 
 Blob& fresh(std::unique_ptr<Blob>& b);   // create or clear()
@@ -330,5 +340,11 @@ struct Ctx {
     }
 
 };
+
+// The allocation-free factor-and-solve (noalloc.cpp): scratchInit records,
+// with `device` current, the device address of its file-scope scratch array.
+hipError_t scratchInit(int device);
+size_t noallocScratchBytes();
+void ldltSolveNoalloc(Ctx& c, int64_t n, double* A, int64_t nrhs, double* B);
 
 }  // namespace amls

@@ -317,5 +317,10 @@ void addSpmm(Plan& plan, const SparseOp& A, const double* X, double* Y, int64_t 
 // min(*err, nodes[m]).
 void addLdltBatch(Plan& plan, const std::vector<double*>& As, const std::vector<int64_t>& ns,
                   const std::vector<double*>& Ws, int* err, const std::vector<int>& nodes);
+// LDL^T of the n x n lower triangle of dA in place and, for nrhs > 0,
+// dX := A^-1 dX; dW: n x 64 scratch, dq: n doubles, *derr: INT_MAX unless a
+// pivot is non-positive or non-finite (then 0).
+void addLdltSolve(Plan& plan, double* dA, int64_t n, double* dX, int64_t nrhs, double* dW,
+                  double* dq, int* derr);
 
 }  // namespace amls

@@ -325,6 +325,17 @@ amls_status amls_hip_gemm(amls_ctx* ctx, int32_t precision, int32_t transA, int3
  * solveWithFactor (src/linalg/sparse_direct.cpp:38-63). */
 amls_status amls_hip_ldlt_solve(amls_ctx* ctx, int64_t n, const double* A, int64_t nrhs,
                                 const double* B, double* LD, double* X);
+/* amls_hip_ldlt_solve without any device allocation, on device buffers:
+ * A (n x n, lower used) is overwritten by L (strict lower) and D (diagonal),
+ * B (n x nrhs) by A^-1 B; the launches and results are those of
+ * amls_hip_ldlt_solve.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Its scratch and launch
+ * descriptors live in one 8 MiB array per device, held until the process
+ * ends; sizes that do not fit it (n up to about 4,000 fit) return
+ * AMLS_E_ARG.  Calls that overlap on one device race on that array. */
+amls_status amls_hip_ldlt_solve_noalloc(amls_ctx* ctx, int64_t n, double* A, int64_t nrhs,
+                                        double* B);
 /* Largest k eigenpairs of a symmetric A (n x n, lower used): lam descending,
  * V n x k orthonormal. */
 amls_status amls_hip_syevx_top(amls_ctx* ctx, int64_t n, const double* A, int64_t k,

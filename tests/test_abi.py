@@ -204,3 +204,18 @@ def test_sil_mode_entry_points():
     o.silMode = 3
     assert L.amls_hip_solve_numeric(ctx._h, ctypes.byref(o), None, None, None, None) == 1
     ctx.close()
+
+
+def test_noalloc_size_limit_on_host_context():
+    """The allocation-free factor-and-solve checks its scratch limit before
+    anything else touches the device: an order whose panel buffer alone
+    exceeds the 8 MiB scratch is AMLS_E_ARG; a fitting one needs a device."""
+    L = _lib.lib()
+    ctx = HipContext(-1)
+    assert L.amls_hip_ldlt_solve_noalloc(ctx._h, 20000, None, 1, None) == 1
+    assert "scratch" in L.amls_hip_last_error(ctx._h).decode()
+    assert L.amls_hip_ldlt_solve_noalloc(ctx._h, 0, None, 1, None) == 1
+    buf = np.zeros(16)
+    p = buf.ctypes.data
+    assert L.amls_hip_ldlt_solve_noalloc(ctx._h, 4, p, 1, p) == 6
+    ctx.close()
