@@ -678,14 +678,16 @@ def amls_kinv(ge: BlockGE, R: np.ndarray) -> np.ndarray:
 
 
 def subspace_iteration(K, M, ge: BlockGE, Q0: np.ndarray, steps: int, nev: int,
-                       perm: np.ndarray):
+                       perm: np.ndarray, every_step: bool = False):
     """Q <- K^-1 M Q, Rayleigh-Ritz with the original K, M after every step
     (Algorithm 5 of the paper, with an RR step per iteration).  Q0 and the
-    result are in the original ordering."""
+    result are in the original ordering.  every_step: also return the list
+    of the lowest nev Ritz values after each step."""
     Kr = sp.csr_matrix(K)[perm][:, perm]
     Mr = sp.csr_matrix(M)[perm][:, perm]
     Q = Q0[perm]
     theta = None
+    hist = []
     for _ in range(steps):
         R = Mr @ Q
         Y = amls_kinv(ge, R)
@@ -695,8 +697,11 @@ def subspace_iteration(K, M, ge: BlockGE, Q0: np.ndarray, steps: int, nev: int,
         Mc = 0.5 * (Mc + Mc.T)
         theta, V = sla.eigh(Kc, Mc)
         Q = Y @ V
+        hist.append(theta[:nev].copy())
     X = np.zeros_like(Q)
     X[perm] = Q
+    if every_step:
+        return theta[:nev], X[:, :nev], hist
     return theta[:nev], X[:, :nev]
 
 

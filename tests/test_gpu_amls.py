@@ -269,7 +269,8 @@ def test_c2_subspace_iteration_matches_oracle():
     whole AMLS basis): the FP64 and mixed-precision GPU eigenvalues agree with
     the oracle's FP64 AMLS + subspace iteration on the same tree within 1e-7
     relative (about 1e-10 measured), and both reach 1e-6 against the
-    committed FP64 ground-truth fixture."""
+    committed FP64 ground-truth fixture; three and four steps match the
+    oracle's after as many steps."""
     import os
     pb = cases.case_hex27_10()
     tree, _ = tree_of(pb.K, pb.M, pb.coords, 6, 500)
@@ -279,7 +280,7 @@ def test_c2_subspace_iteration_matches_oracle():
     z = ref.cond.backtransform(Q)
     X = np.zeros_like(z)
     X[tree.perm] = z
-    t_or, _ = O.subspace_iteration(pb.K, pb.M, ref.ge, X, 2, nev, tree.perm)
+    _, _, hist = O.subspace_iteration(pb.K, pb.M, ref.ge, X, 4, nev, tree.perm, every_step=True)
     exact = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "hex27_10_lowest100.txt"))
     solver = AmlsSolver(0)
     for prec in ("fp64", "mixed"):
@@ -287,8 +288,18 @@ def test_c2_subspace_iteration_matches_oracle():
                            AmlsOptions(nev=nev, nLevels=6, leafSize=500, thresholdFactor=4.0,
                                        precision=prec, siSteps=2, siVectors=10**6, verbosity=0))
         assert sol.p == ref.p
-        assert (np.abs(sol.theta - t_or) / t_or).max() <= 1e-7, prec
+        assert (np.abs(sol.theta - hist[1]) / hist[1]).max() <= 1e-7, prec
         assert O.rel_err(sol.theta, exact).max() <= 1e-6, prec
+    # three and four steps: the basis is M-orthonormalised after the second
+    # step (scaling alone makes the projected stiffness indefinite here)
+    for steps in (3, 4):
+        sol = solver.solve(Pencil(pb.K, pb.M, pb.coords),
+                           AmlsOptions(nev=nev, nLevels=6, leafSize=500, thresholdFactor=4.0,
+                                       precision="mixed", siSteps=steps, siVectors=10**6,
+                                       verbosity=0))
+        t_or = hist[steps - 1]
+        assert (np.abs(sol.theta - t_or) / t_or).max() <= 1e-7, steps
+        assert O.rel_err(sol.theta, exact).max() <= O.rel_err(hist[1], exact).max(), steps
 
 
 def test_context_reuse_smaller_then_larger_problem(blas1):
@@ -349,3 +360,69 @@ def test_duplicate_entries_summed_on_device_path(blas1):
     s.ctx._chk(st)
     assert p.value == ref.p
     assert np.abs(th - ref.theta).max() <= 1e-12 * ref.theta.max()
+
+
+@pytest.mark.parametrize("steps", [3, 4, 6])
+def test_subspace_iteration_many_steps(blas1, steps):
+    """More steps over the whole AMLS basis (q = p): the device
+    M-orthonormalises after every second step, and the Ritz values follow the
+    oracle's (Rayleigh-Ritz after every step) at 1e-9."""
+    pb = small_hex(seed=31, dims=(5, 5, 6))
+    tree, t = tree_of(pb.K, pb.M, pb.coords, 3, 60)
+    nev = 12
+    ref = O.amls_solve(pb.K, pb.M, tree, nev=nev, thresholdFactor=4.0, recoverVectors=False)
+    _, Q = ref.cond.solve_projected(ref.p)
+    z = ref.cond.backtransform(Q)
+    X = np.zeros_like(z)
+    X[tree.perm] = z
+    t_or, _ = O.subspace_iteration(pb.K, pb.M, ref.ge, X, steps, nev, tree.perm)
+    sol = AmlsSolver(0).solve(Pencil(pb.K, pb.M, pb.coords),
+                              AmlsOptions(nev=nev, nLevels=3, leafSize=60, thresholdFactor=4.0,
+                                          siSteps=steps, siVectors=10**6, verbosity=0))
+    assert np.abs(sol.theta - t_or).max() <= 1e-9 * t_or.max()
+    assert np.abs(np.einsum("ij,ij->j", sol.Z, pb.M @ sol.Z) - 1).max() <= 1e-10
+
+
+def test_indefinite_pencil_is_not_spd_end_to_end():
+    """A negated diagonal entry of K (tests/unit/test_linalg.cpp:69-78 in the
+    reference) makes the node holding it indefinite: amls_hip_solve returns
+    AMLS_E_NOT_SPD naming the node, in both precisions, and the context
+    solves a valid pencil afterwards."""
+    from amls_amd import AmlsError
+    pb = small_hex(seed=61, dims=(4, 4, 5))
+    K = pb.K.tolil()
+    K[7, 7] = -K[7, 7]
+    K = K.tocsc()
+    s = AmlsSolver(0)
+    for prec in ("fp64", "mixed"):
+        with pytest.raises(AmlsError) as e:
+            s.solve(Pencil(K, pb.M, pb.coords),
+                    AmlsOptions(nev=6, nLevels=3, leafSize=40, precision=prec, verbosity=0))
+        assert e.value.status == 2 and "node" in str(e.value)
+    sol = s.solve(Pencil(pb.K, pb.M, pb.coords), AmlsOptions(nev=6, nLevels=3, leafSize=40,
+                                                             verbosity=0))
+    exact = O.reference_eigs(pb.K, pb.M, 6)
+    assert np.all(sol.theta >= exact * (1 - 1e-12))
+
+
+def test_empty_or_small_reduced_problem_is_noconv():
+    """A cut below every local eigenvalue empties the reduced problem
+    (src/amls/solver.cpp:148-149 throws there): AMLS_E_NOCONV.  A reduced
+    problem smaller than nev returns its p eigenvalues for plain AMLS, as the
+    reference's solveProjected does, and is AMLS_E_NOCONV when the subspace
+    iteration needs nev vectors."""
+    from amls_amd import AmlsError
+    pb = small_hex(seed=62, dims=(4, 4, 5))
+    s = AmlsSolver(0)
+    with pytest.raises(AmlsError) as e:
+        s.solve(Pencil(pb.K, pb.M, pb.coords),
+                AmlsOptions(nev=6, nLevels=3, leafSize=40, thresholdFactor=1e-6, verbosity=0))
+    assert e.value.status == 3 and "empty" in str(e.value)
+    sol = s.solve(Pencil(pb.K, pb.M, pb.coords),
+                  AmlsOptions(nev=200, nLevels=3, leafSize=40, thresholdFactor=0.7, verbosity=0))
+    assert sol.p < 200 and sol.theta.size == sol.p
+    with pytest.raises(AmlsError) as e:
+        s.solve(Pencil(pb.K, pb.M, pb.coords),
+                AmlsOptions(nev=200, nLevels=3, leafSize=40, thresholdFactor=0.7, siSteps=2,
+                            verbosity=0))
+    assert e.value.status == 3
