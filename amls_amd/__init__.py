@@ -22,7 +22,8 @@ import scipy.sparse as sp
 from . import _lib
 from ._lib import AmlsError, AmlsOptionsC, dptr, iptr
 
-__all__ = ["AmlsOptions", "AmlsSolver", "Solution", "Pencil", "HipContext", "AmlsError"]
+__all__ = ["AmlsOptions", "AmlsSolver", "Solution", "Pencil", "HipContext", "AmlsError",
+           "DenseFactor"]
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
 
@@ -348,6 +349,58 @@ class HipContext:
                                               dptr(LD), dptr(X)))
         return LD, X
 
+    def gemm_gather(self, A, B, C0=None, m=None, n=None, k=None, alpha=1.0, beta=0.0,
+                    transA=False, transB=False, aRows=None, aCols=None, bRows=None, bCols=None,
+                    precision="fp64"):
+        """C = alpha op(A[aRows][:, aCols]) op(B[bRows][:, bCols]) + beta C, the
+        gathers done on the device (the GEMM of the ColCompOps methods)."""
+        A = np.asfortranarray(A, dtype=np.float64)
+        B = np.asfortranarray(B, dtype=np.float64)
+        idx = [None if v is None else np.ascontiguousarray(v, dtype=np.int32)
+               for v in (aRows, aCols, bRows, bCols)]
+        ar = A.shape[0] if idx[0] is None else idx[0].size
+        ac = A.shape[1] if idx[1] is None else idx[1].size
+        br = B.shape[0] if idx[2] is None else idx[2].size
+        bc = B.shape[1] if idx[3] is None else idx[3].size
+        m = (ac if transA else ar) if m is None else m
+        k = (ar if transA else ac) if k is None else k
+        n = (br if transB else bc) if n is None else n
+        Cm = (np.zeros((m, n), order="F") if C0 is None
+              else np.asfortranarray(C0, dtype=np.float64).copy(order="F"))
+        self._chk(self._L.amls_hip_gemm_gather(
+            self._h, PRECISIONS[precision], int(transA), int(transB), m, n, k, alpha, dptr(A),
+            max(A.shape[0], 1), A.shape[1], _lib.i32ptr(idx[0]), _lib.i32ptr(idx[1]), dptr(B),
+            max(B.shape[0], 1), B.shape[1], _lib.i32ptr(idx[2]), _lib.i32ptr(idx[3]), beta,
+            dptr(Cm), max(m, 1)))
+        return Cm
+
+    def ldlt_factor(self, A, pivoting: bool = True) -> "DenseFactor":
+        """ISparseDirectSolver::factorize on the device: the lower triangle of
+        the sparse or dense A, with Eigen LDLT's diagonal pivoting when
+        `pivoting` (DenseLDLTSolver) or none (SimplicialLDLT's natural
+        order)."""
+        kp, ki, kx = _csc(sp.csc_matrix(A))
+        h = C.c_void_p()
+        self._chk(self._L.amls_hip_ldlt_factor(self._h, A.shape[0], iptr(kp), iptr(ki), dptr(kx),
+                                               int(pivoting), C.byref(h)))
+        return DenseFactor(self, h, A.shape[0])
+
+    def dense_eig(self, K, M, range_: str = "all", lo: float = 0.0, hi: float = 0.0,
+                  il: int = 1, iu: int = 0, vectors: bool = True):
+        """IDenseEigensolver::solveAll / solveInterval (lo, hi] / solveIndex
+        [il, iu] (1-based) on K x = lambda M x: (w ascending, V M-orthonormal)."""
+        K = np.asfortranarray(K, dtype=np.float64)
+        M = np.asfortranarray(M, dtype=np.float64)
+        n = K.shape[0]
+        r = {"all": 0, "value": 1, "index": 2}[range_]
+        w = np.zeros(max(n, 1))
+        V = np.zeros((n, max(n, 1)), order="F") if vectors else None
+        m = C.c_int64()
+        self._chk(self._L.amls_hip_dense_eig(self._h, n, dptr(K), dptr(M), r, lo, hi, il, iu,
+                                             C.byref(m), dptr(w), dptr(V)))
+        mm = m.value
+        return w[:mm].copy(), (None if V is None else V[:, :mm].copy(order="F"))
+
     def syevx_top(self, A, k: int):
         A = np.asfortranarray(A, dtype=np.float64)
         n = A.shape[0]
@@ -355,6 +408,40 @@ class HipContext:
         V = np.zeros((n, max(k, 1)), order="F")
         self._chk(self._L.amls_hip_syevx_top(self._h, n, dptr(A), k, dptr(lam), dptr(V)))
         return lam[:k], V[:, :k]
+
+
+class DenseFactor:
+    """A device LDL^T factor (amls_factor): solve(B) = A^-1 B
+    (solveWithFactor, src/linalg/sparse_direct.hpp:23), factor() = (LD of
+    P A P^T, perm with perm[new] = old)."""
+
+    def __init__(self, ctx: HipContext, handle, n: int):
+        self.ctx, self.h, self.n = ctx, handle, n
+
+    def solve(self, B):
+        B = np.asfortranarray(B, dtype=np.float64).reshape(self.n, -1, order="F")
+        X = np.zeros_like(B, order="F")
+        self.ctx._chk(self.ctx._L.amls_hip_ldlt_factor_solve(self.ctx._h, self.h, B.shape[1],
+                                                             dptr(B), dptr(X)))
+        return X
+
+    def factor(self):
+        LD = np.zeros((self.n, self.n), order="F")
+        perm = np.zeros(self.n, dtype=np.int64)
+        self.ctx._chk(self.ctx._L.amls_hip_ldlt_factor_get(self.ctx._h, self.h, dptr(LD),
+                                                           iptr(perm)))
+        return LD, perm
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx._L.amls_hip_ldlt_factor_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class AmlsSolver:

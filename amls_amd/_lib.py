@@ -28,7 +28,9 @@ EXPORTS = [
     "amls_hip_profile", "amls_hip_kernel_stats", "amls_hip_adjoint",
     "amls_hip_set_problem_batch", "amls_hip_batch_info", "amls_hip_solve_batch",
     "amls_hip_autotune", "amls_hip_set_tuning", "amls_hip_set_sil_mode", "amls_hip_get_sil",
-    "amls_hip_ldlt_solve_noalloc",
+    "amls_hip_ldlt_solve_noalloc", "amls_hip_gemm_gather", "amls_hip_ldlt_factor",
+    "amls_hip_ldlt_factor_solve", "amls_hip_ldlt_factor_get", "amls_hip_ldlt_factor_free",
+    "amls_hip_dense_eig",
 ]
 
 
@@ -66,6 +68,7 @@ P = C.c_void_p
 I32, I64, D = C.c_int32, C.c_int64, C.c_double
 PD = C.POINTER(C.c_double)
 PI64 = C.POINTER(C.c_int64)
+PI32 = C.POINTER(C.c_int32)
 PP = C.POINTER(C.c_void_p)      # array of per-problem pointers
 
 
@@ -112,6 +115,13 @@ def lib() -> C.CDLL:
         "amls_hip_syevx_top": ([P, I64, PD, I64, PD, PD], C.c_int),
         "amls_hip_set_sil_mode": ([P, I32], C.c_int),
         "amls_hip_ldlt_solve_noalloc": ([P, I64, P, I64, P], C.c_int),
+        "amls_hip_gemm_gather": ([P, I32, I32, I32, I64, I64, I64, D, PD, I64, I64, PI32, PI32,
+                                  PD, I64, I64, PI32, PI32, D, PD, I64], C.c_int),
+        "amls_hip_ldlt_factor": ([P, I64, PI64, PI64, PD, I32, C.POINTER(P)], C.c_int),
+        "amls_hip_ldlt_factor_solve": ([P, P, I64, PD, PD], C.c_int),
+        "amls_hip_ldlt_factor_get": ([P, P, PD, PI64], C.c_int),
+        "amls_hip_ldlt_factor_free": ([P], C.c_int),
+        "amls_hip_dense_eig": ([P, I64, PD, PD, I32, D, D, I64, I64, PI64, PD, PD], C.c_int),
         "amls_hip_get_sil": ([P, C.POINTER(C.c_int32), PI64], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -127,6 +137,13 @@ def dptr(a):
         return None
     assert a.dtype == np.float64 and (a.flags.c_contiguous or a.flags.f_contiguous)
     return a.ctypes.data_as(PD)
+
+
+def i32ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(PI32)
 
 
 def iptr(a):

@@ -25,6 +25,7 @@ SOURCES = [
     CSRC / "host" / "tree.cpp",
     CSRC / "sil.cpp",
     CSRC / "noalloc.cpp",
+    CSRC / "operators.cpp",
     CSRC / "engine.cpp",
 ]
 

@@ -1910,8 +1910,7 @@ void addLdltBatch(Plan& plan, const std::vector<double*>& As, const std::vector<
 // for nrhs > 0, X := A^-1 X (forward solve, D^-1, backward solve).  W:
 // n x 64 panel scratch, dq: n doubles, *err: set to INT_MAX first, lowered
 // to 0 by a non-positive or non-finite pivot.
-void addLdltSolve(Plan& plan, double* dA, int64_t n, double* dX, int64_t nrhs, double* dW,
-                  double* dq, int* derr) {
+void addLdltFactor(Plan& plan, double* dA, int64_t n, double* dW, int* derr) {
     Blob& blob = *plan.blob;
     plan.fn([=](hipStream_t s) { launch_set_i32(derr, INT_MAX, s); });
     std::vector<dev::FrontDesc> fd(1);
@@ -1954,26 +1953,35 @@ void addLdltSolve(Plan& plan, double* dA, int64_t n, double* dX, int64_t nrhs, d
             plan.gemm(gb, false);
         }
     }
-    if (nrhs > 0) {
-        TrsmBatch t1, t2;
-        t1.add(dA, n, dX, n, n, nrhs, 0);
-        t2.add(dA, n, dX, n, n, nrhs, 1);
-        std::vector<dev::VecDesc> vd(1);
-        std::memset(vd.data(), 0, sizeof(dev::VecDesc));
-        vd[0].src = dA;
-        vd[0].dst = dq;
-        vd[0].ld = n;
-        vd[0].n = static_cast<int32_t>(n);
-        vd[0].op = 0;
-        // dq = D^-1/2 (vecop 0); the row scaling applied twice is D^-1
-        Copy2dBatch sc;
-        sc.add(cdesc(dX, n, dX, n, n, nrhs, 0, dq));
-        plan.trsm(t1);
-        plan.vec(vd);
-        plan.copy(sc);
-        plan.copy(sc);
-        plan.trsm(t2);
-    }
+}
+
+void addLdltSolveWith(Plan& plan, const double* dLD, int64_t n, double* dX, int64_t nrhs,
+                      double* dq) {
+    if (nrhs <= 0) return;
+    TrsmBatch t1, t2;
+    t1.add(dLD, n, dX, n, n, nrhs, 0);
+    t2.add(dLD, n, dX, n, n, nrhs, 1);
+    std::vector<dev::VecDesc> vd(1);
+    std::memset(vd.data(), 0, sizeof(dev::VecDesc));
+    vd[0].src = dLD;
+    vd[0].dst = dq;
+    vd[0].ld = n;
+    vd[0].n = static_cast<int32_t>(n);
+    vd[0].op = 0;
+    // dq = D^-1/2 (vecop 0); the row scaling applied twice is D^-1
+    Copy2dBatch sc;
+    sc.add(cdesc(dX, n, dX, n, n, nrhs, 0, dq));
+    plan.trsm(t1);
+    plan.vec(vd);
+    plan.copy(sc);
+    plan.copy(sc);
+    plan.trsm(t2);
+}
+
+void addLdltSolve(Plan& plan, double* dA, int64_t n, double* dX, int64_t nrhs, double* dW,
+                  double* dq, int* derr) {
+    addLdltFactor(plan, dA, n, dW, derr);
+    addLdltSolveWith(plan, dA, n, dX, nrhs, dq);
 }
 
 // ---------------------------------------------------------------------------
@@ -2491,6 +2499,10 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
 // ===========================================================================
 struct amls_ctx {
     amls::Ctx c;
+};
+
+struct amls_factor {
+    amls::DenseFactor f;
 };
 
 namespace {
@@ -3190,6 +3202,66 @@ amls_status amls_hip_ldlt_solve(amls_ctx* ctx, int64_t n, const double* A, int64
             AMLS_HIP_CHECK(hipMemcpyAsync(X, dX, n * nrhs * 8, hipMemcpyDeviceToHost, c.stream));
         c.sync();
         if (e != INT_MAX) throw StatusError(AMLS_E_NOT_SPD, "LDL^T: non-positive or non-finite pivot");
+    });
+}
+
+amls_status amls_hip_gemm_gather(amls_ctx* ctx, int32_t precision, int32_t transA, int32_t transB,
+                                 int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                                 int64_t lda, int64_t aCols, const int32_t* aRowIdx,
+                                 const int32_t* aColIdx, const double* B, int64_t ldb, int64_t bCols,
+                                 const int32_t* bRowIdx, const int32_t* bColIdx, double beta,
+                                 double* C, int64_t ldc) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (c.gpu) AMLS_HIP_CHECK(hipSetDevice(c.device));
+        amls::gemmGather(c, precision, transA, transB, m, n, k, alpha, A, lda, aCols, aRowIdx,
+                         aColIdx, B, ldb, bCols, bRowIdx, bColIdx, beta, C, ldc);
+    });
+}
+
+amls_status amls_hip_ldlt_factor(amls_ctx* ctx, int64_t n, const int64_t* colptr,
+                                 const int64_t* rowidx, const double* val, int32_t pivoting,
+                                 amls_factor** out) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (!out) throw amls::StatusError(AMLS_E_ARG, "null factor handle");
+        *out = nullptr;
+        if (c.gpu) AMLS_HIP_CHECK(hipSetDevice(c.device));
+        std::unique_ptr<amls_factor> f(new amls_factor());
+        amls::ldltFactor(c, f->f, n, colptr, rowidx, val, pivoting);
+        *out = f.release();
+    });
+}
+
+amls_status amls_hip_ldlt_factor_solve(amls_ctx* ctx, const amls_factor* f, int64_t nrhs,
+                                       const double* B, double* X) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (!f) throw amls::StatusError(AMLS_E_ARG, "null factor handle");
+        if (c.gpu) AMLS_HIP_CHECK(hipSetDevice(c.device));
+        amls::ldltFactorSolve(c, f->f, nrhs, B, X);
+    });
+}
+
+amls_status amls_hip_ldlt_factor_get(amls_ctx* ctx, const amls_factor* f, double* LD,
+                                     int64_t* perm) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (!f) throw amls::StatusError(AMLS_E_ARG, "null factor handle");
+        if (c.gpu) AMLS_HIP_CHECK(hipSetDevice(c.device));
+        amls::ldltFactorGet(c, f->f, LD, perm);
+    });
+}
+
+amls_status amls_hip_ldlt_factor_free(amls_factor* f) {
+    if (!f) return AMLS_E_ARG;
+    if (f->f.device >= 0) (void)hipSetDevice(f->f.device);
+    delete f;
+    return AMLS_OK;
+}
+
+amls_status amls_hip_dense_eig(amls_ctx* ctx, int64_t n, const double* K, const double* M,
+                               int32_t range, double lo, double hi, int64_t il, int64_t iu,
+                               int64_t* m, double* w, double* V) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (c.gpu) AMLS_HIP_CHECK(hipSetDevice(c.device));
+        amls::denseEig(c, n, K, M, range, lo, hi, il, iu, m, w, V);
     });
 }
 

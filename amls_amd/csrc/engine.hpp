@@ -341,6 +341,30 @@ struct Ctx {
 
 };
 
+// Operator-granular entry points (operators.cpp).
+// A reusable dense LDL^T factor of P A P^T on the device (perm[new] = old).
+struct DenseFactor {
+    int device = -1;
+    int64_t n = 0;
+    std::unique_ptr<DevArena> mem;
+    double* LD = nullptr;        // L strict lower, D on the diagonal
+    double* dq = nullptr;        // n doubles of solve scratch
+    int32_t *p32 = nullptr, *ip32 = nullptr;   // perm and its inverse
+    std::vector<int64_t> perm;
+};
+std::vector<int64_t> eigenLdltOrder(const std::vector<double>& diag);
+void gemmGather(Ctx& c, int precision, int ta, int tb, int64_t m, int64_t n, int64_t k,
+                double alpha, const double* A, int64_t lda, int64_t aColsStored,
+                const int32_t* aRows, const int32_t* aCols, const double* B, int64_t ldb,
+                int64_t bColsStored, const int32_t* bRows, const int32_t* bCols, double beta,
+                double* C, int64_t ldc);
+void ldltFactor(Ctx& c, DenseFactor& f, int64_t n, const int64_t* cp, const int64_t* ri,
+                const double* vx, int pivoting);
+void ldltFactorSolve(Ctx& c, const DenseFactor& f, int64_t nrhs, const double* B, double* X);
+void ldltFactorGet(Ctx& c, const DenseFactor& f, double* LD, int64_t* perm);
+void denseEig(Ctx& c, int64_t n, const double* K, const double* M, int range, double lo, double hi,
+              int64_t il, int64_t iu, int64_t* mOut, double* w, double* V);
+
 // The allocation-free factor-and-solve (noalloc.cpp): scratchInit records,
 // with `device` current, the device address of its file-scope scratch array.
 hipError_t scratchInit(int device);

@@ -118,6 +118,17 @@ struct GatherDesc {
     int32_t scatter, pad;
 };
 
+// dst(r, c) = src(ri ? ri[r] : r, ci ? ci[c] : c) for r < rows, c < cols
+// (int32 index lists: the stored-column lists of the reference's ColComp)
+struct Gather2dDesc {
+    const double* src;
+    double* dst;
+    const int32_t* ri;
+    const int32_t* ci;
+    int64_t lds, ldd;
+    int32_t rows, cols;
+};
+
 // Small per-matrix vector operations (one block per descriptor):
 //   op 0: dst[i] = 1 / sqrt(src[i + i*ld])       (D^{-1/2} from an LDL^T diagonal)
 //   op 1: dst[i] = 1 / src[i], dst2[i] = sqrt(1 / src[i])

@@ -257,6 +257,31 @@ __global__ void gather_rows_kernel(const GatherDesc* __restrict__ descs, const i
     }
 }
 
+// dst(r, c) = src(ri[r], ci[c]) (identity where a list is null); work item
+// (descriptor, row block, column block) of 64 x 64
+__global__ void __launch_bounds__(256) gather2d_kernel(const Gather2dDesc* __restrict__ descs,
+                                                       const int4* __restrict__ work) {
+    const int4 w = work[blockIdx.x];
+    const Gather2dDesc d = descs[w.x];
+    const int t = threadIdx.x;
+    const int r = w.y * 64 + (t & 63);
+    if (r >= d.rows) return;
+    const int64_t sr = d.ri ? d.ri[r] : r;
+    const int cb = w.z * 64 + (t >> 6);
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int c = cb + 4 * q;
+        const int64_t sc = c < d.cols ? (d.ci ? d.ci[c] : c) : 0;
+        v[q] = c < d.cols ? d.src[sr + sc * d.lds] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int c = cb + 4 * q;
+        if (c < d.cols) d.dst[r + (int64_t)c * d.ldd] = v[q];
+    }
+}
+
 __global__ void vecop_kernel(const VecDesc* __restrict__ descs) {
     const VecDesc d = descs[blockIdx.x];
     for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
@@ -373,6 +398,11 @@ void launch_fill(double* p, int64_t count, double v, hipStream_t s) {
 void launch_copy2d(const dev::Copy2dDesc* d, const int4* work, int nwork, hipStream_t s) {
     if (nwork <= 0) return;
     hipLaunchKernelGGL(dev::copy2d_kernel, dim3(nwork), dim3(256), 0, s, d, work);
+}
+
+void launch_gather2d(const dev::Gather2dDesc* d, const int4* work, int nwork, hipStream_t s) {
+    if (nwork <= 0) return;
+    hipLaunchKernelGGL(dev::gather2d_kernel, dim3(nwork), dim3(256), 0, s, d, work);
 }
 
 void launch_gather(const dev::GatherDesc* d, const int4* work, int nwork, hipStream_t s) {

@@ -37,6 +37,8 @@ void launch_noop(hipStream_t s);   // empty kernel (dispatch-gap calibration)
 void launch_vecop(const dev::VecDesc* d, int count, hipStream_t s);
 void launch_copy2d(const dev::Copy2dDesc* d, const int4* work, int nwork, hipStream_t s);
 void launch_gather(const dev::GatherDesc* d, const int4* work, int nwork, hipStream_t s);
+// work items (descriptor, 64-row block, 64-column block)
+void launch_gather2d(const dev::Gather2dDesc* d, const int4* work, int nwork, hipStream_t s);
 // Step i of the tiled tridiagonalisation; sv holds 2 soff-strided vectors
 // of n per matrix and sp 3 (soff[m] = element offset / stride unit n_m).
 void launch_sytrd_step(const dev::EigDesc* d, const int4* work, int nwork, int i, double* sv,

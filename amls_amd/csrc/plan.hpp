@@ -322,5 +322,10 @@ void addLdltBatch(Plan& plan, const std::vector<double*>& As, const std::vector<
 // pivot is non-positive or non-finite (then 0).
 void addLdltSolve(Plan& plan, double* dA, int64_t n, double* dX, int64_t nrhs, double* dW,
                   double* dq, int* derr);
+// The two halves of addLdltSolve: the factorisation, and X := A^-1 X with a
+// factor it left (L strict lower, D on the diagonal of dLD).
+void addLdltFactor(Plan& plan, double* dA, int64_t n, double* dW, int* derr);
+void addLdltSolveWith(Plan& plan, const double* dLD, int64_t n, double* dX, int64_t nrhs,
+                      double* dq);
 
 }  // namespace amls

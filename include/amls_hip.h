@@ -314,6 +314,65 @@ amls_status amls_hip_get_local(amls_ctx* ctx, int64_t p, double* mu, double* Z);
 amls_status amls_hip_get_reduced(amls_ctx* ctx, double* Mtilde, int64_t* moff);
 
 /* ---- operator-granular entry points (host buffers, one matrix) ------- */
+/*
+ * The GEMM behind every ColCompOps method (src/linalg/colcomp.hpp:124-171,
+ * backend OpenBlasColCompOps :174-423): C = alpha op(Ag) op(Bg) + beta C
+ * with the gathered operands Ag(i, j) = A(aRowIdx[i], aColIdx[j]) and
+ * Bg(i, j) = B(bRowIdx[i], bColIdx[j]) (a null list is the identity).  A is
+ * the stored lda x aCols block, B the stored ldb x bCols block; Ag is
+ * (transA ? k : m) x (transA ? m : k), Bg is (transB ? n : k) x
+ * (transB ? k : n).  The lists are the int32 stored-column lists of the
+ * reference's ColComp (`cols`): subMul(A, B, C) on compressed blocks is
+ * A.data -= B.data C(B.cols, A.cols), i.e. alpha = -1, beta = 1,
+ * bRowIdx = B.cols, bColIdx = A.cols; the transposed terms (subTransMul,
+ * addTransMul) gather both operands.  precision 0 / 1 as for amls_hip_gemm.
+ */
+amls_status amls_hip_gemm_gather(amls_ctx* ctx, int32_t precision, int32_t transA, int32_t transB,
+                                 int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                                 int64_t lda, int64_t aCols, const int32_t* aRowIdx,
+                                 const int32_t* aColIdx, const double* B, int64_t ldb, int64_t bCols,
+                                 const int32_t* bRowIdx, const int32_t* bColIdx, double beta,
+                                 double* C, int64_t ldc);
+
+/*
+ * ISparseDirectSolver::factorize / solveWithFactor
+ * (src/linalg/sparse_direct.hpp:13-64, .cpp:10-63) with the factor kept on
+ * the device.  A: n x n CSC (int64 indices), its lower triangle used, as
+ * Eigen's LDLT and SimplicialLDLT read it.  pivoting 1: DenseLDLTSolver's
+ * Eigen::LDLT diagonal pivoting, P A P^T = L D L^T with the pivot order
+ * Eigen picks (the first largest remaining |A_ii| at each step: its
+ * factorisation is left-looking, so the order depends on the input
+ * diagonal alone); pivoting 0: no permutation.  A non-positive or
+ * non-finite D entry is AMLS_E_NOT_SPD (the reference's "P10" check).
+ * factor_solve: X = A^-1 B (n x nrhs); factor_get: LD (n x n, L strict
+ * lower, D diagonal, of P A P^T) and perm (perm[new] = old), either may be
+ * null.  factor_free releases the handle (no context needed).
+ */
+typedef struct amls_factor amls_factor;
+amls_status amls_hip_ldlt_factor(amls_ctx* ctx, int64_t n, const int64_t* colptr,
+                                 const int64_t* rowidx, const double* val, int32_t pivoting,
+                                 amls_factor** out);
+amls_status amls_hip_ldlt_factor_solve(amls_ctx* ctx, const amls_factor* f, int64_t nrhs,
+                                       const double* B, double* X);
+amls_status amls_hip_ldlt_factor_get(amls_ctx* ctx, const amls_factor* f, double* LD,
+                                     int64_t* perm);
+amls_status amls_hip_ldlt_factor_free(amls_factor* f);
+
+/*
+ * IDenseEigensolver (src/linalg/dense_eigensolver.hpp:10-31) on the pencil
+ * K x = lambda M x, K and M n x n column-major (lower triangles used), M SPD,
+ * as LAPACK ?sygvd / ?sygvx with itype 1, uplo 'L' compute it
+ * (dense_eigensolver.cpp:11-133): range 0 all pairs (solveAll), 1 lambda in
+ * (lo, hi] (solveInterval, RANGE 'V'), 2 the il-th to iu-th lowest, 1-based
+ * (solveIndex, RANGE 'I', iu clamped to n).  *m receives the count, w (m)
+ * the eigenvalues ascending, V (n x m, may be null) M-orthonormal vectors.
+ * n = 0, hi <= lo or iu < il give m = 0 as in the reference; an M that is
+ * not positive definite is AMLS_E_NOT_SPD.
+ */
+amls_status amls_hip_dense_eig(amls_ctx* ctx, int64_t n, const double* K, const double* M,
+                               int32_t range, double lo, double hi, int64_t il, int64_t iu,
+                               int64_t* m, double* w, double* V);
+
 /* C = alpha op(A) op(B) + beta C (column-major; trans 0/1; precision 0/1).
  * The GEMM of OpenBlasColCompOps::dgemm (src/linalg/colcomp.hpp:355-364). */
 amls_status amls_hip_gemm(amls_ctx* ctx, int32_t precision, int32_t transA, int32_t transB,

@@ -762,3 +762,153 @@ def eig_adjoint(conn: np.ndarray, Ke0: np.ndarray, Me0: np.ndarray, theta: np.nd
     lam = np.asarray(theta) - shift
     gr = -np.einsum("eij,ik,ekj->ej", X, Me0, X) * lam[None, :]
     return gs, gr
+
+
+# --------------------------------------------------------------------------
+# Operator interfaces: ColCompOps (src/linalg/colcomp.hpp:124-423) and the
+# dense LDL^T of DenseLDLTSolver (src/linalg/sparse_direct.cpp:38-63)
+# --------------------------------------------------------------------------
+
+@dataclass
+class ColComp:
+    """ColComp (colcomp.hpp:26-75): data holds the stored columns of a
+    logical rows x full block, cols (ascending) their logical numbers; cols
+    None is the identity map (ident)."""
+    data: np.ndarray
+    full: int
+    cols: Optional[np.ndarray] = None
+
+    @property
+    def ident(self) -> bool:
+        return self.cols is None
+
+    def stored(self) -> int:
+        return self.full if self.ident else len(self.cols)
+
+    def empty(self) -> bool:
+        return self.stored() == 0
+
+    def logical(self) -> np.ndarray:
+        """The rows x full block the compressed storage stands for."""
+        if self.ident:
+            return self.data.copy()
+        out = np.zeros((self.data.shape[0], self.full))
+        out[:, self.cols] = self.data
+        return out
+
+
+class NumpyColCompOps:
+    """OpenBlasColCompOps (colcomp.hpp:174-423) restated on numpy arrays.
+    Operands: Dense = ndarray, ColComp, Sparse = scipy sparse.  The
+    transposed terms restrict (B C)^T to the target's stored columns with
+    S = B.data(A.cols, :) (the rows of B's stored block that the target
+    keeps, all of B's stored columns); the reference's compressed-B branch
+    of accMul (:396-397) writes sub(B.data, A.cols, B.cols), which reads
+    B.data's columns at logical numbers."""
+
+    @staticmethod
+    def _rows(X, idx):
+        return X[np.asarray(idx)]
+
+    def gemm(self, ta: bool, tb: bool, alpha, A, B, beta, C):
+        if isinstance(A, ColComp):
+            if not A.ident and len(A.cols) == 0:
+                return beta * C
+            if A.ident:
+                A = A.data
+            elif not ta:
+                Bg = self._rows(B, A.cols)
+                return alpha * (A.data @ (Bg.T if tb else Bg)) + beta * C
+            else:
+                A = A.data          # stored-row-space result (:132-137)
+        opA = A.T if ta else A
+        opB = B.T if tb else B
+        return alpha * (opA @ opB) + beta * C
+
+    def subMul(self, A, B, C):
+        """A -= B C; returns the updated A (ndarray or ColComp)."""
+        return self._acc(A, B, C, -1.0, False)
+
+    def addMul(self, A, B, C):
+        return self._acc(A, B, C, +1.0, False)
+
+    def subTransMul(self, A, B, C):
+        return self._acc(A, B, C, -1.0, True)
+
+    def addTransMul(self, A, B, C):
+        return self._acc(A, B, C, +1.0, True)
+
+    def _acc(self, A, B, C, sign, trans):
+        if sp.issparse(B):
+            B = ColComp(np.asarray(B.todense()), B.shape[1])
+        if B.empty():
+            return A
+        if isinstance(A, ColComp) and A.empty():
+            return A
+        dense_target = not isinstance(A, ColComp)
+        data = A if dense_target else A.data
+        acols = None if dense_target or A.ident else A.cols
+        if not trans:
+            Cg = C if B.ident else self._rows(C, B.cols)
+            if acols is not None:
+                Cg = Cg[:, acols]
+            data = data + sign * (B.data @ Cg)
+        else:
+            Cg = C if B.ident else self._rows(C, B.cols)
+            S = B.data if acols is None else B.data[acols]
+            data = data + sign * (Cg.T @ S.T)
+        if dense_target:
+            return data
+        return ColComp(data, A.full, A.cols)
+
+    def mul(self, B, C):
+        if sp.issparse(B):
+            return np.asarray(B @ C)
+        return self.gemm(False, False, 1.0, B, C, 0.0, np.zeros((B.data.shape[0], C.shape[1])))
+
+    def expandT(self, B):
+        if sp.issparse(B):
+            return np.asarray(B.T.todense())
+        return B.logical().T
+
+
+def eigen_ldlt(A: np.ndarray):
+    """Eigen's LDLT<Lower> as DenseLDLTSolver uses it (Eigen 3, unblocked
+    left-looking factorisation with diagonal pivoting): step k takes the
+    first largest |diagonal| of positions k..n-1 of the working matrix,
+    swaps it into k (a symmetric permutation of the lower-stored matrix),
+    then forms column k from the columns before it.  Returns perm
+    (perm[new] = old), unit lower L and D with A[perm][:, perm] = L D L^T."""
+    n = A.shape[0]
+    mat = np.tril(np.asarray(A, dtype=np.float64)).copy()
+    perm = np.arange(n)
+    for k in range(n):
+        idx = k + int(np.argmax(np.abs(np.diag(mat)[k:])))
+        if idx != k:
+            mat[[k, idx], :k] = mat[[idx, k], :k]
+            tail = mat[idx + 1:, k].copy()
+            mat[idx + 1:, k] = mat[idx + 1:, idx]
+            mat[idx + 1:, idx] = tail
+            mat[k, k], mat[idx, idx] = mat[idx, idx], mat[k, k]
+            for i in range(k + 1, idx):
+                mat[i, k], mat[idx, i] = mat[idx, i], mat[i, k]
+            perm[k], perm[idx] = perm[idx], perm[k]
+        if k > 0:
+            temp = np.diag(mat)[:k] * mat[k, :k]
+            mat[k, k] -= mat[k, :k] @ temp
+            mat[k + 1:, k] -= mat[k + 1:, :k] @ temp
+        if k + 1 < n and mat[k, k] != 0.0:
+            mat[k + 1:, k] /= mat[k, k]
+    return perm, np.tril(mat, -1) + np.eye(n), np.diag(mat).copy()
+
+
+def diagonal_selection_order(diag: np.ndarray) -> np.ndarray:
+    """The pivot order eigen_ldlt ends with, from the input diagonal alone:
+    the first largest |d| of positions k..n-1 swapped into k, k = 0..n-1."""
+    a = np.abs(np.asarray(diag, dtype=np.float64)).copy()
+    pos = np.arange(a.size)
+    for k in range(a.size):
+        j = k + int(np.argmax(a[k:]))
+        a[[k, j]] = a[[j, k]]
+        pos[[k, j]] = pos[[j, k]]
+    return pos
