@@ -213,6 +213,20 @@ class HipContext:
                                            iptr(steps)))
         return path, steps
 
+    def set_rr_mode(self, mode: int):
+        """Solver of the final Rayleigh-Ritz problem of the subspace
+        iteration: 0 Chebyshev-filtered subspace iteration where it applies
+        (dense fallback), 1 always the dense reduction."""
+        self._chk(self._L.amls_hip_set_rr_mode(self._h, int(mode)))
+
+    def rr_info(self):
+        """(filter iterations, largest relative Ritz residual) of the last
+        final Rayleigh-Ritz; 0 iterations when the dense path gave it."""
+        it = C.c_int32()
+        res = C.c_double()
+        self._chk(self._L.amls_hip_get_rr_info(self._h, C.byref(it), C.byref(res)))
+        return it.value, res.value
+
     def spectral(self, kind: str, value: float) -> np.ndarray:
         N = self.tree()["size"].shape[0]
         k = np.zeros(N, dtype=np.int64)

@@ -30,7 +30,7 @@ EXPORTS = [
     "amls_hip_autotune", "amls_hip_set_tuning", "amls_hip_set_sil_mode", "amls_hip_get_sil",
     "amls_hip_ldlt_solve_noalloc", "amls_hip_gemm_gather", "amls_hip_ldlt_factor",
     "amls_hip_ldlt_factor_solve", "amls_hip_ldlt_factor_get", "amls_hip_ldlt_factor_free",
-    "amls_hip_dense_eig",
+    "amls_hip_dense_eig", "amls_hip_set_rr_mode", "amls_hip_get_rr_info",
 ]
 
 
@@ -123,6 +123,8 @@ def lib() -> C.CDLL:
         "amls_hip_ldlt_factor_free": ([P], C.c_int),
         "amls_hip_dense_eig": ([P, I64, PD, PD, I32, D, D, I64, I64, PI64, PD, PD], C.c_int),
         "amls_hip_get_sil": ([P, C.POINTER(C.c_int32), PI64], C.c_int),
+        "amls_hip_set_rr_mode": ([P, I32], C.c_int),
+        "amls_hip_get_rr_info": ([P, C.POINTER(C.c_int32), PD], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -22,10 +22,12 @@ SOURCES = [
     CSRC / "kernels" / "sparse.hip",
     CSRC / "kernels" / "adjoint.hip",
     CSRC / "kernels" / "lanczos.hip",
+    CSRC / "kernels" / "chfsi.hip",
     CSRC / "host" / "tree.cpp",
     CSRC / "sil.cpp",
     CSRC / "noalloc.cpp",
     CSRC / "operators.cpp",
+    CSRC / "chfsi.cpp",
     CSRC / "engine.cpp",
 ]
 

@@ -535,9 +535,11 @@ void GemmBatch::add(const GemmDesc& d0) {
     for (int i = 0; i < tm; ++i) ntile += d.lower ? std::min(i + 1, tn) : tn;
     // long-k products with few tiles (the Gram matrices of the subspace
     // iteration: k = n) are split over k so the whole device takes part;
-    // the pieces add into C atomically (beta must be 0 or 1)
+    // the pieces add into C atomically (beta 0 or 1, or any beta for a full
+    // output, which the plan scales by beta beforehand)
     int nsplit = 1;
-    const bool splittable = d.triA == 0 && !(d.A2 && d.k2 > 0) && (d.beta == 0.0 || d.beta == 1.0);
+    const bool splittable = d.triA == 0 && !(d.A2 && d.k2 > 0) &&
+                            (d.beta == 0.0 || d.beta == 1.0 || !d.lower);
     if (splittable && forceSplit > 0) {
         nsplit = forceSplit;
     } else if (splittable && d.k >= 8192 && t_splitTable && !t_splitTable->empty()) {
@@ -2445,8 +2447,26 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     plan.gemm(gl2, false);
     plan.vec(vd);
     plan.copy(c3);
-    addSytrd(plan, eds, soff, sv, sp);
-    addSelect(plan, eds, true);
+    // the nev largest eigenpairs of C: Chebyshev-filtered subspace iteration
+    // for one wide problem (q = p), the dense reduction otherwise or when it
+    // does not converge
+    bool dense = true;
+    lastRrIters = 0;
+    lastRrResidual = 0.0;
+    if (NS == 1 && rrMode == 0 && chfsiBlock(q, nev) > 0) {
+        plan.run(stream);
+        dense = !chfsiTop(Cm, q, nev, lam, Yt);
+        plan.steps.clear();
+        plan.blob = &fresh(siBlob2);
+        if (dense) {
+            lastRrIters = 0;
+            lastRrResidual = 0.0;
+        }
+    }
+    if (dense) {
+        addSytrd(plan, eds, soff, sv, sp);
+        addSelect(plan, eds, true);
+    }
     plan.vec(v2);
     plan.copy(cv);
     plan.gemm(gv, false);
@@ -2716,6 +2736,20 @@ amls_status amls_hip_get_sil(amls_ctx* ctx, int32_t* path, int64_t* steps) {
             if (path) path[p] = c.lastPath.empty() ? 0 : c.lastPath[p];
             if (steps) steps[p] = c.lastSteps.empty() ? 0 : c.lastSteps[p];
         }
+    });
+}
+
+amls_status amls_hip_set_rr_mode(amls_ctx* ctx, int32_t mode) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (mode != 0 && mode != 1) throw amls::StatusError(AMLS_E_ARG, "rr mode must be 0 or 1");
+        c.rrMode = mode;
+    });
+}
+
+amls_status amls_hip_get_rr_info(amls_ctx* ctx, int32_t* iters, double* residual) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (iters) *iters = c.lastRrIters;
+        if (residual) *residual = c.lastRrResidual;
     });
 }
 

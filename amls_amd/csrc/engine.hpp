@@ -295,7 +295,17 @@ struct Ctx {
     std::unique_ptr<DevArena> siMem, outMem;
     int64_t outElems = 0;        // doubles outBuf holds (n x nev of the solve that sized it)
     double* outBuf = nullptr;
-    std::unique_ptr<Blob> siBlob;
+    std::unique_ptr<Blob> siBlob, siBlob2;
+
+    // ---- top eigenpairs of the final Rayleigh-Ritz problem by Chebyshev-
+    // filtered subspace iteration (chfsi.cpp); the dense path stays the
+    // fallback
+    std::unique_ptr<DevArena> chfMem;
+    std::unique_ptr<Blob> chfBlob;
+    int rrMode = 0;          // 0: filtered iteration where it applies, 1: dense only
+    int lastRrIters = 0;     // filter iterations of the last final Rayleigh-Ritz, 0: dense path
+    double lastRrResidual = 0.0;
+    bool chfsiTop(double* C, int64_t q, int64_t m, double* lam, double* V);   // C made symmetric from its lower triangle
 
     SplitTable splitTable;   // empty: the built-in split-k rule
     void autotune();
@@ -362,6 +372,9 @@ void ldltFactor(Ctx& c, DenseFactor& f, int64_t n, const int64_t* cp, const int6
                 const double* vx, int pivoting);
 void ldltFactorSolve(Ctx& c, const DenseFactor& f, int64_t nrhs, const double* B, double* X);
 void ldltFactorGet(Ctx& c, const DenseFactor& f, double* LD, int64_t* perm);
+// block width of the filtered iteration for m of the largest eigenpairs of a
+// q x q matrix, 0 where it does not apply (chfsi.cpp)
+int64_t chfsiBlock(int64_t q, int64_t m);
 void denseEig(Ctx& c, int64_t n, const double* K, const double* M, int range, double lo, double hi,
               int64_t il, int64_t iu, int64_t* mOut, double* w, double* V);
 

@@ -138,6 +138,7 @@ struct Gather2dDesc {
 //   op 5: dst[i] = 1 / src[i + i*ld]             (D^{-1} from an LDL^T diagonal)
 //   op 6: dst[i] = sqrt(src[i + i*ld])           (D^{1/2} from an LDL^T diagonal)
 //   op 7: dst[i] = 1 / src[i] + a                (shift-invert Ritz values)
+//   op 8: dst[i + i*ld] += a                     (diagonal shift)
 struct VecDesc {
     const double* src;
     double* dst;

@@ -298,6 +298,7 @@ __global__ void vecop_kernel(const VecDesc* __restrict__ descs) {
             case 5: d.dst[i] = 1.0 / d.src[i + (int64_t)i * d.ld]; break;
             case 6: d.dst[i] = sqrt(d.src[i + (int64_t)i * d.ld]); break;
             case 7: d.dst[i] = 1.0 / d.src[i] + d.a; break;
+            case 8: d.dst[i + (int64_t)i * d.ld] += d.a; break;
             default: d.dst[i] = sqrt(1.0 / d.src[i]); break;
         }
     }

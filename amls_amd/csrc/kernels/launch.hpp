@@ -39,6 +39,11 @@ void launch_copy2d(const dev::Copy2dDesc* d, const int4* work, int nwork, hipStr
 void launch_gather(const dev::GatherDesc* d, const int4* work, int nwork, hipStream_t s);
 // work items (descriptor, 64-row block, 64-column block)
 void launch_gather2d(const dev::Gather2dDesc* d, const int4* work, int nwork, hipStream_t s);
+// x[0 .. n) = reproducible uniform values in [-1, 1) from seed
+void launch_fill_hash(double* x, int64_t n, uint64_t seed, hipStream_t s);
+// res[j] = || R(:, j) - lam[j] V(:, j) || for j < m (q x m blocks, ld q)
+void launch_ritz_resid(const double* R, const double* V, const double* lam, int64_t q, int m,
+                       double* res, hipStream_t s);
 // Step i of the tiled tridiagonalisation; sv holds 2 soff-strided vectors
 // of n per matrix and sp 3 (soff[m] = element offset / stride unit n_m).
 void launch_sytrd_step(const dev::EigDesc* d, const int4* work, int nwork, int i, double* sv,

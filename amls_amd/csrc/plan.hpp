@@ -155,13 +155,16 @@ struct Plan {
 
     void gemm(const GemmBatch& b, bool fp32) {
         if (b.empty()) return;
-        // split-k products accumulate into C: clear it first when beta = 0
+        // split-k products accumulate into C: clear it first when beta = 0,
+        // scale it by beta first otherwise (beta != 1 only for full outputs)
         Copy2dBatch clr;
         for (const auto& d : b.descs)
             if (d.kchunk > 0 && d.beta == 0.0) {
                 Copy2dDesc c = cdesc(nullptr, 0, d.C, d.ldc, d.m, d.n, 3, nullptr, nullptr, 0.0);
                 c.pad = d.lower;
                 clr.add(c);
+            } else if (d.kchunk > 0 && d.beta != 1.0) {
+                clr.add(cdesc(d.C, d.ldc, d.C, d.ldc, d.m, d.n, 0, nullptr, nullptr, d.beta));
             }
         copy(clr);
         const size_t od = blob->add(b.descs), ot = blob->add(b.tiles);

@@ -206,6 +206,22 @@ amls_status amls_hip_subspace_iter(amls_ctx* ctx, int32_t steps, int64_t q, int6
                                    double* theta, double* Z);
 
 /*
+ * Solver of the final Rayleigh-Ritz problem's nev largest eigenpairs (the
+ * q x q matrix C = D^-1/2 L^-1 (Y^T M Y) L^-T D^-1/2 with Y^T K Y = L D L^T):
+ * mode 0 (default) uses Chebyshev-filtered subspace iteration on a block of
+ * at most 192 columns when one problem is solved, nev <= 144 and q is at
+ * least four block widths, and falls back to the dense reduction when it
+ * does not converge (Ritz residuals above 1e-11 of the largest eigenvalue);
+ * mode 1 always takes the dense reduction.  get_rr_info: the filter
+ * iterations of the last Rayleigh-Ritz (0 when the dense path gave the
+ * result) and its largest relative Ritz residual.  Either pointer may be
+ * null.  No counterpart in the reference (its subspace iteration is absent
+ * from the sources).
+ */
+amls_status amls_hip_set_rr_mode(amls_ctx* ctx, int32_t mode);
+amls_status amls_hip_get_rr_info(amls_ctx* ctx, int32_t* iters, double* residual);
+
+/*
  * The whole pipeline of AmlsSolver::solve (src/amls/solver.cpp:100-319),
  * including partition (coords may be null), two-pass truncation, the reduced
  * solve, vector recovery and the optional subspace iteration.

@@ -219,3 +219,13 @@ def test_noalloc_size_limit_on_host_context():
     p = buf.ctypes.data
     assert L.amls_hip_ldlt_solve_noalloc(ctx._h, 4, p, 1, p) == 6
     ctx.close()
+
+
+def test_rr_mode_arguments():
+    from amls_amd import AmlsError, HipContext
+    c = HipContext(-1)
+    with pytest.raises(AmlsError, match="rr mode"):
+        c.set_rr_mode(2)
+    c.set_rr_mode(1)
+    assert c.rr_info() == (0, 0.0)
+    c.close()
