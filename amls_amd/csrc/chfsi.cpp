@@ -16,7 +16,8 @@
 //     above it, through the three-term recurrence with Cs = (4 / nu_c) C - 2 I
 //     (X_1 = Cs X_0 / 2, X_j = Cs X_(j-1) - X_(j-2)), then normalises the
 //     columns and orthonormalises the block by Cholesky QR (G = X^T X =
-//     L D L^T, X <- X L^-T D^-1/2; twice in the first iteration).  Cholesky
+//     L D L^T, X <- X L^-T D^-1/2; twice in the first filtered iteration,
+//     whose block is the worst conditioned).  Cholesky
 //     QR keeps the columns in order, as orthogonal iteration does, so after
 //     the first iterations each column leans towards its own eigenvector and
 //     the filter's growth over the block stays within what one Cholesky QR
@@ -196,12 +197,12 @@ bool Ctx::chfsiTop(double* C, int64_t q, int64_t m, double* lamOut, double* Vout
             launch_set_i32(err, INT_MAX, st);
             launch_fill_hash(X0, qq * si, 0x2545F4914F6CDD1Dull, st);
         });
+        // one Cholesky QR each: the random block and its power step are
+        // well conditioned (cond about 1 and nu_1 / nu_s)
         normalize(plan, X0);
-        cholqr(plan, X0, X1);
         cholqr(plan, X0, X1);
         addGemm(plan, prod(C, q, 0, X0, q, 0, X1, q, q, s, q, 1.0, 0.0), 3);
         normalize(plan, X1);
-        cholqr(plan, X1, X0);
         cholqr(plan, X1, X0);
         addGemm(plan, prod(C, q, 0, X1, q, 0, X0, q, q, s, q, 1.0, 0.0), 3);
         plan.fn([=](hipStream_t st) { launch_coldot(qq, X1, X0, si, qq, rq, st); });

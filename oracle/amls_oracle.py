@@ -134,10 +134,11 @@ class BlockGE:
             o, n = int(off[p]), int(size[p])
             self.Kd.append(K[o:o + n, o:o + n].toarray())
             self.Md.append(M[o:o + n, o:o + n].toarray())
+        Kcsc, Mcsc = K.tocsc(), M.tocsc()
         for p in range(N):
             o, n = int(off[p]), int(size[p])
-            Kcol = K[:, o:o + n]
-            Mcol = M[:, o:o + n]
+            Kcol = Kcsc[:, o:o + n].tocsr()
+            Mcol = Mcsc[:, o:o + n].tocsr()
             for a in self.anc[p]:
                 oa, na = int(off[a]), int(size[a])
                 self.Kc[(a, p)] = Kcol[oa:oa + na].toarray()
@@ -160,74 +161,98 @@ class BlockGE:
         square block, dropped otherwise)."""
         tr = self.tree
         for p in range(tr.N):
-            Kh = self.Kd[p]
-            self.Khat[p] = Kh.copy()
+            self.Khat[p] = self.Kd[p].copy()
             self.Mhat[p] = self.Md[p].copy()
-            try:
-                c, low = sla.cho_factor(Kh, lower=True, check_finite=False)
-            except np.linalg.LinAlgError as e:   # D > 0 check (sparse_direct.cpp:13-22)
-                raise np.linalg.LinAlgError(f"node {int(tr.treeId[p])}: not SPD") from e
-            d = np.diag(c)
-            if not np.all(np.isfinite(d)) or np.any(d <= 0):
-                raise np.linalg.LinAlgError(f"node {int(tr.treeId[p])}: not SPD")
-            self.fac[p] = (c, low)
             anc = self.anc[p]
-            if not anc:
-                continue
-            n_p = int(tr.size[p])
-            W = self.Mhat[p]
-            X, Y, WX, rK, rM = {}, {}, {}, {}, {}
-            for a in anc:                               # solveXYAnc_ (:510-598)
-                Ka, Ma = self.Kc[(a, p)], self.Mc[(a, p)]
-                na = Ka.shape[0]
-                rK[a], rM[a] = _nzrows(Ka), _nzrows(Ma)
-                X[a] = np.zeros((n_p, na))
-                Y[a] = np.zeros((n_p, na))
-                if rK[a].size:
-                    X[a][:, rK[a]] = self.solve(p, Ka[rK[a]].T)
-                if rM[a].size:
-                    Y[a][:, rM[a]] = self.solve(p, Ma[rM[a]].T)
-                WX[a] = np.zeros((n_p, na))             # makeWX_ (:653-676)
-                if rK[a].size:
-                    WX[a][:, rK[a]] = self.solve(p, W @ X[a][:, rK[a]])
-            ix = np.ix_
-            for a in anc:                               # schurKDiagTask_ (:678-693)
-                r = rK[a]
-                if r.size:
-                    upd = self.Kc[(a, p)][r] @ X[a][:, r]
-                    self.Kd[a][ix(r, r)] += upd if plant == "schur_sign" else -upd
-            for u in range(len(anc)):                   # schurKCoupTask_ (:695-715)
-                for v in range(u + 1, len(anc)):
-                    lo, hi = anc[u], anc[v]
-                    rh, rl = rK[hi], rK[lo]
-                    if rh.size and rl.size:
-                        self.Kc[(hi, lo)][ix(rh, rl)] -= self.Kc[(hi, p)][rh] @ X[lo][:, rl]
-            for a in anc:                               # congDiagTask_ (:717-749)
-                B = self.Kc[(a, p)]
-                r, rm = rK[a], rM[a]
-                if r.size and rm.size:
-                    T1 = B[r] @ Y[a][:, rm]
-                    self.Md[a][ix(r, rm)] -= T1
-                    self.Md[a][ix(rm, r)] -= T1.T
-                if r.size and plant != "drop_wx":
-                    self.Md[a][ix(r, r)] += B[r] @ WX[a][:, r]
-            for u in range(len(anc)):                   # congCoupTask_ (:751-781)
-                for v in range(u + 1, len(anc)):
-                    lo, hi = anc[u], anc[v]
-                    rh, rl = rK[hi], rK[lo]
-                    rmh, rml = rM[hi], rM[lo]
-                    tgt = self.Mc[(hi, lo)]
-                    if rh.size and rml.size:
-                        tgt[ix(rh, rml)] -= self.Kc[(hi, p)][rh] @ Y[lo][:, rml]
-                    if rl.size and rmh.size:
-                        T2 = self.Kc[(lo, p)][rl] @ Y[hi][:, rmh]
-                        if plant != "coup_transpose":
-                            tgt[ix(rmh, rl)] -= T2.T
-                        elif T2.shape[0] == T2.shape[1]:
-                            tgt[ix(rmh, rl)] -= T2
-                    if rh.size and rl.size:
-                        tgt[ix(rh, rl)] += self.Kc[(hi, p)][rh] @ WX[lo][:, rl]
+            fac, deltas = ge_node(self.Kd[p], self.Mhat[p], anc,
+                                  [self.Kc[(a, p)] for a in anc], [self.Mc[(a, p)] for a in anc],
+                                  int(tr.treeId[p]), plant)
+            self.fac[p] = fac
+            self.apply(deltas)
         return self
+
+    def apply(self, deltas):
+        """Add the updates ge_node returned into the ancestors' blocks, in
+        order."""
+        for kind, key, (ri, ci), delta in deltas:
+            tgt = {"Kd": self.Kd, "Md": self.Md, "Kc": self.Kc, "Mc": self.Mc}[kind][key]
+            tgt[np.ix_(ri, ci)] += delta
+
+
+def ge_node(Kh: np.ndarray, W: np.ndarray, anc: List[int], Kc_p: List[np.ndarray],
+            Mc_p: List[np.ndarray], node_id: int, plant: Optional[str] = None):
+    """One node of BlockGE.run (gaussian_elimination.cpp:458-826): the
+    factor of K-hat_pp and the updates the node sends to its ancestors'
+    blocks, as (block kind, key, (rows, cols), delta) to add in order.  It
+    reads only the node's own blocks (K-hat_pp, M-hat_pp and its couplings
+    Kc_p[j] = K_(anc[j], p), Mc_p[j] = M_(anc[j], p)), so nodes of one tree
+    level, which have no ancestor in common with each other's own blocks,
+    can run concurrently with the same results."""
+    try:
+        c, low = sla.cho_factor(Kh, lower=True, check_finite=False)
+    except np.linalg.LinAlgError as e:   # D > 0 check (sparse_direct.cpp:13-22)
+        raise np.linalg.LinAlgError(f"node {node_id}: not SPD") from e
+    d = np.diag(c)
+    if not np.all(np.isfinite(d)) or np.any(d <= 0):
+        raise np.linalg.LinAlgError(f"node {node_id}: not SPD")
+    fac = (c, low)
+    deltas = []
+    if not anc:
+        return fac, deltas
+    solve = lambda B: sla.cho_solve(fac, B, check_finite=False)
+    n_p = Kh.shape[0]
+    Kc = dict(zip(anc, Kc_p))
+    Mc = dict(zip(anc, Mc_p))
+    X, Y, WX, rK, rM = {}, {}, {}, {}, {}
+    for a in anc:                               # solveXYAnc_ (:510-598)
+        Ka, Ma = Kc[a], Mc[a]
+        na = Ka.shape[0]
+        rK[a], rM[a] = _nzrows(Ka), _nzrows(Ma)
+        X[a] = np.zeros((n_p, na))
+        Y[a] = np.zeros((n_p, na))
+        if rK[a].size:
+            X[a][:, rK[a]] = solve(Ka[rK[a]].T)
+        if rM[a].size:
+            Y[a][:, rM[a]] = solve(Ma[rM[a]].T)
+        WX[a] = np.zeros((n_p, na))             # makeWX_ (:653-676)
+        if rK[a].size:
+            WX[a][:, rK[a]] = solve(W @ X[a][:, rK[a]])
+    for a in anc:                               # schurKDiagTask_ (:678-693)
+        r = rK[a]
+        if r.size:
+            upd = Kc[a][r] @ X[a][:, r]
+            deltas.append(("Kd", a, (r, r), upd if plant == "schur_sign" else -upd))
+    for u in range(len(anc)):                   # schurKCoupTask_ (:695-715)
+        for v in range(u + 1, len(anc)):
+            lo, hi = anc[u], anc[v]
+            rh, rl = rK[hi], rK[lo]
+            if rh.size and rl.size:
+                deltas.append(("Kc", (hi, lo), (rh, rl), -(Kc[hi][rh] @ X[lo][:, rl])))
+    for a in anc:                               # congDiagTask_ (:717-749)
+        B = Kc[a]
+        r, rm = rK[a], rM[a]
+        if r.size and rm.size:
+            T1 = B[r] @ Y[a][:, rm]
+            deltas.append(("Md", a, (r, rm), -T1))
+            deltas.append(("Md", a, (rm, r), -T1.T))
+        if r.size and plant != "drop_wx":
+            deltas.append(("Md", a, (r, r), B[r] @ WX[a][:, r]))
+    for u in range(len(anc)):                   # congCoupTask_ (:751-781)
+        for v in range(u + 1, len(anc)):
+            lo, hi = anc[u], anc[v]
+            rh, rl = rK[hi], rK[lo]
+            rmh, rml = rM[hi], rM[lo]
+            if rh.size and rml.size:
+                deltas.append(("Mc", (hi, lo), (rh, rml), -(Kc[hi][rh] @ Y[lo][:, rml])))
+            if rl.size and rmh.size:
+                T2 = Kc[lo][rl] @ Y[hi][:, rmh]
+                if plant != "coup_transpose":
+                    deltas.append(("Mc", (hi, lo), (rmh, rl), -T2.T))
+                elif T2.shape[0] == T2.shape[1]:
+                    deltas.append(("Mc", (hi, lo), (rmh, rl), -T2))
+            if rh.size and rl.size:
+                deltas.append(("Mc", (hi, lo), (rh, rl), Kc[hi][rh] @ WX[lo][:, rl]))
+    return fac, deltas
 
 
 # --------------------------------------------------------------------------
@@ -506,30 +531,11 @@ class Condensation:
         blocks = []
         for i in range(N):
             anc = ge.anc[i]
-            ki = int(self.k[i])
             rowKK = rows.get(i, [])
-            if anc:
-                if rowKK and plant != "drop_mid":
-                    RHS = np.hstack([mbar[(i, kk)] for kk in rowKK])
-                    Vs = ge.solve(i, RHS)
-                    off = 0
-                    for kk in rowKK:
-                        kc = mbar[(i, kk)].shape[1]
-                        for a in anc:
-                            mbar[(a, kk)] -= ge.Kc[(a, i)] @ Vs[:, off:off + kc]
-                        off += kc
-                if ki > 0:
-                    Zi = self.Z[i]
-                    WMz = ge.solve(i, ge.Mhat[i] @ Zi)
-                    for a in anc:
-                        mbar[(a, i)] = ge.Mc[(a, i)] @ Zi - ge.Kc[(a, i)] @ WMz
-                        rows.setdefault(a, []).append(i)
-            if ki > 0:
-                for kk in rowKK:
-                    blocks.append((i, kk, self.Z[i].T @ mbar[(i, kk)]))
-            for kk in rowKK:
-                del mbar[(i, kk)]
-            rows.pop(i, None)
+            upd, made, blk = project_node(
+                ge.fac[i], ge.Mhat[i], self.Z[i], anc, [ge.Kc[(a, i)] for a in anc],
+                [ge.Mc[(a, i)] for a in anc], rowKK, [mbar[(i, kk)] for kk in rowKK], plant)
+            self.apply_projection(i, upd, made, blk, mbar, rows, blocks)
         self.blocks = blocks
         # finalize (:120-127): offsets in node-id order
         order = np.argsort(tr.treeId)
@@ -540,6 +546,22 @@ class Condensation:
             acc += int(self.k[p])
         self.p = acc
         return self
+
+    @staticmethod
+    def apply_projection(i, upd, made, blk, mbar, rows, blocks):
+        """Fold what project_node returned for node i into the running
+        contribution table: mid updates, the node's new blocks towards its
+        ancestors, its reduced-M blocks; node i's own entries are consumed."""
+        for key, delta in upd:
+            mbar[key] += delta
+        for a, arr in made:
+            mbar[(a, i)] = arr
+            rows.setdefault(a, []).append(i)
+        for kk, B in blk:
+            blocks.append((i, kk, B))
+        for kk in rows.get(i, []):
+            del mbar[(i, kk)]
+        rows.pop(i, None)
 
     def assemble(self):
         """assembleMprojDense / assembleKprojDiag (solver.cpp:72-96)."""
@@ -587,6 +609,37 @@ class Condensation:
             o, ni = int(tr.offset[i]), int(tr.size[i])
             out[o:o + ni] = z[i]
         return out
+
+
+def project_node(fac, Mhat_i: np.ndarray, Z_i: np.ndarray, anc: List[int], Kc_i: List[np.ndarray],
+                 Mc_i: List[np.ndarray], rowKK: List[int], mbar_i: List[np.ndarray],
+                 plant: Optional[str] = None):
+    """One node of the post-order projection (subspace_condensation.cpp
+    :129-261): the mid updates M_(a, kk) -= K_(a, i) Khat_i^-1 M_(i, kk) it
+    sends up for every earlier node kk in its row (as ((a, kk), delta)), its
+    own blocks M_(a, i) = M_(a, i) Z_i - K_(a, i) Khat_i^-1 M-hat_i Z_i, and
+    the reduced blocks Z_i^T M_(i, kk).  It reads only node i's own blocks
+    and contributions, so the nodes of one tree level can run concurrently."""
+    solve = lambda B: sla.cho_solve(fac, B, check_finite=False)
+    ki = Z_i.shape[1]
+    upd, made, blk = [], [], []
+    if anc:
+        if rowKK and plant != "drop_mid":
+            Vs = solve(np.hstack(mbar_i))
+            off = 0
+            for kk, mb in zip(rowKK, mbar_i):
+                kc = mb.shape[1]
+                for a, Ka in zip(anc, Kc_i):
+                    upd.append(((a, kk), -(Ka @ Vs[:, off:off + kc])))
+                off += kc
+        if ki > 0:
+            WMz = solve(Mhat_i @ Z_i)
+            for a, Ka, Ma in zip(anc, Kc_i, Mc_i):
+                made.append((a, Ma @ Z_i - Ka @ WMz))
+    if ki > 0:
+        for kk, mb in zip(rowKK, mbar_i):
+            blk.append((kk, Z_i.T @ mb))
+    return upd, made, blk
 
 
 def amls_solve(K: sp.spmatrix, M: sp.spmatrix, tree: Tree, *, nev: int = 100,
