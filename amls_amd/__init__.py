@@ -89,10 +89,13 @@ class Pencil:
 
 
 def _csc(A):
-    A = sp.csc_matrix(A, copy=True)
+    A = A if isinstance(A, sp.csc_matrix) else sp.csc_matrix(A)
     # one entry per (row, col), rows ascending (the library sums duplicates
-    # too; doing it here keeps the arrays it receives small)
-    A.sum_duplicates()
+    # too; doing it here keeps the arrays it receives small).  The caller's
+    # matrix is copied only when it is not in that form already.
+    if not A.has_canonical_format:
+        A = A.copy()
+        A.sum_duplicates()
     return (np.ascontiguousarray(A.indptr, dtype=np.int64),
             np.ascontiguousarray(A.indices, dtype=np.int64),
             np.ascontiguousarray(A.data, dtype=np.float64))
@@ -175,6 +178,14 @@ class HipContext:
         nodes = np.zeros(cnt.value + 1, dtype=np.int64)
         self._chk(self._L.amls_hip_batch_info(self._h, None, iptr(rows), iptr(nodes)))
         return rows, nodes
+
+    def setup_timing(self) -> dict:
+        """Seconds of the last set_problem: analysis (canonical columns, graph,
+        nested dissection, symbolic tables), tables (device-side index maps and
+        operator layouts built on the host), upload, total."""
+        t = np.zeros(4)
+        self._chk(self._L.amls_hip_setup_timing(self._h, dptr(t)))
+        return dict(zip(("analysis", "tables", "upload", "total"), t.tolist()))
 
     def tree(self) -> dict:
         nn, ml, n = C.c_int64(), C.c_int64(), C.c_int64()
@@ -270,8 +281,11 @@ class HipContext:
         t = np.ascontiguousarray(np.asarray(table, dtype=np.int64).reshape(-1, 3))
         self._chk(self._L.amls_hip_set_tuning(self._h, iptr(t) if t.size else None, t.shape[0]))
 
-    def profile(self, enable: bool = True):
-        self._chk(self._L.amls_hip_profile(self._h, int(enable)))
+    def profile(self, enable: bool = True, by_phase: bool = False):
+        """HIP-event timing of every launch class; with by_phase the classes
+        are keyed "phase/class" (ge, spectral, select, project, reduced,
+        backtransform, si)."""
+        self._chk(self._L.amls_hip_profile(self._h, 2 if (enable and by_phase) else int(enable)))
 
     def kernel_stats(self) -> dict:
         out = {}

@@ -8,6 +8,7 @@
 // eigenvalue lambda_hat, the per-node counts k_i of pass B, the reduced
 // eigenvalues, and the error word of the factorisations).
 #include "engine.hpp"
+#include "host/par.hpp"
 #include "plan.hpp"
 
 #include <algorithm>
@@ -16,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <numeric>
 
 #include "../../include/amls_hip.h"
 
@@ -200,18 +202,16 @@ struct Bsr3Quads {
     std::vector<double> val;
 };
 
-bool buildQuads(const Bsr3Host& b, Bsr3Quads& out) {
-    const int64_t nb = static_cast<int64_t>(b.ptr.size()) - 1;
-    if (nb <= 0 || 3 * nb >= (int64_t(1) << 28)) return false;
-    const int64_t nq = (nb + 3) / 4;
+// Block rows [q0 * 4, q1 * 4) of b as quads, into out (qptr / qvptr relative
+// to out's own start, first entry 0).
+void buildQuadRange(const Bsr3Host& b, int64_t nb, int64_t q0, int64_t q1, Bsr3Quads& out) {
     out.qptr.assign(1, 0);
     out.qvptr.assign(1, 0);
     out.ucol.clear();
     out.val.clear();
-    out.val.reserve(b.val.size());
     struct Ent { int32_t col; int32_t row; int64_t blk; };
     std::vector<Ent> ents;
-    for (int64_t q = 0; q < nq; ++q) {
+    for (int64_t q = q0; q < q1; ++q) {
         ents.clear();
         for (int r = 0; r < 4 && 4 * q + r < nb; ++r)
             for (int64_t k = b.ptr[4 * q + r]; k < b.ptr[4 * q + r + 1]; ++k)
@@ -233,41 +233,115 @@ bool buildQuads(const Bsr3Host& b, Bsr3Quads& out) {
         out.qptr.push_back(static_cast<int64_t>(out.ucol.size()));
         out.qvptr.push_back(static_cast<int64_t>(out.val.size() / 9));
     }
+}
+
+// pieces[i] built independently (ranges in order) joined into one layout
+void joinQuads(std::vector<Bsr3Quads>& pieces, Bsr3Quads& out, int threads) {
+    int64_t nq = 0, nu = 0, nv = 0;
+    for (const auto& p : pieces) {
+        nq += static_cast<int64_t>(p.qptr.size()) - 1;
+        nu += static_cast<int64_t>(p.ucol.size());
+        nv += static_cast<int64_t>(p.val.size());
+    }
+    out.qptr.assign(static_cast<size_t>(nq) + 1, 0);
+    out.qvptr.assign(static_cast<size_t>(nq) + 1, 0);
+    out.ucol.resize(static_cast<size_t>(nu));
+    out.val.resize(static_cast<size_t>(nv));
+    std::vector<int64_t> oq(pieces.size() + 1, 0), ou(pieces.size() + 1, 0), ov(pieces.size() + 1, 0);
+    for (size_t i = 0; i < pieces.size(); ++i) {
+        oq[i + 1] = oq[i] + static_cast<int64_t>(pieces[i].qptr.size()) - 1;
+        ou[i + 1] = ou[i] + static_cast<int64_t>(pieces[i].ucol.size());
+        ov[i + 1] = ov[i] + static_cast<int64_t>(pieces[i].val.size());
+    }
+    parallelRanges(static_cast<int64_t>(pieces.size()), threads, [&](int64_t lo, int64_t hi, int) {
+        for (int64_t i = lo; i < hi; ++i) {
+            const Bsr3Quads& p = pieces[i];
+            const int64_t m = static_cast<int64_t>(p.qptr.size()) - 1;
+            for (int64_t q = 0; q < m; ++q) {
+                out.qptr[oq[i] + q + 1] = p.qptr[q + 1] + ou[i];
+                out.qvptr[oq[i] + q + 1] = p.qvptr[q + 1] + ov[i] / 9;
+            }
+            std::copy(p.ucol.begin(), p.ucol.end(), out.ucol.begin() + ou[i]);
+            std::copy(p.val.begin(), p.val.end(), out.val.begin() + ov[i]);
+        }
+    });
+}
+
+// Block rows grouped by four (the layout of launch_spmm_bsr3q), over
+// `threads` host threads.  Fails when a column offset does not fit the 28
+// bits of the layout.
+bool buildQuads(const Bsr3Host& b, Bsr3Quads& out, int threads) {
+    const int64_t nb = static_cast<int64_t>(b.ptr.size()) - 1;
+    if (nb <= 0 || 3 * nb >= (int64_t(1) << 28)) return false;
+    const int64_t nq = (nb + 3) / 4;
+    const int pieces = static_cast<int>(std::min<int64_t>(nq, 4 * std::max(threads, 1)));
+    std::vector<Bsr3Quads> part(static_cast<size_t>(pieces));
+    parallelRanges(pieces, threads, [&](int64_t lo, int64_t hi, int) {
+        for (int64_t i = lo; i < hi; ++i)
+            buildQuadRange(b, nb, nq * i / pieces, nq * (i + 1) / pieces, part[i]);
+    });
+    joinQuads(part, out, threads);
     return true;
 }
 
-bool buildBsr3(int64_t n, const int64_t* rp, const int64_t* ci, const double* vx, Bsr3Host& out) {
-    if (n <= 0 || n % 3 != 0 || n / 3 > INT32_MAX) return false;
-    // fast path: the three rows of every triple list the same column triples
-    // in the same order (true for sorted rows and for symmetric permutations
-    // that move whole triples); otherwise sort each row and try again
-    const int64_t nb = n / 3;
+// Block rows [b0, b1) of the fast-path 3 x 3 layout into out (ptr relative,
+// first entry 0); false when a row triple breaks the pattern.
+bool bsr3Range(const int64_t* rp, const int64_t* ci, const double* vx, int64_t b0, int64_t b1,
+               Bsr3Host& out) {
     out.ptr.assign(1, 0);
     out.col.clear();
     out.val.clear();
-    out.col.reserve(static_cast<size_t>(rp[n] / 9));
-    out.val.reserve(static_cast<size_t>(rp[n]));
-    bool ok = true;
-    for (int64_t b = 0; b < nb && ok; ++b) {
+    for (int64_t b = b0; b < b1; ++b) {
         const int64_t r0 = 3 * b, len = rp[r0 + 1] - rp[r0];
-        if (len % 3 != 0 || rp[r0 + 2] - rp[r0 + 1] != len || rp[r0 + 3] - rp[r0 + 2] != len) {
-            ok = false;
-            break;
-        }
-        for (int64_t t = 0; t < len && ok; t += 3) {
+        if (len % 3 != 0 || rp[r0 + 2] - rp[r0 + 1] != len || rp[r0 + 3] - rp[r0 + 2] != len)
+            return false;
+        for (int64_t t = 0; t < len; t += 3) {
             const int64_t c = ci[rp[r0] + t];
-            if (c % 3 != 0) { ok = false; break; }
-            for (int i = 0; i < 3 && ok; ++i)
+            if (c % 3 != 0) return false;
+            for (int i = 0; i < 3; ++i)
                 for (int j = 0; j < 3; ++j)
-                    if (ci[rp[r0 + i] + t + j] != c + j) { ok = false; break; }
-            if (!ok) break;
+                    if (ci[rp[r0 + i] + t + j] != c + j) return false;
             out.col.push_back(static_cast<int32_t>(c / 3));
             for (int i = 0; i < 3; ++i)
                 for (int j = 0; j < 3; ++j) out.val.push_back(vx[rp[r0 + i] + t + j]);
         }
         out.ptr.push_back(static_cast<int64_t>(out.col.size()));
     }
-    return ok || buildBsr3Sorted(n, rp, ci, vx, out);
+    return true;
+}
+
+bool buildBsr3(int64_t n, const int64_t* rp, const int64_t* ci, const double* vx, Bsr3Host& out,
+               int threads) {
+    if (n <= 0 || n % 3 != 0 || n / 3 > INT32_MAX) return false;
+    // fast path: the three rows of every triple list the same column triples
+    // in the same order (true for sorted rows and for symmetric permutations
+    // that move whole triples); otherwise sort each row and try again
+    const int64_t nb = n / 3;
+    const int pieces = static_cast<int>(std::min<int64_t>(nb, 4 * std::max(threads, 1)));
+    std::vector<Bsr3Host> part(static_cast<size_t>(pieces));
+    std::vector<char> ok(static_cast<size_t>(pieces), 0);
+    parallelRanges(pieces, threads, [&](int64_t lo, int64_t hi, int) {
+        for (int64_t i = lo; i < hi; ++i)
+            ok[i] = bsr3Range(rp, ci, vx, nb * i / pieces, nb * (i + 1) / pieces, part[i]);
+    });
+    if (std::find(ok.begin(), ok.end(), 0) != ok.end()) return buildBsr3Sorted(n, rp, ci, vx, out);
+    int64_t nblk = 0;
+    for (const auto& p : part) nblk += static_cast<int64_t>(p.col.size());
+    out.ptr.assign(static_cast<size_t>(nb) + 1, 0);
+    out.col.resize(static_cast<size_t>(nblk));
+    out.val.resize(static_cast<size_t>(9 * nblk));
+    std::vector<int64_t> ob(part.size() + 1, 0);
+    for (size_t i = 0; i < part.size(); ++i) ob[i + 1] = ob[i] + static_cast<int64_t>(part[i].col.size());
+    parallelRanges(pieces, threads, [&](int64_t lo, int64_t hi, int) {
+        for (int64_t i = lo; i < hi; ++i) {
+            const Bsr3Host& p = part[i];
+            const int64_t r0 = nb * i / pieces, m = static_cast<int64_t>(p.ptr.size()) - 1;
+            for (int64_t r = 0; r < m; ++r) out.ptr[r0 + r + 1] = p.ptr[r + 1] + ob[i];
+            std::copy(p.col.begin(), p.col.end(), out.col.begin() + ob[i]);
+            std::copy(p.val.begin(), p.val.end(), out.val.begin() + 9 * ob[i]);
+        }
+    });
+    return true;
 }
 
 bool buildBsr3Sorted(int64_t n, const int64_t* rp, const int64_t* ci, const double* vx,
@@ -302,39 +376,100 @@ bool buildBsr3Sorted(int64_t n, const int64_t* rp, const int64_t* ci, const doub
     return true;
 }
 
-// Appends the nn columns of one CSC matrix (rows shifted by `base`) to the
-// block-diagonal CSC (cp, ri, vx), each column with ascending rows and its
-// duplicate entries summed.  Columns that are already strictly ascending are
-// copied as they are.
-void appendCanonical(int64_t nn, int64_t base, const int64_t* pp, const int64_t* pi,
-                     const double* px, std::vector<int64_t>& cp, std::vector<int64_t>& ri,
-                     std::vector<double>& vx) {
-    std::vector<std::pair<int64_t, double>> col;
-    for (int64_t c = 0; c < nn; ++c) {
-        const int64_t e0 = pp[c], e1 = pp[c + 1];
-        bool sorted = true;
-        for (int64_t e = e0 + 1; e < e1 && sorted; ++e) sorted = pi[e] > pi[e - 1];
-        if (sorted) {
+// One problem's matrix in canonical CSC: every column's rows strictly
+// ascending, duplicate (row, col) entries summed.
+struct CscLocal {
+    std::vector<int64_t> cp, ri;
+    std::vector<double> vx;
+    // the canonical matrix: the vectors above, or the caller's arrays when
+    // those are canonical already
+    const int64_t *pcp = nullptr, *pri = nullptr;
+    const double* pvx = nullptr;
+    void view() {
+        pcp = cp.data();
+        pri = ri.data();
+        pvx = vx.data();
+    }
+};
+
+// Canonical copy of the nn columns (pp, pi, px), over `threads` host threads:
+// a column that is already strictly ascending is copied as it is, any other
+// is sorted (stably) and its repeated rows summed.  Row indices outside
+// [0, nn) are rejected with AMLS_E_ARG.
+void canonicalCsc(int64_t nn, const int64_t* pp, const int64_t* pi, const double* px,
+                  CscLocal& out, int threads, const char* name) {
+    const int T = nn < 4096 ? 1 : threads;
+    out.cp.assign(static_cast<size_t>(nn) + 1, 0);
+    std::vector<char> bad(static_cast<size_t>(std::max(T, 1)), 0);
+    std::vector<char> unsorted(static_cast<size_t>(std::max(T, 1)), 0);
+    auto sortedCol = [&](std::vector<std::pair<int64_t, double>>& col, int64_t c) {
+        col.clear();
+        for (int64_t e = pp[c]; e < pp[c + 1]; ++e) col.push_back({pi[e], px[e]});
+        std::stable_sort(col.begin(), col.end(),
+                         [](const auto& a, const auto& b) { return a.first < b.first; });
+    };
+    parallelRanges(nn, T, [&](int64_t lo, int64_t hi, int chunk) {
+        std::vector<std::pair<int64_t, double>> col;
+        for (int64_t c = lo; c < hi; ++c) {
+            const int64_t e0 = pp[c], e1 = pp[c + 1];
+            bool sorted = true;
             for (int64_t e = e0; e < e1; ++e) {
-                ri.push_back(pi[e] + base);
-                vx.push_back(px[e]);
+                if (pi[e] < 0 || pi[e] >= nn) {
+                    bad[static_cast<size_t>(chunk)] = 1;
+                    return;
+                }
+                if (e > e0 && pi[e] <= pi[e - 1]) sorted = false;
             }
-        } else {
-            col.clear();
-            for (int64_t e = e0; e < e1; ++e) col.push_back({pi[e], px[e]});
-            std::stable_sort(col.begin(), col.end(),
-                             [](const auto& a, const auto& b) { return a.first < b.first; });
+            int64_t cnt = e1 - e0;
+            if (!sorted) {
+                unsorted[static_cast<size_t>(chunk)] = 1;
+                sortedCol(col, c);
+                cnt = 0;
+                for (size_t t = 0; t < col.size(); ++t) cnt += t == 0 || col[t].first != col[t - 1].first;
+            }
+            out.cp[c + 1] = cnt;
+        }
+    });
+    for (char b : bad)
+        if (b) throw StatusError(AMLS_E_ARG, std::string(name) + " row index out of range");
+    for (int64_t c = 0; c < nn; ++c) out.cp[c + 1] += out.cp[c];
+    if (std::find(unsorted.begin(), unsorted.end(), 1) == unsorted.end()) {
+        // every column strictly ascending already: no copy
+        std::vector<int64_t>().swap(out.cp);
+        out.pcp = pp;
+        out.pri = pi;
+        out.pvx = px;
+        return;
+    }
+    out.ri.resize(static_cast<size_t>(out.cp[nn]));
+    out.vx.resize(static_cast<size_t>(out.cp[nn]));
+    parallelRanges(nn, T, [&](int64_t lo, int64_t hi, int) {
+        std::vector<std::pair<int64_t, double>> col;
+        for (int64_t c = lo; c < hi; ++c) {
+            const int64_t e0 = pp[c], e1 = pp[c + 1];
+            int64_t w = out.cp[c];
+            if (e1 - e0 == out.cp[c + 1] - w) {   // strictly ascending: copied
+                bool asc = true;
+                for (int64_t e = e0 + 1; e < e1 && asc; ++e) asc = pi[e] > pi[e - 1];
+                if (asc) {
+                    std::copy(pi + e0, pi + e1, out.ri.begin() + w);
+                    std::copy(px + e0, px + e1, out.vx.begin() + w);
+                    continue;
+                }
+            }
+            sortedCol(col, c);
             for (size_t t = 0; t < col.size();) {
                 double v = 0.0;
                 size_t u = t;
                 for (; u < col.size() && col[u].first == col[t].first; ++u) v += col[u].second;
-                ri.push_back(col[t].first + base);
-                vx.push_back(v);
+                out.ri[w] = col[t].first;
+                out.vx[w] = v;
+                ++w;
                 t = u;
             }
         }
-        cp.push_back(static_cast<int64_t>(ri.size()));
-    }
+    });
+    out.view();
 }
 
 }  // namespace
@@ -461,7 +596,8 @@ hipEvent_t KernelStats::get() {
 
 void KernelStats::run(const std::string& name, double flops, double bytes, hipStream_t s,
                       const std::function<void()>& launch, int64_t count) {
-    auto& en = byName[name];
+    const std::string key = byPhase && *phase ? std::string(phase) + "/" + name : name;
+    auto& en = byName[key];
     en.launches += count;
     en.flops += flops;
     en.bytes += bytes;
@@ -473,7 +609,7 @@ void KernelStats::run(const std::string& name, double flops, double bytes, hipSt
     AMLS_HIP_CHECK(hipEventRecord(a, s));
     launch();
     AMLS_HIP_CHECK(hipEventRecord(b, s));
-    pending.push_back({name, a, b});
+    pending.push_back({key, a, b});
 }
 
 void KernelStats::resolve() {
@@ -686,31 +822,65 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
         segRow[s + 1] = segRow[s] + P.n;
     }
     n = segRow[nseg];
-    // block-diagonal CSC of all problems
-    Kp.assign(1, 0);
-    Mp.assign(1, 0);
-    Ki.clear();
-    Kx.clear();
-    Mi.clear();
-    Mx.clear();
-    std::vector<Symbolic> parts;
-    for (int64_t s = 0; s < nseg; ++s) {
+    const auto tSet0 = Clock::now();
+    // per problem (several at once for a batch, busiest first): canonical
+    // columns (rows ascending, duplicate (row, col) entries summed: the
+    // reference's Eigen matrices hold each entry once and the front assembly
+    // maps one slot per entry), the graph of |K| + |M|, nested dissection and
+    // the symbolic tables
+    std::vector<CscLocal> canK(nseg), canM(nseg);
+    std::vector<Symbolic> parts(nseg);
+    const int T = hostThreads();
+    const int outer = static_cast<int>(std::min<int64_t>(T, nseg));
+    const int inner = nseg == 1 ? T : 1;
+    std::vector<int64_t> order(nseg);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        return probs[a].Kp[probs[a].n] > probs[b].Kp[probs[b].n];
+    });
+    parallelDynamic(nseg, outer, [&](int64_t i) {
+        const int64_t s = order[i];
         const ProblemIn& P = probs[s];
-        const int64_t nn = P.n, base = segRow[s];
-        for (int64_t e = 0; e < P.Kp[nn]; ++e)
-            if (P.Ki[e] < 0 || P.Ki[e] >= nn) throw StatusError(AMLS_E_ARG, "K row index out of range");
-        for (int64_t e = 0; e < P.Mp[nn]; ++e)
-            if (P.Mi[e] < 0 || P.Mi[e] >= nn) throw StatusError(AMLS_E_ARG, "M row index out of range");
-        // canonical columns: rows ascending, duplicate (row, col) entries
-        // summed (the reference's Eigen matrices hold each entry once; the
-        // front assembly maps one slot per entry)
-        appendCanonical(nn, base, P.Kp, P.Ki, P.Kx, Kp, Ki, Kx);
-        appendCanonical(nn, base, P.Mp, P.Mi, P.Mx, Mp, Mi, Mx);
-        const Graph gs = graphFromCsc(nn, P.Kp, P.Ki, P.Mp, P.Mi);
+        const int64_t nn = P.n;
+        canonicalCsc(nn, P.Kp, P.Ki, P.Kx, canK[s], inner, "K");
+        canonicalCsc(nn, P.Mp, P.Mi, P.Mx, canM[s], inner, "M");
+        const Graph gs = graphFromCsc(nn, canK[s].pcp, canK[s].pri, canM[s].pcp, canM[s].pri, inner);
         const PartitionResult ps = recursiveBisection(gs, P.coords, nLevels, leafSize);
-        parts.push_back(analyze(ps, gs));
-        segNode[s + 1] = segNode[s] + parts.back().nNodes;
+        parts[s] = analyze(ps, gs);
+    });
+    // block-diagonal CSC of all problems (one problem: its canonical matrix)
+    CscLocal Kall, Mall;
+    auto concat = [&](std::vector<CscLocal>& loc, CscLocal& all) {
+        if (nseg == 1) {
+            all = std::move(loc[0]);
+            if (!all.cp.empty()) all.view();
+            return;
+        }
+        int64_t tot = 0;
+        for (int64_t sg = 0; sg < nseg; ++sg) tot += loc[sg].pcp[segRow[sg + 1] - segRow[sg]];
+        all.cp.assign(static_cast<size_t>(n) + 1, 0);
+        all.ri.resize(static_cast<size_t>(tot));
+        all.vx.resize(static_cast<size_t>(tot));
+        int64_t e0 = 0;
+        for (int64_t sg = 0; sg < nseg; ++sg) {
+            const CscLocal& c = loc[sg];
+            const int64_t base = segRow[sg], nn = segRow[sg + 1] - base, ne = c.pcp[nn];
+            for (int64_t j = 0; j < nn; ++j) all.cp[base + j + 1] = c.pcp[j + 1] + e0;
+            for (int64_t e = 0; e < ne; ++e) all.ri[e0 + e] = c.pri[e] + base;
+            std::copy(c.pvx, c.pvx + ne, all.vx.begin() + e0);
+            e0 += ne;
+            loc[sg] = CscLocal();
+        }
+        all.view();
+    };
+    {
+        std::vector<std::function<void()>> cat = {[&] { concat(canK, Kall); },
+                                                  [&] { concat(canM, Mall); }};
+        parallelTasks(cat);
     }
+    const int64_t *Kp = Kall.pcp, *Ki = Kall.pri, *Mp = Mall.pcp, *Mi = Mall.pri;
+    const double *Kx = Kall.pvx, *Mx = Mall.pvx;
+    for (int64_t s = 0; s < nseg; ++s) segNode[s + 1] = segNode[s] + parts[s].nNodes;
     S = mergeSymbolic(parts);
     std::vector<Symbolic>().swap(parts);
     nodeSeg.assign(S.nNodes, 0);
@@ -738,81 +908,119 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
         vo[p + 1] = vo[p] + S.size[p];
     }
     haveProblem = true;
+    const auto tSet1 = Clock::now();
+    setupTimes[0] = secs(tSet0, tSet1);
+    setupTimes[1] = setupTimes[2] = 0.0;
+    setupTimes[3] = setupTimes[0];
     if (!gpu) return;
 
-    // assembly maps: lower entries (row >= col in the permuted ordering) of
-    // each matrix -> their slot in the column owner's front
-    auto buildMap = [&](const std::vector<int64_t>& cp, const std::vector<int64_t>& ri,
-                        std::vector<int64_t>& src, std::vector<int64_t>& tgt) {
-        src.clear();
-        tgt.clear();
-        for (int64_t c = 0; c < n; ++c) {
-            const int64_t cn = S.iperm[c];
-            const int64_t p = owner[cn];
-            const int64_t o = S.off[p], np = S.size[p], f = S.f(p);
-            const int64_t* B = S.bidx.data() + S.bptr[p];
-            const int64_t nb = S.b(p);
-            for (int64_t e = cp[c]; e < cp[c + 1]; ++e) {
-                const int64_t rn = S.iperm[ri[e]];
-                if (rn < cn) continue;
-                int64_t lr;
-                if (rn < o + np) {
-                    lr = rn - o;
-                } else {
-                    const int64_t* it = std::lower_bound(B, B + nb, rn);
-                    if (it == B + nb || *it != rn)
-                        throw StatusError(AMLS_E_INTERNAL, "entry outside its front");
-                    lr = np + (it - B);
-                }
-                src.push_back(e);
-                tgt.push_back(fo[p] + lr + (cn - o) * f);
-            }
-        }
-    };
-    std::vector<int64_t> srcK, tgtK, srcM, tgtM;
-    buildMap(Kp, Ki, srcK, tgtK);
-    buildMap(Mp, Mi, srcM, tgtM);
-    nasmK = static_cast<int64_t>(srcK.size());
-    nasmM = static_cast<int64_t>(srcM.size());
-    // permuted CSR (= CSC of the symmetric permuted matrix) of K and M
-    auto permCsr = [&](const std::vector<int64_t>& cp, const std::vector<int64_t>& ri,
-                       const std::vector<double>& vx, std::vector<int64_t>& pp,
-                       std::vector<int64_t>& pi, std::vector<double>& px) {
+    // permuted CSR (= CSC of the symmetric permuted matrix) of K and M:
+    // column c of the input becomes row iperm[c], its entries keep their
+    // order (so an aligned row triple keeps its column triples together)
+    const int TT = hostThreads();
+    auto permCsr = [&](const int64_t* cp, const int64_t* ri, const double* vx,
+                       std::vector<int64_t>& pp, std::vector<int64_t>& pi, std::vector<double>& px,
+                       int threads) {
         pp.assign(n + 1, 0);
         for (int64_t c = 0; c < n; ++c) pp[S.iperm[c] + 1] = cp[c + 1] - cp[c];
         for (int64_t j = 0; j < n; ++j) pp[j + 1] += pp[j];
         pi.resize(cp[n]);
         px.resize(cp[n]);
-        for (int64_t c = 0; c < n; ++c) {
-            int64_t w = pp[S.iperm[c]];
-            for (int64_t e = cp[c]; e < cp[c + 1]; ++e) {
-                pi[w] = S.iperm[ri[e]];
-                px[w] = vx[e];
-                ++w;
+        parallelRanges(n, threads, [&](int64_t lo, int64_t hi, int) {
+            for (int64_t c = lo; c < hi; ++c) {
+                int64_t w = pp[S.iperm[c]];
+                for (int64_t e = cp[c]; e < cp[c + 1]; ++e) {
+                    pi[w] = S.iperm[ri[e]];
+                    px[w] = vx[e];
+                    ++w;
+                }
             }
-        }
+        });
     };
+    // assembly maps: lower entries (row >= col) of each permuted matrix ->
+    // their slot in the column owner's front; src indexes the permuted CSR
+    auto buildMap = [&](const std::vector<int64_t>& pp, const std::vector<int64_t>& pi,
+                        std::vector<int64_t>& src, std::vector<int64_t>& tgt, int threads) {
+        std::vector<int64_t> cnt(static_cast<size_t>(n) + 1, 0);
+        parallelRanges(n, threads, [&](int64_t lo, int64_t hi, int) {
+            for (int64_t cn = lo; cn < hi; ++cn) {
+                int64_t k = 0;
+                for (int64_t e = pp[cn]; e < pp[cn + 1]; ++e) k += pi[e] >= cn;
+                cnt[cn + 1] = k;
+            }
+        });
+        for (int64_t c = 0; c < n; ++c) cnt[c + 1] += cnt[c];
+        src.resize(static_cast<size_t>(cnt[n]));
+        tgt.resize(static_cast<size_t>(cnt[n]));
+        parallelRanges(n, threads, [&](int64_t lo, int64_t hi, int) {
+            for (int64_t cn = lo; cn < hi; ++cn) {
+                const int64_t p = owner[cn];
+                const int64_t o = S.off[p], np = S.size[p], f = S.f(p);
+                const int64_t* B = S.bidx.data() + S.bptr[p];
+                const int64_t nb = S.b(p);
+                int64_t w = cnt[cn];
+                for (int64_t e = pp[cn]; e < pp[cn + 1]; ++e) {
+                    const int64_t rn = pi[e];
+                    if (rn < cn) continue;
+                    int64_t lr;
+                    if (rn < o + np) {
+                        lr = rn - o;
+                    } else {
+                        const int64_t* it = std::lower_bound(B, B + nb, rn);
+                        if (it == B + nb || *it != rn)
+                            throw StatusError(AMLS_E_INTERNAL, "entry outside its front");
+                        lr = np + (it - B);
+                    }
+                    src[w] = e;
+                    tgt[w] = fo[p] + lr + (cn - o) * f;
+                    ++w;
+                }
+            }
+        });
+    };
+    // the device tables, built concurrently per matrix: permuted CSR, then
+    // its assembly map and its 3 x 3 block layout
+    std::vector<int64_t> srcK, tgtK, srcM, tgtM;
     std::vector<int64_t> pkp, pki, pmp, pmi;
     std::vector<double> pkx, pmx;
-    permCsr(Kp, Ki, Kx, pkp, pki, pkx);
-    permCsr(Mp, Mi, Mx, pmp, pmi, pmx);
+    Bsr3Quads qPK, qPM;
+    bool okB[2] = {false, false};
+    auto blocksOf = [&](const std::vector<int64_t>& rp, const std::vector<int64_t>& ci,
+                        const std::vector<double>& vx, Bsr3Quads& q, bool& ok, int threads) {
+        Bsr3Host b;
+        ok = buildBsr3(n, rp.data(), ci.data(), vx.data(), b, threads) && buildQuads(b, q, threads);
+    };
+    {
+        const int half = std::max(1, TT / 2);
+        std::vector<std::function<void()>> tasks = {
+            [&] {
+                permCsr(Kp, Ki, Kx, pkp, pki, pkx, half);
+                std::vector<std::function<void()>> t2 = {
+                    [&] { buildMap(pkp, pki, srcK, tgtK, std::max(1, half / 2)); },
+                    [&] { blocksOf(pkp, pki, pkx, qPK, okB[0], std::max(1, half / 2)); }};
+                parallelTasks(t2);
+            },
+            [&] {
+                permCsr(Mp, Mi, Mx, pmp, pmi, pmx, half);
+                std::vector<std::function<void()>> t2 = {
+                    [&] { buildMap(pmp, pmi, srcM, tgtM, std::max(1, half / 2)); },
+                    [&] { blocksOf(pmp, pmi, pmx, qPM, okB[1], std::max(1, half / 2)); }};
+                parallelTasks(t2);
+            }};
+        parallelTasks(tasks);
+    }
+    nasmK = static_cast<int64_t>(srcK.size());
+    nasmM = static_cast<int64_t>(srcM.size());
+    const auto tSet2 = Clock::now();
 
     auto& A = fresh(probMem);
     const int64_t nk = Kp[n], nm = Mp[n];
-    A.plan(nk * 8 * 2 + (n + 1) * 8);        // Kx, Ki, Kp
-    A.plan(nm * 8 * 2 + (n + 1) * 8);
     A.plan(nasmK * 16 + nasmM * 16);
     A.plan(n * 8);                           // perm
     A.plan(S.bidx.size() * 8 * 2);           // bidx, rel
     A.plan(nk * 16 + (n + 1) * 8 + nm * 16 + (n + 1) * 8);   // permuted CSR
     A.plan((nseg + 1) * 8);
     A.commit();
-    dKx = A.take<double>(nk);
-    dKi = A.take<int64_t>(nk);
-    dKp = A.take<int64_t>(n + 1);
-    dMx = A.take<double>(nm);
-    dMi = A.take<int64_t>(nm);
-    dMp = A.take<int64_t>(n + 1);
     dsrcK = A.take<int64_t>(nasmK);
     dtgtK = A.take<int64_t>(nasmK);
     dsrcM = A.take<int64_t>(nasmM);
@@ -830,12 +1038,6 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
     auto up = [&](void* d, const void* h, size_t bytes) {
         if (bytes) AMLS_HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
     };
-    up(dKx, Kx.data(), nk * 8);
-    up(dKi, Ki.data(), nk * 8);
-    up(dKp, Kp.data(), (n + 1) * 8);
-    up(dMx, Mx.data(), nm * 8);
-    up(dMi, Mi.data(), nm * 8);
-    up(dMp, Mp.data(), (n + 1) * 8);
     up(dsrcK, srcK.data(), nasmK * 8);
     up(dtgtK, tgtK.data(), nasmK * 8);
     up(dsrcM, srcM.data(), nasmM * 8);
@@ -850,7 +1052,8 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
     up(dPMi, pmi.data(), nm * 8);
     up(dPMp, pmp.data(), (n + 1) * 8);
     up(dSegRow, segRow.data(), (nseg + 1) * 8);
-    // operators: CSR always, 3 x 3 blocks when the pattern allows
+    // operators (tree-permuted ordering): CSR always, 3 x 3 blocks when the
+    // pattern allows
     auto csrOp = [&](const int64_t* rp, const int64_t* ci, const double* vx, double nnz) {
         SparseOp o;
         o.n = n;
@@ -860,33 +1063,22 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
         o.nnz = nnz;
         return o;
     };
-    opK = csrOp(dKp, dKi, dKx, static_cast<double>(nk));
-    opM = csrOp(dMp, dMi, dMx, static_cast<double>(nm));
     opPK = csrOp(dPKp, dPKi, dPKx, static_cast<double>(nk));
     opPM = csrOp(dPMp, dPMi, dPMx, static_cast<double>(nm));
-    Bsr3Quads qK, qM, qPK, qPM;
-    bool blocks = false;
-    {
-        Bsr3Host bK, bM, bPK, bPM;
-        blocks = buildBsr3(n, Kp.data(), Ki.data(), Kx.data(), bK) &&
-                 buildBsr3(n, Mp.data(), Mi.data(), Mx.data(), bM) &&
-                 buildBsr3(n, pkp.data(), pki.data(), pkx.data(), bPK) &&
-                 buildBsr3(n, pmp.data(), pmi.data(), pmx.data(), bPM) && buildQuads(bK, qK) &&
-                 buildQuads(bM, qM) && buildQuads(bPK, qPK) && buildQuads(bPM, qPM);
-    }
+    const bool blocks = okB[0] && okB[1];
     bsrMem.reset();
     if (blocks) {
         auto& B = fresh(bsrMem);
-        for (const Bsr3Quads* h : {&qK, &qM, &qPK, &qPM}) {
+        for (const Bsr3Quads* h : {&qPK, &qPM}) {
             B.plan(h->qptr.size() * 8);
             B.plan(h->qvptr.size() * 8);
             B.plan(h->ucol.size() * 4);
             B.plan(h->val.size() * 8);
         }
         B.commit();
-        SparseOp* ops[4] = {&opK, &opM, &opPK, &opPM};
-        const Bsr3Quads* hs[4] = {&qK, &qM, &qPK, &qPM};
-        for (int q = 0; q < 4; ++q) {
+        SparseOp* ops[2] = {&opPK, &opPM};
+        const Bsr3Quads* hs[2] = {&qPK, &qPM};
+        for (int q = 0; q < 2; ++q) {
             int64_t* pq = B.take<int64_t>(hs[q]->qptr.size());
             int64_t* pv = B.take<int64_t>(hs[q]->qvptr.size());
             uint32_t* pu = B.take<uint32_t>(hs[q]->ucol.size());
@@ -904,6 +1096,10 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
         }
     }
     sync();   // host vectors above go out of scope
+    const auto tSet3 = Clock::now();
+    setupTimes[1] = secs(tSet1, tSet2);
+    setupTimes[2] = secs(tSet2, tSet3);
+    setupTimes[3] = secs(tSet0, tSet3);
 }
 
 // ---------------------------------------------------------------------------
@@ -916,6 +1112,7 @@ static void checkPrecision(int prec) {
 void Ctx::runGE(int prec) {
     checkPrecision(prec);
     requireGpu();
+    kstats.phase = "ge";
     sync();   // earlier phases may still use buffers released below
     if (!haveProblem) throw StatusError(AMLS_E_STATE, "no problem set");
     geDone = specReady = selected = projected = reduced = recovered = false;
@@ -960,7 +1157,7 @@ void Ctx::runGE(int prec) {
         double *F = dF, *G = dG, *Li = dLinv, *st = dStat;
         const int64_t nf = ftot, nl = co[N], nk = nasmK, nm = nasmM;
         const int64_t *sK = dsrcK, *tK = dtgtK, *sM = dsrcM, *tM = dtgtM;
-        const double *Kx_ = dKx, *Mx_ = dMx;
+        const double *Kx_ = dPKx, *Mx_ = dPMx;
         const double sh = shift;
         plan.fn([=](hipStream_t s) {
             launch_set_i32(err, INT_MAX, s);
@@ -1260,6 +1457,7 @@ void Ctx::finishGE() {
 // ---------------------------------------------------------------------------
 void Ctx::spectralPrepare() {
     requireGpu();
+    kstats.phase = "spectral";
     sync();   // earlier phases may still use buffers released below
     if (!geDone) throw StatusError(AMLS_E_STATE, "spectral_prepare before ge");
     specReady = selected = projected = reduced = recovered = false;
@@ -1376,6 +1574,7 @@ void Ctx::spectralPrepare() {
 void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
     if (kind != 0 && kind != 1) throw StatusError(AMLS_E_ARG, "truncation kind must be 0 or 1");
     requireGpu();
+    kstats.phase = "select";
     sync();   // earlier phases may still use buffers released below
     if (!specReady) throw StatusError(AMLS_E_STATE, "spectral before spectral_prepare");
     if (static_cast<int64_t>(valueSeg.size()) != nseg)
@@ -1562,6 +1761,7 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
 void Ctx::project(int prec) {
     checkPrecision(prec);
     requireGpu();
+    kstats.phase = "project";
     sync();   // earlier phases may still use buffers released below
     if (!selected) throw StatusError(AMLS_E_STATE, "project before spectral");
     projected = reduced = recovered = false;
@@ -1682,6 +1882,7 @@ void Ctx::project(int prec) {
 // ---------------------------------------------------------------------------
 void Ctx::reducedSolve(int64_t w, bool vectors) {
     requireGpu();
+    kstats.phase = "reduced";
     sync();   // earlier phases may still use buffers released below
     if (!projected) throw StatusError(AMLS_E_STATE, "reduced_solve before project");
     reduced = recovered = false;
@@ -1770,6 +1971,7 @@ void Ctx::reducedSolve(int64_t w, bool vectors) {
 // ---------------------------------------------------------------------------
 void Ctx::backtransform() {
     requireGpu();
+    kstats.phase = "backtransform";
     sync();   // earlier phases may still use buffers released below
     if (!reduced || !redVectors) throw StatusError(AMLS_E_STATE, "backtransform needs reduced vectors");
     const int64_t N = S.nNodes, w = want;
@@ -1835,13 +2037,15 @@ void Ctx::backtransform() {
     const int64_t nn = n;
     int64_t* perm = dperm;
     double *zf = dZfull, *xo_ = dXout;
-    plan.fn([=](hipStream_t s) { launch_permute_rows(nn, perm, zf, xo_, static_cast<int>(w), nn, s); });
-    addSpmm(plan, opM, xo_, Ytmp, w, Tr1, Tr2);
+    // M-normalise in the tree-permuted ordering (x^T M x is the same there),
+    // then un-permute
+    addSpmm(plan, opPM, zf, Ytmp, w, Tr1, Tr2);
     const int64_t* seg = dSegRow;
     const int ns = static_cast<int>(nseg);
     plan.fn([=](hipStream_t s) {
-        launch_colnorm_seg(seg, ns, xo_, Ytmp, static_cast<int>(w), nn, nrm, s);
+        launch_colnorm_seg(seg, ns, zf, Ytmp, static_cast<int>(w), nn, nrm, s);
     });
+    plan.fn([=](hipStream_t s) { launch_permute_rows(nn, perm, zf, xo_, static_cast<int>(w), nn, s); });
     plan.run(stream);
     recovered = true;
 }
@@ -1998,6 +2202,7 @@ void addLdltSolve(Plan& plan, double* dA, int64_t n, double* dX, int64_t nrhs, d
 void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& thetaOut,
                        bool fromBasis) {
     requireGpu();
+    kstats.phase = "si";
     sync();
     if (steps <= 0) throw StatusError(AMLS_E_ARG, "steps must be positive");
     if (fromBasis) {
@@ -2489,15 +2694,15 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     double* nrm = dsq;                     // NS x nev <= NS q
     const int64_t* seg = dSegRow;
     const int nsg = static_cast<int>(NS);
-    plan.fn([=](hipStream_t s) { launch_permute_rows(nn, perm, Q, xo_, nv, nn, s); });
-    addSpmm(plan, opM, xo_, MY, nev, Tr1, Tr2);
-    plan.fn([=](hipStream_t s) { launch_colnorm_seg(seg, nsg, xo_, MY, nv, nn, nrm, s); });
+    addSpmm(plan, opPM, Q, MY, nev, Tr1, Tr2);
+    plan.fn([=](hipStream_t s) { launch_colnorm_seg(seg, nsg, Q, MY, nv, nn, nrm, s); });
     // theta_j = x_j^T K x_j of the M-normalised vectors: the Ritz value
     // restated as the Rayleigh quotient of the vector actually returned, so
     // the pair stays consistent to rounding however the projected pencil
-    // was conditioned
-    addSpmm(plan, opK, xo_, MY, nev, Tr1, Tr2);
-    plan.fn([=](hipStream_t s) { launch_coldot_seg(seg, nsg, xo_, MY, nv, nn, th, s); });
+    // was conditioned.  Both in the tree-permuted ordering, then un-permuted.
+    addSpmm(plan, opPK, Q, MY, nev, Tr1, Tr2);
+    plan.fn([=](hipStream_t s) { launch_coldot_seg(seg, nsg, Q, MY, nv, nn, th, s); });
+    plan.fn([=](hipStream_t s) { launch_permute_rows(nn, perm, Q, xo_, nv, nn, s); });
     plan.run(stream);
     thetaOut.assign(NS * nev, 0.0);
     int ecode = 0;
@@ -2691,6 +2896,15 @@ amls_status amls_hip_tree_info(const amls_ctx* cctx, int64_t* nNodes, int64_t* m
         if (nNodes) *nNodes = c.S.nNodes;
         if (maxLevel) *maxLevel = c.S.maxLevel;
         if (n) *n = c.n;
+    });
+}
+
+amls_status amls_hip_setup_timing(const amls_ctx* cctx, double* t) {
+    auto* ctx = const_cast<amls_ctx*>(cctx);
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (!t) throw amls::StatusError(AMLS_E_ARG, "null output");
+        if (!c.haveProblem) throw amls::StatusError(AMLS_E_STATE, "no problem set");
+        std::copy(c.setupTimes, c.setupTimes + 4, t);
     });
 }
 
@@ -3065,6 +3279,7 @@ amls_status amls_hip_profile(amls_ctx* ctx, int32_t enable) {
     return guarded(ctx, [&](amls::Ctx& c) {
         c.kstats.reset();
         c.kstats.enabled = enable != 0;
+        c.kstats.byPhase = enable == 2;
         if (c.kstats.enabled && c.gpu) {
             // "launch_overhead": 512 back-to-back empty kernels under one
             // event pair, the dispatch gap that a class timed as one bracket

@@ -118,6 +118,10 @@ struct GemmBatch {
 // flops and bytes of the launch computed from its descriptors.
 struct KernelStats {
     bool enabled = false;
+    // byPhase: entries are keyed "phase/class" instead of "class", with the
+    // phase the engine is in when the launch runs
+    bool byPhase = false;
+    const char* phase = "";
     struct Entry {
         int64_t launches = 0;
         double ms = 0.0, flops = 0.0, bytes = 0.0;
@@ -183,11 +187,13 @@ struct Ctx {
     // ---- problem (host)
     int64_t n = 0;
     double shift = 0.0;
-    std::vector<int64_t> Kp, Ki, Mp, Mi;
-    std::vector<double> Kx, Mx;
     Symbolic S;
     std::vector<int64_t> owner;
     bool haveProblem = false;
+    // host set-up of the last setProblems, seconds: canonical columns, graph,
+    // nested dissection and symbolic tables; device tables (assembly maps,
+    // permuted CSR, 3 x 3 block layouts); upload; total
+    double setupTimes[4] = {0.0, 0.0, 0.0, 0.0};
     // independent problems solved together (a forest): problem s owns rows
     // [segRow[s], segRow[s+1]) in both orderings and post-order nodes
     // [segNode[s], segNode[s+1]); one problem is the case nseg = 1
@@ -196,15 +202,13 @@ struct Ctx {
 
     // ---- problem (device)
     std::unique_ptr<DevArena> probMem;
-    double *dKx = nullptr, *dMx = nullptr;
-    int64_t *dKp = nullptr, *dKi = nullptr, *dMp = nullptr, *dMi = nullptr;
     int64_t *dsrcK = nullptr, *dtgtK = nullptr, *dsrcM = nullptr, *dtgtM = nullptr;
     int64_t nasmK = 0, nasmM = 0;
     int64_t *dperm = nullptr, *dbidx = nullptr, *drel = nullptr;
     int64_t* dSegRow = nullptr;   // segRow on the device
     // K and M as operators, original and permuted ordering (3 x 3 blocks when
     // the pattern has them)
-    SparseOp opK, opM, opPK, opPM;
+    SparseOp opPK, opPM;   // K and M in the tree-permuted ordering
     std::unique_ptr<DevArena> bsrMem;
     // permuted CSR of K and M (for the subspace iteration)
     int64_t *dPKp = nullptr, *dPKi = nullptr, *dPMp = nullptr, *dPMi = nullptr;

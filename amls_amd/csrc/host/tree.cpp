@@ -1,6 +1,8 @@
 // Separator tree, nested dissection and symbolic analysis (host side).
 #include "tree.hpp"
 
+#include "par.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <functional>
@@ -113,15 +115,16 @@ bool SeparatorTree::validate(int64_t nTotal, std::string* why) const {
 // ---------------------------------------------------------------------------
 // Graph of |K| + |M|
 // ---------------------------------------------------------------------------
-Graph graphFromCsc(int64_t n, const int64_t* cpK, const int64_t* riK,
-                   const int64_t* cpM, const int64_t* riM) {
+namespace {
+
+// general form: both directions of every entry of either pattern, then one
+// sorted, duplicate-free list per vertex
+Graph symmetrisedGraph(int64_t n, const int64_t* cpK, const int64_t* riK, const int64_t* cpM,
+                       const int64_t* riM) {
     Graph g;
     g.n = n;
     g.ptr.assign(static_cast<size_t>(n) + 1, 0);
     std::vector<int64_t> mark(static_cast<size_t>(n), -1);
-    // union of both patterns, symmetrised, self loops dropped
-    std::vector<std::vector<int64_t>> tmp;  // avoided for large n: two passes
-    // pass 1: count
     auto visit = [&](auto&& fn) {
         for (int64_t c = 0; c < n; ++c) {
             for (int64_t e = cpK[c]; e < cpK[c + 1]; ++e) fn(c, riK[e]);
@@ -130,8 +133,6 @@ Graph graphFromCsc(int64_t n, const int64_t* cpK, const int64_t* riK,
         }
     };
     std::vector<int64_t> deg(static_cast<size_t>(n), 0);
-    // symmetric patterns are the norm; symmetrise anyway by counting both
-    // directions and deduplicating per vertex afterwards
     visit([&](int64_t c, int64_t r) {
         if (r == c) return;
         ++deg[static_cast<size_t>(c)];
@@ -146,7 +147,6 @@ Graph graphFromCsc(int64_t n, const int64_t* cpK, const int64_t* riK,
         adj[static_cast<size_t>(fill[c]++)] = r;
         adj[static_cast<size_t>(fill[r]++)] = c;
     });
-    // dedupe per vertex
     g.adj.reserve(adj.size() / 2 + 16);
     for (int64_t v = 0; v < n; ++v) {
         for (int64_t e = ptr[v]; e < ptr[v + 1]; ++e) {
@@ -159,7 +159,103 @@ Graph graphFromCsc(int64_t n, const int64_t* cpK, const int64_t* riK,
         std::sort(g.adj.begin() + g.ptr[v], g.adj.end());
         g.ptr[v + 1] = static_cast<int64_t>(g.adj.size());
     }
-    (void)tmp;
+    return g;
+}
+
+}  // namespace
+
+Graph graphFromCsc(int64_t n, const int64_t* cpK, const int64_t* riK,
+                   const int64_t* cpM, const int64_t* riM, int threads) {
+    // Fast path for the usual input, both triangles of a symmetric pattern:
+    // vertex c's neighbours are the merged rows of column c of K and M,
+    // self loop dropped.  Columns whose rows are not strictly ascending are
+    // sorted first.  The result is returned when every neighbour relation
+    // holds both ways (checked by binary search); otherwise the pattern is
+    // symmetrised the general way.
+    const int T = n < 4096 ? 1 : threads > 0 ? threads : hostThreads();
+    std::vector<int64_t> deg(static_cast<size_t>(n) + 1, 0);
+    auto column = [&](int64_t c, std::vector<int64_t>& buf, int64_t* out) -> int64_t {
+        const int64_t* a = riK + cpK[c];
+        const int64_t na = cpK[c + 1] - cpK[c];
+        const int64_t* b = cpM ? riM + cpM[c] : nullptr;
+        const int64_t nb = cpM ? cpM[c + 1] - cpM[c] : 0;
+        bool sorted = true;
+        for (int64_t e = 1; e < na && sorted; ++e) sorted = a[e] > a[e - 1];
+        for (int64_t e = 1; e < nb && sorted; ++e) sorted = b[e] > b[e - 1];
+        int64_t m = 0;
+        auto emit = [&](int64_t r) {
+            if (r == c || (m > 0 && (out ? out[m - 1] : buf[0]) == r)) return;
+            if (out) out[m] = r;
+            else buf[0] = r;
+            ++m;
+        };
+        if (sorted) {
+            int64_t i = 0, j = 0;
+            while (i < na || j < nb) {
+                const int64_t r = j >= nb || (i < na && a[i] <= b[j]) ? a[i++] : b[j++];
+                emit(r);
+            }
+        } else {
+            buf.assign(a, a + na);
+            buf.insert(buf.end(), b, b + nb);
+            std::sort(buf.begin(), buf.end());
+            int64_t last = -1;
+            for (int64_t r : buf) {
+                if (r == c || r == last) continue;
+                last = r;
+                if (out) out[m] = r;
+                ++m;
+            }
+        }
+        return m;
+    };
+    parallelRanges(n, T, [&](int64_t lo, int64_t hi, int) {
+        std::vector<int64_t> buf(1);
+        for (int64_t c = lo; c < hi; ++c) deg[c + 1] = column(c, buf, nullptr);
+    });
+    Graph g;
+    g.n = n;
+    g.ptr.assign(static_cast<size_t>(n) + 1, 0);
+    for (int64_t v = 0; v < n; ++v) g.ptr[v + 1] = g.ptr[v] + deg[v + 1];
+    g.adj.resize(static_cast<size_t>(g.ptr[n]));
+    std::vector<char> asym(static_cast<size_t>(T), 0);
+    parallelRanges(n, T, [&](int64_t lo, int64_t hi, int) {
+        std::vector<int64_t> buf(1);
+        for (int64_t c = lo; c < hi; ++c) column(c, buf, g.adj.data() + g.ptr[c]);
+    });
+    // symmetry: the vertices c of a range [lo, hi), visited in ascending
+    // order, must meet each neighbour list adj(r) at its entries in [lo, hi)
+    // in order, and use all of them.  One cursor per touched r, started by a
+    // binary search for lo.
+    parallelRanges(n, T, [&](int64_t lo, int64_t hi, int chunk) {
+        std::vector<int64_t> cur(static_cast<size_t>(n), -1), touched;
+        for (int64_t c = lo; c < hi; ++c)
+            for (int64_t e = g.ptr[c]; e < g.ptr[c + 1]; ++e) {
+                const int64_t r = g.adj[e];
+                int64_t& k = cur[static_cast<size_t>(r)];
+                if (k < 0) {
+                    k = std::lower_bound(g.adj.begin() + g.ptr[r], g.adj.begin() + g.ptr[r + 1], lo) -
+                        g.adj.begin();
+                    touched.push_back(r);
+                }
+                if (k >= g.ptr[r + 1] || g.adj[k] != c) {
+                    asym[static_cast<size_t>(chunk)] = 1;
+                    return;
+                }
+                ++k;
+            }
+        for (int64_t r : touched) {
+            const int64_t k = cur[static_cast<size_t>(r)];
+            if (k < g.ptr[r + 1] && g.adj[k] < hi) {
+                asym[static_cast<size_t>(chunk)] = 1;
+                return;
+            }
+        }
+        // every entry in [lo, hi) of an untouched list would have been met
+        // from its column: such lists hold none
+    });
+    for (char a : asym)
+        if (a) return symmetrisedGraph(n, cpK, riK, cpM, riM);
     return g;
 }
 

@@ -60,8 +60,9 @@ struct Graph {
     std::vector<int64_t> ptr, adj;
 };
 
+// threads: host threads to use (0: all, at most 16).
 Graph graphFromCsc(int64_t n, const int64_t* colptrK, const int64_t* rowK,
-                   const int64_t* colptrM, const int64_t* rowM);
+                   const int64_t* colptrM, const int64_t* rowM, int threads = 0);
 
 // coords: n x 3 row-major (may be null).
 PartitionResult recursiveBisection(const Graph& g, const double* coords,

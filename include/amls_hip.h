@@ -125,6 +125,13 @@ amls_status amls_hip_tree_info(const amls_ctx* ctx, int64_t* nNodes, int64_t* ma
 amls_status amls_hip_tree_get(const amls_ctx* ctx, int64_t* treeId, int64_t* offset,
                               int64_t* size, int64_t* parent, int64_t* level,
                               int64_t* bsize, int64_t* perm);
+/* Host set-up time of the last set_problem(_batch), seconds, t[4]:
+ * t[0] canonical columns, graph of |K| + |M|, nested dissection and symbolic
+ * tables; t[1] device tables (assembly maps, permuted CSR, 3 x 3 block
+ * layouts); t[2] upload to the device; t[3] total.  Host-only contexts do
+ * t[0] only.  The breakdown of the reference's "partition" timing key
+ * (src/amls/solver.cpp:124-134), which covers only t[0]'s partition step. */
+amls_status amls_hip_setup_timing(const amls_ctx* ctx, double* t);
 
 /*
  * Block Gaussian elimination of every tree level (leaves first): per node a
@@ -313,7 +320,9 @@ amls_status amls_hip_set_tuning(amls_ctx* ctx, const int64_t* table, int64_t row
  * CSR product) are always counted.  Enabling also times 512 back-to-back
  * empty kernels as the class "launch_overhead" (the dispatch gap included by
  * classes timed as one bracket around many launches: sytrd_step and
- * sytrd_batch).  No counterpart in the reference. */
+ * sytrd_batch).  enable = 2 keys each class by the solve phase it ran in
+ * ("ge/gemm_f64", "si/spmm", ...; phases ge, spectral, select, project,
+ * reduced, backtransform, si).  No counterpart in the reference. */
 amls_status amls_hip_profile(amls_ctx* ctx, int32_t enable);
 /* Class `index` (0-based, sorted by name); AMLS_E_ARG past the last one. */
 amls_status amls_hip_kernel_stats(amls_ctx* ctx, int32_t index, char* name, int32_t name_len,

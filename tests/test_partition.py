@@ -118,3 +118,29 @@ def test_batch_forest_equals_single_trees():
         par = t["parent"][a:b]
         assert np.array_equal(np.where(par >= 0, par - a, -1), s["parent"])
         assert np.array_equal(t["perm"][rows[i]:rows[i + 1]] - rows[i], s["perm"])
+
+
+@pytest.mark.parametrize("coords", [False, True])
+def test_one_sided_pattern_is_symmetrised(coords):
+    """The graph of |K| + |M| joins i and j when either (i, j) or (j, i) is
+    stored: a pencil given by its lower triangles only (and one with a single
+    extra one-sided entry) is partitioned like the full symmetric one.  The
+    first takes the library's general symmetrising path, the full pencil its
+    merge path."""
+    pb = cases.hex_elasticity(1, 5, 4, 3)
+    xyz = pb.coords if coords else None
+    full = analyse(pb.K, pb.M, xyz, 3, 40)
+    lower = analyse(sp.tril(pb.K).tocsc(), sp.tril(pb.M).tocsc(), xyz, 3, 40)
+    for key in ("treeId", "offset", "size", "parent", "bsize", "perm"):
+        assert np.array_equal(full[key], lower[key]), key
+    # one entry (i, j) whose mirror (j, i) is absent from both matrices
+    K = sp.lil_matrix(pb.K)
+    i, j = 0, pb.n - 1
+    assert pb.K[i, j] == 0 and pb.M[i, j] == 0
+    K[i, j] = 1e-3
+    one = analyse(K.tocsc(), pb.M, xyz, 3, 40)
+    Ks = sp.lil_matrix(K)
+    Ks[j, i] = 1e-3
+    both = analyse(Ks.tocsc(), pb.M, xyz, 3, 40)
+    for key in ("treeId", "offset", "size", "parent", "bsize", "perm"):
+        assert np.array_equal(one[key], both[key]), key
