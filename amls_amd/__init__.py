@@ -281,11 +281,12 @@ class HipContext:
         t = np.ascontiguousarray(np.asarray(table, dtype=np.int64).reshape(-1, 3))
         self._chk(self._L.amls_hip_set_tuning(self._h, iptr(t) if t.size else None, t.shape[0]))
 
-    def profile(self, enable: bool = True, by_phase: bool = False):
+    def profile(self, enable: bool = True, by_phase: bool = False, by_shape: bool = False):
         """HIP-event timing of every launch class; with by_phase the classes
         are keyed "phase/class" (ge, spectral, select, project, reduced,
-        backtransform, si)."""
-        self._chk(self._L.amls_hip_profile(self._h, 2 if (enable and by_phase) else int(enable)))
+        backtransform, si), with by_shape GEMM batches also by their shape."""
+        mode = (3 if by_shape else 2 if by_phase else 1) if enable else 0
+        self._chk(self._L.amls_hip_profile(self._h, mode))
 
     def kernel_stats(self) -> dict:
         out = {}

@@ -3279,7 +3279,8 @@ amls_status amls_hip_profile(amls_ctx* ctx, int32_t enable) {
     return guarded(ctx, [&](amls::Ctx& c) {
         c.kstats.reset();
         c.kstats.enabled = enable != 0;
-        c.kstats.byPhase = enable == 2;
+        c.kstats.byPhase = enable >= 2;
+        c.kstats.byShape = enable == 3;
         if (c.kstats.enabled && c.gpu) {
             // "launch_overhead": 512 back-to-back empty kernels under one
             // event pair, the dispatch gap that a class timed as one bracket

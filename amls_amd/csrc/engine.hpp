@@ -121,6 +121,9 @@ struct KernelStats {
     // byPhase: entries are keyed "phase/class" instead of "class", with the
     // phase the engine is in when the launch runs
     bool byPhase = false;
+    // byShape (with byPhase): GEMM batches are keyed by their largest m, n, k,
+    // descriptor count and operand form as well
+    bool byShape = false;
     const char* phase = "";
     struct Entry {
         int64_t launches = 0;

@@ -172,7 +172,22 @@ struct Plan {
         Blob* bl = blob;
         double fl, by;
         gemmWork(b, fp32, fl, by);
-        timed(fp32 ? "gemm_f32" : "gemm_f64", fl, by, [=](hipStream_t s) {
+        std::string name = fp32 ? "gemm_f32" : "gemm_f64";
+        if (st && st->byShape) {
+            int mm = 0, nn = 0, kk = 0, tri = 0, low = 0, split = 0;
+            for (const auto& d : b.descs) {
+                mm = std::max(mm, d.m);
+                nn = std::max(nn, d.n);
+                kk = std::max(kk, std::max(d.k, d.k2));
+                tri |= d.triA != 0;
+                low |= d.lower != 0;
+                split |= d.kchunk > 0;
+            }
+            name += " m" + std::to_string(mm) + " n" + std::to_string(nn) + " k" + std::to_string(kk) +
+                    " x" + std::to_string(b.descs.size()) + (tri ? " tri" : "") + (low ? " low" : "") +
+                    (split ? " split" : "") + " t" + std::to_string(b.tiles.size());
+        }
+        timed(name, fl, by, [=](hipStream_t s) {
             launch_gemm(bl->at<GemmDesc>(od), bl->at<TileRef>(ot), nt, fp32, s);
         });
     }

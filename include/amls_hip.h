@@ -322,7 +322,9 @@ amls_status amls_hip_set_tuning(amls_ctx* ctx, const int64_t* table, int64_t row
  * classes timed as one bracket around many launches: sytrd_step and
  * sytrd_batch).  enable = 2 keys each class by the solve phase it ran in
  * ("ge/gemm_f64", "si/spmm", ...; phases ge, spectral, select, project,
- * reduced, backtransform, si).  No counterpart in the reference. */
+ * reduced, backtransform, si); enable = 3 also splits GEMM batches by their
+ * largest m, n, k, descriptor count, operand form and tile count.  No
+ * counterpart in the reference. */
 amls_status amls_hip_profile(amls_ctx* ctx, int32_t enable);
 /* Class `index` (0-based, sorted by name); AMLS_E_ARG past the last one. */
 amls_status amls_hip_kernel_stats(amls_ctx* ctx, int32_t index, char* name, int32_t name_len,
