@@ -179,6 +179,16 @@ class HipContext:
         self._chk(self._L.amls_hip_batch_info(self._h, None, iptr(rows), iptr(nodes)))
         return rows, nodes
 
+    def trace_begin(self, path: str):
+        """Record a Chrome trace (host phases and GPU launches) into path."""
+        self._chk(self._L.amls_hip_trace(self._h, str(path).encode(), None))
+
+    def trace_end(self) -> int:
+        """Write the trace started by trace_begin; the number of spans."""
+        n = C.c_int64()
+        self._chk(self._L.amls_hip_trace(self._h, None, C.byref(n)))
+        return n.value
+
     def setup_timing(self) -> dict:
         """Seconds of the last set_problem: analysis (canonical columns, graph,
         nested dissection, symbolic tables), tables (device-side index maps and
@@ -186,6 +196,20 @@ class HipContext:
         t = np.zeros(4)
         self._chk(self._L.amls_hip_setup_timing(self._h, dptr(t)))
         return dict(zip(("analysis", "tables", "upload", "total"), t.tolist()))
+
+    def set_partition(self, method: str = "auto"):
+        """Bisection of the following set_problem calls: "auto" (planes with
+        coordinates, else spectral cut + FM refinement on low-degree graphs,
+        else BFS level sets) or "levels" (BFS level sets only)."""
+        self._chk(self._L.amls_hip_set_partition(self._h, {"auto": 0, "levels": 1}[method]))
+
+    def tree_stats(self) -> dict:
+        """Statistics of the current separator tree: per level (0 = leaves)
+        the node count and the min / max / total node and boundary sizes, and
+        in all the separator total, the front storage sum (n_p + b_p)^2 and
+        the dense elimination flops sum n_p^3 / 3 + n_p^2 b_p + n_p b_p^2
+        (the factor, X and the Schur update of each front)."""
+        return tree_stats(self.tree())
 
     def tree(self) -> dict:
         nn, ml, n = C.c_int64(), C.c_int64(), C.c_int64()
@@ -471,6 +495,26 @@ class DenseFactor:
             self.close()
         except Exception:
             pass
+
+
+def tree_stats(t: dict) -> dict:
+    """Separator-tree statistics from a tree() dictionary (see
+    HipContext.tree_stats)."""
+    size, b, lvl = (np.asarray(t[k], dtype=np.float64) for k in ("size", "bsize", "level"))
+    levels = []
+    for l in range(int(t["maxLevel"]) + 1):
+        m = np.asarray(t["level"]) == l
+        levels.append({"level": l, "nodes": int(m.sum()),
+                       "size_min": int(size[m].min()), "size_max": int(size[m].max()),
+                       "size_sum": int(size[m].sum()),
+                       "bsize_min": int(b[m].min()), "bsize_max": int(b[m].max()),
+                       "bsize_sum": int(b[m].sum())})
+    sep = np.asarray(t["level"]) > 0
+    return {"nodes": int(len(size)), "levels": levels,
+            "separator_total": int(size[sep].sum()),
+            "root_size": int(size[np.asarray(t["parent"]) < 0].max()),
+            "front_storage": float(((size + b) ** 2).sum()),
+            "elim_flops": float((size ** 3 / 3 + size ** 2 * b + size * b ** 2).sum())}
 
 
 class AmlsSolver:

@@ -31,7 +31,7 @@ EXPORTS = [
     "amls_hip_ldlt_solve_noalloc", "amls_hip_gemm_gather", "amls_hip_ldlt_factor",
     "amls_hip_ldlt_factor_solve", "amls_hip_ldlt_factor_get", "amls_hip_ldlt_factor_free",
     "amls_hip_dense_eig", "amls_hip_set_rr_mode", "amls_hip_get_rr_info",
-    "amls_hip_setup_timing",
+    "amls_hip_setup_timing", "amls_hip_trace", "amls_hip_set_partition",
 ]
 
 
@@ -127,6 +127,8 @@ def lib() -> C.CDLL:
         "amls_hip_set_rr_mode": ([P, I32], C.c_int),
         "amls_hip_get_rr_info": ([P, C.POINTER(C.c_int32), PD], C.c_int),
         "amls_hip_setup_timing": ([P, PD], C.c_int),
+        "amls_hip_trace": ([P, C.c_char_p, PI64], C.c_int),
+        "amls_hip_set_partition": ([P, I32], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -28,6 +28,7 @@ SOURCES = [
     CSRC / "noalloc.cpp",
     CSRC / "operators.cpp",
     CSRC / "chfsi.cpp",
+    CSRC / "trace.cpp",
     CSRC / "engine.cpp",
 ]
 

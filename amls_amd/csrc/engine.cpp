@@ -15,6 +15,8 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <numeric>
@@ -121,6 +123,14 @@ void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<i
             if (v > i) bytes += 8.0 * double(v - i - 1) * double(v - i);
     KernelStats* kst = plan.st;
     const int64_t nsteps = i0;
+    // the tail: Householder reduction of each matrix's last t = n - i0
+    // order (4/3 t^3 flops), its lower triangle read and written once
+    double tailFl = 0.0, tailBy = 0.0;
+    for (int v : ns) {
+        const double t = std::max(0, v - i0);
+        tailFl += 4.0 / 3.0 * t * t * t;
+        tailBy += 8.0 * t * (t + 1);
+    }
     const EigDesc d1 = descs[0];
     double* sv1 = sv + 2 * soff[0];
     double* sp1 = sp + 3 * soff[0];
@@ -144,7 +154,7 @@ void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<i
             // are separate kernels and separate statistics
             if (nsteps > 0)
                 kst->run(one ? "sytrd_step" : "sytrd_batch", 2.0 * bytes / 8.0, bytes, s, loop, nsteps);
-            kst->run("sytrd_tail", 0.0, 0.0, s, tail);
+            kst->run("sytrd_tail", tailFl, tailBy, s, tail);
         } else {
             loop();
             tail();
@@ -533,18 +543,38 @@ void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors, bool
     const int nw = static_cast<int>(work.size()), nq = static_cast<int>(owork.size());
     const int ns = static_cast<int>(swork.size());
     KernelStats* kst = plan.st;
+    // algorithmic work of the textbook methods (LAPACK dstebz / dstein /
+    // dormtr), per matrix of order n with k wanted pairs:
+    //   bisection: 53 halvings of one Sturm count (2 flops per row) per
+    //     eigenvalue, the tridiagonal read once;
+    //   inverse iteration: an LU of T - lambda I (6 n flops) and two solves
+    //     (5 n each) per vector, the tridiagonal read and the vector written;
+    //   back-transformation: 4 n^2 k flops, the reflectors (n^2 / 2 words)
+    //     read once and the k columns read and written.
+    double fBis = 0, bBis = 0, fSt = 0, bSt = 0, fOr = 0, bOr = 0;
+    for (const auto& d : descs) {
+        const double n = d.n, k = d.k;
+        if (k <= 0) continue;
+        fBis += 53.0 * 2.0 * n * k;
+        bBis += 16.0 * n;
+        fSt += 16.0 * n * k;
+        bSt += 16.0 * n + 8.0 * n * k;
+        fOr += 4.0 * n * n * k;
+        bOr += 4.0 * n * n + 16.0 * n * k;
+    }
     plan.fn([=](hipStream_t s) {
-        auto run = [&](const char* name, const std::function<void()>& f) {
-            if (kst) kst->run(name, 0.0, 0.0, s, f);
+        auto run = [&](const char* name, double fl, double by, const std::function<void()>& f) {
+            if (kst) kst->run(name, fl, by, s, f);
             else f();
         };
-        run("bisect", [&] { launch_bisect(bl->at<EigDesc>(od), bl->at<int2>(ow), nw, nmax, s); });
+        run("bisect", fBis, bBis,
+            [&] { launch_bisect(bl->at<EigDesc>(od), bl->at<int2>(ow), nw, nmax, s); });
         if (vectors) {
-            run("stein", [&] {
+            run("stein", fSt, bSt, [&] {
                 launch_stein(bl->at<EigDesc>(od), bl->at<int2>(os), ns, bl->at<int64_t>(oo), nmax, s);
             });
             if (backTransform)
-                run("ormtr", [&] {
+                run("ormtr", fOr, bOr, [&] {
                     if (wave) launch_ormtr_wave(bl->at<EigDesc>(od), nmat, bl->at<int4>(ov), nv, nmax, s);
                     else launch_ormtr(bl->at<EigDesc>(od), bl->at<int4>(oq), nq, lds, s);
                 });
@@ -601,22 +631,37 @@ void KernelStats::run(const std::string& name, double flops, double bytes, hipSt
     en.launches += count;
     en.flops += flops;
     en.bytes += bytes;
-    if (!enabled) {
+    if (!enabled && !trace) {
         launch();
         return;
+    }
+    if (trace && !base) {
+        // the base of the GPU timeline: an event reached now, read on the host
+        AMLS_HIP_CHECK(hipEventCreate(&base));
+        AMLS_HIP_CHECK(hipEventRecord(base, s));
+        AMLS_HIP_CHECK(hipEventSynchronize(base));
+        baseUs = std::chrono::duration_cast<std::chrono::microseconds>(
+                     std::chrono::steady_clock::now().time_since_epoch())
+                     .count();
     }
     hipEvent_t a = get(), b = get();
     AMLS_HIP_CHECK(hipEventRecord(a, s));
     launch();
     AMLS_HIP_CHECK(hipEventRecord(b, s));
-    pending.push_back({key, a, b});
+    pending.push_back({key, a, b, phase, s == lane1 ? 1 : 2});
 }
 
 void KernelStats::resolve() {
     for (auto& p : pending) {
-        float ms = 0.f;
-        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess)
+        float ms = 0.f, m0 = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
             byName[p.name].ms += ms;
+            if (trace && base && hipEventElapsedTime(&m0, base, p.a) == hipSuccess) {
+                const int64_t t0 = baseUs + static_cast<int64_t>(1000.0 * m0);
+                spans.push_back({"gpu", p.name, p.phase ? p.phase : "", p.tid, t0,
+                                 t0 + static_cast<int64_t>(1000.0 * ms)});
+            }
+        }
         pool.push_back(p.a);
         pool.push_back(p.b);
     }
@@ -629,6 +674,7 @@ void KernelStats::reset() {
 }
 
 KernelStats::~KernelStats() {
+    if (base) (void)hipEventDestroy(base);
     for (auto& p : pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
@@ -845,7 +891,7 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
         canonicalCsc(nn, P.Kp, P.Ki, P.Kx, canK[s], inner, "K");
         canonicalCsc(nn, P.Mp, P.Mi, P.Mx, canM[s], inner, "M");
         const Graph gs = graphFromCsc(nn, canK[s].pcp, canK[s].pri, canM[s].pcp, canM[s].pri, inner);
-        const PartitionResult ps = recursiveBisection(gs, P.coords, nLevels, leafSize);
+        const PartitionResult ps = recursiveBisection(gs, P.coords, nLevels, leafSize, partMethod);
         parts[s] = analyze(ps, gs);
     });
     // block-diagonal CSC of all problems (one problem: its canonical matrix)
@@ -912,6 +958,7 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
     setupTimes[0] = secs(tSet0, tSet1);
     setupTimes[1] = setupTimes[2] = 0.0;
     setupTimes[3] = setupTimes[0];
+    traceHost("setup", "ANALYSIS", tSet0, tSet1);
     if (!gpu) return;
 
     // permuted CSR (= CSC of the symmetric permuted matrix) of K and M:
@@ -1100,6 +1147,8 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
     setupTimes[1] = secs(tSet1, tSet2);
     setupTimes[2] = secs(tSet2, tSet3);
     setupTimes[3] = secs(tSet0, tSet3);
+    traceHost("setup", "TABLES", tSet1, tSet2);
+    traceHost("setup", "UPLOAD", tSet2, tSet3);
 }
 
 // ---------------------------------------------------------------------------
@@ -1143,7 +1192,7 @@ void Ctx::runGE(int prec) {
     // arena's address (the values come from the device): the first run for
     // them is eager, the second captures a graph, later runs replay it
     const bool sameKey = geGraphPrec == prec && geGraphGen == problemGen && geGraphBase == A.base();
-    if (geGraph && sameKey && !kstats.enabled) {
+    if (geGraph && sameKey && !kstats.enabled && !kstats.trace) {
         AMLS_HIP_CHECK(hipGraphLaunch(geGraph, stream));
         finishGE();
         return;
@@ -1402,7 +1451,7 @@ void Ctx::runGE(int prec) {
         plan.gemm(gFm, f32);
         plan.gemm(gG22, f32);
     }
-    if (kstats.enabled || !sameKey) {
+    if (kstats.enabled || kstats.trace || !sameKey) {
         plan.run(stream);
     } else {
         // descriptors go up first, outside the graph; the graph holds the
@@ -2805,6 +2854,12 @@ amls_status amls_hip_create(int32_t device, uint32_t flags, amls_ctx** out) {
         }
         ctx->c.device = device;
         ctx->c.gpu = true;
+        ctx->c.kstats.lane1 = ctx->c.stream;
+    }
+    // AMLS_TRACE=<file.json>: trace this context, written when it is destroyed
+    if (const char* tp = std::getenv("AMLS_TRACE"); tp && *tp) {
+        ctx->c.tracePath = tp;
+        ctx->c.kstats.trace = true;
     }
     *out = ctx;
     return AMLS_OK;
@@ -2812,6 +2867,12 @@ amls_status amls_hip_create(int32_t device, uint32_t flags, amls_ctx** out) {
 
 amls_status amls_hip_destroy(amls_ctx* ctx) {
     if (!ctx) return AMLS_E_ARG;
+    if (ctx->c.kstats.trace && !ctx->c.tracePath.empty()) {
+        if (ctx->c.gpu) (void)hipSetDevice(ctx->c.device);
+        const int64_t n = ctx->c.traceWrite(ctx->c.tracePath);
+        std::fprintf(stderr, "[trace] %lld events -> %s\n", static_cast<long long>(n),
+                     ctx->c.tracePath.c_str());
+    }
     if (ctx->c.gpu) {
         (void)hipSetDevice(ctx->c.device);
         (void)hipStreamSynchronize(ctx->c.stream);
@@ -2896,6 +2957,13 @@ amls_status amls_hip_tree_info(const amls_ctx* cctx, int64_t* nNodes, int64_t* m
         if (nNodes) *nNodes = c.S.nNodes;
         if (maxLevel) *maxLevel = c.S.maxLevel;
         if (n) *n = c.n;
+    });
+}
+
+amls_status amls_hip_set_partition(amls_ctx* ctx, int32_t method) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (method < 0 || method > 1) throw amls::StatusError(AMLS_E_ARG, "method must be 0 or 1");
+        c.partMethod = method;
     });
 }
 
@@ -3120,6 +3188,15 @@ static void solveNumeric(Ctx& c, amls_options o, double* theta, double* Z, int64
     c.lastVectors = vec;
     if (p_out)
         for (int64_t sg = 0; sg < NS; ++sg) p_out[sg] = c.segP[sg];
+    if (c.kstats.trace) {
+        c.traceHost("solver", "SETUP", t0, t1);
+        c.traceHost("ge", "GE", t1, t2);
+        c.traceHost("cond", "CONDENSE", t2, t3);
+        c.traceHost("solver", "REDUCED", t3, t4);
+        c.traceHost("cond", "BACKTRANSFORM", t4, t5);
+        c.traceHost("solver", "SUBSPACE", t5, t6);
+        c.traceHost("solver", "SOLVE", t0, Clock::now());
+    }
     if (timing) {
         timing[0] = secs(t0, t1);
         timing[1] = secs(t1, t2);
@@ -3272,6 +3349,25 @@ amls_status amls_hip_set_tuning(amls_ctx* ctx, const int64_t* table, int64_t row
             t.push_back({tiles, k, sp});
         }
         c.splitTable = std::move(t);
+    });
+}
+
+amls_status amls_hip_trace(amls_ctx* ctx, const char* path, int64_t* events) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (path) {
+            c.tracePath = path;
+            c.kstats.trace = true;
+            c.kstats.spans.clear();
+            if (events) *events = 0;
+            return;
+        }
+        if (!c.kstats.trace) throw amls::StatusError(AMLS_E_STATE, "no trace in progress");
+        const int64_t n = c.traceWrite(c.tracePath);
+        if (n < 0) throw amls::StatusError(AMLS_E_ARG, "cannot write " + c.tracePath);
+        if (events) *events = n;
+        c.kstats.trace = false;
+        c.kstats.spans.clear();
+        c.tracePath.clear();
     });
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -130,9 +131,25 @@ struct KernelStats {
         double ms = 0.0, flops = 0.0, bytes = 0.0;
     };
     std::map<std::string, Entry> byName;
+    // trace: every launch is bracketed by events (as when enabled) and its
+    // span kept on the host's steady-clock microseconds, through a base
+    // event synchronised once; lane 1 is the engine's main stream (lane1),
+    // lane 2 any other stream
+    bool trace = false;
+    hipStream_t lane1 = nullptr;
+    hipEvent_t base = nullptr;
+    int64_t baseUs = 0;
+    struct Span {
+        std::string cat, name, phase;
+        int tid;
+        int64_t t0, t1;
+    };
+    std::vector<Span> spans;
     struct Pending {
         std::string name;
         hipEvent_t a, b;
+        const char* phase;
+        int tid;
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
@@ -193,6 +210,7 @@ struct Ctx {
     Symbolic S;
     std::vector<int64_t> owner;
     bool haveProblem = false;
+    int partMethod = 0;   // recursiveBisection's method for the next set-up
     // host set-up of the last setProblems, seconds: canonical columns, graph,
     // nested dissection and symbolic tables; device tables (assembly maps,
     // permuted CSR, 3 x 3 block layouts); upload; total
@@ -313,6 +331,12 @@ struct Ctx {
     int lastRrIters = 0;     // filter iterations of the last final Rayleigh-Ritz, 0: dense path
     double lastRrResidual = 0.0;
     bool chfsiTop(double* C, int64_t q, int64_t m, double* lam, double* V);   // C made symmetric from its lower triangle
+
+    // Chrome trace of host phases and GPU launches (trace.cpp)
+    std::string tracePath;
+    void traceHost(const char* cat, const char* name, std::chrono::steady_clock::time_point a,
+                   std::chrono::steady_clock::time_point b);
+    int64_t traceWrite(const std::string& path);
 
     SplitTable splitTable;   // empty: the built-in split-k rule
     void autotune();

@@ -64,9 +64,13 @@ struct Graph {
 Graph graphFromCsc(int64_t n, const int64_t* colptrK, const int64_t* rowK,
                    const int64_t* colptrM, const int64_t* rowM, int threads = 0);
 
-// coords: n x 3 row-major (may be null).
+// coords: n x 3 row-major (may be null).  method 0: plane separators where
+// coordinates allow, else the spectral cut with FM refinement for graphs of
+// average degree <= 32 and subsets of <= 20,000 vertices, else BFS level
+// sets (refined likewise on low-degree graphs); method 1: BFS level sets
+// only, unrefined (the plain graph bisection, for comparison).
 PartitionResult recursiveBisection(const Graph& g, const double* coords,
-                                   int nLevels, int64_t leafSize);
+                                   int nLevels, int64_t leafSize, int method = 0);
 
 // ---------------------------------------------------------------------------
 // Symbolic analysis on the post-order permuted pencil.

@@ -125,6 +125,15 @@ amls_status amls_hip_tree_info(const amls_ctx* ctx, int64_t* nNodes, int64_t* ma
 amls_status amls_hip_tree_get(const amls_ctx* ctx, int64_t* treeId, int64_t* offset,
                               int64_t* size, int64_t* parent, int64_t* level,
                               int64_t* bsize, int64_t* perm);
+/* Bisection method of the following set_problem(_batch) calls: 0 (default)
+ * plane separators where coordinates allow, else for graphs of average
+ * degree <= 32 a spectral cut (Fiedler vector by Lanczos, median split,
+ * smaller cut boundary) with Fiduccia-Mattheyses vertex-separator
+ * refinement, else BFS level sets from a pseudo-peripheral vertex; 1 BFS
+ * level sets only, unrefined.  Stands in for MetisPartitioner's
+ * METIS_ComputeVertexSeparator (src/partition/metis_partitioner.cpp:80-103),
+ * keeping its recursion and stop rules (:110-179). */
+amls_status amls_hip_set_partition(amls_ctx* ctx, int32_t method);
 /* Host set-up time of the last set_problem(_batch), seconds, t[4]:
  * t[0] canonical columns, graph of |K| + |M|, nested dissection and symbolic
  * tables; t[1] device tables (assembly maps, permuted CSR, 3 x 3 block
@@ -326,6 +335,18 @@ amls_status amls_hip_set_tuning(amls_ctx* ctx, const int64_t* table, int64_t row
  * largest m, n, k, descriptor count, operand form and tile count.  No
  * counterpart in the reference. */
 amls_status amls_hip_profile(amls_ctx* ctx, int32_t enable);
+
+/* Chrome trace (the reference's AMLS_TRACE, src/core/trace.hpp:82-181):
+ * path != null starts recording into that file (previous spans dropped);
+ * path == null writes it and stops (AMLS_E_STATE when none is in progress;
+ * *events, if given, receives the number of spans).  The trace holds the
+ * host phases (tid 0: set-up analysis / tables / upload, GE, CONDENSE,
+ * REDUCED, BACKTRANSFORM, SUBSPACE, SOLVE) and every GPU launch with its
+ * solve phase (tid 1 main stream, tid 2 second stream), on the host's
+ * steady clock in microseconds; tracing brackets every launch with events.
+ * A context created while AMLS_TRACE=<file> is set records from the start
+ * and writes the file when it is destroyed. */
+amls_status amls_hip_trace(amls_ctx* ctx, const char* path, int64_t* events);
 /* Class `index` (0-based, sorted by name); AMLS_E_ARG past the last one. */
 amls_status amls_hip_kernel_stats(amls_ctx* ctx, int32_t index, char* name, int32_t name_len,
                                   int64_t* launches, double* ms, double* flops, double* bytes);

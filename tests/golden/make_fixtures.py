@@ -1,6 +1,6 @@
 """Write the FP64 ground-truth eigenvalue fixtures used by tests and bench.py.
 
-    python tests/golden/make_fixtures.py [--only hex8_10|hex27_10]
+    python tests/golden/make_fixtures.py [--only hex8_10|hex27_10|hex27_14]
 
 Ground truth follows ReferenceSolver::smallestEigenvalues
 (src/reference/reference_solver.cpp:17-95): dense generalized eigh for
@@ -22,6 +22,7 @@ from oracle.amls_oracle import reference_eigs  # noqa: E402
 FIXTURES = {
     "hex8_10": (cases.case_hex8_10, 20),
     "hex27_10": (cases.case_hex27_10, 100),
+    "hex27_14": (cases.case_hex27_14, 50),
 }
 
 

@@ -265,8 +265,10 @@ def test_ge_graph_replay_matches_eager():
 
 def test_c2_subspace_iteration_matches_oracle():
     """The benchmark workload at full size (27-node hexahedra, 10^3 elements,
-    n = 26,460, lowest 100, eta = 4, two subspace-iteration steps over the
-    whole AMLS basis): the FP64 and mixed-precision GPU eigenvalues agree with
+    n = 26,460, lowest 100, eta = 4): plain FP64 AMLS matches the oracle's
+    eigenvalues within 1e-9 and its reduced size p exactly; with two
+    subspace-iteration steps over the whole AMLS basis the FP64 and
+    mixed-precision GPU eigenvalues agree with
     the oracle's FP64 AMLS + subspace iteration on the same tree within 1e-7
     relative (about 1e-10 measured), and both reach 1e-6 against the
     committed FP64 ground-truth fixture; three and four steps match the
@@ -283,6 +285,14 @@ def test_c2_subspace_iteration_matches_oracle():
     _, _, hist = O.subspace_iteration(pb.K, pb.M, ref.ge, X, 4, nev, tree.perm, every_step=True)
     exact = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "hex27_10_lowest100.txt"))
     solver = AmlsSolver(0)
+    # plain AMLS in FP64 (the drop-in form): the oracle's eigenvalues and p
+    plain = solver.solve(Pencil(pb.K, pb.M, pb.coords),
+                         AmlsOptions(nev=nev, nLevels=6, leafSize=500, thresholdFactor=4.0,
+                                     precision="fp64", siSteps=0, verbosity=0))
+    assert plain.p == ref.p
+    assert (np.abs(plain.theta - ref.theta) / ref.theta).max() <= 1e-9
+    mz = np.einsum("ij,ij->j", plain.Z, pb.M @ plain.Z)
+    assert np.abs(mz - 1).max() <= 1e-10
     for prec in ("fp64", "mixed"):
         sol = solver.solve(Pencil(pb.K, pb.M, pb.coords),
                            AmlsOptions(nev=nev, nLevels=6, leafSize=500, thresholdFactor=4.0,

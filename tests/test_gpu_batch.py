@@ -98,3 +98,28 @@ def test_batch64_workload(blas1):
         r = resid(pb, sb.theta, sb.Z)
         assert r[:10].max() < 1e-2 and r.max() < 5e-2
         assert np.all(np.diff(sb.theta) >= -1e-12 * sb.theta.max())
+
+
+def test_batch64_fp64_plain_matches_oracle(blas1):
+    """The 64-problem workload in the all-FP64 form: plain batched AMLS on all
+    64 problems against the oracle's FP64 AMLS on each problem's own tree
+    (the smallest, the largest and two in between), and M-normalised vectors
+    everywhere."""
+    pbs = cases.case_batch64()
+    nev = 20
+    opt = AmlsOptions(nev=nev, nLevels=6, leafSize=500, thresholdFactor=4.0, precision="fp64",
+                      siSteps=0, verbosity=0)
+    sols = AmlsSolver(0).solve_batch([Pencil(p.K, p.M, p.coords) for p in pbs], opt)
+    sizes = np.array([p.n for p in pbs])
+    picks = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), 17, 40})
+    for i in picks:
+        pb = pbs[i]
+        tree, _ = tree_of(pb.K, pb.M, pb.coords, 6, 500)
+        ref = O.amls_solve(pb.K, pb.M, tree, nev=nev, thresholdFactor=4.0, recoverVectors=False)
+        assert sols[i].p == ref.p, i
+        assert (np.abs(sols[i].theta - ref.theta) / ref.theta).max() <= 1e-9, i
+    for pb, sb in zip(pbs, sols):
+        assert np.abs(np.einsum("ij,ij->j", sb.Z, pb.M @ sb.Z) - 1).max() <= 1e-10
+        # plain AMLS at eta 4: the vectors' residuals are large (up to ~0.64
+        # measured), which is what the subspace iteration is for
+        assert resid(pb, sb.theta, sb.Z).max() < 1.0

@@ -112,7 +112,11 @@ def test_syevx_graded_spectrum(gpu_ctx, scale):
     lam, V = gpu_ctx.syevx_top(A, n)
     ref = np.sort(np.linalg.eigvalsh(A))[::-1]
     assert np.abs(lam - ref).max() <= 1e-13 * n * scale
-    assert np.abs(V.T @ V - np.eye(n)).max() <= 1e-10
+    # inverse iteration re-orthogonalises within clusters of 1e-7 |T|, so two
+    # vectors just outside one overlap by up to eps |T| / (1e-7 |T|) ~ 2e-9
+    # (1e-10 to 1.3e-10 measured here, varying with the order of the
+    # reduction's atomic sums)
+    assert np.abs(V.T @ V - np.eye(n)).max() <= 2.3e-9
     assert np.linalg.norm(A @ V - V * lam, axis=0).max() <= 1e-11 * n * scale
 
 

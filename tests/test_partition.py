@@ -144,3 +144,94 @@ def test_one_sided_pattern_is_symmetrised(coords):
     both = analyse(Ks.tocsc(), pb.M, xyz, 3, 40)
     for key in ("treeId", "offset", "size", "parent", "bsize", "perm"):
         assert np.array_equal(one[key], both[key]), key
+
+
+def _separator_is_minimal(K, t, p):
+    """Every vertex of separator node p touches both of its children's
+    subtrees (no vertex could leave the separator)."""
+    A = sp.csr_matrix(K)
+    perm = t["perm"]
+    off, size, parent = t["offset"], t["size"], t["parent"]
+    N = len(size)
+    owner = np.zeros(A.shape[0], dtype=np.int64)
+    for q in range(N):
+        owner[perm[off[q]:off[q] + size[q]]] = q
+    kids = [q for q in range(N) if parent[q] == p]
+    assert len(kids) == 2
+
+    def subtree(r):
+        out, st = set(), [r]
+        while st:
+            x = st.pop()
+            out.add(x)
+            st.extend(q for q in range(N) if parent[q] == x)
+        return out
+
+    sub = [subtree(k) for k in kids]
+    for v in perm[off[p]:off[p] + size[p]]:
+        nb = {owner[u] for u in A.indices[A.indptr[v]:A.indptr[v + 1]] if u != v}
+        if not (nb & sub[0]) or not (nb & sub[1]):
+            return False
+    return True
+
+
+def test_graph_separators_without_coordinates():
+    """Without coordinates a low-degree graph is cut by the spectral
+    bisection with FM refinement: on the reference's 64 x 64 grid case the
+    root separator is one grid line (64 vertices, as a straight METIS cut),
+    separators are minimal, and the fronts are smaller than with plain BFS
+    level sets (the comparison amls_hip_set_partition(1) makes)."""
+    K, M = cases.grid_graph_2d(64, 64)
+    stats = {}
+    for meth in ("auto", "levels"):
+        c = HipContext(-1)
+        c.set_partition(meth)
+        c.set_problem(K, M, None, 4, 1000)
+        t = c.tree()
+        stats[meth] = c.tree_stats()
+        c.close()
+        if meth == "auto":
+            root = int(np.argmax(t["level"]))
+            assert t["size"][root] == 64
+            for p in range(len(t["size"])):
+                if t["level"][p] > 0:
+                    assert _separator_is_minimal(K, t, p), p
+    assert stats["auto"]["front_storage"] < stats["levels"]["front_storage"]
+    assert stats["auto"]["elim_flops"] < stats["levels"]["elim_flops"]
+    lv = stats["auto"]["levels"]
+    assert sum(l["nodes"] for l in lv) == stats["auto"]["nodes"]
+    assert lv[0]["size_max"] <= 1000
+
+
+def test_spectral_cut_meets_the_accuracy_rule(blas1):
+    """amls_verify's grid64x64 case (nev 50, eta 2, 4 levels, leaves of
+    1000; src/apps/amls_verify.cpp:206-212) meets the 4 % rule on the
+    graph-cut tree and misses it on BFS level sets, whose triangular parts
+    resolve the wanted modes worse."""
+    from oracle import amls_oracle as O
+    K, M = cases.grid_graph_2d(64, 64)
+    ref = cases.grid_graph_2d_spectrum(64, 64)[:50]
+    err = {}
+    for meth in ("auto", "levels"):
+        c = HipContext(-1)
+        c.set_partition(meth)
+        c.set_problem(K, M, None, 4, 1000)
+        tree = O.Tree.from_dict(c.tree())
+        c.close()
+        r = O.amls_solve(K, M, tree, nev=50, thresholdFactor=2.0, recoverVectors=False)
+        err[meth] = float(np.max(np.abs(r.theta - ref) / ref))
+    assert err["auto"] <= 0.04 < err["levels"]
+
+
+def test_tree_stats_fields():
+    pb = cases.hex_elasticity(1, 4, 4, 5)
+    c = HipContext(-1)
+    c.set_problem(pb.K, pb.M, pb.coords, 3, 40)
+    t, st = c.tree(), c.tree_stats()
+    c.close()
+    size, b = t["size"], t["bsize"]
+    assert st["nodes"] == len(size)
+    assert sum(l["size_sum"] for l in st["levels"]) == pb.n
+    assert st["separator_total"] == pb.n - st["levels"][0]["size_sum"]
+    assert np.isclose(st["front_storage"], ((size + b) ** 2).sum())
+    assert st["root_size"] == size[t["parent"] < 0].max()
