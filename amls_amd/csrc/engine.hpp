@@ -266,6 +266,7 @@ struct Ctx {
         int n, cap;                  // order, Lanczos capacity min(n, 1536)
         double sigma;                // -max(1e-10, 1e-9 max |K-hat_jj|)
         double *S, *Mf, *X, *U, *alpha, *beta, *u0, *dmhi, *dai, *dhalf, *Wp;
+        double *W, *C, *part;        // Lanczos step scratch: w, coefficients, partial sums
         int32_t* state;              // (steps done, breakdown flag) on the device
         int mdone = 0;
         bool broke = false;

@@ -161,6 +161,9 @@ struct LanczosDesc {
     const double* u0;
     int32_t* state;   // [0] steps done, [1] breakdown flag
     int32_t n, ncv;
+    double* W;        // n: the step's vector w
+    double* C;        // min(n, 1536): reorthogonalisation coefficients
+    double* part;     // 64: per-block partial sums of |w|^2
 };
 
 // Statistics of a front's leading n x n block before its factorisation:

@@ -7,14 +7,23 @@
 // the original coordinates x.  With M = R R^T, u = R^T x turns it into the
 // standard Lanczos process on the symmetric S = R^T (K - sigma M)^-1 R: the
 // same alpha, beta and Ritz values, and x = R^-T u.  The host forms S by
-// batched factorisations and GEMMs; this kernel runs the sequential part,
-// one workgroup per local problem:
-//   w = S u_j;  alpha_j = u_j^T w;
-//   two passes of full reorthogonalisation against u_0 .. u_j (classical
-//   Gram-Schmidt: coefficients first, then one update; the reference's
-//   modified form computes the same numbers in exact arithmetic);
-//   beta_j = ||w||; stop at beta_j < 1e-13 (breakdown) or after step ncv-1;
-//   u_{j+1} = w / beta_j.
+// batched factorisations and GEMMs; these kernels run the sequential part.
+// One Lanczos step of every active node is six launches, each spread over
+// the whole device (a node's basis is n x up to 1,536 doubles, and the
+// step reads it twice per reorthogonalisation pass, which one workgroup
+// per node could only stream at a single CU's bandwidth):
+//   lz_matvec:  w = S u_j, four rows per wave (S is stored symmetric, so
+//               row r is the contiguous column r);
+//   lz_coef:    c_l = u_l^T w, one wave per basis vector (alpha_j = c_j of
+//               the first pass);
+//   lz_update:  w -= U c, a thread per row, c staged in LDS; after the
+//               second pass each block's part of |w|^2;
+//   (lz_coef and lz_update twice: two passes of classical Gram-Schmidt;
+//   the reference's modified form computes the same numbers in exact
+//   arithmetic)
+//   lz_finish:  beta_j = |w| from the block parts in order; stop at
+//               beta_j < 1e-13 (breakdown) or after step ncv-1;
+//               u_{j+1} = w / beta_j.
 // u_{j+1} and beta_j are stored after the last requested step as well, so a
 // later call with a larger ncv continues the same run; every reduction has a
 // fixed order, so continuing gives the numbers a fresh, longer run gives.
@@ -24,10 +33,10 @@
 namespace amls {
 namespace dev {
 
-constexpr int LZ_THREADS = 1024;
-constexpr int LZ_WAVES = LZ_THREADS / 64;
 constexpr int LZ_NMAX = 3200;
 constexpr int LZ_CMAX = 1536;
+constexpr int LZ_ROWS = 16;    // rows per lz_matvec workgroup (four per wave)
+constexpr int LZ_UPD = 256;    // rows per lz_update workgroup
 
 __device__ __forceinline__ double lz_wave_sum(double v) {
 #pragma unroll
@@ -35,121 +44,154 @@ __device__ __forceinline__ double lz_wave_sum(double v) {
     return v;
 }
 
-// sum over the workgroup, the same value (same order) on every thread
-__device__ __forceinline__ double lz_block_sum(double v, double* red) {
-    v = lz_wave_sum(v);
-    __syncthreads();   // earlier readers of red are done
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < LZ_WAVES; ++i) s += red[i];
-    return s;
+// the step j of node d, or -1 when the node has stopped (uniform per block)
+__device__ __forceinline__ int lz_step(const LanczosDesc& d) {
+    const int j = d.state[0];
+    return (d.state[1] != 0 || j >= d.ncv || d.n <= 0) ? -1 : j;
 }
 
-__global__ void __launch_bounds__(LZ_THREADS) lanczos_kernel(const LanczosDesc* __restrict__ descs) {
-    extern __shared__ double lds[];
+// u_0 = u0 / sqrt(max(u0^T u0, 1e-300)) (the reference's q0 normalised in
+// the M inner product, u0 = R^T q0) for nodes that have not started
+__global__ void __launch_bounds__(256) lz_init_kernel(const LanczosDesc* __restrict__ descs) {
+    __shared__ double red[4];
     const LanczosDesc d = descs[blockIdx.x];
+    if (lz_step(d) != 0) return;
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int r = t; r < d.n; r += 256) s = fma(d.u0[r], d.u0[r], s);
+    s = lz_wave_sum(s);
+    if ((t & 63) == 0) red[t >> 6] = s;
+    __syncthreads();
+    const double nrm = sqrt(fmax(((red[0] + red[1]) + red[2]) + red[3], 1e-300));
+    for (int r = t; r < d.n; r += 256) d.U[r] = d.u0[r] / nrm;
+}
+
+__global__ void __launch_bounds__(256) lz_matvec_kernel(const LanczosDesc* __restrict__ descs) {
+    const LanczosDesc d = descs[blockIdx.y];
+    const int j = lz_step(d);
+    const int r0 = blockIdx.x * LZ_ROWS + 4 * (threadIdx.x >> 6);
+    if (j < 0 || r0 >= d.n) return;
+    const int lane = threadIdx.x & 63, n = d.n;
+    const double* u = d.U + (int64_t)j * n;
+    const double* c0 = d.S + (int64_t)r0 * n;
+    const int nr = min(4, n - r0);
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = lane; c < n; c += 64) {
+        const double uc = u[c];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (i < nr) a[i] = fma(c0[(int64_t)i * n + c], uc, a[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double v = lz_wave_sum(a[i]);
+        if (lane == 0 && i < nr) d.W[r0 + i] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) lz_coef_kernel(const LanczosDesc* __restrict__ descs,
+                                                      int pass) {
+    const LanczosDesc d = descs[blockIdx.y];
+    const int j = lz_step(d);
+    const int l = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j < 0 || l > j) return;
+    const int lane = threadIdx.x & 63, n = d.n;
+    const double* ul = d.U + (int64_t)l * n;
+    double acc = 0.0;
+    for (int r = lane; r < n; r += 64) acc = fma(ul[r], d.W[r], acc);
+    acc = lz_wave_sum(acc);
+    if (lane == 0) {
+        d.C[l] = acc;
+        if (pass == 0 && l == j) d.alpha[j] = acc;
+    }
+}
+
+__global__ void __launch_bounds__(LZ_UPD) lz_update_kernel(const LanczosDesc* __restrict__ descs,
+                                                           int last) {
+    __shared__ double sc[LZ_CMAX];
+    __shared__ double red[LZ_UPD / 64];
+    const LanczosDesc d = descs[blockIdx.y];
+    const int j = lz_step(d);
     const int n = d.n;
-    int m = d.state[0];
-    if (d.state[1] != 0 || m >= d.ncv || n <= 0) return;   // uniform over the block
-    double* red = lds;            // LZ_WAVES partial sums
-    double* su = lds + 32;        // u_j
-    double* sw = su + n;          // w
-    double* sc = sw + n;          // reorthogonalisation coefficients (<= ncv)
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int cap = min(n, LZ_CMAX);
-    if (m == 0) {
+    if (j < 0 || (int)blockIdx.x * LZ_UPD >= n) return;
+    const int t = threadIdx.x;
+    for (int l = t; l <= j; l += LZ_UPD) sc[l] = d.C[l];
+    __syncthreads();
+    const int r = blockIdx.x * LZ_UPD + t;
+    double w = 0.0;
+    if (r < n) {
+        double a0 = 0.0, a1 = 0.0;
+        int l = 0;
+        for (; l + 2 <= j + 1; l += 2) {
+            a0 = fma(d.U[r + (int64_t)l * n], sc[l], a0);
+            a1 = fma(d.U[r + (int64_t)(l + 1) * n], sc[l + 1], a1);
+        }
+        if (l <= j) a0 = fma(d.U[r + (int64_t)l * n], sc[l], a0);
+        w = d.W[r] - (a0 + a1);
+        d.W[r] = w;
+    }
+    if (last) {
+        double s = lz_wave_sum(w * w);
+        if ((t & 63) == 0) red[t >> 6] = s;
+        __syncthreads();
+        if (t == 0) d.part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+    }
+}
+
+__global__ void __launch_bounds__(256) lz_finish_kernel(const LanczosDesc* __restrict__ descs) {
+    __shared__ double sb;
+    const LanczosDesc d = descs[blockIdx.x];
+    const int j = lz_step(d);
+    if (j < 0) return;
+    const int n = d.n, t = threadIdx.x;
+    if (t == 0) {
         double s = 0.0;
-        for (int r = t; r < n; r += LZ_THREADS) {
-            const double v = d.u0[r];
-            su[r] = v;
-            s += v * v;
-        }
-        // the reference's q0 / sqrt(max(q0^T M q0, 1e-300)), q0^T M q0 = |u0|^2
-        const double nrm = sqrt(fmax(lz_block_sum(s, red), 1e-300));
-        for (int r = t; r < n; r += LZ_THREADS) {
-            const double v = su[r] / nrm;
-            su[r] = v;
-            d.U[r] = v;
-        }
-    } else {
-        for (int r = t; r < n; r += LZ_THREADS) su[r] = d.U[r + (int64_t)m * n];
+        for (int b = 0; b * LZ_UPD < n; ++b) s += d.part[b];
+        sb = sqrt(fmax(s, 0.0));
     }
     __syncthreads();
-    for (int j = m; j < d.ncv; ++j) {
-        // w = S u_j: thread per row (column-major S, consecutive rows on
-        // consecutive lanes), u_j broadcast from LDS
-        for (int r = t; r < n; r += LZ_THREADS) {
-            const double* row = d.S + r;
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-            int c = 0;
-            for (; c + 4 <= n; c += 4) {
-                a0 = fma(row[(int64_t)c * n], su[c], a0);
-                a1 = fma(row[(int64_t)(c + 1) * n], su[c + 1], a1);
-                a2 = fma(row[(int64_t)(c + 2) * n], su[c + 2], a2);
-                a3 = fma(row[(int64_t)(c + 3) * n], su[c + 3], a3);
-            }
-            for (; c < n; ++c) a0 = fma(row[(int64_t)c * n], su[c], a0);
-            sw[r] = (a0 + a1) + (a2 + a3);
+    const double bj = sb;
+    const int cap = min(n, LZ_CMAX);
+    if (bj < 1e-13) {
+        __syncthreads();   // every thread has read the state before it changes
+        if (t == 0) {
+            d.state[1] = 1;
+            d.state[0] = j + 1;
         }
-        __syncthreads();
-        double s = 0.0;
-        for (int r = t; r < n; r += LZ_THREADS) s += su[r] * sw[r];
-        const double aj = lz_block_sum(s, red);
-        for (int pass = 0; pass < 2; ++pass) {
-            // sc[l] = u_l^T w, one wave per l (coalesced column reads)
-            for (int l = wv; l <= j; l += LZ_WAVES) {
-                const double* ul = d.U + (int64_t)l * n;
-                double acc = 0.0;
-                for (int r = lane; r < n; r += 64) acc = fma(ul[r], sw[r], acc);
-                acc = lz_wave_sum(acc);
-                if (lane == 0) sc[l] = acc;
-            }
-            __syncthreads();
-            // w -= U sc, thread per row
-            for (int r = t; r < n; r += LZ_THREADS) {
-                double a0 = 0.0, a1 = 0.0;
-                int l = 0;
-                for (; l + 2 <= j + 1; l += 2) {
-                    a0 = fma(d.U[r + (int64_t)l * n], sc[l], a0);
-                    a1 = fma(d.U[r + (int64_t)(l + 1) * n], sc[l + 1], a1);
-                }
-                if (l <= j) a0 = fma(d.U[r + (int64_t)l * n], sc[l], a0);
-                sw[r] -= a0 + a1;
-            }
-            __syncthreads();
-        }
-        s = 0.0;
-        for (int r = t; r < n; r += LZ_THREADS) s += sw[r] * sw[r];
-        const double bj = sqrt(fmax(lz_block_sum(s, red), 0.0));
-        if (t == 0) d.alpha[j] = aj;
-        m = j + 1;
-        if (bj < 1e-13) {
-            if (t == 0) d.state[1] = 1;
-            break;
-        }
-        if (t == 0) d.beta[j] = bj;
-        if (j + 1 < cap) {
-            for (int r = t; r < n; r += LZ_THREADS) {
-                const double v = sw[r] / bj;
-                su[r] = v;
-                d.U[r + (int64_t)(j + 1) * n] = v;
-            }
-        }
-        __syncthreads();
+        return;
     }
-    if (t == 0) d.state[0] = m;
+    if (j + 1 < cap)
+        for (int r = t; r < n; r += 256) d.U[r + (int64_t)(j + 1) * n] = d.W[r] / bj;
+    __syncthreads();
+    if (t == 0) {
+        d.beta[j] = bj;
+        d.state[0] = j + 1;
+    }
 }
 
 }  // namespace dev
 
 int lanczos_max_order() { return dev::LZ_NMAX; }
+int lanczos_max_ncv() { return dev::LZ_CMAX; }
 
-void launch_lanczos(const dev::LanczosDesc* d, int count, int nmax, int cmax, hipStream_t s) {
+void launch_lanczos_init(const dev::LanczosDesc* d, int count, hipStream_t s) {
     if (count <= 0) return;
-    const size_t lds = (size_t)(32 + 2 * nmax + cmax) * sizeof(double);
-    hipLaunchKernelGGL(dev::lanczos_kernel, dim3(count), dim3(dev::LZ_THREADS), lds, s, d);
+    hipLaunchKernelGGL(dev::lz_init_kernel, dim3(count), dim3(256), 0, s, d);
+}
+
+// one step of every active node: nmax = largest order, jmax = largest step
+// index any node may be at
+void launch_lanczos_step(const dev::LanczosDesc* d, int count, int nmax, int jmax, hipStream_t s) {
+    if (count <= 0 || nmax <= 0) return;
+    const unsigned nb = (unsigned)count;
+    hipLaunchKernelGGL(dev::lz_matvec_kernel, dim3((nmax + dev::LZ_ROWS - 1) / dev::LZ_ROWS, nb),
+                       dim3(256), 0, s, d);
+    const unsigned gc = (unsigned)((jmax + 1 + 3) / 4), gu = (unsigned)((nmax + dev::LZ_UPD - 1) / dev::LZ_UPD);
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(dev::lz_coef_kernel, dim3(gc, nb), dim3(256), 0, s, d, pass);
+        hipLaunchKernelGGL(dev::lz_update_kernel, dim3(gu, nb), dim3(dev::LZ_UPD), 0, s, d, pass);
+    }
+    hipLaunchKernelGGL(dev::lz_finish_kernel, dim3(nb), dim3(256), 0, s, d);
 }
 
 }  // namespace amls

@@ -28,7 +28,9 @@ void launch_front_stats(const dev::FrontStatDesc* d, const int2* work, int nwork
 // Shift-invert Lanczos steps, one workgroup per descriptor (n <= 8192 and
 // ncv <= 1536: LDS holds two n-vectors and the step's coefficients).
 int lanczos_max_order();
-void launch_lanczos(const dev::LanczosDesc* d, int count, int nmax, int cmax, hipStream_t s);
+int lanczos_max_ncv();
+void launch_lanczos_init(const dev::LanczosDesc* d, int count, hipStream_t s);
+void launch_lanczos_step(const dev::LanczosDesc* d, int count, int nmax, int jmax, hipStream_t s);
 void launch_scatter(const int64_t* src, const int64_t* tgt, const double* vals, double* dst,
                     int64_t count, double scale, bool accumulate, hipStream_t s);
 void launch_fill(double* p, int64_t count, double v, hipStream_t s);

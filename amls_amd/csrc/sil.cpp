@@ -86,6 +86,9 @@ void Ctx::silPrepare() {
         for (int q = 0; q < 4; ++q) A.plan(n * 8);        // u0, dmhi, dai, dhalf
         A.plan(n * 64 * 8);                               // panel scratch
         A.plan(16);                                       // state
+        A.plan(n * 8);                                    // W
+        A.plan(s.cap * 8);                                // C
+        A.plan(64 * 8);                                   // part
     }
     A.plan(nmax * 8);                   // q0
     A.plan(256);                        // error word
@@ -104,6 +107,9 @@ void Ctx::silPrepare() {
         s.dhalf = A.take<double>(n);
         s.Wp = A.take<double>(n * 64);
         s.state = A.take<int32_t>(4);
+        s.W = A.take<double>(n);
+        s.C = A.take<double>(s.cap);
+        s.part = A.take<double>(64);
         s.mdone = 0;
         s.broke = false;
     }
@@ -245,7 +251,7 @@ void Ctx::silPrepare() {
 // breakdown); the host copies of the step counts are updated.
 void Ctx::silAdvance(const std::vector<int>& idx, const std::vector<int>& ncv) {
     std::vector<dev::LanczosDesc> ds;
-    int nmax = 1, cmax = 1;
+    int nmax = 1, steps = 0, jtop = 0;
     for (size_t i = 0; i < idx.size(); ++i) {
         SilNode& s = sil[idx[i]];
         const int target = std::min(ncv[i], s.cap);
@@ -260,9 +266,13 @@ void Ctx::silAdvance(const std::vector<int>& idx, const std::vector<int>& ncv) {
         d.state = s.state;
         d.n = s.n;
         d.ncv = target;
+        d.W = s.W;
+        d.C = s.C;
+        d.part = s.part;
         ds.push_back(d);
         nmax = std::max(nmax, s.n);
-        cmax = std::max(cmax, target);
+        steps = std::max(steps, target - s.mdone);
+        jtop = std::max(jtop, target - 1);
     }
     if (ds.empty()) return;
     Blob blob;
@@ -279,9 +289,21 @@ void Ctx::silAdvance(const std::vector<int>& idx, const std::vector<int>& ncv) {
             bytes += 8.0 * (double(s.n) * s.n + 4.0 * s.n * (j + 1));
         }
     }
+    // one step of every active node per round; a node that has reached its
+    // target or broken down drops out inside the kernels
+    std::vector<int> mstart;
+    for (size_t i = 0; i < idx.size(); ++i) {
+        const SilNode& s = sil[idx[i]];
+        if (!(s.broke || s.mdone >= std::min(ncv[i], s.cap))) mstart.push_back(s.mdone);
+    }
+    // round k: node i is at step mdone_i + k at most
+    const int mmax = *std::max_element(mstart.begin(), mstart.end());
     kstats.run("lanczos", flops, bytes, stream, [&] {
-        launch_lanczos(blob.at<dev::LanczosDesc>(od), cnt, nmax, cmax, stream);
-    });
+        const dev::LanczosDesc* d = blob.at<dev::LanczosDesc>(od);
+        launch_lanczos_init(d, cnt, stream);
+        for (int k = 0; k < steps; ++k)
+            launch_lanczos_step(d, cnt, nmax, std::min(jtop, mmax + k), stream);
+    }, std::max(steps, 1));
     std::vector<int32_t> st(4 * sil.size());
     for (size_t i = 0; i < sil.size(); ++i)
         AMLS_HIP_CHECK(hipMemcpyAsync(st.data() + 4 * i, sil[i].state, 2 * sizeof(int32_t),
