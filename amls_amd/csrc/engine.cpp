@@ -134,7 +134,8 @@ void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<i
     const EigDesc d1 = descs[0];
     double* sv1 = sv + 2 * soff[0];
     double* sp1 = sp + 3 * soff[0];
-    plan.fn([=](hipStream_t s) {
+    // the classes inside time themselves
+    plan.hostStep([=](hipStream_t s) {
         launch_fill(sp, spLen, 0.0, s);
         const int4* work = cached ? cached : (one ? nullptr : bl->at<int4>(ow));
         // the steps run back to back; one event pair brackets the loop
@@ -562,7 +563,7 @@ void addSelect(Plan& plan, const std::vector<EigDesc>& descs, bool vectors, bool
         fOr += 4.0 * n * n * k;
         bOr += 4.0 * n * n + 16.0 * n * k;
     }
-    plan.fn([=](hipStream_t s) {
+    plan.hostStep([=](hipStream_t s) {   // the classes inside time themselves
         auto run = [&](const char* name, double fl, double by, const std::function<void()>& f) {
             if (kst) kst->run(name, fl, by, s, f);
             else f();

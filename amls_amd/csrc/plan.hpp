@@ -196,7 +196,9 @@ struct Plan {
         const size_t od = blob->add(b.descs), ow = blob->add(b.work);
         const int nw = static_cast<int>(b.work.size());
         Blob* bl = blob;
-        steps.push_back([=](hipStream_t s) {
+        double by = 0.0;   // each element read (unless a fill) and written once
+        for (const auto& d : b.descs) by += double(d.rows) * d.cols * (d.src ? 16.0 : 8.0);
+        timed("copy2d", 0.0, by, [=](hipStream_t s) {
             launch_copy2d(bl->at<Copy2dDesc>(od), bl->at<int4>(ow), nw, s);
         });
     }
@@ -205,7 +207,13 @@ struct Plan {
         const size_t od = blob->add(b.descs), ow = blob->add(b.work);
         const int nw = static_cast<int>(b.work.size());
         Blob* bl = blob;
-        steps.push_back([=](hipStream_t s) {
+        double fl = 0.0, by = 0.0;   // source read, target read and written
+        for (const auto& d : b.descs) {
+            const double e = double(d.rows) * d.cols * (d.symmetric ? 0.5 : 1.0);
+            fl += e;
+            by += 24.0 * e;
+        }
+        timed("extadd", fl, by, [=](hipStream_t s) {
             launch_extadd(bl->at<ExtAddDesc>(od), bl->at<int4>(ow), nw, s);
         });
     }
@@ -284,7 +292,9 @@ struct Plan {
         const size_t od = blob->add(b.descs), ow = blob->add(b.work);
         const int nw = static_cast<int>(b.work.size());
         Blob* bl = blob;
-        steps.push_back([=](hipStream_t s) {
+        double by = 0.0;   // each gathered element read and written once
+        for (const auto& d : b.descs) by += double(d.rows) * d.cols * 16.0;
+        timed("gather", 0.0, by, [=](hipStream_t s) {
             launch_gather(bl->at<GatherDesc>(od), bl->at<int4>(ow), nw, s);
         });
     }
@@ -293,9 +303,12 @@ struct Plan {
         const size_t od = blob->add(v);
         const int nv = static_cast<int>(v.size());
         Blob* bl = blob;
-        steps.push_back([=](hipStream_t s) { launch_vecop(bl->at<VecDesc>(od), nv, s); });
+        timed("vecop", 0.0, 0.0, [=](hipStream_t s) { launch_vecop(bl->at<VecDesc>(od), nv, s); });
     }
-    void fn(std::function<void(hipStream_t)> f) { steps.push_back(std::move(f)); }
+    // other launches (fills, column norms, permutations), timed as "misc"
+    void fn(std::function<void(hipStream_t)> f) { timed("misc", 0.0, 0.0, std::move(f)); }
+    // steps that time their own launches (or launch nothing)
+    void hostStep(std::function<void(hipStream_t)> f) { steps.push_back(std::move(f)); }
     // side's steps run on s2 from this point of the plan on, concurrently
     // with the steps added here until join(); side shares this plan's blob
     void fork(Plan& side, hipStream_t s2, hipEvent_t ef, hipEvent_t ej) {
