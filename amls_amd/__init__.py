@@ -133,6 +133,8 @@ class HipContext:
             raise AmlsError(st, f"amls_hip_create(device={device}) failed")
         self._h = h
         self.device = device
+        self.n = 0
+        self.count = 1   # problems of the last set_problem / set_problem_batch
 
     def close(self):
         if getattr(self, "_h", None):
@@ -161,6 +163,7 @@ class HipContext:
                                                iptr(mp), iptr(mi), dptr(mx), dptr(cc),
                                                nLevels, leafSize, shift))
         self.n = K.shape[0]
+        self.count = 1
 
     def set_problem_batch(self, pencils, nLevels: int = 6, leafSize: int = 500,
                           shift: float = 0.0):
@@ -170,6 +173,7 @@ class HipContext:
         self._chk(self._L.amls_hip_set_problem_batch(self._h, *args[:9], nLevels, leafSize,
                                                      shift))
         self.n = int(sum(pc.K.shape[0] for pc in pencils))
+        self.count = len(pencils)
 
     def batch_info(self):
         cnt = C.c_int64()
@@ -274,10 +278,14 @@ class HipContext:
         self._chk(self._L.amls_hip_project(self._h, PRECISIONS[precision], C.byref(p)))
         return p.value
 
+    def _per_problem(self, v: np.ndarray, k: int) -> np.ndarray:
+        # one problem: k values; a batch: count x k, problem-major
+        return v if self.count == 1 else v.reshape(self.count, k)
+
     def reduced_solve(self, want: int, vectors: bool = True) -> np.ndarray:
-        th = np.zeros(want, dtype=np.float64)
+        th = np.zeros(self.count * want, dtype=np.float64)
         self._chk(self._L.amls_hip_reduced_solve(self._h, want, int(vectors), dptr(th)))
-        return th
+        return self._per_problem(th, want)
 
     def backtransform(self, want: int) -> np.ndarray:
         Z = np.zeros((self.n, want), dtype=np.float64, order="F")
@@ -285,10 +293,10 @@ class HipContext:
         return Z
 
     def subspace_iter(self, steps: int, q: int, nev: int):
-        th = np.zeros(nev, dtype=np.float64)
+        th = np.zeros(self.count * nev, dtype=np.float64)
         Z = np.zeros((self.n, nev), dtype=np.float64, order="F")
         self._chk(self._L.amls_hip_subspace_iter(self._h, steps, q, nev, dptr(th), dptr(Z)))
-        return th, Z
+        return self._per_problem(th, nev), Z
 
     # ---- measurement -----------------------------------------------------
     def autotune(self) -> np.ndarray:
@@ -328,16 +336,20 @@ class HipContext:
         return out
 
     def solve_numeric(self, opt: "AmlsOptions", vectors: bool = False):
-        """Numeric solve on the problem already set (inputs resident in HBM)."""
-        nev = opt.nev
-        theta = np.zeros(max(nev, 1))
-        Z = np.zeros((self.n, max(nev, 1)), order="F") if vectors else None
-        p = C.c_int64()
+        """Numeric solve on the problem (or batch) already set (inputs resident
+        in HBM).  After set_problem_batch, theta is count x nev and p holds
+        the count reduced sizes."""
+        nev = max(opt.nev, 1)
+        theta = np.zeros(self.count * nev)
+        Z = np.zeros((self.n, nev), order="F") if vectors else None
+        p = np.zeros(self.count, dtype=np.int64)
         timing = np.zeros(8)
         oc = opt.to_c()
         self._chk(self._L.amls_hip_solve_numeric(self._h, C.byref(oc), dptr(theta), dptr(Z),
-                                                 C.byref(p), dptr(timing)))
-        return theta, Z, p.value, dict(zip(AmlsSolver.TIMING_KEYS, timing.tolist()))
+                                                 iptr(p), dptr(timing)))
+        pv = int(p[0]) if self.count == 1 else p
+        return (self._per_problem(theta, nev), Z, pv,
+                dict(zip(AmlsSolver.TIMING_KEYS, timing.tolist())))
 
     def adjoint(self, conn, Ke0, Me0, theta, Z=None):
         """Element sensitivities (grad_s, grad_r), each nelem x nev:

@@ -7,7 +7,7 @@ import time
 import numpy as np
 import pytest
 
-from amls_amd import AmlsOptions, AmlsSolver, Pencil, cases
+from amls_amd import AmlsOptions, AmlsSolver, HipContext, Pencil, cases
 from oracle import amls_oracle as O
 from tests.helpers import tree_of
 
@@ -69,6 +69,29 @@ def test_batch_subspace_iteration(blas1, prec):
         assert O.rel_err(sb.theta, exact).max() < O.rel_err(sp_.theta, exact).max()
         assert np.abs(np.einsum("ij,ij->j", sb.Z, pb.M @ sb.Z) - 1).max() <= 1e-10
         assert np.all(resid(pb, sb.theta, sb.Z) <= resid(pb, ss.theta, ss.Z) * 1.1 + 1e-12)
+
+
+def test_batch_numeric_entry_points_size_outputs_per_problem(blas1):
+    """set_problem_batch + solve_numeric return count x nev values and count
+    reduced sizes (the output buffers are sized per problem), the numbers
+    solve_batch gives."""
+    pbs = small_batch(7, 5)
+    nev = 6
+    opt = AmlsOptions(nev=nev, nLevels=3, leafSize=40, precision="mixed", siSteps=2,
+                      siVectors=12, recoverVectors=False, verbosity=0)
+    pens = [Pencil(p.K, p.M, p.coords) for p in pbs]
+    sols = AmlsSolver(0).solve_batch(pens, opt)
+    ctx = HipContext(0)
+    ctx.set_problem_batch(pens, 3, 40)
+    th, Z, p, _ = ctx.solve_numeric(opt, vectors=True)
+    assert th.shape == (len(pbs), nev) and p.shape == (len(pbs),)
+    assert Z.shape == (sum(pb.K.shape[0] for pb in pbs), nev)
+    rows, _ = ctx.batch_info()
+    for s, sb in enumerate(sols):
+        assert p[s] == sb.p
+        assert np.array_equal(th[s], sb.theta)
+        assert np.array_equal(Z[rows[s]:rows[s + 1]], sb.Z)
+    ctx.close()
 
 
 def test_batch64_workload(blas1):
