@@ -23,7 +23,7 @@ from . import _lib
 from ._lib import AmlsError, AmlsOptionsC, dptr, iptr
 
 __all__ = ["AmlsOptions", "AmlsSolver", "Solution", "Pencil", "HipContext", "AmlsError",
-           "DenseFactor"]
+           "DenseFactor", "ContributionStore"]
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
 
@@ -587,3 +587,6 @@ class AmlsSolver:
                                 p=int(p[s_]), timing=tm))
             r0 += n_s
         return out
+
+
+from .store import ContributionStore  # noqa: E402  (the store only needs _lib)

@@ -32,6 +32,9 @@ EXPORTS = [
     "amls_hip_ldlt_factor_solve", "amls_hip_ldlt_factor_get", "amls_hip_ldlt_factor_free",
     "amls_hip_dense_eig", "amls_hip_set_rr_mode", "amls_hip_get_rr_info",
     "amls_hip_setup_timing", "amls_hip_trace", "amls_hip_set_partition",
+    "amls_hip_store_open", "amls_hip_store_close", "amls_hip_store_last_error",
+    "amls_hip_store_append", "amls_hip_store_read_sum", "amls_hip_store_shape",
+    "amls_hip_store_remove", "amls_hip_store_reset", "amls_hip_store_disk_bytes",
 ]
 
 
@@ -129,6 +132,16 @@ def lib() -> C.CDLL:
         "amls_hip_setup_timing": ([P, PD], C.c_int),
         "amls_hip_trace": ([P, C.c_char_p, PI64], C.c_int),
         "amls_hip_set_partition": ([P, I32], C.c_int),
+        # block pointers as void*: host arrays or device addresses
+        "amls_hip_store_open": ([C.c_char_p, I32, I64, C.POINTER(P)], C.c_int),
+        "amls_hip_store_close": ([P], C.c_int),
+        "amls_hip_store_last_error": ([P], C.c_char_p),
+        "amls_hip_store_append": ([P, I64, P, I64, I64, I64], C.c_int),
+        "amls_hip_store_read_sum": ([P, I64, P, I64, I64, I64, I32, PI32], C.c_int),
+        "amls_hip_store_shape": ([P, I64, PI64, PI64, PI64], C.c_int),
+        "amls_hip_store_remove": ([P, I64], C.c_int),
+        "amls_hip_store_reset": ([P], C.c_int),
+        "amls_hip_store_disk_bytes": ([P, PD], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

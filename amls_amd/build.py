@@ -23,12 +23,14 @@ SOURCES = [
     CSRC / "kernels" / "adjoint.hip",
     CSRC / "kernels" / "lanczos.hip",
     CSRC / "kernels" / "chfsi.hip",
+    CSRC / "kernels" / "store.hip",
     CSRC / "host" / "tree.cpp",
     CSRC / "sil.cpp",
     CSRC / "noalloc.cpp",
     CSRC / "operators.cpp",
     CSRC / "chfsi.cpp",
     CSRC / "trace.cpp",
+    CSRC / "store.cpp",
     CSRC / "engine.cpp",
 ]
 

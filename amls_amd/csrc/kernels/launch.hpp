@@ -34,6 +34,9 @@ void launch_lanczos_step(const dev::LanczosDesc* d, int count, int nmax, int jma
 void launch_scatter(const int64_t* src, const int64_t* tgt, const double* vals, double* dst,
                     int64_t count, double scale, bool accumulate, hipStream_t s);
 void launch_fill(double* p, int64_t count, double v, hipStream_t s);
+// out(r, c) += src(r, c) over rows x cols (column-major, leading dims ldo, lds)
+void launch_add2d(double* out, int64_t ldo, const double* src, int64_t lds, int64_t rows,
+                  int64_t cols, hipStream_t s);
 void launch_set_i32(int* p, int v, hipStream_t s);   // *p = v, as a kernel (graph-safe)
 void launch_noop(hipStream_t s);   // empty kernel (dispatch-gap calibration)
 void launch_vecop(const dev::VecDesc* d, int count, hipStream_t s);

@@ -448,6 +448,49 @@ amls_status amls_hip_ldlt_solve_noalloc(amls_ctx* ctx, int64_t n, double* A, int
 amls_status amls_hip_syevx_top(amls_ctx* ctx, int64_t n, const double* A, int64_t k,
                                double* lam, double* V);
 
+/*
+ * Out-of-core contribution store: ContributionStore of
+ * src/amls/contribution_store.hpp:23-124 (append-only, one file per target
+ * block, <dir>/t<target>.bin, records "int64 rows, int64 cols, rows*cols
+ * doubles column-major" in native byte order, the reference's layout).
+ *
+ * store_open creates dir.  device >= 0: the blocks passed to append and
+ * read_sum are device pointers of that GPU; they are staged through two
+ * pinned host buffers of staging_bytes each (0: 64 MiB), and read_sum adds
+ * the records on the GPU, on a stream of the store's own that is ordered
+ * after the null stream (work on other streams must be finished by the
+ * caller).  device < 0: host pointers, no HIP call.  Every call returns when
+ * its data are on disk / in the output block.
+ *
+ * append(target, m, rows, cols, ld): ContributionStore::append (:34-49);
+ * appends to one target serialise, any number of threads may append.
+ * read_sum: ContributionStore::readSum (:53-92): out (rows x cols, leading
+ * dimension ld) = the records of target summed in append order (one
+ * addition per element and record: the host's element-wise sum bit for
+ * bit); *found = 0 and out untouched when the target has no file; a
+ * truncated trailing record is ignored; a record of another shape is
+ * AMLS_E_ARG; consume != 0 deletes the file afterwards.  shape: rows / cols
+ * of the first complete record and the record count (-1: no file), to size
+ * the output.  remove / reset: :94-103 (reset deletes every file this store
+ * wrote; close does too, as the reference's destructor).  disk_bytes: the
+ * payload bytes appended since the last reset (:106).  last_error: text of
+ * the store's last failure (valid until its next failing call).
+ */
+typedef struct amls_store amls_store;
+amls_status amls_hip_store_open(const char* dir, int32_t device, int64_t staging_bytes,
+                                amls_store** out);
+amls_status amls_hip_store_close(amls_store* st);
+const char* amls_hip_store_last_error(amls_store* st);
+amls_status amls_hip_store_append(amls_store* st, int64_t target, const double* m, int64_t rows,
+                                  int64_t cols, int64_t ld);
+amls_status amls_hip_store_read_sum(amls_store* st, int64_t target, double* out, int64_t rows,
+                                    int64_t cols, int64_t ld, int32_t consume, int32_t* found);
+amls_status amls_hip_store_shape(amls_store* st, int64_t target, int64_t* rows, int64_t* cols,
+                                 int64_t* records);
+amls_status amls_hip_store_remove(amls_store* st, int64_t target);
+amls_status amls_hip_store_reset(amls_store* st);
+amls_status amls_hip_store_disk_bytes(amls_store* st, double* bytes);
+
 #ifdef __cplusplus
 }
 #endif
