@@ -652,6 +652,15 @@ void KernelStats::run(const std::string& name, double flops, double bytes, hipSt
     pending.push_back({key, a, b, phase, s == lane1 ? 1 : 2});
 }
 
+void KernelStats::hostSpan(const char* name, std::chrono::steady_clock::time_point a,
+                           std::chrono::steady_clock::time_point b) {
+    if (!trace) return;
+    auto us = [](std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration_cast<std::chrono::microseconds>(t.time_since_epoch()).count();
+    };
+    spans.push_back({"plan", name, phase ? phase : "", 0, us(a), us(b)});
+}
+
 void KernelStats::resolve() {
     for (auto& p : pending) {
         float ms = 0.f, m0 = 0.f;
@@ -2253,7 +2262,9 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                        bool fromBasis) {
     requireGpu();
     kstats.phase = "si";
-    sync();
+    // no sync: the plan below depends only on host state, so it is built
+    // while the device finishes the previous phase; its launches follow that
+    // phase's on the stream
     if (steps <= 0) throw StatusError(AMLS_E_ARG, "steps must be positive");
     if (fromBasis) {
         if (!projected) throw StatusError(AMLS_E_STATE, "subspace_iter needs the AMLS basis");
@@ -2293,6 +2304,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     A.plan(NS * q * 8 * 13);               // eig vectors
     A.plan(NS * q * nev * 8 * 2);          // Yt, V
     A.plan(NS * nev * 6 * q * 8);          // inverse-iteration scratch
+    A.plan(NS * q * 8);                    // column factors of a deferred scaling
     A.plan(256);
     A.commit();
     double* Q = A.take<double>(n * q);
@@ -2320,6 +2332,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     double* Yt = A.take<double>(NS * q * nev);
     double* V = A.take<double>(NS * q * nev);
     double* wk = A.take<double>(NS * nev * 6 * q);
+    double* fac = A.take<double>(NS * q);
     const int64_t qq = q * q;
     std::vector<double*> McS, KcS, WpS;
     for (int64_t sg = 0; sg < NS; ++sg) {
@@ -2330,10 +2343,9 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     // rows of problem s: [r0, r0 + ns) in the permuted ordering
     auto rowsOf = [&](int64_t sg) { return std::make_pair(segRow[sg], segRow[sg + 1] - segRow[sg]); };
     int* err = A.take<int>(1);
-    const int big = INT_MAX;
-    AMLS_HIP_CHECK(hipMemcpyAsync(err, &big, sizeof(int), hipMemcpyHostToDevice, stream));
     fresh(siBlob);
     Plan plan(siBlob.get(), &kstats);
+    plan.hostStep([=](hipStream_t s) { launch_set_i32(err, INT_MAX, s); });
     const int qi = static_cast<int>(q);
     const int64_t nn = n;
 
@@ -2389,37 +2401,60 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     }
     addSpmm(plan, opPM, Q, R, q, Tr1, Tr2);     // R = M Q
 
+    // Children's updates into the parent fronts of the forward solves.  A
+    // child's relative index list is monotone, so the rows of its update
+    // that land in the parent's own block are a prefix: those are added
+    // first (own = true, before the parent's y_p is used), the rest after
+    // the parent's u_p = -X^T y_p has initialised its boundary block
+    // (beta = 0), so T needs no clearing.  Lists found unsorted fall back to
+    // clearing T and adding every row first.
+    bool relSorted = true;
+    for (int64_t c = 0; c < N && relSorted; ++c)
+        relSorted = std::is_sorted(S.rel.begin() + S.bptr[c], S.rel.begin() + S.bptr[c] + S.b(c));
+    auto addChildUpdates = [&](const std::vector<int64_t>& nodes, bool own) {
+        for (int slot = 0; slot < 2; ++slot) {
+            ExtBatch eb;
+            for (int64_t p : nodes) {
+                const int64_t c = slot == 0 ? S.child0[p] : S.child1[p];
+                if (c < 0 || S.b(c) == 0) continue;
+                const int64_t bc = S.b(c);
+                const int64_t* r = S.rel.data() + S.bptr[c];
+                const int64_t split = relSorted ? std::lower_bound(r, r + bc, S.size[p]) - r : bc;
+                const int64_t r0 = own ? 0 : split, r1 = own ? split : bc;
+                if (r1 <= r0) continue;
+                ExtAddDesc x;
+                std::memset(&x, 0, sizeof(x));
+                x.src = T + to[c] + S.size[c] + r0;
+                x.lds = S.f(c);
+                x.dst = T + to[p];
+                x.ldd = S.f(p);
+                x.rowmap = drel + S.bptr[c] + r0;
+                x.rows = static_cast<int32_t>(r1 - r0);
+                x.cols = qi;
+                eb.add(x);
+            }
+            plan.ext(eb);
+        }
+    };
+    // the column factors the previous step's M Y still has to be scaled by
+    // (null: M Y is scaled already)
+    const double* pend = nullptr;
     for (int it = 0; it < steps; ++it) {
         const bool last = it + 1 == steps;
         // ---- forward: fronts take their own rows of M Q (R in the first
         // step, the scaled M Y of the previous step after that), children's
         // updates
         const double* src = it == 0 ? R : MY;
-        plan.fn([=](hipStream_t s) { launch_fill(T, to[S.nNodes], 0.0, s); });
+        if (!relSorted) plan.fn([=](hipStream_t s) { launch_fill(T, to[S.nNodes], 0.0, s); });
         Copy2dBatch cown;
         for (int64_t p = 0; p < N; ++p)
-            cown.add(cdesc(src + S.off[p], n, T + to[p], S.f(p), S.size[p], q, 0));
+            cown.add(cdesc(src + S.off[p], n, T + to[p], S.f(p), S.size[p], q, 0, nullptr,
+                           pend ? pend + nodeSeg[p] * q : nullptr));
+        pend = nullptr;
         plan.copy(cown);
         for (int64_t l = 0; l <= S.maxLevel; ++l) {
             const auto& nodes = S.byLevel[l];
-            for (int slot = 0; slot < 2; ++slot) {
-                ExtBatch eb;
-                for (int64_t p : nodes) {
-                    const int64_t c = slot == 0 ? S.child0[p] : S.child1[p];
-                    if (c < 0 || S.b(c) == 0) continue;
-                    ExtAddDesc x;
-                    std::memset(&x, 0, sizeof(x));
-                    x.src = T + to[c] + S.size[c];
-                    x.lds = S.f(c);
-                    x.dst = T + to[p];
-                    x.ldd = S.f(p);
-                    x.rowmap = drel + S.bptr[c];
-                    x.rows = static_cast<int32_t>(S.b(c));
-                    x.cols = qi;
-                    eb.add(x);
-                }
-                plan.ext(eb);
-            }
+            addChildUpdates(nodes, true);
             // u_p = T_B - X^T y_p, then w_p = L^-T D^-1 L^-1 y_p in place
             // (explicit L^-1: two GEMMs through the scratch Tr1)
             GemmBatch gu, gl1, gl2;
@@ -2436,7 +2471,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                     g.ldb = f;
                     g.k = static_cast<int32_t>(np);
                     g.alpha1 = -1.0;
-                    g.beta = 1.0;
+                    g.beta = relSorted ? 0.0 : 1.0;   // T_B is not cleared when sorted
                     g.C = Tp + np;
                     g.ldc = f;
                     g.m = static_cast<int32_t>(b);
@@ -2471,6 +2506,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                 gl2.add(b2);
             }
             plan.gemm(gu, false);
+            addChildUpdates(nodes, false);
             plan.gemm(gl1, false);
             plan.copy(sc);
             plan.gemm(gl2, false);
@@ -2490,11 +2526,22 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
         // after every second step that is not the last one the basis is
         // M-orthonormalised (below), and the projected pencil of the final
         // Rayleigh-Ritz stays well conditioned however many steps run.
+        // Y and M Y are read scaled only by the final Rayleigh-Ritz and the
+        // Cholesky QR; before another plain step only the next step's copy
+        // of M Y's own rows needs the factors, and it applies them itself
+        // (the same product per element as scaling M Y first), while Y is
+        // overwritten unread
         const int64_t* seg = dSegRow;
         const int nsg = static_cast<int>(NS);
         double* nrmS = dsq;
-        plan.fn([=](hipStream_t s) { launch_colnorm_seg(seg, nsg, MY, Yv, qi, nn, nrmS, s, true); });
-        if (!last && (it + 1) % 2 == 0) {
+        const bool chol = !last && (it + 1) % 2 == 0;
+        if (last || chol) {
+            plan.fn([=](hipStream_t s) { launch_colnorm_seg(seg, nsg, MY, Yv, qi, nn, nrmS, s, true); });
+        } else {
+            plan.fn([=](hipStream_t s) { launch_colnorm_factors(seg, nsg, MY, Yv, qi, nn, nrmS, fac, s); });
+            pend = fac;
+        }
+        if (chol) {
             // Cholesky QR in the M inner product, per problem:
             // G = Y^T M Y = L D L^T, Y <- Y L^-T D^-1/2 (through Q), M Y again
             GemmBatch gg, gy;
@@ -3141,7 +3188,12 @@ static void solveNumeric(Ctx& c, amls_options o, double* theta, double* Z, int64
         c.spectralSelect(1, cut);
         c.project(o.precision);
     }
-    c.sync();
+    // the whole-basis subspace iteration starts without waiting for the
+    // condensation's launches (its plan is built meanwhile); otherwise the
+    // host reads the reduced sizes of the finished condensation first
+    const int64_t qWant = o.siVectors > 0 ? o.siVectors : std::max<int64_t>(2 * o.nev, o.nev + 8);
+    const bool basisNext = o.siSteps > 0 && NS == 1 && std::max(qWant, o.nev) >= c.ptot;
+    if (!basisNext) c.sync();
     const auto t3 = Clock::now();
     int64_t want = std::min<int64_t>(o.nev, minP());
     bool vec = o.recoverVectors != 0;
@@ -3151,8 +3203,7 @@ static void solveNumeric(Ctx& c, amls_options o, double* theta, double* Z, int64
         // subspace-iteration paper), never more than p.  q = p iterates on
         // the whole AMLS subspace, which needs no reduced eigenvectors (one
         // problem only: a batch iterates its lowest Ritz vectors).
-        const int64_t q = o.siVectors > 0 ? o.siVectors : std::max<int64_t>(2 * o.nev, o.nev + 8);
-        want = std::min<int64_t>(std::max<int64_t>(q, o.nev), minP());
+        want = std::min<int64_t>(std::max<int64_t>(qWant, o.nev), minP());
         if (want < o.nev) throw StatusError(AMLS_E_NOCONV, "reduced problem smaller than nev");
         fromBasis = NS == 1 && want == c.ptot;
         vec = !fromBasis;

@@ -159,6 +159,9 @@ struct KernelStats {
     // (one event pair brackets them all)
     void run(const std::string& name, double flops, double bytes, hipStream_t s,
              const std::function<void()>& launch, int64_t count = 1);
+    // a host span (lane 0) in the current phase, kept when tracing
+    void hostSpan(const char* name, std::chrono::steady_clock::time_point a,
+                  std::chrono::steady_clock::time_point b);
     void resolve();   // after a stream sync
     void reset();
     ~KernelStats();

@@ -141,6 +141,9 @@ struct Plan {
     Blob* blob;
     KernelStats* st;
     std::vector<std::function<void(hipStream_t)>> steps;
+    // traced runs record the host's planning ("build": from the plan's
+    // creation or last run), the descriptor upload and the launch enqueue
+    std::chrono::steady_clock::time_point born = std::chrono::steady_clock::now();
     Plan(Blob* b, KernelStats* k) : blob(b), st(k) {}
 
     void timed(const std::string& name, double flops, double bytes,
@@ -325,9 +328,18 @@ struct Plan {
         steps.push_back([=](hipStream_t s) { AMLS_HIP_CHECK(hipStreamWaitEvent(s, ej, 0)); });
     }
     void run(hipStream_t s) {
+        const auto t1 = std::chrono::steady_clock::now();
         blob->upload(s);
+        const auto t2 = std::chrono::steady_clock::now();
         for (auto& f : steps) f(s);
         AMLS_HIP_CHECK(hipGetLastError());
+        const auto t3 = std::chrono::steady_clock::now();
+        if (st && st->trace) {
+            st->hostSpan("build", born, t1);
+            st->hostSpan("upload", t1, t2);
+            st->hostSpan("enqueue", t2, t3);
+        }
+        born = t3;
     }
 };
 
