@@ -25,8 +25,9 @@ void launch_extadd(const dev::ExtAddDesc* d, const int4* work, int nwork, hipStr
 // items (front index, 64-column block); stat (2 per node, cleared first).
 void launch_front_stats(const dev::FrontStatDesc* d, const int2* work, int nwork, double* stat,
                         hipStream_t s);
-// Shift-invert Lanczos steps, one workgroup per descriptor (n <= 8192 and
-// ncv <= 1536: LDS holds two n-vectors and the step's coefficients).
+// Shift-invert Lanczos steps of every descriptor's node (order n <=
+// lanczos_max_order(), ncv <= lanczos_max_ncv()): init normalises the start
+// vectors of nodes at step 0; one step is six launches over the device.
 int lanczos_max_order();
 int lanczos_max_ncv();
 void launch_lanczos_init(const dev::LanczosDesc* d, int count, hipStream_t s);
