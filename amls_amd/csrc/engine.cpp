@@ -730,7 +730,7 @@ void GemmBatch::add(const GemmDesc& d0) {
     // the pieces add into C atomically (beta 0 or 1, or any beta for a full
     // output, which the plan scales by beta beforehand)
     int nsplit = 1;
-    const bool splittable = d.triA == 0 && !(d.A2 && d.k2 > 0) &&
+    const bool splittable = d.triA == 0 && !(d.A2 && d.k2 > 0) && !d.rsC &&
                             (d.beta == 0.0 || d.beta == 1.0 || !d.lower);
     if (splittable && forceSplit > 0) {
         nsplit = forceSplit;
@@ -2458,7 +2458,6 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
             // u_p = T_B - X^T y_p, then w_p = L^-T D^-1 L^-1 y_p in place
             // (explicit L^-1: two GEMMs through the scratch Tr1)
             GemmBatch gu, gl1, gl2;
-            Copy2dBatch sc;
             for (int64_t p : nodes) {
                 const int64_t np = S.size[p], f = S.f(p), b = S.b(p);
                 double* Tp = T + to[p];
@@ -2489,8 +2488,8 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                 a.ldc = n;
                 a.m = static_cast<int32_t>(np);
                 a.n = qi;
+                a.rsC = dDinv + vo[p];          // D^-1 L^-1 y_p: the row scaling in the store
                 gl1.add(a);
-                sc.add(cdesc(Tr1 + S.off[p], n, Tr1 + S.off[p], n, np, q, 0, dDinv + vo[p]));
                 GemmDesc b2 = gdesc();
                 b2.A = dLinv + co[p];
                 b2.lda = np;
@@ -2508,7 +2507,6 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
             plan.gemm(gu, false);
             addChildUpdates(nodes, false);
             plan.gemm(gl1, false);
-            plan.copy(sc);
             plan.gemm(gl2, false);
         }
         // ---- backward: Y = w, then Y(p) -= X_p Y(B_p) top-down

@@ -31,6 +31,8 @@ struct GemmDesc {
                         // [pad kchunk, (pad + 1) kchunk) and adds its part to C
                         // atomically (C must already hold beta C)
     int32_t pad0;
+    const double* rsC;  // non-null: row i of the product is scaled by rsC[i]
+                        // before it is stored (beta = 0, no split-k)
 };
 
 // One 64 x 64 output tile of descriptor `desc`.

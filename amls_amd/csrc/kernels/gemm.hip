@@ -190,10 +190,12 @@ __global__ void __launch_bounds__(256) gemm_batched_kernel(const GemmDesc* __res
     double* C = d.C;
     const int64_t ldc = d.ldc;
     const bool split = d.kchunk > 0;
+    const double* rs = d.rsC;
     tile_foreach<T>(acc, [&](int r, int c, double v) {
         const int gr = r0 + r, gc = c0 + c;
         if (r < mv && c < nv && (!lower || gr >= gc)) {
             double* p = C + gr + (int64_t)gc * ldc;
+            if (rs) v *= rs[gr];
             if (split) atomicAdd(p, v);
             else *p = beta == 0.0 ? v : v + beta * *p;
         }
