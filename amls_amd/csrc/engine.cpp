@@ -164,7 +164,8 @@ void addSytrd(Plan& plan, const std::vector<EigDesc>& descs, const std::vector<i
 }
 
 // Y = A X for column-major X, Y (n x ncols, ld n): transpose X into the
-// row-major scratch t1, sparse product into t2, transpose back into Y.
+// row-major scratch t1, then the 3 x 3 block product stores Y column-major
+// itself; the scalar CSR product goes through t2 and a transpose back.
 // Algorithmic bytes: the matrix once (values and indices) plus X and Y.
 void addSpmm(Plan& plan, const SparseOp& A, const double* X, double* Y, int64_t ncols, double* t1,
              double* t2) {
@@ -176,12 +177,14 @@ void addSpmm(Plan& plan, const SparseOp& A, const double* X, double* Y, int64_t 
     const int nc = static_cast<int>(ncols);
     const SparseOp a = A;
     if (a.nq > 0) {
+        // the block kernel stores Y column-major itself (no transpose back)
         const double nblk = a.nnz / 9.0;
         plan.timed("spmm", 2.0 * a.nnz * ncols,
                    nblk * 72.0 + 2.0 * n * ncols * 8.0,
                    [=](hipStream_t s) {
-                       launch_spmm_bsr3q(a.nq, a.nb, a.qptr, a.ucol, a.qvptr, a.qval, t1, t2, nc, s);
+                       launch_spmm_bsr3q(a.nq, a.nb, a.qptr, a.ucol, a.qvptr, a.qval, t1, Y, nc, s, n);
                    });
+        return;
     } else {
         plan.timed("spmm", 2.0 * a.nnz * ncols, a.nnz * 16.0 + (n + 1) * 8.0 + 2.0 * n * ncols * 8.0,
                    [=](hipStream_t s) { launch_spmm_rm(n, a.rp, a.ci, a.vx, t1, t2, nc, s); });

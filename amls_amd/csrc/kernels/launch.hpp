@@ -84,13 +84,14 @@ void launch_spmm(int64_t n, const int64_t* rowptr, const int64_t* colidx, const 
 // Y = A X with X, Y row-major (n x ncols, leading dimension ncols)
 void launch_spmm_rm(int64_t n, const int64_t* rowptr, const int64_t* colidx, const double* vals,
                     const double* x, double* y, int ncols, hipStream_t s);
-// Y = A X with X, Y row-major (3 nb x ncols) and A in 3 x 3 blocks grouped
+// Y = A X with X row-major (3 nb x ncols) and A in 3 x 3 blocks grouped
 // by four block rows (merged column lists): qptr (nq + 1) into ucol
 // (3 c | row mask << 28), qvptr (nq + 1) into val (9 doubles per block,
-// row-major).
+// row-major).  Y row-major (ldy = 0) or column-major with leading
+// dimension ldy.
 void launch_spmm_bsr3q(int64_t nq, int64_t nb, const int64_t* qptr, const uint32_t* ucol,
                        const int64_t* qvptr, const double* val, const double* x, double* y, int ncols,
-                       hipStream_t s);
+                       hipStream_t s, int64_t ldy = 0);
 // out[c] = sum_r x(r, c) * y(r, c)
 void launch_coldot(int64_t n, const double* x, const double* y, int ncols, int64_t ld, double* out,
                    hipStream_t s);

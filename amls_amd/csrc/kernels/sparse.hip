@@ -115,7 +115,7 @@ __device__ __forceinline__ void bsr3q_quad(const uint32_t* __restrict__ uc, int 
                                            const double* __restrict__ av,
                                            const double* __restrict__ x, double* __restrict__ y,
                                            int64_t q, int64_t nb, int ncols, int lane, int c_first,
-                                           int c_step) {
+                                           int c_step, int64_t ldy) {
     for (int c0 = c_first; c0 < ncols; c0 += c_step) {
         int cc[SPM_CPL];
         bool on[SPM_CPL];
@@ -194,8 +194,12 @@ __device__ __forceinline__ void bsr3q_quad(const uint32_t* __restrict__ uc, int 
                 for (int k = 0; k < SPM_CPL; ++k)
                     if (on[k]) {
                         const int c = c0 + lane + 64 * k;
+                        // ldy = 0: row-major Y; else column-major with leading dimension ldy
 #pragma unroll
-                        for (int i = 0; i < 3; ++i) y[(3 * br + i) * ncols + c] = acc[r][i][k];
+                        for (int i = 0; i < 3; ++i) {
+                            const int64_t e = ldy ? (3 * br + i) + (int64_t)c * ldy : (3 * br + i) * ncols + c;
+                            y[e] = acc[r][i][k];
+                        }
                     }
             }
         }
@@ -208,7 +212,8 @@ __global__ void __launch_bounds__(256) spmm_bsr3q_kernel(int64_t nq, int64_t nb,
                                                          const int64_t* __restrict__ qvptr,
                                                          const double* __restrict__ val,
                                                          const double* __restrict__ x,
-                                                         double* __restrict__ y, int ncols) {
+                                                         double* __restrict__ y, int ncols,
+                                                         int64_t ldy) {
     __shared__ double sv[SPQ_MAXB * 9];
     __shared__ uint32_t su[SPQ_MAXU];
     const unsigned per = gridDim.x / NXCD;
@@ -220,13 +225,13 @@ __global__ void __launch_bounds__(256) spmm_bsr3q_kernel(int64_t nq, int64_t nb,
     // column group blockIdx.y (four chunks), one chunk per wave
     const int cs = 64 * SPM_CPL;
     if (nu > SPQ_MAXU || nbk > SPQ_MAXB) {
-        bsr3q_quad<false>(ucol + u0, nu, val + 9 * b0, x, y, q, nb, ncols, lane, (blockIdx.y * 4 + w) * cs, 1 << 30);
+        bsr3q_quad<false>(ucol + u0, nu, val + 9 * b0, x, y, q, nb, ncols, lane, (blockIdx.y * 4 + w) * cs, 1 << 30, ldy);
         return;
     }
     for (int e = t; e < nbk * 9; e += 256) sv[e] = val[9 * b0 + e];
     for (int e = t; e < nu; e += 256) su[e] = ucol[u0 + e];
     __syncthreads();
-    bsr3q_quad<true>(su, nu, sv, x, y, q, nb, ncols, lane, (blockIdx.y * 4 + w) * cs, 1 << 30);
+    bsr3q_quad<true>(su, nu, sv, x, y, q, nb, ncols, lane, (blockIdx.y * 4 + w) * cs, 1 << 30, ldy);
 }
 
 // One block per column.
@@ -318,12 +323,12 @@ void launch_spmm_rm(int64_t n, const int64_t* rowptr, const int64_t* colidx, con
 
 void launch_spmm_bsr3q(int64_t nq, int64_t nb, const int64_t* qptr, const uint32_t* ucol,
                        const int64_t* qvptr, const double* val, const double* x, double* y, int ncols,
-                       hipStream_t s) {
+                       hipStream_t s, int64_t ldy) {
     if (nq <= 0 || ncols <= 0) return;
     const unsigned gx = (unsigned)((nq + dev::NXCD - 1) / dev::NXCD * dev::NXCD);
     const unsigned gy = (unsigned)((ncols + 4 * 64 * dev::SPM_CPL - 1) / (4 * 64 * dev::SPM_CPL));
     hipLaunchKernelGGL(dev::spmm_bsr3q_kernel, dim3(gx, gy), dim3(256), 0, s, nq, nb, qptr, ucol, qvptr,
-                       val, x, y, ncols);
+                       val, x, y, ncols, ldy);
 }
 
 void launch_coldot(int64_t n, const double* x, const double* y, int ncols, int64_t ld, double* out,
