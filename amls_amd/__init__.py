@@ -273,6 +273,26 @@ class HipContext:
                                             float(value), iptr(k)))
         return k
 
+    def spectral_pinned(self, k) -> np.ndarray:
+        """Every node keeps exactly min(k[p], n_p) modes from the dense
+        reduction (post-order counts); returns the counts kept."""
+        N = self.tree()["size"].shape[0]
+        kin = np.ascontiguousarray(np.asarray(k, dtype=np.int64).reshape(-1))
+        if kin.size != N:
+            raise ValueError(f"one count per node expected ({N})")
+        kout = np.zeros(N, dtype=np.int64)
+        self._chk(self._L.amls_hip_spectral_pinned(self._h, iptr(kin), iptr(kout)))
+        return kout
+
+    def truncation_margin(self):
+        """(margin, node) of the last threshold truncation: the smallest
+        relative distance of a node's last kept or first dropped local
+        eigenvalue to the cut; (inf, -1) after a fixed-count selection."""
+        m = C.c_double()
+        p = C.c_int64()
+        self._chk(self._L.amls_hip_truncation_margin(self._h, C.byref(m), C.byref(p)))
+        return m.value, p.value
+
     def project(self, precision: str = "fp64") -> int:
         p = C.c_int64()
         self._chk(self._L.amls_hip_project(self._h, PRECISIONS[precision], C.byref(p)))

@@ -35,6 +35,7 @@ EXPORTS = [
     "amls_hip_store_open", "amls_hip_store_close", "amls_hip_store_last_error",
     "amls_hip_store_append", "amls_hip_store_read_sum", "amls_hip_store_shape",
     "amls_hip_store_remove", "amls_hip_store_reset", "amls_hip_store_disk_bytes",
+    "amls_hip_spectral_pinned", "amls_hip_truncation_margin",
 ]
 
 
@@ -142,6 +143,8 @@ def lib() -> C.CDLL:
         "amls_hip_store_remove": ([P, I64], C.c_int),
         "amls_hip_store_reset": ([P], C.c_int),
         "amls_hip_store_disk_bytes": ([P, PD], C.c_int),
+        "amls_hip_spectral_pinned": ([P, PI64, PI64], C.c_int),
+        "amls_hip_truncation_margin": ([P, PD, PI64], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

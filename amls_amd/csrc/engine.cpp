@@ -1633,7 +1633,8 @@ void Ctx::spectralPrepare() {
 // ---------------------------------------------------------------------------
 // Local eigenpairs + truncation (both passes)
 // ---------------------------------------------------------------------------
-void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
+void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg,
+                         const std::vector<int64_t>* pinned) {
     if (kind != 0 && kind != 1) throw StatusError(AMLS_E_ARG, "truncation kind must be 0 or 1");
     requireGpu();
     kstats.phase = "select";
@@ -1643,8 +1644,16 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
         throw StatusError(AMLS_E_INTERNAL, "one truncation value per problem expected");
     selected = projected = reduced = recovered = false;
     const int64_t N = S.nNodes;
+    if (pinned && static_cast<int64_t>(pinned->size()) != N)
+        throw StatusError(AMLS_E_ARG, "pinned counts: one per node expected");
     k.assign(N, 0);
-    if (kind == 0) {
+    lastSelKind = -1;
+    if (pinned) {
+        for (int64_t p = 0; p < N; ++p) {
+            if ((*pinned)[p] < 0) throw StatusError(AMLS_E_ARG, "pinned counts must not be negative");
+            k[p] = std::min<int64_t>((*pinned)[p], S.size[p]);
+        }
+    } else if (kind == 0) {
         for (int64_t p = 0; p < N; ++p) {
             const double value = valueSeg[nodeSeg[p]];
             k[p] = value <= 0 ? S.size[p] : std::min<int64_t>(static_cast<int64_t>(value), S.size[p]);
@@ -1679,8 +1688,18 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
         for (int64_t p = 0; p < N; ++p) k[p] = cnt[p];
     }
     // nodes on the shift-invert Lanczos path: their k and Lanczos steps
+    // (pinned counts come from the dense reduction of every node)
     std::vector<int> mUse;
-    silSelect(kind, valueSeg, mUse);
+    if (pinned) {
+        mUse.assign(N, 0);
+        std::fill(lastPath.begin(), lastPath.end(), 0);
+        std::fill(lastSteps.begin(), lastSteps.end(), 0);
+    } else {
+        silSelect(kind, valueSeg, mUse);
+    }
+    lastSelKind = pinned ? 2 : kind;
+    lastSelValue = valueSeg;
+    lastMUse = mUse;
     moff.assign(N + 1, 0);
     zo.assign(N + 1, 0);
     int64_t wk = 0;
@@ -1815,6 +1834,80 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg) {
     plan.trsm(str);
     plan.run(stream);
     selected = true;
+}
+
+// The margin of the last threshold selection (the discreteness of the
+// truncation: a local eigenvalue within rounding of lambda_cut may be kept by
+// one implementation and dropped by another).  Per node, the last kept
+// eigenvalue w_k (from the selection's output) and the first dropped one
+// w_k+1 (one more bisection on the node's tridiagonal form, or on T_m of a
+// shift-invert Lanczos node) give min(|w_k - cut|, |w_k+1 - cut|) / cut.
+void Ctx::truncationMargin(double& margin, int64_t& node) {
+    requireGpu();
+    margin = INFINITY;
+    node = -1;
+    if (!selected || lastSelKind != 1) return;
+    sync();
+    const int64_t N = S.nNodes;
+    std::vector<EigDesc> ed;
+    std::vector<int2> work;
+    std::vector<int64_t> who;
+    int nmax = 1;
+    DevArena A;
+    A.plan(vo[N] * 8);
+    A.commit();
+    double* drop = A.take<double>(vo[N]);
+    for (int64_t p = 0; p < N; ++p) {
+        const int m = lastMUse[p];
+        const bool silNode = m > 0;
+        const int64_t n = silNode ? m : S.size[p];
+        if (k[p] >= n) continue;                      // nothing dropped
+        EigDesc e;
+        std::memset(&e, 0, sizeof(e));
+        e.d = silNode ? sil[silOf[p]].alpha : dD + vo[p];
+        e.e = silNode ? sil[silOf[p]].beta : dE + vo[p];
+        e.n = static_cast<int32_t>(n);
+        e.kcap = e.n;
+        e.lam = drop + vo[p];                         // lam[k_p] = the first dropped nu
+        const int q = static_cast<int>(ed.size());
+        ed.push_back(e);
+        who.push_back(p);
+        work.push_back(make_int2(q, static_cast<int>(k[p])));
+        while (work.size() % 4) work.push_back(make_int2(q, -1));
+        nmax = std::max<int>(nmax, static_cast<int>(n));
+    }
+    std::vector<double> dropped(vo[N], 0.0), mu(ptot, 0.0);
+    if (!ed.empty()) {
+        Blob b;
+        const size_t od = b.add(ed), ow = b.add(work);
+        b.upload(stream);
+        launch_bisect(b.at<EigDesc>(od), b.at<int2>(ow), static_cast<int>(work.size()), nmax, stream);
+        AMLS_HIP_CHECK(hipGetLastError());
+        AMLS_HIP_CHECK(hipMemcpyAsync(dropped.data(), drop, vo[N] * 8, hipMemcpyDeviceToHost, stream));
+    }
+    if (ptot > 0)
+        AMLS_HIP_CHECK(hipMemcpyAsync(mu.data(), dMu, ptot * 8, hipMemcpyDeviceToHost, stream));
+    sync();
+    std::vector<char> hasDrop(N, 0);
+    for (int64_t p : who) hasDrop[p] = 1;
+    for (int64_t p = 0; p < N; ++p) {
+        const double cut = lastSelValue[nodeSeg[p]];
+        if (!(cut > 0.0)) continue;
+        const double sigma = lastMUse[p] > 0 ? sil[silOf[p]].sigma : 0.0;
+        double mp = INFINITY;
+        if (k[p] > 0) mp = std::fabs(mu[moff[p] + k[p] - 1] - cut) / cut;
+        if (hasDrop[p]) {
+            const double nu = dropped[vo[p] + k[p]];
+            const double w = 1.0 / nu + sigma;
+            // a first dropped nu <= 0 lies at w = +infinity (dense) or outside
+            // the interval (shift-invert): no margin from that side
+            if (nu > 0.0) mp = std::min(mp, std::fabs(w - cut) / cut);
+        }
+        if (mp < margin) {
+            margin = mp;
+            node = p;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -3089,6 +3182,26 @@ amls_status amls_hip_spectral(amls_ctx* ctx, int32_t kind, double value, int64_t
         c.spectralSelect(kind, value);
         c.sync();
         if (k_out) std::copy(c.k.begin(), c.k.end(), k_out);
+    });
+}
+
+amls_status amls_hip_spectral_pinned(amls_ctx* ctx, const int64_t* k_in, int64_t* k_out) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        if (!k_in) throw amls::StatusError(AMLS_E_ARG, "null pinned counts");
+        const std::vector<int64_t> pin(k_in, k_in + c.S.nNodes);
+        c.spectralSelect(0, std::vector<double>(static_cast<size_t>(c.nseg), 0.0), &pin);
+        c.sync();
+        if (k_out) std::copy(c.k.begin(), c.k.end(), k_out);
+    });
+}
+
+amls_status amls_hip_truncation_margin(amls_ctx* ctx, double* margin, int64_t* node) {
+    return guarded(ctx, [&](amls::Ctx& c) {
+        double m = 0.0;
+        int64_t p = -1;
+        c.truncationMargin(m, p);
+        if (margin) *margin = m;
+        if (node) *node = p;
     });
 }
 

@@ -278,6 +278,14 @@ struct Ctx {
     std::vector<int> silOf;          // node -> index into sil, or -1
     std::vector<int32_t> lastPath;   // per node, last spectral call: 0 dense, 1 SIL, 2 SIL fell back
     std::vector<int64_t> lastSteps;  // per node, Lanczos steps the last call used
+    // the last selection: kind (-1 none, 2 pinned), its values per problem and
+    // the Lanczos steps whose Ritz pairs each node kept (0: dense path)
+    int lastSelKind = -1;
+    std::vector<double> lastSelValue;
+    std::vector<int> lastMUse;
+    // smallest relative distance of a node's last kept or first dropped local
+    // eigenvalue to the cut of the last threshold selection, and its node
+    void truncationMargin(double& margin, int64_t& node);
     std::unique_ptr<DevArena> silMem;
     std::unique_ptr<Blob> silBlob;
     void silPrepare();               // S = R^T (K-hat - sigma M-hat)^-1 R per SIL node
@@ -369,8 +377,11 @@ struct Ctx {
     void runGE(int prec);
     void spectralPrepare();
     // kind 0: keep min(value, n_i) modes per node (value <= 0 keeps all);
-    // kind 1: keep modes with w <= value; one value per problem
-    void spectralSelect(int kind, const std::vector<double>& valuePerSeg);
+    // kind 1: keep modes with w <= value; one value per problem.
+    // pinned (non-null, one count per node): keep exactly min(pinned[p], n_p)
+    // modes of every node from the dense reduction (kind and values unused)
+    void spectralSelect(int kind, const std::vector<double>& valuePerSeg,
+                        const std::vector<int64_t>* pinned = nullptr);
     void spectralSelect(int kind, double value) {
         spectralSelect(kind, std::vector<double>(static_cast<size_t>(nseg), value));
     }

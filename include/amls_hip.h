@@ -129,8 +129,9 @@ amls_status amls_hip_tree_get(const amls_ctx* ctx, int64_t* treeId, int64_t* off
  * plane separators where coordinates allow, else for graphs of average
  * degree <= 32 a spectral cut (Fiedler vector by Lanczos, median split,
  * smaller cut boundary) with Fiduccia-Mattheyses vertex-separator
- * refinement, else BFS level sets from a pseudo-peripheral vertex; 1 BFS
- * level sets only, unrefined.  Stands in for MetisPartitioner's
+ * refinement, else BFS level sets from a pseudo-peripheral vertex; graph
+ * separators drop the vertices adjacent to one side only; 1 BFS level sets
+ * only, unrefined.  Stands in for MetisPartitioner's
  * METIS_ComputeVertexSeparator (src/partition/metis_partitioner.cpp:80-103),
  * keeping its recursion and stop rules (:110-179). */
 amls_status amls_hip_set_partition(amls_ctx* ctx, int32_t method);
@@ -187,6 +188,28 @@ amls_status amls_hip_get_sil(amls_ctx* ctx, int32_t* path, int64_t* steps);
  * node the modes are in ascending order.
  */
 amls_status amls_hip_spectral(amls_ctx* ctx, int32_t kind, double value, int64_t* k_out);
+
+/*
+ * amls_hip_spectral with pinned counts: node p (post-order) keeps exactly
+ * min(k_in[p], n_p) modes, its lowest, from the dense reduction (no node
+ * takes the shift-invert Lanczos path).  For parity runs: pinning the k_i of
+ * another implementation's run removes the discreteness of the truncation
+ * (a local eigenvalue within rounding of lambda_cut).  k_out (may be null)
+ * receives the counts kept.  No counterpart in the reference, whose
+ * truncation policies take one count or one cut for all nodes.
+ */
+amls_status amls_hip_spectral_pinned(amls_ctx* ctx, const int64_t* k_in, int64_t* k_out);
+
+/*
+ * Margin of the last threshold truncation (amls_hip_spectral kind 1, or
+ * pass B of a solve): the smallest min(|w_k - cut|, |w_k+1 - cut|) / cut over
+ * the nodes, w_k the node's last kept and w_k+1 its first dropped local
+ * eigenvalue (Ritz value on the shift-invert Lanczos path), and the node
+ * (post-order) where it occurs.  A margin near rounding flags a truncation
+ * that another implementation may decide the other way.  After a fixed-count
+ * or pinned selection: margin = +infinity, node = -1.
+ */
+amls_status amls_hip_truncation_margin(amls_ctx* ctx, double* margin, int64_t* node);
 
 /* Projection of M onto the retained modes, all levels; returns p.
  * Replaces SubspaceCondensation::projectLevel/projectNode_/finalize
