@@ -859,7 +859,11 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
     if (nLevels < 0 || nLevels > 40) throw StatusError(AMLS_E_ARG, "nLevels must be in 0..40");
     if (leafSize < 1) throw StatusError(AMLS_E_ARG, "leafSize must be positive");
     ++problemGen;
-    if (gpu) dropGeGraph();
+    if (gpu) {
+        dropGeGraph();
+        dropSiGraph();
+        siKeySeen = false;
+    }
     haveProblem = false;
     geDone = specReady = selected = projected = reduced = recovered = false;
     lastVectors = false;
@@ -1489,6 +1493,38 @@ void Ctx::runGE(int prec) {
     geGraphGen = problemGen;
     geGraphBase = A.base();
     finishGE();
+}
+
+void Ctx::dropSiGraph() {
+    if (siGraph) (void)hipGraphExecDestroy(siGraph);
+    siGraph = nullptr;
+}
+
+// Uploads the plan's descriptors, then captures its launches on stream s as
+// a HIP graph (kernel nodes only: no memset or host-copy nodes), instantiates
+// and launches it; the caller owns the returned executable graph.
+hipGraphExec_t captureAndLaunch(Plan& plan, hipStream_t s) {
+    plan.blob->upload(s);
+    AMLS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    hipGraph_t g = nullptr;
+    try {
+        for (auto& f : plan.steps) f(s);
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    AMLS_HIP_CHECK(hipStreamEndCapture(s, &g));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    AMLS_HIP_CHECK(ie);
+    const hipError_t le = hipGraphLaunch(exec, s);
+    if (le != hipSuccess) {
+        (void)hipGraphExecDestroy(exec);
+        AMLS_HIP_CHECK(le);
+    }
+    return exec;
 }
 
 void Ctx::dropGeGraph() {
@@ -2439,305 +2475,8 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     // rows of problem s: [r0, r0 + ns) in the permuted ordering
     auto rowsOf = [&](int64_t sg) { return std::make_pair(segRow[sg], segRow[sg + 1] - segRow[sg]); };
     int* err = A.take<int>(1);
-    fresh(siBlob);
-    Plan plan(siBlob.get(), &kstats);
-    plan.hostStep([=](hipStream_t s) { launch_set_i32(err, INT_MAX, s); });
     const int qi = static_cast<int>(q);
     const int64_t nn = n;
-
-    // top-down application of L^-T: rows of node p become
-    // Q(p) = Q(p) - X_p Q(B_p), ancestors first
-    auto addBacksolve = [&](double* X) {
-        for (int64_t l = S.maxLevel; l >= 0; --l) {
-            GatherBatch gt;
-            GemmBatch gb;
-            for (int64_t p : S.byLevel[l]) {
-                const int64_t np = S.size[p], b = S.b(p);
-                if (b == 0) continue;
-                GatherDesc gd;
-                std::memset(&gd, 0, sizeof(gd));
-                gd.src = X;
-                gd.lds = n;
-                gd.dst = zB + zbl[p];
-                gd.ldd = b;
-                gd.rows_idx = dbidx + S.bptr[p];
-                gd.rows = static_cast<int32_t>(b);
-                gd.cols = qi;
-                gt.add(gd);
-                GemmDesc g = gdesc();
-                g.A = dXb + xo[p];
-                g.lda = np;
-                g.B = zB + zbl[p];
-                g.ldb = b;
-                g.k = static_cast<int32_t>(b);
-                g.alpha1 = -1.0;
-                g.beta = 1.0;
-                g.C = X + S.off[p];
-                g.ldc = n;
-                g.m = static_cast<int32_t>(np);
-                g.n = qi;
-                gb.add(g);
-            }
-            plan.gather(gt);
-            plan.gemm(gb, false);
-        }
-    };
-
-    if (fromBasis) {
-        // Q0 = L^-T blockdiag(Z_p): Z_p in its rows and mode columns, then the
-        // top-down solve
-        plan.fn([=](hipStream_t s) { launch_fill(Q, nn * q, 0.0, s); });
-        Copy2dBatch cz;
-        for (int64_t p = 0; p < N; ++p)
-            cz.add(cdesc(dZ + zo[p], S.size[p], Q + S.off[p] + moff[p] * n, n, S.size[p], k[p], 0));
-        plan.copy(cz);
-        addBacksolve(Q);
-    } else {
-        AMLS_HIP_CHECK(hipMemcpyAsync(Q, dZfull, n * q * 8, hipMemcpyDeviceToDevice, stream));
-    }
-    addSpmm(plan, opPM, Q, R, q, Tr1, Tr2);     // R = M Q
-
-    // Children's updates into the parent fronts of the forward solves.  A
-    // child's relative index list is monotone, so the rows of its update
-    // that land in the parent's own block are a prefix: those are added
-    // first (own = true, before the parent's y_p is used), the rest after
-    // the parent's u_p = -X^T y_p has initialised its boundary block
-    // (beta = 0), so T needs no clearing.  Lists found unsorted fall back to
-    // clearing T and adding every row first.
-    bool relSorted = true;
-    for (int64_t c = 0; c < N && relSorted; ++c)
-        relSorted = std::is_sorted(S.rel.begin() + S.bptr[c], S.rel.begin() + S.bptr[c] + S.b(c));
-    auto addChildUpdates = [&](const std::vector<int64_t>& nodes, bool own) {
-        for (int slot = 0; slot < 2; ++slot) {
-            ExtBatch eb;
-            for (int64_t p : nodes) {
-                const int64_t c = slot == 0 ? S.child0[p] : S.child1[p];
-                if (c < 0 || S.b(c) == 0) continue;
-                const int64_t bc = S.b(c);
-                const int64_t* r = S.rel.data() + S.bptr[c];
-                const int64_t split = relSorted ? std::lower_bound(r, r + bc, S.size[p]) - r : bc;
-                const int64_t r0 = own ? 0 : split, r1 = own ? split : bc;
-                if (r1 <= r0) continue;
-                ExtAddDesc x;
-                std::memset(&x, 0, sizeof(x));
-                x.src = T + to[c] + S.size[c] + r0;
-                x.lds = S.f(c);
-                x.dst = T + to[p];
-                x.ldd = S.f(p);
-                x.rowmap = drel + S.bptr[c] + r0;
-                x.rows = static_cast<int32_t>(r1 - r0);
-                x.cols = qi;
-                eb.add(x);
-            }
-            plan.ext(eb);
-        }
-    };
-    // the column factors the previous step's M Y still has to be scaled by
-    // (null: M Y is scaled already)
-    const double* pend = nullptr;
-    for (int it = 0; it < steps; ++it) {
-        const bool last = it + 1 == steps;
-        // ---- forward: fronts take their own rows of M Q (R in the first
-        // step, the scaled M Y of the previous step after that), children's
-        // updates
-        const double* src = it == 0 ? R : MY;
-        if (!relSorted) plan.fn([=](hipStream_t s) { launch_fill(T, to[S.nNodes], 0.0, s); });
-        Copy2dBatch cown;
-        for (int64_t p = 0; p < N; ++p)
-            cown.add(cdesc(src + S.off[p], n, T + to[p], S.f(p), S.size[p], q, 0, nullptr,
-                           pend ? pend + nodeSeg[p] * q : nullptr));
-        pend = nullptr;
-        plan.copy(cown);
-        for (int64_t l = 0; l <= S.maxLevel; ++l) {
-            const auto& nodes = S.byLevel[l];
-            addChildUpdates(nodes, true);
-            // u_p = T_B - X^T y_p, then w_p = L^-T D^-1 L^-1 y_p in place
-            // (explicit L^-1: two GEMMs through the scratch Tr1)
-            GemmBatch gu, gl1, gl2;
-            for (int64_t p : nodes) {
-                const int64_t np = S.size[p], f = S.f(p), b = S.b(p);
-                double* Tp = T + to[p];
-                if (b > 0) {
-                    GemmDesc g = gdesc();
-                    g.A = dXb + xo[p];
-                    g.lda = np;
-                    g.ta = 1;
-                    g.B = Tp;
-                    g.ldb = f;
-                    g.k = static_cast<int32_t>(np);
-                    g.alpha1 = -1.0;
-                    g.beta = relSorted ? 0.0 : 1.0;   // T_B is not cleared when sorted
-                    g.C = Tp + np;
-                    g.ldc = f;
-                    g.m = static_cast<int32_t>(b);
-                    g.n = qi;
-                    gu.add(g);
-                }
-                GemmDesc a = gdesc();
-                a.A = dLinv + co[p];
-                a.lda = np;
-                a.triA = 1;
-                a.B = Tp;
-                a.ldb = f;
-                a.k = static_cast<int32_t>(np);
-                a.C = Tr1 + S.off[p];
-                a.ldc = n;
-                a.m = static_cast<int32_t>(np);
-                a.n = qi;
-                a.rsC = dDinv + vo[p];          // D^-1 L^-1 y_p: the row scaling in the store
-                gl1.add(a);
-                GemmDesc b2 = gdesc();
-                b2.A = dLinv + co[p];
-                b2.lda = np;
-                b2.ta = 1;
-                b2.triA = 2;
-                b2.B = Tr1 + S.off[p];
-                b2.ldb = n;
-                b2.k = static_cast<int32_t>(np);
-                b2.C = Tp;
-                b2.ldc = f;
-                b2.m = static_cast<int32_t>(np);
-                b2.n = qi;
-                gl2.add(b2);
-            }
-            plan.gemm(gu, false);
-            addChildUpdates(nodes, false);
-            plan.gemm(gl1, false);
-            plan.gemm(gl2, false);
-        }
-        // ---- backward: Y = w, then Y(p) -= X_p Y(B_p) top-down
-        Copy2dBatch cw;
-        for (int64_t p = 0; p < N; ++p)
-            cw.add(cdesc(T + to[p], S.f(p), Yv + S.off[p], n, S.size[p], q, 0));
-        plan.copy(cw);
-        addBacksolve(Yv);
-        addSpmm(plan, opPM, Yv, MY, q, Tr1, Tr2);   // MY = M Y
-        // scale every column of Y and M Y to unit M-norm (per problem): the
-        // next step applies K^-1 to (M Y) S, the Rayleigh-Ritz at the end
-        // projects onto Y S.  Scaling only changes the basis of the same
-        // subspace, but the columns turn towards the dominant modes: on the
-        // whole AMLS basis cond(Y^T M Y) grows by 1e3 - 1e4 per step, so
-        // after every second step that is not the last one the basis is
-        // M-orthonormalised (below), and the projected pencil of the final
-        // Rayleigh-Ritz stays well conditioned however many steps run.
-        // Y and M Y are read scaled only by the final Rayleigh-Ritz and the
-        // Cholesky QR; before another plain step only the next step's copy
-        // of M Y's own rows needs the factors, and it applies them itself
-        // (the same product per element as scaling M Y first), while Y is
-        // overwritten unread
-        const int64_t* seg = dSegRow;
-        const int nsg = static_cast<int>(NS);
-        double* nrmS = dsq;
-        const bool chol = !last && (it + 1) % 2 == 0;
-        if (last || chol) {
-            plan.fn([=](hipStream_t s) { launch_colnorm_seg(seg, nsg, MY, Yv, qi, nn, nrmS, s, true); });
-        } else {
-            plan.fn([=](hipStream_t s) { launch_colnorm_factors(seg, nsg, MY, Yv, qi, nn, nrmS, fac, s); });
-            pend = fac;
-        }
-        if (chol) {
-            // Cholesky QR in the M inner product, per problem:
-            // G = Y^T M Y = L D L^T, Y <- Y L^-T D^-1/2 (through Q), M Y again
-            GemmBatch gg, gy;
-            Copy2dBatch cz0, cy;
-            std::vector<VecDesc> vid0(NS), vdq(NS);
-            TrsmBatch ti;
-            for (int64_t sg = 0; sg < NS; ++sg) {
-                const auto [r0, ns] = rowsOf(sg);
-                double *g = KcS[sg], *li = McS[sg], *ds = dsq + sg * q;
-                GemmDesc d = gdesc();
-                d.A = Yv + r0;
-                d.lda = n;
-                d.ta = 1;
-                d.B = MY + r0;
-                d.ldb = n;
-                d.k = static_cast<int32_t>(ns);
-                d.C = g;
-                d.ldc = q;
-                d.m = d.n = qi;
-                d.lower = 1;
-                gg.add(d);
-                cz0.add(cdesc(nullptr, 0, li, q, q, q, 3, nullptr, nullptr, 0.0));
-                std::memset(&vid0[sg], 0, sizeof(VecDesc));
-                vid0[sg].dst = li;
-                vid0[sg].ld = q;
-                vid0[sg].n = qi;
-                vid0[sg].op = 2;
-                ti.add(g, q, li, q, q, q, 0);
-                ti.descs.back().rhsLower = 1;
-                std::memset(&vdq[sg], 0, sizeof(VecDesc));
-                vdq[sg].src = g;
-                vdq[sg].dst = ds;
-                vdq[sg].ld = q;
-                vdq[sg].n = qi;
-                vdq[sg].op = 0;
-                GemmDesc e = gdesc();               // Q = Y L^-T
-                e.A = Yv + r0;
-                e.lda = n;
-                e.B = li;
-                e.ldb = q;
-                e.tb = 1;
-                e.k = qi;
-                e.C = Q + r0;
-                e.ldc = n;
-                e.m = static_cast<int32_t>(ns);
-                e.n = qi;
-                gy.add(e);
-                cy.add(cdesc(Q + r0, n, Yv + r0, n, ns, q, 0, nullptr, ds));   // Y = Q D^-1/2
-            }
-            plan.gemm(gg, false);
-            addLdltBatch(plan, KcS, std::vector<int64_t>(NS, q), WpS, err,
-                         std::vector<int>(NS, -2));
-            plan.copy(cz0);
-            plan.vec(vid0);
-            plan.trsm(ti);
-            plan.vec(vdq);
-            plan.gemm(gy, false);
-            plan.copy(cy);
-            addSpmm(plan, opPM, Yv, MY, q, Tr1, Tr2);
-        }
-    }
-    // ---- final Rayleigh-Ritz with the original FP64 K and M, per problem:
-    // Kc = Y^T K Y, Mc = Y^T M Y; lowest nev pairs of (Kc, Mc) through
-    // Kc = L D L^T, C = D^-1/2 L^-1 Mc L^-T D^-1/2, nu = 1/theta.  L^-1 is
-    // formed once (one blocked triangular solve on the identity, into the
-    // storage of Mc after Mc is copied out), and the two congruence products
-    // and the back-transformation of the eigenvectors are GEMMs with it.
-    addSpmm(plan, opPK, Yv, R, q, Tr1, Tr2);
-    {
-        GemmBatch gr;
-        for (int64_t sg = 0; sg < NS; ++sg) {
-            const auto [r0, ns] = rowsOf(sg);
-            GemmDesc g1 = gdesc();
-            g1.A = Yv + r0;
-            g1.lda = n;
-            g1.ta = 1;
-            g1.B = R + r0;
-            g1.ldb = n;
-            g1.k = static_cast<int32_t>(ns);
-            g1.C = KcS[sg];
-            g1.ldc = q;
-            g1.m = g1.n = qi;
-            g1.lower = 1;
-            GemmDesc g2 = g1;
-            g2.B = MY + r0;
-            g2.C = McS[sg];
-            if (shift != 0.0) {            // (K + shift M): Kc += shift Y^T M Y
-                g1.A2 = Yv + r0;
-                g1.lda2 = n;
-                g1.ta2 = 1;
-                g1.B2 = MY + r0;
-                g1.ldb2 = n;
-                g1.k2 = static_cast<int32_t>(ns);
-                g1.alpha2 = shift;
-            }
-            gr.add(g1);
-            gr.add(g2);
-        }
-        plan.gemm(gr, false);
-    }
-    addLdltBatch(plan, KcS, std::vector<int64_t>(KcS.size(), q), WpS, err,
-                 std::vector<int>(KcS.size(), -1));
     const int nv = static_cast<int>(nev);
     Copy2dBatch c1, cz, c3, cv;
     TrsmBatch tinv;
@@ -2835,25 +2574,351 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
         g3.n = nv;
         gx.add(g3);
     }
-    plan.copy(c1);
-    plan.copy(cz);
-    plan.vec(vid);
-    plan.trsm(tinv);
-    plan.gemm(gl1, false);
-    plan.gemm(gl2, false);
-    plan.vec(vd);
-    plan.copy(c3);
-    // the nev largest eigenpairs of C: Chebyshev-filtered subspace iteration
-    // for one wide problem (q = p), the dense reduction otherwise or when it
-    // does not converge
+
+    // The steps and the set-up of the final Rayleigh-Ritz problem (part 1)
+    // depend only on the tree, the widths, the start, the shift, the node
+    // mode counts and buffer addresses: the first run for a key is eager, the
+    // second captures part 1 as a HIP graph, later runs replay it (runs with
+    // statistics or tracing stay eager).
+    SiKey key;
+    key.gen = problemGen;
+    key.q = q;
+    key.nev = nev;
+    key.steps = steps;
+    key.fromBasis = fromBasis;
+    key.shift = shift;
+    key.ptrs = {A.base(), dXb, dLinv, dDinv, dZ, dZfull, opPM.qval, opPK.qval, opPM.vx, opPK.vx,
+                dSegRow, drel, dbidx};
+    if (fromBasis) key.k = k;
+    const bool eagerOnly = kstats.enabled || kstats.trace;
+    const bool sameKey = siKeySeen && key == siKey;
+    if (!eagerOnly && sameKey && siGraph) {
+        AMLS_HIP_CHECK(hipGraphLaunch(siGraph, stream));
+    } else {
+        fresh(siBlob);
+        Plan plan(siBlob.get(), &kstats);
+        plan.hostStep([=](hipStream_t s) { launch_set_i32(err, INT_MAX, s); });
+
+        // top-down application of L^-T: rows of node p become
+        // Q(p) = Q(p) - X_p Q(B_p), ancestors first
+        auto addBacksolve = [&](double* X) {
+            for (int64_t l = S.maxLevel; l >= 0; --l) {
+                GatherBatch gt;
+                GemmBatch gb;
+                for (int64_t p : S.byLevel[l]) {
+                    const int64_t np = S.size[p], b = S.b(p);
+                    if (b == 0) continue;
+                    GatherDesc gd;
+                    std::memset(&gd, 0, sizeof(gd));
+                    gd.src = X;
+                    gd.lds = n;
+                    gd.dst = zB + zbl[p];
+                    gd.ldd = b;
+                    gd.rows_idx = dbidx + S.bptr[p];
+                    gd.rows = static_cast<int32_t>(b);
+                    gd.cols = qi;
+                    gt.add(gd);
+                    GemmDesc g = gdesc();
+                    g.A = dXb + xo[p];
+                    g.lda = np;
+                    g.B = zB + zbl[p];
+                    g.ldb = b;
+                    g.k = static_cast<int32_t>(b);
+                    g.alpha1 = -1.0;
+                    g.beta = 1.0;
+                    g.C = X + S.off[p];
+                    g.ldc = n;
+                    g.m = static_cast<int32_t>(np);
+                    g.n = qi;
+                    gb.add(g);
+                }
+                plan.gather(gt);
+                plan.gemm(gb, false);
+            }
+        };
+
+        if (fromBasis) {
+            // Q0 = L^-T blockdiag(Z_p): Z_p in its rows and mode columns, then the
+            // top-down solve
+            plan.fn([=](hipStream_t s) { launch_fill(Q, nn * q, 0.0, s); });
+            Copy2dBatch cz;
+            for (int64_t p = 0; p < N; ++p)
+                cz.add(cdesc(dZ + zo[p], S.size[p], Q + S.off[p] + moff[p] * n, n, S.size[p], k[p], 0));
+            plan.copy(cz);
+            addBacksolve(Q);
+        } else {
+            Copy2dBatch cq;
+            cq.add(cdesc(dZfull, n, Q, n, n, q, 0));
+            plan.copy(cq);
+        }
+        addSpmm(plan, opPM, Q, R, q, Tr1, Tr2);     // R = M Q
+
+        // Children's updates into the parent fronts of the forward solves.  A
+        // child's relative index list is monotone, so the rows of its update
+        // that land in the parent's own block are a prefix: those are added
+        // first (own = true, before the parent's y_p is used), the rest after
+        // the parent's u_p = -X^T y_p has initialised its boundary block
+        // (beta = 0), so T needs no clearing.  Lists found unsorted fall back to
+        // clearing T and adding every row first.
+        bool relSorted = true;
+        for (int64_t c = 0; c < N && relSorted; ++c)
+            relSorted = std::is_sorted(S.rel.begin() + S.bptr[c], S.rel.begin() + S.bptr[c] + S.b(c));
+        auto addChildUpdates = [&](const std::vector<int64_t>& nodes, bool own) {
+            for (int slot = 0; slot < 2; ++slot) {
+                ExtBatch eb;
+                for (int64_t p : nodes) {
+                    const int64_t c = slot == 0 ? S.child0[p] : S.child1[p];
+                    if (c < 0 || S.b(c) == 0) continue;
+                    const int64_t bc = S.b(c);
+                    const int64_t* r = S.rel.data() + S.bptr[c];
+                    const int64_t split = relSorted ? std::lower_bound(r, r + bc, S.size[p]) - r : bc;
+                    const int64_t r0 = own ? 0 : split, r1 = own ? split : bc;
+                    if (r1 <= r0) continue;
+                    ExtAddDesc x;
+                    std::memset(&x, 0, sizeof(x));
+                    x.src = T + to[c] + S.size[c] + r0;
+                    x.lds = S.f(c);
+                    x.dst = T + to[p];
+                    x.ldd = S.f(p);
+                    x.rowmap = drel + S.bptr[c] + r0;
+                    x.rows = static_cast<int32_t>(r1 - r0);
+                    x.cols = qi;
+                    eb.add(x);
+                }
+                plan.ext(eb);
+            }
+        };
+        // the column factors the previous step's M Y still has to be scaled by
+        // (null: M Y is scaled already)
+        const double* pend = nullptr;
+        for (int it = 0; it < steps; ++it) {
+            const bool last = it + 1 == steps;
+            // ---- forward: fronts take their own rows of M Q (R in the first
+            // step, the scaled M Y of the previous step after that), children's
+            // updates
+            const double* src = it == 0 ? R : MY;
+            if (!relSorted) plan.fn([=](hipStream_t s) { launch_fill(T, to[S.nNodes], 0.0, s); });
+            Copy2dBatch cown;
+            for (int64_t p = 0; p < N; ++p)
+                cown.add(cdesc(src + S.off[p], n, T + to[p], S.f(p), S.size[p], q, 0, nullptr,
+                               pend ? pend + nodeSeg[p] * q : nullptr));
+            pend = nullptr;
+            plan.copy(cown);
+            for (int64_t l = 0; l <= S.maxLevel; ++l) {
+                const auto& nodes = S.byLevel[l];
+                addChildUpdates(nodes, true);
+                // u_p = T_B - X^T y_p, then w_p = L^-T D^-1 L^-1 y_p in place
+                // (explicit L^-1: two GEMMs through the scratch Tr1)
+                GemmBatch gu, gl1, gl2;
+                for (int64_t p : nodes) {
+                    const int64_t np = S.size[p], f = S.f(p), b = S.b(p);
+                    double* Tp = T + to[p];
+                    if (b > 0) {
+                        GemmDesc g = gdesc();
+                        g.A = dXb + xo[p];
+                        g.lda = np;
+                        g.ta = 1;
+                        g.B = Tp;
+                        g.ldb = f;
+                        g.k = static_cast<int32_t>(np);
+                        g.alpha1 = -1.0;
+                        g.beta = relSorted ? 0.0 : 1.0;   // T_B is not cleared when sorted
+                        g.C = Tp + np;
+                        g.ldc = f;
+                        g.m = static_cast<int32_t>(b);
+                        g.n = qi;
+                        gu.add(g);
+                    }
+                    GemmDesc a = gdesc();
+                    a.A = dLinv + co[p];
+                    a.lda = np;
+                    a.triA = 1;
+                    a.B = Tp;
+                    a.ldb = f;
+                    a.k = static_cast<int32_t>(np);
+                    a.C = Tr1 + S.off[p];
+                    a.ldc = n;
+                    a.m = static_cast<int32_t>(np);
+                    a.n = qi;
+                    a.rsC = dDinv + vo[p];          // D^-1 L^-1 y_p: the row scaling in the store
+                    gl1.add(a);
+                    GemmDesc b2 = gdesc();
+                    b2.A = dLinv + co[p];
+                    b2.lda = np;
+                    b2.ta = 1;
+                    b2.triA = 2;
+                    b2.B = Tr1 + S.off[p];
+                    b2.ldb = n;
+                    b2.k = static_cast<int32_t>(np);
+                    b2.C = Tp;
+                    b2.ldc = f;
+                    b2.m = static_cast<int32_t>(np);
+                    b2.n = qi;
+                    gl2.add(b2);
+                }
+                plan.gemm(gu, false);
+                addChildUpdates(nodes, false);
+                plan.gemm(gl1, false);
+                plan.gemm(gl2, false);
+            }
+            // ---- backward: Y = w, then Y(p) -= X_p Y(B_p) top-down
+            Copy2dBatch cw;
+            for (int64_t p = 0; p < N; ++p)
+                cw.add(cdesc(T + to[p], S.f(p), Yv + S.off[p], n, S.size[p], q, 0));
+            plan.copy(cw);
+            addBacksolve(Yv);
+            addSpmm(plan, opPM, Yv, MY, q, Tr1, Tr2);   // MY = M Y
+            // scale every column of Y and M Y to unit M-norm (per problem): the
+            // next step applies K^-1 to (M Y) S, the Rayleigh-Ritz at the end
+            // projects onto Y S.  Scaling only changes the basis of the same
+            // subspace, but the columns turn towards the dominant modes: on the
+            // whole AMLS basis cond(Y^T M Y) grows by 1e3 - 1e4 per step, so
+            // after every second step that is not the last one the basis is
+            // M-orthonormalised (below), and the projected pencil of the final
+            // Rayleigh-Ritz stays well conditioned however many steps run.
+            // Y and M Y are read scaled only by the final Rayleigh-Ritz and the
+            // Cholesky QR; before another plain step only the next step's copy
+            // of M Y's own rows needs the factors, and it applies them itself
+            // (the same product per element as scaling M Y first), while Y is
+            // overwritten unread
+            const int64_t* seg = dSegRow;
+            const int nsg = static_cast<int>(NS);
+            double* nrmS = dsq;
+            const bool chol = !last && (it + 1) % 2 == 0;
+            if (last || chol) {
+                plan.fn([=](hipStream_t s) { launch_colnorm_seg(seg, nsg, MY, Yv, qi, nn, nrmS, s, true); });
+            } else {
+                plan.fn([=](hipStream_t s) { launch_colnorm_factors(seg, nsg, MY, Yv, qi, nn, nrmS, fac, s); });
+                pend = fac;
+            }
+            if (chol) {
+                // Cholesky QR in the M inner product, per problem:
+                // G = Y^T M Y = L D L^T, Y <- Y L^-T D^-1/2 (through Q), M Y again
+                GemmBatch gg, gy;
+                Copy2dBatch cz0, cy;
+                std::vector<VecDesc> vid0(NS), vdq(NS);
+                TrsmBatch ti;
+                for (int64_t sg = 0; sg < NS; ++sg) {
+                    const auto [r0, ns] = rowsOf(sg);
+                    double *g = KcS[sg], *li = McS[sg], *ds = dsq + sg * q;
+                    GemmDesc d = gdesc();
+                    d.A = Yv + r0;
+                    d.lda = n;
+                    d.ta = 1;
+                    d.B = MY + r0;
+                    d.ldb = n;
+                    d.k = static_cast<int32_t>(ns);
+                    d.C = g;
+                    d.ldc = q;
+                    d.m = d.n = qi;
+                    d.lower = 1;
+                    gg.add(d);
+                    cz0.add(cdesc(nullptr, 0, li, q, q, q, 3, nullptr, nullptr, 0.0));
+                    std::memset(&vid0[sg], 0, sizeof(VecDesc));
+                    vid0[sg].dst = li;
+                    vid0[sg].ld = q;
+                    vid0[sg].n = qi;
+                    vid0[sg].op = 2;
+                    ti.add(g, q, li, q, q, q, 0);
+                    ti.descs.back().rhsLower = 1;
+                    std::memset(&vdq[sg], 0, sizeof(VecDesc));
+                    vdq[sg].src = g;
+                    vdq[sg].dst = ds;
+                    vdq[sg].ld = q;
+                    vdq[sg].n = qi;
+                    vdq[sg].op = 0;
+                    GemmDesc e = gdesc();               // Q = Y L^-T
+                    e.A = Yv + r0;
+                    e.lda = n;
+                    e.B = li;
+                    e.ldb = q;
+                    e.tb = 1;
+                    e.k = qi;
+                    e.C = Q + r0;
+                    e.ldc = n;
+                    e.m = static_cast<int32_t>(ns);
+                    e.n = qi;
+                    gy.add(e);
+                    cy.add(cdesc(Q + r0, n, Yv + r0, n, ns, q, 0, nullptr, ds));   // Y = Q D^-1/2
+                }
+                plan.gemm(gg, false);
+                addLdltBatch(plan, KcS, std::vector<int64_t>(NS, q), WpS, err,
+                             std::vector<int>(NS, -2));
+                plan.copy(cz0);
+                plan.vec(vid0);
+                plan.trsm(ti);
+                plan.vec(vdq);
+                plan.gemm(gy, false);
+                plan.copy(cy);
+                addSpmm(plan, opPM, Yv, MY, q, Tr1, Tr2);
+            }
+        }
+        // ---- final Rayleigh-Ritz with the original FP64 K and M, per problem:
+        // Kc = Y^T K Y, Mc = Y^T M Y; lowest nev pairs of (Kc, Mc) through
+        // Kc = L D L^T, C = D^-1/2 L^-1 Mc L^-T D^-1/2, nu = 1/theta.  L^-1 is
+        // formed once (one blocked triangular solve on the identity, into the
+        // storage of Mc after Mc is copied out), and the two congruence products
+        // and the back-transformation of the eigenvectors are GEMMs with it.
+        addSpmm(plan, opPK, Yv, R, q, Tr1, Tr2);
+        {
+            GemmBatch gr;
+            for (int64_t sg = 0; sg < NS; ++sg) {
+                const auto [r0, ns] = rowsOf(sg);
+                GemmDesc g1 = gdesc();
+                g1.A = Yv + r0;
+                g1.lda = n;
+                g1.ta = 1;
+                g1.B = R + r0;
+                g1.ldb = n;
+                g1.k = static_cast<int32_t>(ns);
+                g1.C = KcS[sg];
+                g1.ldc = q;
+                g1.m = g1.n = qi;
+                g1.lower = 1;
+                GemmDesc g2 = g1;
+                g2.B = MY + r0;
+                g2.C = McS[sg];
+                if (shift != 0.0) {            // (K + shift M): Kc += shift Y^T M Y
+                    g1.A2 = Yv + r0;
+                    g1.lda2 = n;
+                    g1.ta2 = 1;
+                    g1.B2 = MY + r0;
+                    g1.ldb2 = n;
+                    g1.k2 = static_cast<int32_t>(ns);
+                    g1.alpha2 = shift;
+                }
+                gr.add(g1);
+                gr.add(g2);
+            }
+            plan.gemm(gr, false);
+        }
+        addLdltBatch(plan, KcS, std::vector<int64_t>(KcS.size(), q), WpS, err,
+                     std::vector<int>(KcS.size(), -1));
+        plan.copy(c1);
+        plan.copy(cz);
+        plan.vec(vid);
+        plan.trsm(tinv);
+        plan.gemm(gl1, false);
+        plan.gemm(gl2, false);
+        plan.vec(vd);
+        plan.copy(c3);
+        dropSiGraph();
+        if (!eagerOnly && sameKey)
+            siGraph = captureAndLaunch(plan, stream);
+        else
+            plan.run(stream);
+        siKey = std::move(key);
+        siKeySeen = true;
+    }
+    // part 2: the nev largest eigenpairs of C by the Chebyshev-filtered
+    // subspace iteration for one wide problem (q = p), the dense reduction
+    // otherwise or when it does not converge; the output block
+    Plan plan(&fresh(siBlob2), &kstats);
     bool dense = true;
     lastRrIters = 0;
     lastRrResidual = 0.0;
     if (NS == 1 && rrMode == 0 && chfsiBlock(q, nev) > 0) {
-        plan.run(stream);
         dense = !chfsiTop(Cm, q, nev, lam, Yt);
-        plan.steps.clear();
-        plan.blob = &fresh(siBlob2);
         if (dense) {
             lastRrIters = 0;
             lastRrResidual = 0.0;
@@ -3020,6 +3085,7 @@ amls_status amls_hip_destroy(amls_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->c.stream);
         (void)hipStreamSynchronize(ctx->c.stream2);
         ctx->c.dropGeGraph();
+        ctx->c.dropSiGraph();
     }
     hipStream_t s = ctx->c.stream, s2 = ctx->c.stream2;
     hipEvent_t e1 = ctx->c.evFork, e2 = ctx->c.evJoin;

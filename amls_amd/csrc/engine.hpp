@@ -333,6 +333,28 @@ struct Ctx {
     int64_t outElems = 0;        // doubles outBuf holds (n x nev of the solve that sized it)
     double* outBuf = nullptr;
     std::unique_ptr<Blob> siBlob, siBlob2;
+    // The steps and the set-up of the final Rayleigh-Ritz problem as a HIP
+    // graph, keyed by everything their launch sequence depends on: the tree
+    // (problemGen), q, nev, the step count, the start, the shift, the node
+    // mode counts of a whole-basis start, and the addresses of the buffers
+    // the launches touch.  First run eager, second captures, later replay.
+    struct SiKey {
+        uint64_t gen = 0;
+        int64_t q = -1, nev = -1;
+        int steps = 0;
+        bool fromBasis = false;
+        double shift = 0.0;
+        std::vector<const void*> ptrs;
+        std::vector<int64_t> k;
+        bool operator==(const SiKey& o) const {
+            return gen == o.gen && q == o.q && nev == o.nev && steps == o.steps &&
+                   fromBasis == o.fromBasis && shift == o.shift && ptrs == o.ptrs && k == o.k;
+        }
+    };
+    SiKey siKey;
+    bool siKeySeen = false;
+    hipGraphExec_t siGraph = nullptr;
+    void dropSiGraph();
 
     // ---- top eigenpairs of the final Rayleigh-Ritz problem by Chebyshev-
     // filtered subspace iteration (chfsi.cpp); the dense path stays the

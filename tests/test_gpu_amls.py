@@ -263,6 +263,47 @@ def test_ge_graph_replay_matches_eager():
     ctx.close()
 
 
+@pytest.mark.parametrize("q", [0, 20])
+def test_subspace_graph_replay_matches_eager(blas1, q):
+    """The subspace iteration's steps and the set-up of its final
+    Rayleigh-Ritz problem run eagerly the first time for a key, are captured
+    as a HIP graph the second time and replayed after that.  Every captured
+    and replayed solve, with other device work between them, gives the eager
+    (profiled) solve's eigenvalues and vectors to rounding (the split-k Gram
+    products add in any order); changing the width or the problem builds
+    again.  q = 0 starts from the whole AMLS basis, q = 20 from recovered Ritz
+    vectors."""
+    pb = small_hex(seed=8, dims=(4, 5, 6))
+    ctx = HipContext(0)
+    ctx.set_problem(pb.K, pb.M, pb.coords, 3, 40)
+    opt = AmlsOptions(nev=8, nLevels=3, leafSize=40, thresholdFactor=3.0, siSteps=3,
+                      siVectors=q, precision="mixed", verbosity=0)
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((60, 40))
+    B = rng.standard_normal((40, 30))
+    ctx.profile(True)
+    th0, Z0, p0, _ = ctx.solve_numeric(opt, vectors=True)
+    ctx.profile(False)
+    for rep in range(4):              # rep 0 captures, 1..3 replay
+        th, Z, p, _ = ctx.solve_numeric(opt, vectors=True)
+        assert p == p0
+        assert np.abs(th - th0).max() <= 1e-11 * th0.max(), rep
+        s = np.sign(np.einsum("ij,ij->j", Z, pb.M @ Z0))
+        assert np.abs(Z * s - Z0).max() <= 1e-8 * np.abs(Z0).max(), rep
+        C = ctx.gemm(A, B)            # eager work between replays
+        assert np.abs(C - A @ B).max() <= 1e-12 * np.abs(A @ B).max()
+    # another width, then a new problem of the same sizes: built again
+    th1 = ctx.solve_numeric(AmlsOptions(**{**opt.__dict__, "nev": 6}))[0]
+    assert np.abs(th1 - th0[:6]).max() <= 1e-6 * th0.max()
+    pb2 = small_hex(seed=9, dims=(4, 5, 6))
+    ctx.set_problem(pb2.K, pb2.M, pb2.coords, 3, 40)
+    ref = AmlsSolver(0).solve(Pencil(pb2.K, pb2.M, pb2.coords), opt)
+    for rep in range(3):
+        th2 = ctx.solve_numeric(opt)[0]
+        assert np.abs(th2 - ref.theta).max() <= 1e-11 * ref.theta.max(), rep
+    ctx.close()
+
+
 def test_c2_subspace_iteration_matches_oracle():
     """The benchmark workload at full size (27-node hexahedra, 10^3 elements,
     n = 26,460, lowest 100, eta = 4): plain FP64 AMLS matches the oracle's

@@ -66,8 +66,9 @@ def test_filtered_rr_matches_dense_and_oracle_medium(blas1):
 def test_filtered_rr_c2_matches_dense():
     """The benchmark workload (n = 26,460, lowest 100, eta = 4, two steps,
     q = p = 1,499, block 192), mixed precision: filtered and dense final
-    Rayleigh-Ritz give the same eigenvalues (1e-10) and vectors of the same
-    quality, both within 1e-6 of the FP64 ground-truth fixture."""
+    Rayleigh-Ritz give the same eigenvalues (within the filter's residual
+    rule) and vectors of the same quality, both within 1e-6 of the FP64
+    ground-truth fixture."""
     pb = cases.case_hex27_10()
     exact = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "hex27_10_lowest100.txt"))
     ctx = HipContext(0)
@@ -78,7 +79,11 @@ def test_filtered_rr_c2_matches_dense():
     th_f, Z_f, _, info_f = _solve(ctx, pb, opt, 0)
     assert p == 1499 and info_d[0] == 0
     assert 1 <= info_f[0] <= 14 and info_f[1] <= 1e-11
-    assert np.abs(th_f - th_d).max() <= 1e-10 * th_d.max()
+    # the filter stops at residuals 1e-11 nu_1 of C (nu = 1 / theta): an
+    # eigenvalue error of at most 1e-11 nu_1 in nu, 1e-11 theta_j^2 / theta_1
+    # in theta (6e-10 at the top of this spectrum); the dense path is exact
+    # to rounding
+    assert np.all(np.abs(th_f - th_d) <= 1e-11 * th_d ** 2 / th_d[0] + 1e-14)
     assert O.rel_err(th_f, exact).max() <= 1e-6
     r_f, r_d = _resid(pb, th_f, Z_f), _resid(pb, th_d, Z_d)
     # the two small-problem solvers leave eigenvector errors of ~1e-11 / gap:
