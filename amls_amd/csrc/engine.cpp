@@ -2410,12 +2410,9 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     }
     if (nev > q) throw StatusError(AMLS_E_ARG, "nev exceeds the subspace width");
     const int64_t N = S.nNodes;
-    std::vector<int64_t> to(N + 1, 0), zbo(N + 1, 0);
+    std::vector<int64_t> zbo(N + 1, 0);   // boundary blocks b_p x q of the forward solve
     int64_t zbLevel = 0;
-    for (int64_t p = 0; p < N; ++p) {
-        to[p + 1] = to[p] + S.f(p) * q;
-        zbo[p + 1] = zbo[p] + S.b(p) * q;
-    }
+    for (int64_t p = 0; p < N; ++p) zbo[p + 1] = zbo[p] + S.b(p) * q;
     // gathered x_B blocks are only live within a level: one buffer per level
     std::vector<int64_t> zbl(N, 0);
     for (int64_t l = 0; l <= S.maxLevel; ++l) {
@@ -2427,8 +2424,8 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
         zbLevel = std::max(zbLevel, acc);
     }
     auto& A = fresh(siMem);
-    A.plan(n * q * 8 * 6);                 // Q, R, Y, MY, two transposes
-    A.plan(to[N] * 8);                     // fronts of the forward solve
+    A.plan(n * q * 8 * 5);                 // Q, two step blocks, two transposes
+    A.plan(zbo[N] * 8);                    // boundary blocks of the forward solve
     A.plan(zbLevel * 8);                   // gathered x_B of one level
     const int64_t NS = nseg;               // one q x q problem per segment
     A.plan(NS * q * q * 8 * 4);            // Kc, Mc, C, T
@@ -2436,16 +2433,18 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     A.plan(NS * q * 8 * 13);               // eig vectors
     A.plan(NS * q * nev * 8 * 2);          // Yt, V
     A.plan(NS * nev * 6 * q * 8);          // inverse-iteration scratch
-    A.plan(NS * q * 8);                    // column factors of a deferred scaling
     A.plan(256);
     A.commit();
     double* Q = A.take<double>(n * q);
+    // the steps alternate between two blocks: step `it` turns R (even) or
+    // MYb (odd) into Y in place and puts M Y into the other one
     double* R = A.take<double>(n * q);
-    double* Yv = A.take<double>(n * q);
-    double* MY = A.take<double>(n * q);
+    double* MYb = A.take<double>(n * q);
+    double* const Yv = (steps - 1) % 2 == 0 ? R : MYb;   // Y and M Y of the last step
+    double* const MY = (steps - 1) % 2 == 0 ? MYb : R;
     double* Tr1 = A.take<double>(n * q);
     double* Tr2 = A.take<double>(n * q);
-    double* T = A.take<double>(to[N]);
+    double* Tb = A.take<double>(zbo[N]);
     double* zB = A.take<double>(zbLevel);
     double* Kc = A.take<double>(NS * q * q);
     double* Mc = A.take<double>(NS * q * q);
@@ -2464,7 +2463,6 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     double* Yt = A.take<double>(NS * q * nev);
     double* V = A.take<double>(NS * q * nev);
     double* wk = A.take<double>(NS * nev * 6 * q);
-    double* fac = A.take<double>(NS * q);
     const int64_t qq = q * q;
     std::vector<double*> McS, KcS, WpS;
     for (int64_t sg = 0; sg < NS; ++sg) {
@@ -2653,33 +2651,42 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
         }
         addSpmm(plan, opPM, Q, R, q, Tr1, Tr2);     // R = M Q
 
-        // Children's updates into the parent fronts of the forward solves.  A
-        // child's relative index list is monotone, so the rows of its update
-        // that land in the parent's own block are a prefix: those are added
-        // first (own = true, before the parent's y_p is used), the rest after
-        // the parent's u_p = -X^T y_p has initialised its boundary block
-        // (beta = 0), so T needs no clearing.  Lists found unsorted fall back to
-        // clearing T and adding every row first.
-        bool relSorted = true;
-        for (int64_t c = 0; c < N && relSorted; ++c)
-            relSorted = std::is_sorted(S.rel.begin() + S.bptr[c], S.rel.begin() + S.bptr[c] + S.b(c));
-        auto addChildUpdates = [&](const std::vector<int64_t>& nodes, bool own) {
+        // The forward solve works in place: the own rows of node p are the
+        // rows of the step's block X (R in the first step, the scaled M Y of
+        // the previous step after that), read nowhere else afterwards; only
+        // the boundary blocks (b_p x q, the updates for the ancestors) live in
+        // Tb.  A child's relative index list is monotone, so the rows of its
+        // update that land in the parent's own block are a prefix: those are
+        // added first (own = true, before the parent's y_p is used), the rest
+        // after the parent's u_p = -X^T y_p has initialised its boundary block
+        // (beta = 0), so no block needs clearing.
+        for (int64_t c = 0; c < N; ++c)
+            if (!std::is_sorted(S.rel.begin() + S.bptr[c], S.rel.begin() + S.bptr[c] + S.b(c)))
+                throw StatusError(AMLS_E_INTERNAL, "relative index list of node " +
+                                                       std::to_string(c) + " is not monotone");
+        auto addChildUpdates = [&](const std::vector<int64_t>& nodes, bool own, double* X) {
             for (int slot = 0; slot < 2; ++slot) {
                 ExtBatch eb;
                 for (int64_t p : nodes) {
                     const int64_t c = slot == 0 ? S.child0[p] : S.child1[p];
                     if (c < 0 || S.b(c) == 0) continue;
-                    const int64_t bc = S.b(c);
+                    const int64_t bc = S.b(c), np = S.size[p];
                     const int64_t* r = S.rel.data() + S.bptr[c];
-                    const int64_t split = relSorted ? std::lower_bound(r, r + bc, S.size[p]) - r : bc;
+                    const int64_t split = std::lower_bound(r, r + bc, np) - r;
                     const int64_t r0 = own ? 0 : split, r1 = own ? split : bc;
                     if (r1 <= r0) continue;
                     ExtAddDesc x;
                     std::memset(&x, 0, sizeof(x));
-                    x.src = T + to[c] + S.size[c] + r0;
-                    x.lds = S.f(c);
-                    x.dst = T + to[p];
-                    x.ldd = S.f(p);
+                    x.src = Tb + zbo[c] + r0;
+                    x.lds = bc;
+                    if (own) {                      // front rows [0, n_p): X's rows of p
+                        x.dst = X + S.off[p];
+                        x.ldd = n;
+                    } else {                        // front rows [n_p, f_p): p's boundary block
+                        x.dst = Tb + zbo[p];
+                        x.ldd = S.b(p);
+                        x.rowoff = static_cast<int32_t>(np);
+                    }
                     x.rowmap = drel + S.bptr[c] + r0;
                     x.rows = static_cast<int32_t>(r1 - r0);
                     x.cols = qi;
@@ -2688,43 +2695,33 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                 plan.ext(eb);
             }
         };
-        // the column factors the previous step's M Y still has to be scaled by
-        // (null: M Y is scaled already)
-        const double* pend = nullptr;
         for (int it = 0; it < steps; ++it) {
             const bool last = it + 1 == steps;
-            // ---- forward: fronts take their own rows of M Q (R in the first
-            // step, the scaled M Y of the previous step after that), children's
-            // updates
-            const double* src = it == 0 ? R : MY;
-            if (!relSorted) plan.fn([=](hipStream_t s) { launch_fill(T, to[S.nNodes], 0.0, s); });
-            Copy2dBatch cown;
-            for (int64_t p = 0; p < N; ++p)
-                cown.add(cdesc(src + S.off[p], n, T + to[p], S.f(p), S.size[p], q, 0, nullptr,
-                               pend ? pend + nodeSeg[p] * q : nullptr));
-            pend = nullptr;
-            plan.copy(cown);
+            // the step's block (it becomes Y in place) and where M Y goes
+            double* X = it % 2 == 0 ? R : MYb;
+            double* MYn = it % 2 == 0 ? MYb : R;
+            // ---- forward: children's updates, then u_p = T_B - X^T y_p and
+            // w_p = L^-T D^-1 L^-1 y_p in place (explicit L^-1: two GEMMs
+            // through the scratch Tr1, D^-1 applied in the first one's store)
             for (int64_t l = 0; l <= S.maxLevel; ++l) {
                 const auto& nodes = S.byLevel[l];
-                addChildUpdates(nodes, true);
-                // u_p = T_B - X^T y_p, then w_p = L^-T D^-1 L^-1 y_p in place
-                // (explicit L^-1: two GEMMs through the scratch Tr1)
+                addChildUpdates(nodes, true, X);
                 GemmBatch gu, gl1, gl2;
                 for (int64_t p : nodes) {
-                    const int64_t np = S.size[p], f = S.f(p), b = S.b(p);
-                    double* Tp = T + to[p];
+                    const int64_t np = S.size[p], b = S.b(p);
+                    double* Xp = X + S.off[p];
                     if (b > 0) {
                         GemmDesc g = gdesc();
                         g.A = dXb + xo[p];
                         g.lda = np;
                         g.ta = 1;
-                        g.B = Tp;
-                        g.ldb = f;
+                        g.B = Xp;
+                        g.ldb = n;
                         g.k = static_cast<int32_t>(np);
                         g.alpha1 = -1.0;
-                        g.beta = relSorted ? 0.0 : 1.0;   // T_B is not cleared when sorted
-                        g.C = Tp + np;
-                        g.ldc = f;
+                        g.beta = 0.0;
+                        g.C = Tb + zbo[p];
+                        g.ldc = b;
                         g.m = static_cast<int32_t>(b);
                         g.n = qi;
                         gu.add(g);
@@ -2733,8 +2730,8 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                     a.A = dLinv + co[p];
                     a.lda = np;
                     a.triA = 1;
-                    a.B = Tp;
-                    a.ldb = f;
+                    a.B = Xp;
+                    a.ldb = n;
                     a.k = static_cast<int32_t>(np);
                     a.C = Tr1 + S.off[p];
                     a.ldc = n;
@@ -2750,47 +2747,37 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                     b2.B = Tr1 + S.off[p];
                     b2.ldb = n;
                     b2.k = static_cast<int32_t>(np);
-                    b2.C = Tp;
-                    b2.ldc = f;
+                    b2.C = Xp;
+                    b2.ldc = n;
                     b2.m = static_cast<int32_t>(np);
                     b2.n = qi;
                     gl2.add(b2);
                 }
                 plan.gemm(gu, false);
-                addChildUpdates(nodes, false);
+                addChildUpdates(nodes, false, X);
                 plan.gemm(gl1, false);
                 plan.gemm(gl2, false);
             }
-            // ---- backward: Y = w, then Y(p) -= X_p Y(B_p) top-down
-            Copy2dBatch cw;
-            for (int64_t p = 0; p < N; ++p)
-                cw.add(cdesc(T + to[p], S.f(p), Yv + S.off[p], n, S.size[p], q, 0));
-            plan.copy(cw);
-            addBacksolve(Yv);
-            addSpmm(plan, opPM, Yv, MY, q, Tr1, Tr2);   // MY = M Y
-            // scale every column of Y and M Y to unit M-norm (per problem): the
-            // next step applies K^-1 to (M Y) S, the Rayleigh-Ritz at the end
-            // projects onto Y S.  Scaling only changes the basis of the same
-            // subspace, but the columns turn towards the dominant modes: on the
-            // whole AMLS basis cond(Y^T M Y) grows by 1e3 - 1e4 per step, so
-            // after every second step that is not the last one the basis is
-            // M-orthonormalised (below), and the projected pencil of the final
-            // Rayleigh-Ritz stays well conditioned however many steps run.
-            // Y and M Y are read scaled only by the final Rayleigh-Ritz and the
-            // Cholesky QR; before another plain step only the next step's copy
-            // of M Y's own rows needs the factors, and it applies them itself
-            // (the same product per element as scaling M Y first), while Y is
-            // overwritten unread
+            // ---- backward in place: Y(p) -= X_p Y(B_p) top-down
+            addBacksolve(X);
+            addSpmm(plan, opPM, X, MYn, q, Tr1, Tr2);   // M Y
+            // scale every column of M Y (and of Y where it is read scaled) to
+            // unit M-norm, per problem: the next step applies K^-1 to (M Y) S,
+            // the Rayleigh-Ritz at the end projects onto Y S.  Scaling only
+            // changes the basis of the same subspace, but the columns turn
+            // towards the dominant modes: on the whole AMLS basis cond(Y^T M Y)
+            // grows by 1e3 - 1e4 per step, so after every second step that is
+            // not the last one the basis is M-orthonormalised (below), and the
+            // projected pencil of the final Rayleigh-Ritz stays well
+            // conditioned however many steps run.  Y is read scaled only by
+            // the final Rayleigh-Ritz and the Cholesky QR; before another plain
+            // step the next step overwrites it unread.
             const int64_t* seg = dSegRow;
             const int nsg = static_cast<int>(NS);
             double* nrmS = dsq;
             const bool chol = !last && (it + 1) % 2 == 0;
-            if (last || chol) {
-                plan.fn([=](hipStream_t s) { launch_colnorm_seg(seg, nsg, MY, Yv, qi, nn, nrmS, s, true); });
-            } else {
-                plan.fn([=](hipStream_t s) { launch_colnorm_factors(seg, nsg, MY, Yv, qi, nn, nrmS, fac, s); });
-                pend = fac;
-            }
+            const bool scaleY = last || chol;
+            plan.fn([=](hipStream_t s) { launch_colnorm_seg(seg, nsg, MYn, X, qi, nn, nrmS, s, scaleY); });
             if (chol) {
                 // Cholesky QR in the M inner product, per problem:
                 // G = Y^T M Y = L D L^T, Y <- Y L^-T D^-1/2 (through Q), M Y again
@@ -2802,10 +2789,10 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                     const auto [r0, ns] = rowsOf(sg);
                     double *g = KcS[sg], *li = McS[sg], *ds = dsq + sg * q;
                     GemmDesc d = gdesc();
-                    d.A = Yv + r0;
+                    d.A = X + r0;
                     d.lda = n;
                     d.ta = 1;
-                    d.B = MY + r0;
+                    d.B = MYn + r0;
                     d.ldb = n;
                     d.k = static_cast<int32_t>(ns);
                     d.C = g;
@@ -2828,7 +2815,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                     vdq[sg].n = qi;
                     vdq[sg].op = 0;
                     GemmDesc e = gdesc();               // Q = Y L^-T
-                    e.A = Yv + r0;
+                    e.A = X + r0;
                     e.lda = n;
                     e.B = li;
                     e.ldb = q;
@@ -2839,7 +2826,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                     e.m = static_cast<int32_t>(ns);
                     e.n = qi;
                     gy.add(e);
-                    cy.add(cdesc(Q + r0, n, Yv + r0, n, ns, q, 0, nullptr, ds));   // Y = Q D^-1/2
+                    cy.add(cdesc(Q + r0, n, X + r0, n, ns, q, 0, nullptr, ds));   // Y = Q D^-1/2
                 }
                 plan.gemm(gg, false);
                 addLdltBatch(plan, KcS, std::vector<int64_t>(NS, q), WpS, err,
@@ -2850,16 +2837,15 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                 plan.vec(vdq);
                 plan.gemm(gy, false);
                 plan.copy(cy);
-                addSpmm(plan, opPM, Yv, MY, q, Tr1, Tr2);
+                addSpmm(plan, opPM, X, MYn, q, Tr1, Tr2);
             }
         }
-        // ---- final Rayleigh-Ritz with the original FP64 K and M, per problem:
         // Kc = Y^T K Y, Mc = Y^T M Y; lowest nev pairs of (Kc, Mc) through
         // Kc = L D L^T, C = D^-1/2 L^-1 Mc L^-T D^-1/2, nu = 1/theta.  L^-1 is
         // formed once (one blocked triangular solve on the identity, into the
         // storage of Mc after Mc is copied out), and the two congruence products
         // and the back-transformation of the eigenvectors are GEMMs with it.
-        addSpmm(plan, opPK, Yv, R, q, Tr1, Tr2);
+        addSpmm(plan, opPK, Yv, Q, q, Tr1, Tr2);   // K Y (Q is free until X = Y V)
         {
             GemmBatch gr;
             for (int64_t sg = 0; sg < NS; ++sg) {
@@ -2868,7 +2854,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                 g1.A = Yv + r0;
                 g1.lda = n;
                 g1.ta = 1;
-                g1.B = R + r0;
+                g1.B = Q + r0;
                 g1.ldb = n;
                 g1.k = static_cast<int32_t>(ns);
                 g1.C = KcS[sg];

@@ -78,6 +78,8 @@ struct ExtAddDesc {
     int32_t rows, cols;
     int32_t coloff;          // used when symmetric == 0
     int32_t symmetric;       // 1: square src, colmap = rowmap, lower part only
+    int32_t rowoff;          // subtracted from every rowmap entry (dst row = rowmap[s] - rowoff)
+    int32_t pad;
 };
 
 // Symmetric tridiagonal eigen-problem bookkeeping for one dense matrix.

@@ -125,13 +125,13 @@ __global__ void __launch_bounds__(256) extadd_kernel(const ExtAddDesc* __restric
     const int t = threadIdx.x;
     const int s = w.y * 64 + (t & 63);
     if (s >= d.rows) return;
-    const int64_t ds = d.rowmap ? d.rowmap[s] : s;
+    const int64_t ds = (d.rowmap ? d.rowmap[s] : s) - d.rowoff;
     for (int q = 0; q < 16; ++q) {
         const int c = w.z * 64 + (t >> 6) + 4 * q;
         if (c >= d.cols) break;
         if (d.symmetric) {
             if (c > s) continue;
-            const int64_t dc = d.rowmap ? d.rowmap[c] : c;
+            const int64_t dc = (d.rowmap ? d.rowmap[c] : c) - d.rowoff;
             d.dst[ds + dc * d.ldd] += d.src[s + (int64_t)c * d.lds];
         } else {
             d.dst[ds + (int64_t)(d.coloff + c) * d.ldd] += d.src[s + (int64_t)c * d.lds];

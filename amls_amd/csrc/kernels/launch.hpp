@@ -100,11 +100,6 @@ void launch_coldot(int64_t n, const double* x, const double* y, int ncols, int64
 // too if scaleY); nrm is nseg x ncols
 void launch_colnorm_seg(const int64_t* seg, int nseg, double* x, double* y, int ncols,
                         int64_t ld, double* nrm, hipStream_t s, bool scaleY = false);
-// nrm as launch_colnorm_seg computes it, and the factors it would scale the
-// columns by, problem-major: fac[s ncols + c] = 1 / sqrt(max(nrm[s + c nseg],
-// 1e-300)); nothing is scaled
-void launch_colnorm_factors(const int64_t* seg, int nseg, const double* x, const double* y,
-                            int ncols, int64_t ld, double* nrm, double* fac, hipStream_t s);
 // out[s ncols + c] = x_s(:, c)^T y_s(:, c) over rows [seg[s], seg[s+1])
 void launch_coldot_seg(const int64_t* seg, int nseg, const double* x, const double* y, int ncols,
                        int64_t ld, double* out, hipStream_t s);

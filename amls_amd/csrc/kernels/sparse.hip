@@ -277,16 +277,6 @@ __global__ void __launch_bounds__(256) colscale_seg_kernel(const int64_t* __rest
     for (int64_t r = seg[sg] + threadIdx.x; r < seg[sg + 1]; r += 256) x[r + c * ld] *= sc;
 }
 
-// the factors colscale_seg_kernel applies, stored problem-major:
-// fac[s ncols + c] = 1 / sqrt(max(nrm[s + c nseg], 1e-300))
-__global__ void colscale_factors_kernel(const double* __restrict__ nrm, int nseg, int ncols,
-                                        double* __restrict__ fac) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (int64_t)nseg * ncols) return;
-    const int64_t sg = i / ncols, c = i - sg * ncols;
-    fac[i] = 1.0 / sqrt(fmax(nrm[sg + c * nseg], 1e-300));
-}
-
 __global__ void colscale_kernel(int64_t n, double* __restrict__ x, int64_t ld,
                                 const double* __restrict__ nrm) {
     const int c = blockIdx.y;
@@ -347,16 +337,6 @@ void launch_colnorm_seg(const int64_t* seg, int nseg, double* x, double* y, int 
     if (scaleY)
         hipLaunchKernelGGL(dev::colscale_seg_kernel, dim3(nseg, ncols), dim3(256), 0, s, seg, nseg,
                            y, ld, nrm);
-}
-
-void launch_colnorm_factors(const int64_t* seg, int nseg, const double* x, const double* y,
-                            int ncols, int64_t ld, double* nrm, double* fac, hipStream_t s) {
-    if (ncols <= 0 || nseg <= 0) return;
-    hipLaunchKernelGGL(dev::coldot_seg_kernel, dim3(nseg, ncols), dim3(256), 0, s, seg, nseg, x, y,
-                       ld, nrm, 0);
-    const unsigned nb = (unsigned)(((int64_t)nseg * ncols + 255) / 256);
-    hipLaunchKernelGGL(dev::colscale_factors_kernel, dim3(nb), dim3(256), 0, s, nrm, nseg, ncols,
-                       fac);
 }
 
 void launch_coldot_seg(const int64_t* seg, int nseg, const double* x, const double* y, int ncols,
