@@ -860,6 +860,7 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
     if (leafSize < 1) throw StatusError(AMLS_E_ARG, "leafSize must be positive");
     ++problemGen;
     if (gpu) {
+        drainChain();   // a pending root reduction still reads the old arenas
         dropGeGraph();
         dropSiGraph();
         siKeySeen = false;
@@ -1362,10 +1363,7 @@ void Ctx::runGE(int prec) {
             double pf = 0.0, pb = 0.0;
             for (int32_t q : list) {
                 const int64_t p = nodes[q];
-                const double jb = static_cast<double>(std::min<int64_t>(64, S.size[p] - s0));
-                const double rows = static_cast<double>(S.f(p) - s0 - jb);
-                pf += jb * jb * jb / 3.0 + rows * jb * jb;
-                pb += (jb * jb + 3.0 * rows * jb) * 8.0;
+                ldltPanelWork(S.size[p], s0, pf, pb, S.f(p));
             }
             plan.timed("ldlt_panel", pf, pb, [=](hipStream_t s) {
                 launch_ldlt_panel(bl->at<FrontDesc>(ofd), bl->at<int32_t>(ol), cnt, s0i, err,
@@ -1553,17 +1551,60 @@ void Ctx::finishGE() {
 // ---------------------------------------------------------------------------
 // Local standard-form pencils and their tridiagonal reduction
 // ---------------------------------------------------------------------------
+bool Ctx::silEligible(int64_t p) const {
+    if (silMode < 0 || statHost.size() < static_cast<size_t>(2 * (p + 1))) return false;
+    const int64_t n = S.size[p];
+    const double nnz = statHost[2 * p];
+    const bool use = silMode > 0 || (n >= 200 && nnz < 0.15 * double(n) * double(n));
+    return use && n >= 1 && n <= lanczos_max_order();
+}
+
+void Ctx::joinChain() {
+    joinBatch();
+    if (!chainPending) return;
+    AMLS_HIP_CHECK(hipStreamWaitEvent(stream, evChain, 0));
+    chainPending = false;
+}
+
+void Ctx::joinBatch() {
+    if (!batchPending) return;
+    AMLS_HIP_CHECK(hipStreamWaitEvent(stream, evBatch, 0));
+    batchPending = false;
+}
+
+void Ctx::drainChain() {
+    if (batchPending) {
+        AMLS_HIP_CHECK(hipStreamSynchronize(stream3));
+        batchPending = false;
+    }
+    if (!chainPending) return;
+    AMLS_HIP_CHECK(hipStreamSynchronize(stream2));
+    chainPending = false;
+}
+
 void Ctx::spectralPrepare() {
     requireGpu();
     kstats.phase = "spectral";
+    drainChain();
     sync();   // earlier phases may still use buffers released below
     if (!geDone) throw StatusError(AMLS_E_STATE, "spectral_prepare before ge");
     specReady = selected = projected = reduced = recovered = false;
     const int64_t N = S.nNodes;
+    // a solve's pass A may take the largest node's (the root separator's)
+    // few modes from a copy of its C by the Chebyshev filter while that
+    // node's reduction runs on stream2 until pass B needs it
+    int64_t big = 0;
+    for (int64_t p = 1; p < N; ++p)
+        if (S.size[p] > S.size[big]) big = p;
+    const int64_t nbig = N > 0 ? S.size[big] : 0;
+    const bool overlapWanted = overlapK0 > 0 && N > 1 && stream2 && stream3 && nbig >= 512 &&
+                               !silEligible(big) &&
+                               chfsiBlock(nbig, std::min<int64_t>(overlapK0, nbig)) > 0;
     auto& A = fresh(spMem);
     A.plan(co[N] * 8 * 2);
     A.plan(vo[N] * 8 * 3);
     A.plan(vo[N] * 8 * 5);
+    if (overlapWanted) A.plan(nbig * nbig * 8);
     A.commit();
     dC = A.take<double>(co[N]);
     dT = A.take<double>(co[N]);
@@ -1572,6 +1613,7 @@ void Ctx::spectralPrepare() {
     dTau = A.take<double>(vo[N]);
     dSv = A.take<double>(2 * vo[N]);
     dSp = A.take<double>(3 * vo[N]);
+    dCroot = overlapWanted ? A.take<double>(nbig * nbig) : nullptr;
     fresh(spBlob);
     Plan plan(spBlob.get(), &kstats);
     // C_i = D^-1/2 L^-1 Mhat L^-T D^-1/2 with the explicit L^-1 (two GEMMs)
@@ -1633,15 +1675,13 @@ void Ctx::spectralPrepare() {
     plan.copy(c3);
     plan.copy(c4);
     // The largest node's reduction is the longest chain of latency-bound
-    // steps; it runs alone on the main stream while the other nodes' batch
-    // runs on the second stream.  The split needs the largest node's
-    // scratch (sp is cleared per batch) to lie after every other node's.
-    int big = 0;
-    for (int q = 1; q < static_cast<int>(ed.size()); ++q)
-        if (ed[q].n > ed[big].n) big = q;
+    // steps; it runs alone on one stream while the other nodes' batch runs
+    // on the other.  The split needs the largest node's scratch (sp is
+    // cleared per batch) to lie after every other node's.
     bool split = ed.size() > 1 && stream2;
     for (int q = 0; q < static_cast<int>(ed.size()) && split; ++q)
         if (q != big && soff[q] + ed[q].n > soff[big]) split = false;
+    const bool overlap = split && overlapWanted;
     if (split) {
         std::vector<EigDesc> er;
         std::vector<int64_t> sr;
@@ -1650,16 +1690,36 @@ void Ctx::spectralPrepare() {
                 er.push_back(ed[q]);
                 sr.push_back(soff[q]);
             }
-        Plan side(plan.blob, plan.st);
-        addSytrd(side, er, sr, dSv, dSp, &syLocal);
-        plan.fork(side, stream2, evFork, evJoin);
-        addSytrd(plan, {ed[big]}, {0}, dSv + 2 * soff[big], dSp + 3 * soff[big]);
-        plan.join(evJoin);
+        if (overlap) {
+            // the root's C copied before its reduction overwrites it; the
+            // reduction runs on stream2 and is joined only by pass B
+            Copy2dBatch cc;
+            cc.add(cdesc(ed[big].A, nbig, dCroot, nbig, nbig, nbig, 0));
+            plan.copy(cc);
+            // the batch is enqueued first: the host takes longer to enqueue
+            // the root's 1,000-odd steps than the GPU takes to start them,
+            // and the batch leads the main stream's path to pass A
+            Plan chain(plan.blob, plan.st), batch(plan.blob, plan.st);
+            addSytrd(batch, er, sr, dSv, dSp, &syLocal);
+            plan.fork(batch, stream3, evFork3, evBatch);
+            addSytrd(chain, {ed[big]}, {0}, dSv + 2 * soff[big], dSp + 3 * soff[big]);
+            plan.fork(chain, stream2, evFork, evChain);
+        } else {
+            Plan side(plan.blob, plan.st);
+            addSytrd(side, er, sr, dSv, dSp, &syLocal);
+            plan.fork(side, stream2, evFork, evJoin);
+            addSytrd(plan, {ed[big]}, {0}, dSv + 2 * soff[big], dSp + 3 * soff[big]);
+            plan.join(evJoin);
+        }
     } else {
         addSytrd(plan, ed, soff, dSv, dSp, &syLocal);
     }
     plan.run(stream);
-    sync();
+    if (overlap) {
+        chainPending = batchPending = true;
+        chainRoot = big;
+    }
+    sync();   // the main stream only: a pending root reduction keeps running
     // operators of the nodes the density switch sends to shift-invert
     // Lanczos (the dense reduction above stays their fallback)
     silPrepare();
@@ -1682,6 +1742,8 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg,
     const int64_t N = S.nNodes;
     if (pinned && static_cast<int64_t>(pinned->size()) != N)
         throw StatusError(AMLS_E_ARG, "pinned counts: one per node expected");
+    // a threshold or pinned selection reads every node's tridiagonal form
+    if (pinned || kind != 0) joinChain();
     k.assign(N, 0);
     lastSelKind = -1;
     if (pinned) {
@@ -1765,6 +1827,21 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg,
     dY = A.take<double>(zo[N]);
     dZ = A.take<double>(zo[N]);
     double* dWk = A.take<double>(wk);
+    // the root's pass-A modes by the Chebyshev filter on the copy of its C
+    // while its reduction still runs (the filter's own statistics describe
+    // the final Rayleigh-Ritz, so they are kept); otherwise, or when the
+    // filter does not converge, the reduction is waited for
+    bool rootDone = false;
+    if (chainPending && kind == 0 && !pinned && k[chainRoot] > 0 && mUse[chainRoot] == 0) {
+        const int64_t r = chainRoot;
+        const int itSave = lastRrIters;
+        const double resSave = lastRrResidual;
+        rootDone = chfsiTop(dCroot, S.size[r], k[r], dLam + moff[r], dY + zo[r]);
+        lastRrIters = itSave;
+        lastRrResidual = resSave;
+    }
+    if (!rootDone) joinChain();
+    joinBatch();   // the other nodes' tridiagonal forms
     fresh(passBlob);
     Plan plan(passBlob.get(), &kstats);
     std::vector<EigDesc> ed;
@@ -1834,7 +1911,7 @@ void Ctx::spectralSelect(int kind, const std::vector<double>& valueSeg,
         e.work = dWk + wo2;
         e.k = e.kcap = static_cast<int32_t>(kp);
         e.node = static_cast<int32_t>(p);
-        ed.push_back(e);
+        if (!(rootDone && p == chainRoot)) ed.push_back(e);
         wo2 += 6 * np * kp;
         VecDesc v;
         std::memset(&v, 0, sizeof(v));
@@ -2244,6 +2321,19 @@ void Ctx::backtransform() {
 // ---------------------------------------------------------------------------
 // Dense helpers (appended to a plan)
 // ---------------------------------------------------------------------------
+// Algorithmic work of panel s0 (64 columns) of an LDL^T whose leading block
+// has order n inside a matrix of order f (f = n for a plain LDL^T): the
+// diagonal block's jb^3 / 3 flops and the rows below it, jb^2 per row; bytes
+// = the block read and written once and each row read, written and its W
+// copy written.
+void ldltPanelWork(int64_t n, int64_t s0, double& flops, double& bytes, int64_t f) {
+    if (f < 0) f = n;
+    const double jb = static_cast<double>(std::min<int64_t>(64, n - s0));
+    const double rows = static_cast<double>(f - s0) - jb;
+    flops += jb * jb * jb / 3.0 + rows * jb * jb;
+    bytes += (jb * jb + 3.0 * rows * jb) * 8.0;
+}
+
 // In-place LDL^T of the lower triangles of the square matrices As[m] (order
 // and leading dimension ns[m]), 64-column panels over all matrices at once;
 // Ws[m]: ns[m] x 64 scratch.  A bad pivot of matrix m sets *err to at most
@@ -2295,7 +2385,9 @@ void addLdltBatch(Plan& plan, const std::vector<double*>& As, const std::vector<
         const size_t ol = plan.blob->add(list), orw = plan.blob->add(rows);
         const int cnt = static_cast<int>(list.size()), nrw = static_cast<int>(rows.size());
         const int s0i = static_cast<int>(s0);
-        plan.fn([=](hipStream_t s) {
+        double pf = 0.0, pb = 0.0;
+        for (int32_t m : list) ldltPanelWork(ns[m], s0, pf, pb);
+        plan.timed("ldlt_panel", pf, pb, [=](hipStream_t s) {
             launch_ldlt_panel(bl->at<FrontDesc>(ofd), bl->at<int32_t>(ol), cnt, s0i, err,
                               bl->at<int2>(orw), nrw, s);
         });
@@ -2327,7 +2419,9 @@ void addLdltFactor(Plan& plan, double* dA, int64_t n, double* dW, int* derr) {
         for (int c = 0; c < cdiv(n - r1, 256); ++c) rows.push_back(make_int2(0, c));
         const size_t orw = blob.add(rows);
         const int nrw = static_cast<int>(rows.size());
-        plan.fn([=](hipStream_t s) {
+        double pf = 0.0, pb = 0.0;
+        ldltPanelWork(n, s0, pf, pb);
+        plan.timed("ldlt_panel", pf, pb, [=](hipStream_t s) {
             launch_ldlt_panel(bl->at<dev::FrontDesc>(ofd), bl->at<int32_t>(ol), 1, s0i, derr,
                               bl->at<int2>(orw), nrw, s);
         });
@@ -3038,8 +3132,17 @@ amls_status amls_hip_create(int32_t device, uint32_t flags, amls_ctx** out) {
             hipStreamCreate(&ctx->c.stream) != hipSuccess ||
             hipStreamCreate(&ctx->c.stream2) != hipSuccess ||
             hipEventCreateWithFlags(&ctx->c.evFork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&ctx->c.evJoin, hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&ctx->c.evJoin, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->c.evChain, hipEventDisableTiming) != hipSuccess ||
+            hipStreamCreate(&ctx->c.stream3) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->c.evFork3, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->c.evBatch, hipEventDisableTiming) != hipSuccess) {
             if (ctx->c.evFork) (void)hipEventDestroy(ctx->c.evFork);
+            if (ctx->c.evJoin) (void)hipEventDestroy(ctx->c.evJoin);
+            if (ctx->c.evChain) (void)hipEventDestroy(ctx->c.evChain);
+            if (ctx->c.evFork3) (void)hipEventDestroy(ctx->c.evFork3);
+            if (ctx->c.evBatch) (void)hipEventDestroy(ctx->c.evBatch);
+            if (ctx->c.stream3) (void)hipStreamDestroy(ctx->c.stream3);
             if (ctx->c.stream2) (void)hipStreamDestroy(ctx->c.stream2);
             if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
             delete ctx;
@@ -3070,16 +3173,22 @@ amls_status amls_hip_destroy(amls_ctx* ctx) {
         (void)hipSetDevice(ctx->c.device);
         (void)hipStreamSynchronize(ctx->c.stream);
         (void)hipStreamSynchronize(ctx->c.stream2);
+        (void)hipStreamSynchronize(ctx->c.stream3);
         ctx->c.dropGeGraph();
         ctx->c.dropSiGraph();
     }
-    hipStream_t s = ctx->c.stream, s2 = ctx->c.stream2;
-    hipEvent_t e1 = ctx->c.evFork, e2 = ctx->c.evJoin;
+    hipStream_t s = ctx->c.stream, s2 = ctx->c.stream2, s3 = ctx->c.stream3;
+    hipEvent_t e1 = ctx->c.evFork, e2 = ctx->c.evJoin, e3 = ctx->c.evChain;
+    hipEvent_t e4 = ctx->c.evFork3, e5 = ctx->c.evBatch;
     const bool gpu = ctx->c.gpu;
     delete ctx;
     if (gpu) {
         if (e1) (void)hipEventDestroy(e1);
         if (e2) (void)hipEventDestroy(e2);
+        if (e3) (void)hipEventDestroy(e3);
+        if (e4) (void)hipEventDestroy(e4);
+        if (e5) (void)hipEventDestroy(e5);
+        if (s3) (void)hipStreamDestroy(s3);
         if (s2) (void)hipStreamDestroy(s2);
         if (s) (void)hipStreamDestroy(s);
     }
@@ -3327,7 +3436,20 @@ static void solveNumeric(Ctx& c, amls_options o, double* theta, double* Z, int64
     auto minP = [&] { return *std::min_element(c.segP.begin(), c.segP.end()); };
     c.runGE(o.precision);
     const auto t2 = Clock::now();
-    c.spectralPrepare();
+    // pass A's count per node, known before the local reductions: the
+    // largest node's few modes can come from the filter while its reduction
+    // runs on (one problem, two-pass truncation)
+    if (NS == 1 && o.fixedKeep == 0 && o.twoPass) {
+        const int64_t Ns = c.segNode[1] - c.segNode[0];
+        c.overlapK0 = std::max<int64_t>(4, (3 * o.nev + Ns - 1) / Ns);
+    }
+    try {
+        c.spectralPrepare();
+    } catch (...) {
+        c.overlapK0 = 0;
+        throw;
+    }
+    c.overlapK0 = 0;
     int64_t p0 = 0;
     if (o.fixedKeep > 0) {
         c.spectralSelect(0, static_cast<double>(o.fixedKeep));

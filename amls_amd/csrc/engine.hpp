@@ -205,6 +205,27 @@ struct Ctx {
     // `stream` within one plan, and the fork / join events
     hipStream_t stream2 = nullptr;
     hipEvent_t evFork = nullptr, evJoin = nullptr;
+    // The root separator's tridiagonalisation left running on stream2 past
+    // the end of spectralPrepare (a solve's pass A takes the root's few
+    // modes from a copy of its C by the Chebyshev filter meanwhile):
+    // evChain marks its end; joinChain makes the main stream wait for it
+    // (pass B reads the tridiagonal form), drainChain waits on the host
+    // (before anything may release the spectral arena).
+    hipEvent_t evChain = nullptr;
+    bool chainPending = false;
+    // meanwhile the other nodes' reductions run on stream3 (evBatch marks
+    // their end), so the filter on the main stream overlaps them too;
+    // spectralSelect joins them before it reads their tridiagonal forms
+    hipStream_t stream3 = nullptr;
+    hipEvent_t evFork3 = nullptr, evBatch = nullptr;
+    bool batchPending = false;
+    int64_t chainRoot = -1;
+    double* dCroot = nullptr;
+    int64_t overlapK0 = 0;       // > 0: the next spectralPrepare may leave the root's chain running
+    void joinChain();   // also joins the batch
+    void joinBatch();
+    void drainChain();  // both, on the host
+    bool silEligible(int64_t p) const;
     std::string err;
 
     // ---- problem (host)

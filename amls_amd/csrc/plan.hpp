@@ -358,6 +358,9 @@ void addSpmm(Plan& plan, const SparseOp& A, const double* X, double* Y, int64_t 
 // and leading dimension ns[m]) in 64-column panels; Ws[m]: ns[m] x 64
 // scratch.  A non-positive or non-finite pivot of matrix m sets *err to
 // min(*err, nodes[m]).
+// algorithmic flops / bytes of LDL^T panel s0 of a leading block of order n
+// inside a matrix of order f (-1: f = n), added to flops / bytes
+void ldltPanelWork(int64_t n, int64_t s0, double& flops, double& bytes, int64_t f = -1);
 void addLdltBatch(Plan& plan, const std::vector<double*>& As, const std::vector<int64_t>& ns,
                   const std::vector<double*>& Ws, int* err, const std::vector<int>& nodes);
 // LDL^T of the n x n lower triangle of dA in place and, for nrhs > 0,

@@ -59,13 +59,11 @@ void Ctx::silPrepare() {
     if (silMode < 0) return;
     int64_t nmax = 0;
     for (int64_t p = 0; p < N; ++p) {
+        // the density switch; nodes above the Lanczos kernels' order limit
+        // stay on the dense path
+        if (!silEligible(p)) continue;
         const int64_t n = S.size[p];
-        const double nnz = statHost[2 * p], dmax = statHost[2 * p + 1];
-        bool use = silMode > 0 || (n >= 200 && nnz < 0.15 * double(n) * double(n));
-        // the Lanczos kernel keeps two n-vectors in LDS: larger sparse
-        // nodes stay on the dense path
-        if (n > lanczos_max_order() || n < 1) use = false;
-        if (!use) continue;
+        const double dmax = statHost[2 * p + 1];
         SilNode s{};
         s.p = p;
         s.n = static_cast<int>(n);
