@@ -664,8 +664,15 @@ void KernelStats::hostSpan(const char* name, std::chrono::steady_clock::time_poi
     spans.push_back({"plan", name, phase ? phase : "", 0, us(a), us(b)});
 }
 
-void KernelStats::resolve() {
+void KernelStats::resolve(bool wait) {
+    std::vector<Pending> later;
     for (auto& p : pending) {
+        // without wait, launches still running (on a stream the caller did
+        // not synchronise) stay pending, so statistics never stall a stream
+        if (!wait && hipEventQuery(p.b) == hipErrorNotReady) {
+            later.push_back(p);
+            continue;
+        }
         float ms = 0.f, m0 = 0.f;
         if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
             byName[p.name].ms += ms;
@@ -678,11 +685,11 @@ void KernelStats::resolve() {
         pool.push_back(p.a);
         pool.push_back(p.b);
     }
-    pending.clear();
+    pending.swap(later);
 }
 
 void KernelStats::reset() {
-    resolve();
+    resolve(true);
     byName.clear();
 }
 
@@ -847,7 +854,14 @@ void Ctx::autotune() {
 
 void Ctx::sync() {
     AMLS_HIP_CHECK(hipStreamSynchronize(stream));
-    kstats.resolve();
+    kstats.resolve(false);
+}
+
+void Ctx::syncAll() {
+    AMLS_HIP_CHECK(hipStreamSynchronize(stream));
+    if (stream2) AMLS_HIP_CHECK(hipStreamSynchronize(stream2));
+    if (stream3) AMLS_HIP_CHECK(hipStreamSynchronize(stream3));
+    kstats.resolve(true);
 }
 
 // ---------------------------------------------------------------------------
@@ -3733,7 +3747,7 @@ amls_status amls_hip_profile(amls_ctx* ctx, int32_t enable) {
 amls_status amls_hip_kernel_stats(amls_ctx* ctx, int32_t index, char* name, int32_t name_len,
                                   int64_t* launches, double* ms, double* flops, double* bytes) {
     return guarded(ctx, [&](amls::Ctx& c) {
-        if (c.gpu) c.sync();
+        if (c.gpu) c.syncAll();
         if (index < 0 || index >= static_cast<int32_t>(c.kstats.byName.size()))
             throw amls::StatusError(AMLS_E_ARG, "kernel class index out of range");
         auto it = c.kstats.byName.begin();

@@ -162,7 +162,9 @@ struct KernelStats {
     // a host span (lane 0) in the current phase, kept when tracing
     void hostSpan(const char* name, std::chrono::steady_clock::time_point a,
                   std::chrono::steady_clock::time_point b);
-    void resolve();   // after a stream sync
+    // turns finished launches' event pairs into times (and trace spans);
+    // wait: also those still running
+    void resolve(bool wait);
     void reset();
     ~KernelStats();
 };
@@ -433,7 +435,8 @@ struct Ctx {
     void backtransform();
     void subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& thetaOut,
                       bool fromBasis);
-    void sync();
+    void sync();        // the main stream; statistics of finished launches
+    void syncAll();     // every stream of the context; all statistics
     void requireGpu() const {
         if (!gpu) throw StatusError(6, "context has no device");
     }

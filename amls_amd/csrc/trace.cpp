@@ -46,7 +46,8 @@ int64_t Ctx::traceWrite(const std::string& path) {
     if (gpu) {
         (void)hipStreamSynchronize(stream);
         (void)hipStreamSynchronize(stream2);
-        kstats.resolve();
+        if (stream3) (void)hipStreamSynchronize(stream3);
+        kstats.resolve(true);
     }
     std::FILE* f = std::fopen(path.c_str(), "w");
     if (!f) return -1;
