@@ -31,9 +31,12 @@ __device__ __forceinline__ double lane_value(double v, int src) {
 //
 // Right-looking: step j computes l_i = a_ij / d_j on every lane i > j and
 // keeps it in register j, then subtracts a_ij l_k from a_ik for every k > j,
-// l_k read from lane k as a uniform value.  Registers k > i of lane i (the
-// upper part) collect harmless junk and are never read or stored.  No LDS
-// and no barrier: the whole step chain is straight-line code.
+// with l_k broadcast to every lane through LDS (lane i writes l_i, every
+// lane reads l_k at the same address).  The workgroup is one wave, whose LDS
+// operations complete in issue order, so no barrier is needed; the two
+// alternating columns let one step's reads overlap the next step's write.
+// Registers k > i of lane i (the upper part) collect harmless junk and are
+// never read or stored.
 __global__ void __launch_bounds__(64) ldlt_diag_kernel(const FrontDesc* __restrict__ descs,
                                                        const int32_t* __restrict__ list,
                                                        int s0, int* __restrict__ err) {
@@ -42,6 +45,7 @@ __global__ void __launch_bounds__(64) ldlt_diag_kernel(const FrontDesc* __restri
     const int jb = min(NB, d.n - s0);
     double* F = d.F + s0 + (int64_t)s0 * f;
     const int i = threadIdx.x;
+    __shared__ double sl[2][NB];
     double a[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) a[k] = (k <= i && i < jb) ? F[i + (int64_t)k * f] : 0.0;
@@ -54,8 +58,10 @@ __global__ void __launch_bounds__(64) ldlt_diag_kernel(const FrontDesc* __restri
             const double aij = a[j];
             const double l = aij / dj;
             if (i > j) a[j] = l;
+            double* lj = sl[j & 1];
+            lj[i] = l;
 #pragma unroll
-            for (int k = j + 1; k < NB; ++k) a[k] -= aij * lane_value(a[j], k);
+            for (int k = j + 1; k < NB; ++k) a[k] -= aij * lj[k];
         }
     }
     if (bad && i == 0) atomicMin(err, d.node);
