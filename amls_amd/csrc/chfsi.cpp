@@ -8,7 +8,8 @@
 // Householder reduction of C is q dependent launches; this solver needs a
 // few dozen GEMMs with C instead:
 //   - a block X of s columns (s = min(192, max(2 nev, nev + 48)) rounded up
-//     to 16) starts from a reproducible uniform block and one power step;
+//     to 16) starts from a reproducible uniform block (columns normalised)
+//     and one power step, orthonormalised;
 //     the Rayleigh quotient of its last column estimates the s-th eigenvalue
 //     and sets the cut nu_c of the filter;
 //   - each iteration applies the Chebyshev polynomial T_d of
@@ -197,10 +198,10 @@ bool Ctx::chfsiTop(double* C, int64_t q, int64_t m, double* lamOut, double* Vout
             launch_set_i32(err, INT_MAX, st);
             launch_fill_hash(X0, qq * si, 0x2545F4914F6CDD1Dull, st);
         });
-        // one Cholesky QR each: the random block and its power step are
-        // well conditioned (cond about 1 and nu_1 / nu_s)
+        // the random block only normalised (its columns are nearly
+        // orthogonal already: cond about 2 for s << q), its power step
+        // orthonormalised by one Cholesky QR (cond about 2 nu_1 / nu_s)
         normalize(plan, X0);
-        cholqr(plan, X0, X1);
         addGemm(plan, prod(C, q, 0, X0, q, 0, X1, q, q, s, q, 1.0, 0.0), 3);
         normalize(plan, X1);
         cholqr(plan, X1, X0);
