@@ -147,29 +147,35 @@ bool Ctx::chfsiTop(double* C, int64_t q, int64_t m, double* lamOut, double* Vout
     auto normalize = [&](Plan& plan, double* X) {
         plan.fn([=](hipStream_t st) { launch_colnorm_seg(seg, 1, X, X, si, qq, nrm, st, false); });
     };
-    // X <- X L^-T D^-1/2 with X^T X = L D L^T (T: scratch block)
+    // X <- X L^-T D^-1/2 with X^T X = L D L^T (T: scratch block); L^-1 by
+    // the forward solve on the identity, which follows the factorisation
+    // panel by panel on stream2 when that stream is free
     auto cholqr = [&](Plan& plan, double* X, double* T) {
         addGemm(plan, prod(X, q, 1, X, q, 0, G, s, s, s, q, 1.0, 0.0, 1), 8);
-        addLdltBatch(plan, {G}, {s}, {Wp}, err, {-3});
         Copy2dBatch cz;
         cz.add(cdesc(nullptr, 0, Li, s, s, s, 3, nullptr, nullptr, 0.0));
         plan.copy(cz);
-        std::vector<VecDesc> v(2);
-        std::memset(v.data(), 0, 2 * sizeof(VecDesc));
+        std::vector<VecDesc> v(1), vq(1);
+        std::memset(v.data(), 0, sizeof(VecDesc));
+        std::memset(vq.data(), 0, sizeof(VecDesc));
         v[0].dst = Li;
         v[0].ld = s;
         v[0].n = si;
         v[0].op = 2;
-        v[1].src = G;
-        v[1].dst = dq;
-        v[1].ld = s;
-        v[1].n = si;
-        v[1].op = 0;
         plan.vec(v);
         TrsmBatch tb;
         tb.add(G, s, Li, s, s, s, 0);
         tb.descs.back().rhsLower = 1;
-        plan.trsm(tb);
+        const bool piped = pipelineOk();
+        const InvPipeline pipe{&tb, stream2, evFork, evJoin};
+        addLdltBatch(plan, {G}, {s}, {Wp}, err, {-3}, piped ? &pipe : nullptr);
+        if (!piped) plan.trsm(tb);
+        vq[0].src = G;
+        vq[0].dst = dq;
+        vq[0].ld = s;
+        vq[0].n = si;
+        vq[0].op = 0;
+        plan.vec(vq);
         addGemm(plan, prod(X, q, 0, Li, s, 1, T, q, q, s, s, 1.0, 0.0));
         Copy2dBatch sc;
         sc.add(cdesc(T, q, X, q, q, s, 0, nullptr, dq));

@@ -1314,6 +1314,27 @@ void Ctx::runGE(int prec) {
                 launch_front_stats(bl->at<dev::FrontStatDesc>(osd), bl->at<int2>(osw), nsw, st, s);
             });
         }
+        // L^-1 of every node by the forward solve on the identity, row block
+        // j on stream2 as soon as panel j is factored (it reads only block
+        // column j of L, which the later panels leave alone)
+        TrsmBatch tinv;
+        std::vector<VecDesc> vid;
+        for (int64_t p : nodes) {
+            const int64_t np = S.size[p];
+            VecDesc v;
+            std::memset(&v, 0, sizeof(v));
+            v.n = static_cast<int32_t>(np);
+            v.dst = dLinv + co[p];
+            v.ld = np;
+            v.op = 2;
+            vid.push_back(v);
+            tinv.add(dF + fo[p], S.f(p), dLinv + co[p], np, np, np, 0);
+            tinv.descs.back().rhsLower = 1;
+        }
+        plan.vec(vid);
+        const bool pipe = pipelineOk();
+        const Plan::TrsmPlan tp = plan.trsmSetup(tinv);
+        int panel = 0;
         for (int64_t s0 = 0; s0 < nmaxl; s0 += 64) {
             std::vector<int32_t> list;
             GemmBatch g64, gp;
@@ -1383,13 +1404,22 @@ void Ctx::runGE(int prec) {
                 launch_ldlt_panel(bl->at<FrontDesc>(ofd), bl->at<int32_t>(ol), cnt, s0i, err,
                                   bl->at<int2>(orw), nrw, s);
             });
+            if (pipe) {
+                Plan side(plan.blob, plan.st);
+                side.trsmStep(tinv, tp, panel);
+                plan.fork(side, stream2, evFork, evJoin);
+            }
+            ++panel;
             plan.gemm(g64, false);
             plan.gemm(gp, true);
         }
-        // L^-1 of every node (forward solve on the identity), D^-1/2, D^-1;
-        // X = Khat^-1 K_iB = L^-T L21^T, Fm = M_iB - Mhat X, M congruence
+        if (pipe)
+            plan.join(evJoin);
+        else
+            for (int j = 0; j < tp.maxnb; ++j) plan.trsmStep(tinv, tp, j);
+        // D^-1/2, D^-1; X = Khat^-1 K_iB = L^-T L21^T, Fm = M_iB - Mhat X,
+        // M congruence
         std::vector<VecDesc> vdiag;
-        TrsmBatch tinv;
         GemmBatch gX;
         for (int64_t p : nodes) {
             const int64_t np = S.size[p], f = S.f(p), b = S.b(p);
@@ -1398,10 +1428,6 @@ void Ctx::runGE(int prec) {
             VecDesc v;
             std::memset(&v, 0, sizeof(v));
             v.n = static_cast<int32_t>(np);
-            v.dst = Li;
-            v.ld = np;
-            v.op = 2;
-            vdiag.push_back(v);
             v.src = F;
             v.ld = f;
             v.dst = dDsq + vo[p];
@@ -1410,8 +1436,6 @@ void Ctx::runGE(int prec) {
             v.dst = dDinv + vo[p];
             v.op = 5;
             vdiag.push_back(v);
-            tinv.add(F, f, Li, np, np, np, 0);
-            tinv.descs.back().rhsLower = 1;
             if (b > 0) {
                 GemmDesc g = gdesc();
                 g.A = Li;
@@ -1430,7 +1454,6 @@ void Ctx::runGE(int prec) {
             }
         }
         plan.vec(vdiag);
-        plan.trsm(tinv);
         plan.gemm(gX, false);
         Copy2dBatch cb;
         GemmBatch gFm, gG22;
@@ -2353,9 +2376,13 @@ void ldltPanelWork(int64_t n, int64_t s0, double& flops, double& bytes, int64_t 
 // Ws[m]: ns[m] x 64 scratch.  A bad pivot of matrix m sets *err to at most
 // nodes[m].
 void addLdltBatch(Plan& plan, const std::vector<double*>& As, const std::vector<int64_t>& ns,
-                  const std::vector<double*>& Ws, int* err, const std::vector<int>& nodes) {
+                  const std::vector<double*>& Ws, int* err, const std::vector<int>& nodes,
+                  const InvPipeline* pipe) {
     const int nm = static_cast<int>(As.size());
     if (nm == 0) return;
+    Plan::TrsmPlan tp;
+    if (pipe) tp = plan.trsmSetup(*pipe->inv);
+    int panel = 0;
     std::vector<FrontDesc> fd(nm);
     int64_t nmax = 0;
     for (int m = 0; m < nm; ++m) {
@@ -2405,7 +2432,23 @@ void addLdltBatch(Plan& plan, const std::vector<double*>& As, const std::vector<
             launch_ldlt_panel(bl->at<FrontDesc>(ofd), bl->at<int32_t>(ol), cnt, s0i, err,
                               bl->at<int2>(orw), nrw, s);
         });
+        if (pipe) {
+            Plan side(plan.blob, plan.st);
+            side.trsmStep(*pipe->inv, tp, panel);
+            plan.fork(side, pipe->s2, pipe->ef, pipe->ej);
+        }
+        ++panel;
         plan.gemm(gb, false);
+    }
+    if (pipe) {
+        // row blocks past the last panel (none when inv's L are these
+        // factors), after the others on the same stream
+        for (; panel < tp.maxnb; ++panel) {
+            Plan side(plan.blob, plan.st);
+            side.trsmStep(*pipe->inv, tp, panel);
+            plan.fork(side, pipe->s2, pipe->ef, pipe->ej);
+        }
+        plan.join(pipe->ej);
     }
 }
 
@@ -2986,12 +3029,16 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
             }
             plan.gemm(gr, false);
         }
-        addLdltBatch(plan, KcS, std::vector<int64_t>(KcS.size(), q), WpS, err,
-                     std::vector<int>(KcS.size(), -1));
+        // Mc copied out and L^-1's storage set to the identity first, so that
+        // the forward solve for L^-1 can follow the factorisation panel by
+        // panel on stream2
         plan.copy(c1);
         plan.copy(cz);
         plan.vec(vid);
-        plan.trsm(tinv);
+        const InvPipeline pipe{&tinv, stream2, evFork, evJoin};
+        addLdltBatch(plan, KcS, std::vector<int64_t>(KcS.size(), q), WpS, err,
+                     std::vector<int>(KcS.size(), -1), pipelineOk() ? &pipe : nullptr);
+        if (!pipelineOk()) plan.trsm(tinv);
         plan.gemm(gl1, false);
         plan.gemm(gl2, false);
         plan.vec(vd);

@@ -228,6 +228,9 @@ struct Ctx {
     void joinBatch();
     void drainChain();  // both, on the host
     bool silEligible(int64_t p) const;
+    // stream2 is free for a plan's side work (no root reduction or batch of
+    // the condensation still on the side streams)
+    bool pipelineOk() const { return stream2 && !chainPending && !batchPending; }
     std::string err;
 
     // ---- problem (host)

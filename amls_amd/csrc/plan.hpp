@@ -223,20 +223,38 @@ struct Plan {
     // Blocked right-looking unit-triangular solve of every descriptor: per
     // 64-row block (in solve order) one launch of diagonal-block solves, then
     // one GEMM batch applying the solved block to the rows still to come.
-    void trsm(const TrsmBatch& b) {
-        if (b.descs.empty()) return;
-        const size_t od = blob->add(b.descs);
-        Blob* bl = blob;
+    // The blocked solves of b, one 64-row block of every descriptor per step:
+    // trsmSetup adds the descriptors, trsmStep(s) step s's diagonal-block
+    // solves and the update of the rows after (forward) or before (backward)
+    // them.  Step s of a forward solve reads only block column s of L.
+    struct TrsmPlan {
+        size_t od = 0;
         int maxnb = 0;
-        for (const auto& d : b.descs) maxnb = std::max(maxnb, cdiv(d.n, 64));
-        double fl = 0, by = 0;
+        double fl = 0.0, by = 0.0;
+    };
+    TrsmPlan trsmSetup(const TrsmBatch& b) {
+        TrsmPlan t;
+        if (b.descs.empty()) return t;
+        t.od = blob->add(b.descs);
         for (const auto& d : b.descs) {
+            t.maxnb = std::max(t.maxnb, cdiv(d.n, 64));
             const double n = d.n, r = d.nrhs;
-            fl += n * (n - 1) * r;
-            by += (n * (n + 1) / 2 + 2 * n * r) * 8.0;
+            t.fl += n * (n - 1) * r;
+            t.by += (n * (n + 1) / 2 + 2 * n * r) * 8.0;
         }
+        return t;
+    }
+    void trsm(const TrsmBatch& b) {
+        const TrsmPlan t = trsmSetup(b);
+        for (int s = 0; s < t.maxnb; ++s) trsmStep(b, t, s);
+    }
+    void trsmStep(const TrsmBatch& b, const TrsmPlan& t, int s) {
+        if (s >= t.maxnb) return;
+        const size_t od = t.od;
+        const double fl = t.fl, by = t.by;
+        Blob* bl = blob;
         const int nd = static_cast<int>(b.descs.size());
-        for (int s = 0; s < maxnb; ++s) {
+        {
             std::vector<TrsmWork> work[2];   // by direction: forward, transposed
             GemmBatch gb;
             for (int id = 0; id < nd; ++id) {
@@ -361,8 +379,18 @@ void addSpmm(Plan& plan, const SparseOp& A, const double* X, double* Y, int64_t 
 // algorithmic flops / bytes of LDL^T panel s0 of a leading block of order n
 // inside a matrix of order f (-1: f = n), added to flops / bytes
 void ldltPanelWork(int64_t n, int64_t s0, double& flops, double& bytes, int64_t f = -1);
+// With a pipeline: the forward solves of inv (their L being these factors)
+// run on the pipeline's stream, row block j as soon as panel j is factored,
+// concurrently with the later panels; the plan joins them after the last
+// panel.
+struct InvPipeline {
+    const TrsmBatch* inv = nullptr;
+    hipStream_t s2 = nullptr;
+    hipEvent_t ef = nullptr, ej = nullptr;
+};
 void addLdltBatch(Plan& plan, const std::vector<double*>& As, const std::vector<int64_t>& ns,
-                  const std::vector<double*>& Ws, int* err, const std::vector<int>& nodes);
+                  const std::vector<double*>& Ws, int* err, const std::vector<int>& nodes,
+                  const InvPipeline* pipe = nullptr);
 // LDL^T of the n x n lower triangle of dA in place and, for nrhs > 0,
 // dX := A^-1 dX; dW: n x 64 scratch, dq: n doubles, *derr: INT_MAX unless a
 // pivot is non-positive or non-finite (then 0).
