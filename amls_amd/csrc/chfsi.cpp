@@ -106,7 +106,7 @@ bool Ctx::chfsiTop(double* C, int64_t q, int64_t m, double* lamOut, double* Vout
     for (int i = 0; i < 2; ++i) A.plan(q * s * 8);       // X blocks
     A.plan(q * q * 8);                                   // Cs
     for (int i = 0; i < 3; ++i) A.plan(s * s * 8);       // G, L^-1, H
-    A.plan(s * 64 * 8);                                  // LDL^T panel scratch
+    A.plan(2 * s * 64 * 8);                              // LDL^T panel scratch (two panels)
     for (int i = 0; i < 3; ++i) A.plan(s * 8);           // D^-1/2, column norms, Rayleigh quotients
     A.plan(2 * 8);                                       // row segment [0, q)
     for (int i = 0; i < 3; ++i) A.plan(s * 8);           // d, e, tau
@@ -123,7 +123,7 @@ bool Ctx::chfsiTop(double* C, int64_t q, int64_t m, double* lamOut, double* Vout
     double* G = A.take<double>(s * s);
     double* Li = A.take<double>(s * s);
     double* H = A.take<double>(s * s);
-    double* Wp = A.take<double>(s * 64);
+    double* Wp = A.take<double>(2 * s * 64);
     double* dq = A.take<double>(s);
     double* nrm = A.take<double>(s);
     double* rq = A.take<double>(s);
@@ -167,7 +167,7 @@ bool Ctx::chfsiTop(double* C, int64_t q, int64_t m, double* lamOut, double* Vout
         tb.add(G, s, Li, s, s, s, 0);
         tb.descs.back().rhsLower = 1;
         const bool piped = pipelineOk();
-        const InvPipeline pipe{&tb, stream2, evFork, evJoin};
+        const InvPipeline pipe{&tb, stream2, evFork, evJoin, stream3, evFork3, evBatch};
         addLdltBatch(plan, {G}, {s}, {Wp}, err, {-3}, piped ? &pipe : nullptr);
         if (!piped) plan.trsm(tb);
         vq[0].src = G;

@@ -2380,10 +2380,13 @@ void addLdltBatch(Plan& plan, const std::vector<double*>& As, const std::vector<
                   const InvPipeline* pipe) {
     const int nm = static_cast<int>(As.size());
     if (nm == 0) return;
+    const bool inv = pipe && pipe->inv;
+    const bool la = pipe && pipe->s3;
     Plan::TrsmPlan tp;
-    if (pipe) tp = plan.trsmSetup(*pipe->inv);
+    if (inv) tp = plan.trsmSetup(*pipe->inv);
     int panel = 0;
-    std::vector<FrontDesc> fd(nm);
+    // descriptors [nm, 2 nm): the same matrices with the second W (look-ahead)
+    std::vector<FrontDesc> fd(la ? 2 * nm : nm);
     int64_t nmax = 0;
     for (int m = 0; m < nm; ++m) {
         std::memset(&fd[m], 0, sizeof(FrontDesc));
@@ -2392,55 +2395,85 @@ void addLdltBatch(Plan& plan, const std::vector<double*>& As, const std::vector<
         fd[m].f = ns[m];
         fd[m].n = static_cast<int32_t>(ns[m]);
         fd[m].node = nodes[m];
+        if (la) {
+            fd[nm + m] = fd[m];
+            fd[nm + m].W = Ws[m] + ns[m] * 64;
+        }
         nmax = std::max(nmax, ns[m]);
     }
     const size_t ofd = plan.blob->add(fd);
     Blob* bl = plan.blob;
     for (int64_t s0 = 0; s0 < nmax; s0 += 64) {
+        const int par = la ? (panel & 1) : 0;
         std::vector<int32_t> list;
         std::vector<int2> rows;
-        GemmBatch gb;
+        GemmBatch gb, gnext;   // (look-ahead: the rest, the next panel's columns)
         for (int m = 0; m < nm; ++m) {
             const int64_t n = ns[m];
             if (n <= s0) continue;
             const int64_t jb = std::min<int64_t>(64, n - s0), r1 = s0 + jb;
-            list.push_back(m);
-            for (int c = 0; c < cdiv(n - r1, 256); ++c) rows.push_back(make_int2(m, c));
-            if (n > r1) {
+            const int di = m + par * nm;
+            double* W = Ws[m] + par * n * 64;
+            list.push_back(di);
+            for (int c = 0; c < cdiv(n - r1, 256); ++c) rows.push_back(make_int2(di, c));
+            // C(r, c) -= sum_k L(r, k) W(c - r1, k) over rows and columns from
+            // c0 on (lower part); W row i belongs to row r1 + i
+            auto update = [&](GemmBatch& g, int64_t c0, int64_t c1) {
+                if (c1 <= c0) return;
                 GemmDesc d = gdesc();
-                d.A = As[m] + r1 + s0 * n;
+                d.A = As[m] + c0 + s0 * n;
                 d.lda = n;
-                d.B = Ws[m];
+                d.B = W + (c0 - r1);
                 d.ldb = n;
                 d.tb = 1;
-                d.C = As[m] + r1 + r1 * n;
+                d.C = As[m] + c0 + c0 * n;
                 d.ldc = n;
-                d.m = d.n = static_cast<int32_t>(n - r1);
+                d.m = static_cast<int32_t>(n - c0);
+                d.n = static_cast<int32_t>(c1 - c0);
                 d.k = static_cast<int32_t>(jb);
                 d.alpha1 = -1.0;
                 d.beta = 1.0;
                 d.lower = 1;
-                gb.add(d);
+                g.add(d);
+            };
+            if (la) {
+                const int64_t r2 = std::min<int64_t>(n, r1 + 64);
+                update(gnext, r1, r2);
+                update(gb, r2, n);
+            } else {
+                update(gb, r1, n);
             }
         }
         const size_t ol = plan.blob->add(list), orw = plan.blob->add(rows);
         const int cnt = static_cast<int>(list.size()), nrw = static_cast<int>(rows.size());
         const int s0i = static_cast<int>(s0);
         double pf = 0.0, pb = 0.0;
-        for (int32_t m : list) ldltPanelWork(ns[m], s0, pf, pb);
+        for (int32_t di : list) ldltPanelWork(ns[di % nm], s0, pf, pb);
         plan.timed("ldlt_panel", pf, pb, [=](hipStream_t s) {
             launch_ldlt_panel(bl->at<FrontDesc>(ofd), bl->at<int32_t>(ol), cnt, s0i, err,
                               bl->at<int2>(orw), nrw, s);
         });
-        if (pipe) {
+        if (inv) {
             Plan side(plan.blob, plan.st);
             side.trsmStep(*pipe->inv, tp, panel);
             plan.fork(side, pipe->s2, pipe->ef, pipe->ej);
         }
+        if (la) {
+            // the previous panel's rest wrote these columns too (and this
+            // panel's factorisation did not need it: its own columns came
+            // from the previous next-panel update)
+            if (panel > 0) plan.join(pipe->eb);
+            plan.gemm(gnext, false);
+            Plan side(plan.blob, plan.st);
+            side.gemm(gb, false);
+            plan.fork(side, pipe->s3, pipe->ef3, pipe->eb);
+        } else {
+            plan.gemm(gb, false);
+        }
         ++panel;
-        plan.gemm(gb, false);
     }
-    if (pipe) {
+    if (la && panel > 0) plan.join(pipe->eb);
+    if (inv) {
         // row blocks past the last panel (none when inv's L are these
         // factors), after the others on the same stream
         for (; panel < tp.maxnb; ++panel) {
@@ -2580,7 +2613,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     A.plan(zbLevel * 8);                   // gathered x_B of one level
     const int64_t NS = nseg;               // one q x q problem per segment
     A.plan(NS * q * q * 8 * 4);            // Kc, Mc, C, T
-    A.plan(NS * q * 64 * 8);               // panel scratch
+    A.plan(NS * q * 128 * 8);              // panel scratch (two panels each)
     A.plan(NS * q * 8 * 13);               // eig vectors
     A.plan(NS * q * nev * 8 * 2);          // Yt, V
     A.plan(NS * nev * 6 * q * 8);          // inverse-iteration scratch
@@ -2601,7 +2634,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     double* Mc = A.take<double>(NS * q * q);
     double* Cm = A.take<double>(NS * q * q);
     double* Tm = A.take<double>(NS * q * q);
-    double* Wp = A.take<double>(NS * q * 64);
+    double* Wp = A.take<double>(NS * q * 128);
     double* d = A.take<double>(NS * q);
     double* e = A.take<double>(NS * q);
     double* tau = A.take<double>(NS * q);
@@ -2619,7 +2652,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
     for (int64_t sg = 0; sg < NS; ++sg) {
         McS.push_back(Mc + sg * qq);
         KcS.push_back(Kc + sg * qq);
-        WpS.push_back(Wp + sg * q * 64);
+        WpS.push_back(Wp + sg * q * 128);
     }
     // rows of problem s: [r0, r0 + ns) in the permuted ordering
     auto rowsOf = [&](int64_t sg) { return std::make_pair(segRow[sg], segRow[sg + 1] - segRow[sg]); };
@@ -2980,11 +3013,12 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                     cy.add(cdesc(Q + r0, n, X + r0, n, ns, q, 0, nullptr, ds));   // Y = Q D^-1/2
                 }
                 plan.gemm(gg, false);
-                addLdltBatch(plan, KcS, std::vector<int64_t>(NS, q), WpS, err,
-                             std::vector<int>(NS, -2));
                 plan.copy(cz0);
                 plan.vec(vid0);
-                plan.trsm(ti);
+                const InvPipeline pq{&ti, stream2, evFork, evJoin, stream3, evFork3, evBatch};
+                addLdltBatch(plan, KcS, std::vector<int64_t>(NS, q), WpS, err,
+                             std::vector<int>(NS, -2), pipelineOk() ? &pq : nullptr);
+                if (!pipelineOk()) plan.trsm(ti);
                 plan.vec(vdq);
                 plan.gemm(gy, false);
                 plan.copy(cy);
@@ -3035,7 +3069,7 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
         plan.copy(c1);
         plan.copy(cz);
         plan.vec(vid);
-        const InvPipeline pipe{&tinv, stream2, evFork, evJoin};
+        const InvPipeline pipe{&tinv, stream2, evFork, evJoin, stream3, evFork3, evBatch};
         addLdltBatch(plan, KcS, std::vector<int64_t>(KcS.size(), q), WpS, err,
                      std::vector<int>(KcS.size(), -1), pipelineOk() ? &pipe : nullptr);
         if (!pipelineOk()) plan.trsm(tinv);

@@ -230,7 +230,7 @@ struct Ctx {
     bool silEligible(int64_t p) const;
     // stream2 is free for a plan's side work (no root reduction or batch of
     // the condensation still on the side streams)
-    bool pipelineOk() const { return stream2 && !chainPending && !batchPending; }
+    bool pipelineOk() const { return stream2 && stream3 && !chainPending && !batchPending; }
     std::string err;
 
     // ---- problem (host)

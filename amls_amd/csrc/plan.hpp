@@ -379,14 +379,19 @@ void addSpmm(Plan& plan, const SparseOp& A, const double* X, double* Y, int64_t 
 // algorithmic flops / bytes of LDL^T panel s0 of a leading block of order n
 // inside a matrix of order f (-1: f = n), added to flops / bytes
 void ldltPanelWork(int64_t n, int64_t s0, double& flops, double& bytes, int64_t f = -1);
-// With a pipeline: the forward solves of inv (their L being these factors)
-// run on the pipeline's stream, row block j as soon as panel j is factored,
-// concurrently with the later panels; the plan joins them after the last
-// panel.
+// With a pipeline: the forward solves of inv (their L being these factors,
+// or null) run on s2, row block j as soon as panel j is factored,
+// concurrently with the later panels.  With s3 as well (look-ahead), each
+// panel's trailing update is split: the next panel's 64 columns on the
+// plan's stream, the rest on s3 while the next panel is factored; the W
+// scratch then holds two panels (2 x n x 64, alternating).  The plan joins
+// both after the last panel.
 struct InvPipeline {
     const TrsmBatch* inv = nullptr;
     hipStream_t s2 = nullptr;
     hipEvent_t ef = nullptr, ej = nullptr;
+    hipStream_t s3 = nullptr;
+    hipEvent_t ef3 = nullptr, eb = nullptr;
 };
 void addLdltBatch(Plan& plan, const std::vector<double*>& As, const std::vector<int64_t>& ns,
                   const std::vector<double*>& Ws, int* err, const std::vector<int>& nodes,
