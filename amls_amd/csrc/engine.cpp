@@ -877,6 +877,7 @@ void Ctx::setProblems(const std::vector<ProblemIn>& probs, int nLevels, int64_t 
         drainChain();   // a pending root reduction still reads the old arenas
         dropGeGraph();
         dropSiGraph();
+        dropSpGraphs();
         siKeySeen = false;
     }
     haveProblem = false;
@@ -1562,6 +1563,53 @@ hipGraphExec_t captureAndLaunch(Plan& plan, hipStream_t s) {
     return exec;
 }
 
+void Ctx::dropSpGraphs() {
+    if (spGraphB) (void)hipGraphExecDestroy(spGraphB);
+    if (spGraphC) (void)hipGraphExecDestroy(spGraphC);
+    spGraphB = spGraphC = nullptr;
+    spKeySeen = false;
+}
+
+// The forks of a solve's spectralPrepare after the main stream's part: the
+// other nodes' reductions on stream3 (evBatch at their end), the root's on
+// stream2 (evChain), each replayed from its graph when the key is unchanged.
+void Ctx::launchSpectralSide(const std::vector<EigDesc>& er, const std::vector<int64_t>& sr,
+                             const EigDesc& eroot, int64_t rootOff, int64_t big) {
+    const bool same = spKeySeen && spGraphGen == problemGen && spGraphBase == spMem->base() &&
+                      spGraphBig == big;
+    if (!same) dropSpGraphs();
+    AMLS_HIP_CHECK(hipEventRecord(evFork3, stream));
+    AMLS_HIP_CHECK(hipStreamWaitEvent(stream3, evFork3, 0));
+    if (same && spGraphB) {
+        AMLS_HIP_CHECK(hipGraphLaunch(spGraphB, stream3));
+    } else {
+        Plan batch(&fresh(spBlobB), &kstats);
+        addSytrd(batch, er, sr, dSv, dSp, &syLocal);
+        if (same)
+            spGraphB = captureAndLaunch(batch, stream3);
+        else
+            batch.run(stream3);
+    }
+    AMLS_HIP_CHECK(hipEventRecord(evBatch, stream3));
+    AMLS_HIP_CHECK(hipEventRecord(evFork, stream));
+    AMLS_HIP_CHECK(hipStreamWaitEvent(stream2, evFork, 0));
+    if (same && spGraphC) {
+        AMLS_HIP_CHECK(hipGraphLaunch(spGraphC, stream2));
+    } else {
+        Plan chain(&fresh(spBlobC), &kstats);
+        addSytrd(chain, {eroot}, {0}, dSv + 2 * rootOff, dSp + 3 * rootOff);
+        if (same)
+            spGraphC = captureAndLaunch(chain, stream2);
+        else
+            chain.run(stream2);
+    }
+    AMLS_HIP_CHECK(hipEventRecord(evChain, stream2));
+    spGraphGen = problemGen;
+    spGraphBase = spMem->base();
+    spGraphBig = big;
+    spKeySeen = true;
+}
+
 void Ctx::dropGeGraph() {
     if (geGraph) (void)hipGraphExecDestroy(geGraph);
     geGraph = nullptr;
@@ -1719,6 +1767,9 @@ void Ctx::spectralPrepare() {
     for (int q = 0; q < static_cast<int>(ed.size()) && split; ++q)
         if (q != big && soff[q] + ed[q].n > soff[big]) split = false;
     const bool overlap = split && overlapWanted;
+    bool sideGraphs = false;
+    std::vector<EigDesc> sideEr;
+    std::vector<int64_t> sideSr;
     if (split) {
         std::vector<EigDesc> er;
         std::vector<int64_t> sr;
@@ -1733,14 +1784,21 @@ void Ctx::spectralPrepare() {
             Copy2dBatch cc;
             cc.add(cdesc(ed[big].A, nbig, dCroot, nbig, nbig, nbig, 0));
             plan.copy(cc);
-            // the batch is enqueued first: the host takes longer to enqueue
-            // the root's 1,000-odd steps than the GPU takes to start them,
-            // and the batch leads the main stream's path to pass A
-            Plan chain(plan.blob, plan.st), batch(plan.blob, plan.st);
-            addSytrd(batch, er, sr, dSv, dSp, &syLocal);
-            plan.fork(batch, stream3, evFork3, evBatch);
-            addSytrd(chain, {ed[big]}, {0}, dSv + 2 * soff[big], dSp + 3 * soff[big]);
-            plan.fork(chain, stream2, evFork, evChain);
+            if (kstats.enabled || kstats.trace) {
+                // the batch is enqueued first: the host takes longer to
+                // enqueue the root's 1,000-odd steps than the GPU takes to
+                // start them, and the batch leads the main stream's path to
+                // pass A
+                Plan chain(plan.blob, plan.st), batch(plan.blob, plan.st);
+                addSytrd(batch, er, sr, dSv, dSp, &syLocal);
+                plan.fork(batch, stream3, evFork3, evBatch);
+                addSytrd(chain, {ed[big]}, {0}, dSv + 2 * soff[big], dSp + 3 * soff[big]);
+                plan.fork(chain, stream2, evFork, evChain);
+            } else {
+                sideGraphs = true;   // after the main part below
+                sideEr = er;
+                sideSr = sr;
+            }
         } else {
             Plan side(plan.blob, plan.st);
             addSytrd(side, er, sr, dSv, dSp, &syLocal);
@@ -1752,6 +1810,7 @@ void Ctx::spectralPrepare() {
         addSytrd(plan, ed, soff, dSv, dSp, &syLocal);
     }
     plan.run(stream);
+    if (sideGraphs) launchSpectralSide(sideEr, sideSr, ed[big], soff[big], big);
     if (overlap) {
         chainPending = batchPending = true;
         chainRoot = big;
@@ -3271,6 +3330,7 @@ amls_status amls_hip_destroy(amls_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->c.stream3);
         ctx->c.dropGeGraph();
         ctx->c.dropSiGraph();
+        ctx->c.dropSpGraphs();
     }
     hipStream_t s = ctx->c.stream, s2 = ctx->c.stream2, s3 = ctx->c.stream3;
     hipEvent_t e1 = ctx->c.evFork, e2 = ctx->c.evJoin, e3 = ctx->c.evChain;

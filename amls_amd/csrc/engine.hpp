@@ -321,6 +321,20 @@ struct Ctx {
     // ---- local spectra
     std::unique_ptr<DevArena> spMem;
     std::unique_ptr<Blob> spBlob;
+    // In a solve, the other nodes' reductions (stream3) and the root's
+    // (stream2) as two HIP graphs, with descriptor blobs of their own: the
+    // first run for a problem, arena address and root is eager, the second
+    // captures them, later ones replay them (runs with statistics or tracing
+    // stay eager).
+    std::unique_ptr<Blob> spBlobB, spBlobC;
+    hipGraphExec_t spGraphB = nullptr, spGraphC = nullptr;
+    uint64_t spGraphGen = 0;
+    const void* spGraphBase = nullptr;
+    int64_t spGraphBig = -1;
+    bool spKeySeen = false;
+    void dropSpGraphs();
+    void launchSpectralSide(const std::vector<dev::EigDesc>& er, const std::vector<int64_t>& sr,
+                            const dev::EigDesc& eroot, int64_t rootOff, int64_t big);
     std::vector<int64_t> co, vo;       // n_p^2 and n_p offsets
     double *dC = nullptr, *dT = nullptr, *dD = nullptr, *dE = nullptr, *dTau = nullptr;
     double *dSv = nullptr, *dSp = nullptr, *dDsq = nullptr;
