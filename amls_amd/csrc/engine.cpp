@@ -2996,9 +2996,22 @@ void Ctx::subspaceIter(int steps, int64_t q, int64_t nev, std::vector<double>& t
                     b2.n = qi;
                     gl2.add(b2);
                 }
-                plan.gemm(gu, false);
-                addChildUpdates(nodes, false, X);
-                plan.gemm(gl1, false);
+                // L^-1 y_p reads only the fronts' own rows, complete now: it
+                // runs on stream2 beside u_p and the children's boundary
+                // updates; the in-place L^-T step overwrites y_p, so it waits
+                // for both
+                if (pipelineOk()) {
+                    Plan side(plan.blob, plan.st);
+                    side.gemm(gl1, false);
+                    plan.fork(side, stream2, evFork, evJoin);
+                    plan.gemm(gu, false);
+                    addChildUpdates(nodes, false, X);
+                    plan.join(evJoin);
+                } else {
+                    plan.gemm(gu, false);
+                    addChildUpdates(nodes, false, X);
+                    plan.gemm(gl1, false);
+                }
                 plan.gemm(gl2, false);
             }
             // ---- backward in place: Y(p) -= X_p Y(B_p) top-down
