@@ -3319,6 +3319,10 @@ amls_status amls_hip_create(int32_t device, uint32_t flags, amls_ctx** out) {
         ctx->c.gpu = true;
         ctx->c.kstats.lane1 = ctx->c.stream;
     }
+    // AMLS_SERIAL_SCHEDULE (any non-empty value other than 0): panel chains
+    // and sweeps on one stream, for per-kernel profiling
+    if (const char* ss = std::getenv("AMLS_SERIAL_SCHEDULE"); ss && *ss && std::strcmp(ss, "0") != 0)
+        ctx->c.serialSchedule = true;
     // AMLS_TRACE=<file.json>: trace this context, written when it is destroyed
     if (const char* tp = std::getenv("AMLS_TRACE"); tp && *tp) {
         ctx->c.tracePath = tp;

@@ -229,8 +229,15 @@ struct Ctx {
     void drainChain();  // both, on the host
     bool silEligible(int64_t p) const;
     // stream2 is free for a plan's side work (no root reduction or batch of
-    // the condensation still on the side streams)
-    bool pipelineOk() const { return stream2 && stream3 && !chainPending && !batchPending; }
+    // the condensation still on the side streams).  With per-launch
+    // statistics on, or serialSchedule (environment AMLS_SERIAL_SCHEDULE at
+    // creation, for profilers), the panel chains and the forward sweep keep
+    // to one stream, so that each launch's measured duration is its own.
+    bool serialSchedule = false;
+    bool pipelineOk() const {
+        return !serialSchedule && !kstats.enabled && stream2 && stream3 && !chainPending &&
+               !batchPending;
+    }
     std::string err;
 
     // ---- problem (host)
